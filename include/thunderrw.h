@@ -1,0 +1,229 @@
+/*
+ * thunderrw.h -- C ABI of the MI355X-native Gather-Move-Update step engine.
+ *
+ * This is the boundary a caller of the reference's step engine binds to.
+ * The reference has no FFI of its own: its apps call the C++ template
+ *   template<class F> void compute(Graph&, std::vector<WalkerMeta>&, F)
+ * (random_walk/creeper.h:689), which reads the CSR arrays of class Graph
+ * (util/graph/graph.h:51-83) and the globals g_para / g_num_threads
+ * (random_walk/types.h:72-77,109-110).  Each entry point below names the part
+ * of that interface it replaces.  Plain pointers and sizes only; every call
+ * returns an int status (TRW_OK = 0, negative on error) and never aborts;
+ * trw_last_error() gives a thread-local message for the last failure.
+ */
+#ifndef THUNDERRW_H
+#define THUNDERRW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define TRW_OK 0
+#define TRW_E_INVALID (-1)     /* bad argument */
+#define TRW_E_HIP (-2)         /* HIP runtime error (message has the HIP error string) */
+#define TRW_E_NOMEM (-3)       /* device or host allocation failed */
+#define TRW_E_UNSUPPORTED (-4) /* valid in the reference, not provided on the device */
+#define TRW_E_STATE (-5)       /* call order: e.g. walking before the tables are built */
+#define TRW_E_IO (-6)          /* file missing, short or inconsistent */
+
+/* ExecutionMode, random_walk/types.h:41-45 (values preserved). */
+#define TRW_EXEC_UNIFORM 0
+#define TRW_EXEC_STATIC 1
+#define TRW_EXEC_DYNAMIC 2
+
+/* SamplingMethod, random_walk/types.h:53-59 (values preserved). */
+#define TRW_SM_UNIFORM 0
+#define TRW_SM_ITS 1
+#define TRW_SM_ALIAS 2
+#define TRW_SM_REJECTION 3
+#define TRW_SM_MAX_WEIGHT_REJECTION 4
+
+/* Built-in apps: the functors of random_walk/apps/{deepwalk,ppr,node2vec,metapath,genericwalk}.cpp. */
+#define TRW_APP_DEEPWALK 0
+#define TRW_APP_PPR 1
+#define TRW_APP_NODE2VEC 2
+#define TRW_APP_METAPATH 3
+#define TRW_APP_GENERIC 4
+
+/* Random stream.  REPLAY: walker i's move draws are words [i*K, (i+1)*K) of the
+ * SFMT-19937 stream init_gen_rand(seed) (PPR termination draws: a second stream),
+ * handed to the device by the caller (trw_replay_fill).  PHILOX: counter-based
+ * Philox-4x32-10 keyed by (seed, global walker id, step, trial). */
+#define TRW_RNG_REPLAY 0
+#define TRW_RNG_PHILOX 1
+
+/* Kernel form.  AUTO picks FUSED for fixed-length apps and QUEUE otherwise. */
+#define TRW_FORM_AUTO 0
+#define TRW_FORM_FUSED 1 /* thread per walker, whole walk in one launch, LDS-staged path tiles */
+#define TRW_FORM_QUEUE 2 /* persistent kernel, lanes refill from a device work queue */
+#define TRW_FORM_STEP 3  /* plain form: one launch per step over walker state in HBM */
+
+/* Per-walker outcome (status[] output). */
+#define TRW_WALKER_DONE 0              /* update() ended it (length reached / PPR stop draw) */
+#define TRW_WALKER_GATHER_TERMINATED 1 /* all transition weights were zero (creeper.h:267-293) */
+#define TRW_WALKER_DEAD_END 2          /* current vertex has degree 0 (undefined in the reference) */
+#define TRW_WALKER_REPLAY_EXHAUSTED 3  /* replay draws ran out: an error, never a wrap */
+#define TRW_WALKER_TRIAL_CAP 4         /* rejection trial cap hit */
+#define TRW_WALKER_BAD_SOURCE 5        /* source vertex outside [0, num_vertices) */
+
+/* Weights F::weight gives static_initialization (creeper.h:24-99): DeepWalk and
+ * GenericWalk return edge_weight_[eid] (deepwalk.cpp:19-21), PPR returns 1.0 (ppr.cpp:23-25). */
+#define TRW_WEIGHTS_EDGE 0
+#define TRW_WEIGHTS_UNIT 1
+
+/* AliasSlot, config/type_config.h:22-26 (16 bytes, identical layout). */
+typedef struct {
+    double alias_value;
+    int32_t first;
+    int32_t second;
+} trw_alias_slot;
+
+/* g_para (types.h:72-77) plus the app functor's constructor arguments. */
+typedef struct {
+    int32_t app;            /* TRW_APP_* */
+    int32_t execution;      /* TRW_EXEC_*, -em */
+    int32_t sampling;       /* TRW_SM_*, -sm */
+    int32_t length;         /* g_para.length_ (-l; PPR uses 1, ppr.cpp:51) */
+    double stop_probability;/* PPR -sp (ppr.cpp:46-49) */
+    double p, q;            /* Node2Vec -p, -q as given; inverted as node2vec.cpp:69,76 */
+    int32_t schema[16];     /* MetaPath -s (metapath.cpp:69-83) */
+    int32_t schema_length;
+    int32_t rng;            /* TRW_RNG_* */
+    uint64_t seed;          /* Philox key (PHILOX) */
+    int32_t form;           /* TRW_FORM_* */
+    int32_t max_trials;     /* rejection trials per step before TRW_WALKER_TRIAL_CAP; 0 = no cap */
+} trw_walk_params;
+
+/* Walker batch: all pointers are DEVICE pointers (nullable where noted). */
+typedef struct {
+    const int32_t *sources;        /* n_walkers start vertices (WalkerMeta.source_, walker.h:9-15) */
+    int32_t n_walkers;
+    int32_t path_stride;           /* ints per path row, >= max(length, 1) */
+    int64_t walker_id_base;        /* global id of sources[0] (keys the Philox stream) */
+    const uint32_t *draws;         /* REPLAY move draws, layout [k][n_walkers], k < draws_per_walker */
+    int64_t draws_per_walker;
+    const uint32_t *term_draws;    /* REPLAY PPR termination draws, [k][n_walkers] */
+    int64_t term_draws_per_walker;
+    int32_t *paths;                /* nullable: n_walkers x path_stride, seq[0] = source, -1 padded */
+    int32_t *lengths;              /* nullable: final WalkerMeta.length_ */
+    int32_t *ends;                 /* nullable: final current vertex */
+    int32_t *status;               /* nullable: TRW_WALKER_* */
+    int32_t *draws_used;           /* nullable: REPLAY move draws consumed */
+    uint64_t *counters;            /* nullable: [0] steps [1] gather-terminated [2] dead ends [3] errors; accumulated */
+} trw_walk_io;
+
+typedef struct trw_context trw_context;
+typedef struct trw_graph trw_graph;
+
+/* ---- errors and devices ---- */
+const char *trw_last_error(void);
+int trw_version(int32_t *major, int32_t *minor);
+int trw_device_count(int32_t *count);
+
+/* One context per device; replaces the per-process globals (creeper_main.h:12). */
+int trw_context_create(int32_t device, trw_context **out);
+int trw_context_destroy(trw_context *ctx);
+/* The HIP stream the blocking entry points use (hipStream_t as void*). */
+int trw_context_stream(trw_context *ctx, void **stream);
+
+/* ---- device memory and timing ----
+ * Plumbing with no counterpart in the reference (its arrays live in host memory):
+ * buffers in HBM for walker batches, copies (synchronous on the context stream),
+ * a zero fill that is a kernel (safe under stream capture), and a device-time
+ * measurement with HIP events recorded on a given stream (NULL = context stream). */
+int trw_device_alloc(trw_context *ctx, uint64_t bytes, void **out);
+int trw_device_free(trw_context *ctx, void *ptr);
+int trw_copy_to_device(trw_context *ctx, void *dst, const void *src, uint64_t bytes);
+int trw_copy_to_host(trw_context *ctx, void *dst, const void *src, uint64_t bytes);
+int trw_device_zero(trw_context *ctx, void *ptr, uint64_t bytes, void *stream);
+int trw_synchronize(trw_context *ctx, void *stream);
+int trw_timer_begin(trw_context *ctx, void *stream);
+int trw_timer_end(trw_context *ctx, void *stream, double *elapsed_ms);
+
+/* ---- graph ----
+ * Graph::load_csr (util/graph/graph.cpp:45-101) from host arrays: degree[nv]
+ * (b_degree.bin), adj[ne] sorted per vertex (b_adj.bin), optional FP64 weights
+ * (b_edge_weight.bin) and int32 labels (b_edge_label.bin).  offset_ is the
+ * exclusive prefix of degree (graph.cpp:258-263).  The library copies the arrays
+ * to HBM; the caller keeps ownership of its host memory. */
+int trw_graph_upload(trw_context *ctx, int32_t num_vertices, int64_t num_edges, const int32_t *degree,
+                     const int32_t *adj, const double *edge_weight, const int32_t *edge_label, trw_graph **out);
+int trw_graph_free(trw_graph *g);
+int trw_graph_info(const trw_graph *g, int32_t *num_vertices, int64_t *num_edges, int32_t *max_degree);
+
+/* ---- static tables ----
+ * static_initialization (creeper.h:102-175) on the device: alias tables
+ * (alias_initialization, alias_sampling.h:12-54), ITS prefix sums
+ * (its_initialization, inverse_transformation_sampling.h:19-24) or rejection
+ * maxima (rs_initialization, rejection_sampling.h:12-14), byte-identical to the
+ * reference's.  weights: TRW_WEIGHTS_EDGE or TRW_WEIGHTS_UNIT. */
+int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights);
+/* Copies of the built tables into host memory (b_edge_weight_alias_table.bin,
+ * b_edge_weight_prefix_sum.bin, b_edge_weight_max.bin layouts, graph.cpp:184-206). */
+int trw_copy_alias_table(const trw_graph *g, trw_alias_slot *host_out);
+int trw_copy_prefix_sum(const trw_graph *g, double *host_out);
+int trw_copy_rejection_max(const trw_graph *g, double *host_out);
+/* Device time of the last trw_build_tables call in milliseconds. */
+int trw_last_build_ms(const trw_graph *g, double *ms);
+
+/* ---- walks ----
+ * compute<F>(graph, walkers, f) (creeper.h:689-831) for a built-in app, on a
+ * caller stream (hipStream_t as void*; NULL = the context stream).  Asynchronous:
+ * returns after the launches are queued. */
+int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream);
+
+/* Two walker kinds in one batch: walkers [0, split) run params_a, [split, n) run
+ * params_b (the uneven PPR + MetaPath mix).  form: QUEUE or STEP. */
+int trw_walk_mix_async(trw_graph *g, const trw_walk_params *params_a, const trw_walk_params *params_b,
+                       int32_t split, const trw_walk_io *io, void *stream);
+
+/* Blocking host-memory convenience: copies sources/draws in and results out.
+ * draws/term_draws are host arrays in the [k][n_walkers] layout (may be NULL for PHILOX);
+ * paths is n_walkers x max(length,1); outputs are nullable. */
+int trw_walk_host(trw_graph *g, const trw_walk_params *params, const int32_t *sources, int32_t n_walkers,
+                  const uint32_t *draws, int64_t draws_per_walker, const uint32_t *term_draws,
+                  int64_t term_draws_per_walker, int32_t *paths, int32_t *lengths, int32_t *ends, int32_t *status,
+                  uint64_t *steps);
+
+/* ---- replay stream (host) ----
+ * Walker i's draws are words [i*K, (i+1)*K) of the SFMT-19937 stream
+ * init_gen_rand(seed) (SFMT.c:349-362); written for walkers
+ * [first_walker, first_walker + n_walkers) into out[k * n_walkers + (i - first_walker)]. */
+int trw_replay_fill(uint32_t seed, int64_t first_walker, int32_t n_walkers, int64_t draws_per_walker,
+                    uint32_t *host_out);
+
+/* ---- walker init (walker_generator.cpp) ---- */
+/* generate_all_source (walker_generator.cpp:9-25): non-isolated vertices,
+ * shuffled by std::shuffle with std::mt19937(0) (utility.h:96-100).  out has room for nv. */
+int trw_sources_all(const int32_t *degree, int32_t num_vertices, int32_t *out, int32_t *count);
+/* generate_single_source (walker_generator.cpp:42-63): n copies of the first max-degree vertex. */
+int trw_sources_single(const int32_t *degree, int32_t num_vertices, int32_t n, int32_t *out);
+/* generate_random_source (walker_generator.cpp:27-40): SFMT(seed) u % nv, skipping degree 0. */
+int trw_sources_random(const int32_t *degree, int32_t num_vertices, int32_t n, uint32_t seed, int32_t *out);
+
+/* ---- host graphs: synthetic generator and the reference's CSR files ---- */
+typedef struct trw_host_graph trw_host_graph;
+/* Seeded Chung-Lu power-law graph: expected degrees dmin*(1-U)^(-1/(gamma-1)) capped at dmax,
+ * endpoint pairs drawn proportional to them until the undirected edge count reaches the target,
+ * self-loops and duplicates dropped, symmetrised and sorted as EdgeList2CSR does
+ * (graph.cpp:463-519).  Weights U[1,5) and labels U{0..n_labels-1} per undirected edge, mirrored. */
+int trw_host_graph_chung_lu(int32_t num_vertices, int64_t undirected_edges, double gamma, double dmin,
+                            double dmax, uint64_t seed, int32_t with_weights, int32_t n_labels,
+                            trw_host_graph **out);
+/* Graph::load_csr file set (graph.cpp:233-426): b_degree.bin, b_adj.bin, optional
+ * b_edge_weight.bin / b_edge_label.bin. */
+int trw_host_graph_load(const char *dir, int32_t with_weights, int32_t with_labels, trw_host_graph **out);
+int trw_host_graph_store(const trw_host_graph *h, const char *dir);
+int trw_host_graph_arrays(const trw_host_graph *h, int32_t *num_vertices, int64_t *num_edges,
+                          const int32_t **degree, const int32_t **adj, const double **edge_weight,
+                          const int32_t **edge_label);
+int trw_host_graph_free(trw_host_graph *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* THUNDERRW_H */

@@ -443,6 +443,10 @@ int32_t orc_walk(const orc_graph *g, const orc_params *p, const int32_t *sources
         int32_t cur = sources[i], prev = -1, len = 1, st = ORC_ST_DONE;
         int32_t *seq = paths ? paths + (int64_t)i * (L > 0 ? L : 1) : NULL;
         if (seq && L > 0) seq[0] = cur;
+        if (cur < 0 || cur >= g->nv) {
+            st = ORC_ST_BAD_SOURCE;
+            goto walker_end;
+        }
 
 #define NEXT_DRAW(var)                                  \
     do {                                                \
@@ -459,6 +463,16 @@ int32_t orc_walk(const orc_graph *g, const orc_params *p, const int32_t *sources
             int64_t off = g->offset[cur];
             const int32_t *nb = g->adj + off;
             double dyn_max = 0.0;
+
+            /* Static tables of a vertex whose weights sum to zero: the reference
+             * reads uninitialised or out-of-segment memory here; the engine defines
+             * it as the dynamic gather's termination (creeper.h:267-293). */
+            if (exec == ORC_EXEC_STATIC &&
+                ((sm == ORC_SM_ALIAS && g->alias[off].first < 0) || (sm == ORC_SM_ITS && g->prefix[off + d - 1] == 0.0) ||
+                 (sm == ORC_SM_REJECTION && g->rs_max[cur] == 0.0))) {
+                st = ORC_ST_GATHER_TERMINATED;
+                break;
+            }
 
             if (exec == ORC_EXEC_DYNAMIC && sm != ORC_SM_MAXREJECTION) {
                 for (int32_t j = 0; j < d; ++j) wbuf[j] = app_weight(&c, len, prev, nb[j], off + j);
@@ -494,7 +508,8 @@ int32_t orc_walk(const orc_graph *g, const orc_params *p, const int32_t *sources
                 NEXT_DRAW(u);
                 const double *P = (exec == ORC_EXEC_STATIC) ? g->prefix + off : wbuf;
                 double target = orc_to_real2(u) * P[d - 1];
-                next = nb[orc_upper_bound(P, d, target)];
+                int32_t pos = orc_upper_bound(P, d, target);
+                next = nb[pos < d ? pos : d - 1];   /* pos == d only for negative weights */
             } else if (sm == ORC_SM_REJECTION) {
                 const double *W = (exec == ORC_EXEC_STATIC) ? g->rs_weight + off : wbuf;
                 double m = (exec == ORC_EXEC_STATIC) ? g->rs_max[cur] : dyn_max;

@@ -52,7 +52,7 @@ enum { ORC_APP_DEEPWALK = 0, ORC_APP_PPR = 1, ORC_APP_NODE2VEC = 2, ORC_APP_META
 enum { ORC_EXEC_UNIFORM = 0, ORC_EXEC_STATIC = 1, ORC_EXEC_DYNAMIC = 2 };
 enum { ORC_SM_UNIFORM = 0, ORC_SM_ITS = 1, ORC_SM_ALIAS = 2, ORC_SM_REJECTION = 3, ORC_SM_MAXREJECTION = 4 };
 enum { ORC_ST_DONE = 0, ORC_ST_GATHER_TERMINATED = 1, ORC_ST_DEAD_END = 2, ORC_ST_REPLAY_EXHAUSTED = 3,
-       ORC_ST_TRIAL_CAP = 4 };
+       ORC_ST_TRIAL_CAP = 4, ORC_ST_BAD_SOURCE = 5 };
 
 typedef struct {
     int32_t app;

@@ -1,0 +1,159 @@
+"""The C-ABI library loads, exports what include/thunderrw.h declares, keeps the
+struct layouts the header defines, and its host-side parts (replay stream,
+walker sources, generator, CSR files, argument checks) behave.  No GPU compute."""
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import graphs
+import oracle_bind as ob
+import thunderrw_amd as T
+from conftest import ROOT
+from thunderrw_amd import _abi as A
+
+HEADER = os.path.join(ROOT, "include", "thunderrw.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(trw_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_function():
+    lib = T.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), name
+    assert sorted(A.exported_names()) == names
+
+
+def test_struct_layouts_match_the_header():
+    src = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "thunderrw.h"
+#define O(t, f) printf("%s.%s %zu\n", #t, #f, offsetof(t, f))
+int main(void) {
+  printf("params %zu io %zu slot %zu\n", sizeof(trw_walk_params), sizeof(trw_walk_io), sizeof(trw_alias_slot));
+  O(trw_walk_params, stop_probability); O(trw_walk_params, schema); O(trw_walk_params, schema_length);
+  O(trw_walk_params, rng); O(trw_walk_params, seed); O(trw_walk_params, form); O(trw_walk_params, max_trials);
+  O(trw_walk_io, n_walkers); O(trw_walk_io, walker_id_base); O(trw_walk_io, draws_per_walker);
+  O(trw_walk_io, term_draws); O(trw_walk_io, paths); O(trw_walk_io, counters);
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as tmp:
+        c = os.path.join(tmp, "layout.c")
+        open(c, "w").write(src)
+        exe = os.path.join(tmp, "layout")
+        subprocess.run(["gcc", "-I", os.path.dirname(HEADER), "-o", exe, c], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    sizes = out[0].split()
+    import ctypes
+    assert int(sizes[1]) == ctypes.sizeof(A.WalkParams)
+    assert int(sizes[3]) == ctypes.sizeof(A.WalkIO)
+    assert int(sizes[5]) == ctypes.sizeof(A.AliasSlot) == 16
+    for line in out[1:]:
+        if not line:
+            continue
+        name, off = line.split()
+        t, f = name.split(".")
+        cls = A.WalkParams if t == "trw_walk_params" else A.WalkIO
+        assert getattr(cls, f).offset == int(off), name
+
+
+def test_replay_fill_is_the_chunked_sfmt_stream():
+    n, K = 53, 23
+    d = T.replay_fill(1234, n, K)
+    s = ob.sfmt_stream(1234, n * K).reshape(n, K)
+    assert d.shape == (K, n)
+    assert np.array_equal(d, s.T)
+    assert np.array_equal(T.replay_fill(1234, 7, K, first_walker=40), s[40:47].T)
+
+
+def test_sources_single_and_random_follow_walker_generator():
+    g, _ = graphs.chung_lu(500, 1500, 2.1, 2, 60, 3), None
+    deg = g.degree.copy()
+    deg[:50] = 0
+    assert np.all(T.sources_single(deg, 10) == int(np.argmax(deg)))
+    got = T.sources_random(deg, 300, seed=9)
+    # generate_random_source (walker_generator.cpp:27-40): SFMT(seed) u % n, skipping degree 0.
+    stream = ob.sfmt_stream(9, 5000)
+    expect = [int(u % len(deg)) for u in stream if deg[int(u % len(deg))] != 0][:300]
+    assert got.tolist() == expect
+
+
+def test_sources_all_is_a_permutation_of_live_vertices():
+    deg = np.array([3, 0, 1, 0, 2, 5, 0, 1], dtype=np.int32)
+    s = T.sources_all(deg)
+    assert sorted(s.tolist()) == [0, 2, 4, 5, 7]
+    assert np.array_equal(s, T.sources_all(deg))
+
+
+def test_generator_is_a_simple_symmetric_sorted_graph():
+    h = T.HostGraph.chung_lu(20000, 80000, 2.1, 2, 500, 7, True, 5)
+    deg, adj, w, lab = h.arrays()
+    n = len(deg)
+    assert abs(deg.sum() / 2 - 80000) <= 0.02 * 80000
+    off = np.zeros(n + 1, np.int64)
+    np.cumsum(deg, out=off[1:])
+    u = np.repeat(np.arange(n), deg)
+    assert np.all(u != adj)
+    row_break = np.zeros(len(adj), bool)
+    row_break[off[1:-1][deg[1:] > 0]] = True
+    assert np.all((np.diff(adj) > 0) | row_break[1:])
+    key = np.minimum(u, adj).astype(np.int64) * n + np.maximum(u, adj)
+    o = np.argsort(key, kind="stable")
+    assert np.array_equal(key[o][0::2], key[o][1::2])
+    assert np.array_equal(w[o][0::2], w[o][1::2]) and np.array_equal(lab[o][0::2], lab[o][1::2])
+    assert w.min() >= 1.0 and w.max() < 5.0 and set(np.unique(lab)) <= set(range(5))
+    h2 = T.HostGraph.chung_lu(20000, 80000, 2.1, 2, 500, 7, True, 5)
+    assert all(np.array_equal(a, b) for a, b in zip(h.arrays(), h2.arrays()))
+
+
+def test_reference_csr_files_round_trip():
+    g = graphs.chung_lu(300, 900, 2.1, 2, 50, 5)
+    with tempfile.TemporaryDirectory() as tmp:
+        graphs.write_reference_csr(g, tmp)
+        h = T.HostGraph.load(tmp, True, True)
+        deg, adj, w, lab = h.arrays()
+        assert np.array_equal(deg, g.degree) and np.array_equal(adj, g.adj)
+        assert np.array_equal(w, g.weight) and np.array_equal(lab, g.label)
+        out = os.path.join(tmp, "copy")
+        h.store(out)
+        for f in ("b_degree.bin", "b_adj.bin", "b_edge_weight.bin", "b_edge_label.bin"):
+            assert open(os.path.join(tmp, f), "rb").read() == open(os.path.join(out, f), "rb").read()
+
+
+def test_reference_csr_load_rejects_inconsistent_files():
+    g = graphs.chung_lu(100, 300, 2.1, 2, 30, 5)
+    with tempfile.TemporaryDirectory() as tmp:
+        graphs.write_reference_csr(g, tmp)
+        with open(os.path.join(tmp, "b_adj.bin"), "r+b") as f:
+            f.truncate(8)
+        with pytest.raises(T.TrwError) as e:
+            T.HostGraph.load(tmp, False, False)
+        assert e.value.code == A.E_IO
+        with pytest.raises(T.TrwError):
+            T.HostGraph.load(os.path.join(tmp, "missing"), False, False)
+
+
+def test_argument_errors_are_status_codes():
+    assert T.lib().trw_replay_fill(1, 0, -1, 3, None) == A.E_INVALID
+    assert b"bad argument" in T.lib().trw_last_error()
+    with pytest.raises(T.TrwError):
+        T.HostGraph.chung_lu(10, 1000, 2.1, 2, 5, 1)
+    with pytest.raises(ValueError):
+        T.make_params("metapath", schema=list(range(17)))
+
+
+@pytest.mark.skipif(T.device_count() > 0, reason="checks the no-device path")
+def test_context_without_device_reports_hip_error():
+    with pytest.raises(T.TrwError) as e:
+        T.Context(0)
+    assert e.value.code == A.E_HIP

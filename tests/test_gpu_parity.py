@@ -1,0 +1,283 @@
+"""Parity of the HIP engine with the reference (golden fixtures) and the oracle.
+
+Bar: bit-exact tables, paths, lengths, end vertices, statuses, draw counts and
+step counts in replay mode; statistical agreement in free-running (Philox) mode.
+All calls go through the C ABI (libthunderrw.so)."""
+import os
+
+import numpy as np
+import pytest
+
+import graphs
+import oracle_bind as ob
+import thunderrw_amd as T
+from conftest import GOLDEN, golden_case_args, load_golden_graph
+
+pytestmark = pytest.mark.gpu
+
+STATIC_SM = {1: "its", 2: "alias", 3: "rejection"}
+
+
+def dgraph(ctx, g):
+    return T.DeviceGraph(ctx, g.degree, g.adj, g.weight, g.label)
+
+
+def case_params(case, form="auto", rng="replay", seed=0):
+    a = golden_case_args(case)
+    return T.make_params(case["app"], execution=case["execution"], sampling=case["sampling"], length=case["length"],
+                         stop_probability=a["stop_prob"], p=a["p"], q=a["q"], schema=a["schema"], rng=rng,
+                         seed=seed, form=form)
+
+
+def forms_for(app):
+    return ["queue", "step"] if app == "ppr" else ["fused", "queue", "step"]
+
+
+@pytest.mark.parametrize("gname", ["small", "hub"])
+@pytest.mark.parametrize("app", ["deepwalk", "ppr"])
+def test_device_tables_are_byte_identical_to_reference(gpu_ctx, gname, app):
+    g, _ = load_golden_graph(gname)
+    z = np.load(os.path.join(GOLDEN, f"tables_{gname}_{app}.npz"))
+    dg = dgraph(gpu_ctx, g)
+    w = "edge" if app == "deepwalk" else "unit"
+    try:
+        for sm in ("alias", "its", "rejection"):
+            dg.build_tables(sm, w)
+        assert np.array_equal(dg.alias_table().view(np.uint8).reshape(-1, 16), z["alias"])
+        assert np.array_equal(dg.prefix_sum().view(np.uint64), z["prefix"].view(np.uint64))
+        assert np.array_equal(dg.rejection_max(), z["rs_max"])
+    finally:
+        dg.close()
+
+
+def test_golden_walks_in_every_form(gpu_ctx, golden_index, golden_graphs):
+    seed, tseed = golden_index["seed"], golden_index["tseed"]
+    devs = {name: dgraph(gpu_ctx, g) for name, (g, _) in golden_graphs.items()}
+    try:
+        for case in golden_index["walks"]:
+            if case["app"] == "metapath" and case["sampling"] != 1:
+                continue  # dynamic alias over label weights is not provided on the device
+            g, src = golden_graphs[case["graph"]]
+            z = np.load(os.path.join(GOLDEN, f"walk_{case['case']}.npz"))
+            n = len(src)
+            draws = T.replay_fill(seed, n, case["K"])
+            tdraws = T.replay_fill(tseed, n, max(case["KT"], 1))
+            for form in forms_for(case["app"]):
+                r = T.walk(devs[case["graph"]], case_params(case, form), src, draws, tdraws)
+                tag = (case["case"], form)
+                assert np.array_equal(r["paths"], z["paths"]), tag
+                assert np.array_equal(r["lengths"], z["lengths"]), tag
+                assert np.array_equal(r["ends"], z["ends"]), tag
+                assert np.array_equal(r["draws_used"], z["draws_used"]), tag
+                assert np.array_equal((r["status"] == 1).astype(np.int32), z["status"]), tag
+                assert np.all((r["status"] == 0) | (r["status"] == 1)), tag
+                assert r["steps"] == int(z["steps"][0]), tag
+    finally:
+        for d in devs.values():
+            d.close()
+
+
+def oracle_walk(g, prm_case, src, K, KT, seed, tseed):
+    w = g.weight if prm_case["app"] != "ppr" else np.ones(g.ne)
+    alias, _ = ob.alias_build(g, w)
+    og = ob.Graph(g, alias, ob.its_build(g, w), w, ob.rs_build(g, w))
+    n = len(src)
+    d = ob.sfmt_stream(seed, n * K).reshape(n, K)
+    td = ob.sfmt_stream(tseed, n * KT).reshape(n, KT)
+    a = golden_case_args(prm_case)
+    return ob.walk(og, prm_case["app"], prm_case["execution"], prm_case["sampling"], prm_case["length"], src, d, K,
+                   td, KT, stop_prob=a["stop_prob"], p=a["p"], q=a["q"], schema=a["schema"])
+
+
+RANDOM_CASES = [
+    ("deepwalk", 1, 2, 30, {}), ("deepwalk", 1, 1, 30, {}), ("deepwalk", 1, 3, 30, {}), ("deepwalk", 0, 0, 30, {}),
+    ("deepwalk", 2, 1, 30, {}), ("genericwalk", 1, 2, 17, {}), ("ppr", 0, 0, 1, {"-sp": "0.15"}),
+    ("ppr", 1, 2, 3, {"-sp": "0.2"}), ("node2vec", 2, 4, 30, {"-p": "2", "-q": "0.5"}),
+    ("node2vec", 2, 4, 30, {"-p": "0.25", "-q": "4"}), ("metapath", 2, 1, 30, {"-s": "0,1,2,3,4"}),
+    ("metapath", 2, 1, 30, {"-s": "3"}),
+]
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_random_graphs_match_oracle(gpu_ctx, seed):
+    g = graphs.chung_lu(6000, 30000, 2.0, 2, 2500, seed)
+    g2 = graphs.hub_graph(4000, 3500, 9000, seed + 1)
+    for gg in (g, g2):
+        dg = dgraph(gpu_ctx, gg)
+        try:
+            live = np.nonzero(gg.degree > 0)[0].astype(np.int32)
+            rng = np.random.default_rng(seed)
+            src = np.concatenate([live, rng.choice(live, 3000), np.full(64, int(np.argmax(gg.degree)), np.int32)])
+            src = src.astype(np.int32)
+            for app, em, sm, L, extra in RANDOM_CASES:
+                case = {"app": app, "execution": em, "sampling": sm, "length": L, "extra": extra}
+                K, KT = 4 * L + 256, 160
+                ref = oracle_walk(gg, case, src, K, KT, seed, seed + 7)
+                draws = T.replay_fill(seed, len(src), K)
+                tdraws = T.replay_fill(seed + 7, len(src), KT)
+                for form in forms_for(app):
+                    r = T.walk(dg, case_params(case, form), src, draws, tdraws)
+                    tag = (app, em, sm, form)
+                    for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                        assert np.array_equal(r[key], ref[key]), (tag, key)
+                    assert r["steps"] == ref["steps"], tag
+        finally:
+            dg.close()
+
+
+def test_replay_exhaustion_and_bad_sources_are_reported(gpu_ctx):
+    g, src = load_golden_graph("small")
+    dg = dgraph(gpu_ctx, g)
+    try:
+        src = np.concatenate([src[:50], np.array([-1, g.nv, 10 ** 6], np.int32)]).astype(np.int32)
+        case = {"app": "deepwalk", "execution": 1, "sampling": 2, "length": 20, "extra": {}}
+        K = 11  # fewer than the 38 draws a 20-vertex alias walk needs
+        ref = oracle_walk(g, case, src, K, 1, 5, 6)
+        assert np.all(ref["status"][:50] == T._abi.WALKER_REPLAY_EXHAUSTED)
+        assert np.all(ref["status"][50:] == T._abi.WALKER_BAD_SOURCE)
+        draws = T.replay_fill(5, len(src), K)
+        for form in forms_for("deepwalk"):
+            r = T.walk(dg, case_params(case, form), src, draws, T.replay_fill(6, len(src), 1))
+            for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                assert np.array_equal(r[key], ref[key]), (form, key)
+            assert r["errors"] == len(src)
+    finally:
+        dg.close()
+
+
+def test_forms_agree_in_free_running_mode(gpu_ctx):
+    g = graphs.chung_lu(20000, 120000, 2.1, 2, 3000, 41)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        src = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        for app, em, sm, L in [("deepwalk", 1, 2, 40), ("deepwalk", 1, 1, 40), ("node2vec", 2, 4, 40),
+                               ("ppr", 0, 0, 1), ("metapath", 2, 1, 40)]:
+            outs = []
+            for form in forms_for(app):
+                prm = T.make_params(app, execution=em, sampling=sm, length=L, rng="philox", seed=77, form=form,
+                                    schema=[0, 1, 2, 3, 4], stop_probability=0.15)
+                outs.append(T.walk(dg, prm, src))
+            for o in outs[1:]:
+                for key in ("paths", "lengths", "ends", "status"):
+                    assert np.array_equal(o[key], outs[0][key]), (app, key)
+                assert o["steps"] == outs[0]["steps"]
+    finally:
+        dg.close()
+
+
+def _chi2_pvalue(counts, probs):
+    from scipy.stats import chisquare
+    n = counts.sum()
+    return chisquare(counts, probs * n).pvalue
+
+
+@pytest.mark.parametrize("sm", ["alias", "its", "rejection", "uniform"])
+def test_free_running_first_step_transitions_chi_square(gpu_ctx, sm):
+    g = graphs.hub_graph(3000, 400, 6000, 51)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        hub = int(np.argmax(g.degree))
+        n = 400000
+        src = np.full(n, hub, np.int32)
+        em = "uniform" if sm == "uniform" else "static"
+        prm = T.make_params("deepwalk", execution=em, sampling=sm, length=2, rng="philox", seed=123)
+        r = T.walk(dg, prm, src)
+        nb = g.neighbors(hub)
+        lo, hi = g.offset[hub], g.offset[hub + 1]
+        w = np.ones(len(nb)) if sm == "uniform" else g.weight[lo:hi]
+        pos = np.searchsorted(nb, r["paths"][:, 1])
+        counts = np.bincount(pos, minlength=len(nb)).astype(np.float64)
+        assert _chi2_pvalue(counts, w / w.sum()) > 1e-4
+    finally:
+        dg.close()
+
+
+def test_free_running_ppr_matches_power_iteration(gpu_ctx):
+    g = graphs.chung_lu(5000, 20000, 2.1, 2, 500, 61)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        live = np.nonzero(g.degree > 0)[0]
+        sources = live[:7].astype(np.int32)
+        n = 1 << 20
+        src = np.resize(sources, n).astype(np.int32)
+        for prm, weighted in [(T.make_params("ppr", rng="philox", seed=9, stop_probability=0.2), False),
+                              (T.make_params("ppr", execution="static", sampling="alias", rng="philox", seed=9,
+                                             stop_probability=0.2), False)]:
+            r = T.walk(dg, prm, src, want_paths=False)
+            assert np.all(r["status"] == 0)
+            pi, _ = ob.ppr_power(g, 0.2, sources, weighted=weighted)
+            est = np.bincount(r["ends"], minlength=g.nv) / n
+            bound = np.sum(np.sqrt(pi * (1 - pi) / n))
+            assert np.abs(est - pi).sum() < 1.5 * bound
+            mean_len = (r["lengths"] - 1).mean()
+            assert abs(mean_len - 1 / 0.2) < 0.05
+    finally:
+        dg.close()
+
+
+def test_mixed_batch_queue_equals_step_and_single_kinds(gpu_ctx):
+    g = graphs.chung_lu(30000, 200000, 2.1, 2, 4000, 71)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        live = np.nonzero(g.degree > 0)[0]
+        rng = np.random.default_rng(3)
+        n = 200000
+        src = rng.choice(live, n).astype(np.int32)
+        split = n // 2
+        res = {}
+        for form in ("queue", "step"):
+            a = T.make_params("ppr", length=80, stop_probability=0.15, rng="philox", seed=5, form=form)
+            b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=5, form=form)
+            res[form] = T.walk(dg, a, src, mix=(b, split))
+        for key in ("paths", "lengths", "ends", "status"):
+            assert np.array_equal(res["queue"][key], res["step"][key]), key
+        # Each kind alone (keyed by the same global walker ids) gives the same walks as in the mix.
+        a = T.make_params("ppr", length=80, stop_probability=0.15, rng="philox", seed=5)
+        b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=5)
+        ra = T.walk(dg, a, src[:split])
+        rb = T.walk(dg, b, src[split:], walker_id_base=split)
+        for key in ("paths", "lengths", "ends", "status"):
+            assert np.array_equal(res["queue"][key][:split], ra[key]), key
+            assert np.array_equal(res["queue"][key][split:], rb[key]), key
+        assert res["queue"]["steps"] == ra["steps"] + rb["steps"]
+    finally:
+        dg.close()
+
+
+def test_full_size_deepwalk_alias_graph(gpu_ctx):
+    """2^22-vertex Chung-Lu graph: device tables equal the oracle's, replay paths of a
+    walker subset equal the oracle's, and every free-running step follows an edge."""
+    h = T.HostGraph.chung_lu(1 << 22, 8 << 22, 2.1, 2, 65536, 2, True, 0)
+    deg, adj, w, _ = h.arrays()
+    g = graphs.CSR(deg, adj, w)
+    dg = T.DeviceGraph(gpu_ctx, deg, adj, w)
+    try:
+        dg.build_tables("alias", "edge")
+        table = dg.alias_table()
+        ref_table, _ = ob.alias_build(g, w)
+        assert np.array_equal(table.view(np.uint8), ref_table.view(np.uint8))
+
+        src = T.sources_all(deg)
+        sub = src[:: max(1, len(src) // 65536)][:65536].astype(np.int32)
+        case = {"app": "deepwalk", "execution": 1, "sampling": 2, "length": 80, "extra": {}}
+        K = 158
+        og = ob.Graph(g, ref_table)
+        d = ob.sfmt_stream(1234, len(sub) * K).reshape(len(sub), K)
+        ref = ob.walk(og, "deepwalk", 1, 2, 80, sub, d, K)
+        r = T.walk(dg, case_params(case, "fused"), sub, T.replay_fill(1234, len(sub), K))
+        for key in ("paths", "lengths", "ends", "status", "draws_used"):
+            assert np.array_equal(r[key], ref[key]), key
+
+        prm = T.make_params("deepwalk", rng="philox", seed=3)
+        r = T.walk(dg, prm, src)
+        assert np.all(r["lengths"] == 80) and np.all(r["status"] == 0)
+        assert r["steps"] == 79 * len(src)
+        rows = r["paths"][::97]
+        assert np.array_equal(rows[:, 0], src[::97])
+        nv = np.int64(len(deg))
+        edge_keys = np.repeat(np.arange(nv, dtype=np.int64), deg) * nv + adj  # sorted: rows are sorted
+        q = rows[:, :-1].ravel().astype(np.int64) * nv + rows[:, 1:].ravel()
+        pos = np.minimum(np.searchsorted(edge_keys, q), len(edge_keys) - 1)
+        assert np.all(edge_keys[pos] == q)
+    finally:
+        dg.close()

@@ -1,0 +1,116 @@
+"""ctypes view of the C ABI in include/thunderrw.h (libthunderrw.so)."""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libthunderrw.so")
+
+OK = 0
+E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_STATE, E_IO = -1, -2, -3, -4, -5, -6
+
+EXEC_UNIFORM, EXEC_STATIC, EXEC_DYNAMIC = 0, 1, 2
+SM_UNIFORM, SM_ITS, SM_ALIAS, SM_REJECTION, SM_MAX_WEIGHT_REJECTION = 0, 1, 2, 3, 4
+APP_DEEPWALK, APP_PPR, APP_NODE2VEC, APP_METAPATH, APP_GENERIC = 0, 1, 2, 3, 4
+RNG_REPLAY, RNG_PHILOX = 0, 1
+FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP = 0, 1, 2, 3
+WEIGHTS_EDGE, WEIGHTS_UNIT = 0, 1
+WALKER_DONE, WALKER_GATHER_TERMINATED, WALKER_DEAD_END, WALKER_REPLAY_EXHAUSTED, WALKER_TRIAL_CAP, \
+    WALKER_BAD_SOURCE = 0, 1, 2, 3, 4, 5
+
+APPS = {"deepwalk": APP_DEEPWALK, "ppr": APP_PPR, "node2vec": APP_NODE2VEC, "metapath": APP_METAPATH,
+        "genericwalk": APP_GENERIC}
+
+
+class TrwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("thunderrw error %d: %s" % (code, msg))
+        self.code = code
+
+
+class AliasSlot(C.Structure):
+    _fields_ = [("alias_value", C.c_double), ("first", C.c_int32), ("second", C.c_int32)]
+
+
+class WalkParams(C.Structure):
+    _fields_ = [("app", C.c_int32), ("execution", C.c_int32), ("sampling", C.c_int32), ("length", C.c_int32),
+                ("stop_probability", C.c_double), ("p", C.c_double), ("q", C.c_double),
+                ("schema", C.c_int32 * 16), ("schema_length", C.c_int32), ("rng", C.c_int32),
+                ("seed", C.c_uint64), ("form", C.c_int32), ("max_trials", C.c_int32)]
+
+
+class WalkIO(C.Structure):
+    _fields_ = [("sources", C.c_void_p), ("n_walkers", C.c_int32), ("path_stride", C.c_int32),
+                ("walker_id_base", C.c_int64), ("draws", C.c_void_p), ("draws_per_walker", C.c_int64),
+                ("term_draws", C.c_void_p), ("term_draws_per_walker", C.c_int64), ("paths", C.c_void_p),
+                ("lengths", C.c_void_p), ("ends", C.c_void_p), ("status", C.c_void_p), ("draws_used", C.c_void_p),
+                ("counters", C.c_void_p)]
+
+
+_SIGS = {
+    "trw_last_error": (C.c_char_p, []),
+    "trw_version": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_device_count": (C.c_int, [C.c_void_p]),
+    "trw_context_create": (C.c_int, [C.c_int32, C.c_void_p]),
+    "trw_context_destroy": (C.c_int, [C.c_void_p]),
+    "trw_context_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_device_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "trw_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "trw_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "trw_device_zero": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "trw_synchronize": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_timer_begin": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_timer_end": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "trw_graph_upload": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
+    "trw_graph_free": (C.c_int, [C.c_void_p]),
+    "trw_graph_info": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "trw_build_tables": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "trw_copy_alias_table": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_copy_prefix_sum": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_copy_rejection_max": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_last_build_ms": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_walk_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "trw_walk_mix_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "trw_walk_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
+                                C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "trw_replay_fill": (C.c_int, [C.c_uint32, C.c_int64, C.c_int32, C.c_int64, C.c_void_p]),
+    "trw_sources_all": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "trw_sources_single": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "trw_sources_random": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_uint32, C.c_void_p]),
+    "trw_host_graph_chung_lu": (C.c_int, [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                          C.c_int32, C.c_int32, C.c_void_p]),
+    "trw_host_graph_load": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "trw_host_graph_store": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "trw_host_graph_arrays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]),
+    "trw_host_graph_free": (C.c_int, [C.c_void_p]),
+}
+
+_LIB = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built (there is no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("thunderrw_amd/libthunderrw.so is missing: build it with "
+                               "`python -m thunderrw_amd.build` (there is no CPU fallback)")
+        lib_ = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib_, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib_
+    return _LIB
+
+
+def check(rc):
+    if rc != OK:
+        raise TrwError(rc, lib().trw_last_error().decode(errors="replace"))
+    return rc
+
+
+def exported_names():
+    return list(_SIGS)

@@ -1,0 +1,815 @@
+// C ABI of the step engine (include/thunderrw.h).
+#include "../../include/thunderrw.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "hostgraph.h"
+#include "sfmt.h"
+
+using trw::AliasSlot;
+using trw::BatchIO;
+using trw::DevGraph;
+using trw::KindParams;
+using trw::WalkerState;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? TRW_E_NOMEM : TRW_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define TRW_HIP(call, what)                       \
+    do {                                          \
+        hipError_t e_ = (call);                   \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+template <class T>
+hipError_t dmalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * count);
+}
+
+// Device allocations freed together when the owner goes out of scope.
+struct ScratchBuffers {
+    std::vector<void *> ptrs;
+    ~ScratchBuffers() {
+        for (void *q : ptrs) (void)hipFree(q);
+    }
+    template <class T>
+    hipError_t get(T **p, size_t count) {
+        *p = nullptr;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * (count ? count : 1));
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+};
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct trw_context {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t timer[2] = {nullptr, nullptr};
+};
+
+struct trw_graph {
+    trw_context *ctx = nullptr;
+    int32_t nv = 0;
+    int64_t ne = 0;
+    int32_t max_degree = 0;
+    int64_t *offset = nullptr;
+    int32_t *adj = nullptr;
+    int32_t *degree = nullptr;
+    double *weight = nullptr;
+    int32_t *label = nullptr;
+    int32_t *lbl_sorted = nullptr;
+    int32_t *lbl_pos = nullptr;
+    int32_t *order = nullptr;
+    AliasSlot *alias = nullptr;
+    double *prefix = nullptr;
+    double *rs_max = nullptr;
+    int32_t alias_weights = -1, its_weights = -1, rs_weights = -1;  // TRW_WEIGHTS_* the table was built from
+    unsigned int *counters = nullptr;                                 // [0] queue head [1] alive [2] zero-sum count
+    WalkerState *state = nullptr;
+    int64_t state_cap = 0;
+    double last_build_ms = 0.0;
+
+    DevGraph view() const {
+        DevGraph g;
+        g.nv = nv;
+        g.ne = ne;
+        g.offset = offset;
+        g.adj = adj;
+        g.degree = degree;
+        g.weight = weight;
+        g.label = label;
+        g.alias = alias;
+        g.prefix = prefix;
+        g.rs_max = rs_max;
+        g.lbl_sorted = lbl_sorted;
+        g.lbl_pos = lbl_pos;
+        return g;
+    }
+};
+
+struct trw_host_graph {
+    trw::HostGraph g;
+};
+
+// The entry points take their C linkage from the declarations in thunderrw.h.
+
+const char *trw_last_error(void) { return g_error.c_str(); }
+
+int trw_version(int32_t *major, int32_t *minor) {
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    return TRW_OK;
+}
+
+int trw_device_count(int32_t *count) {
+    if (!count) return fail(TRW_E_INVALID, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return TRW_OK;
+}
+
+int trw_context_create(int32_t device, trw_context **out) {
+    if (!out) return fail(TRW_E_INVALID, "out is null");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(TRW_E_HIP, "no HIP device is visible");
+    if (device < 0 || device >= n) return fail(TRW_E_INVALID, "device %d outside [0, %d)", device, n);
+    TRW_HIP(hipSetDevice(device), "hipSetDevice");
+    std::unique_ptr<trw_context> ctx(new trw_context);
+    ctx->device = device;
+    TRW_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    *out = ctx.release();
+    return TRW_OK;
+}
+
+int trw_context_destroy(trw_context *ctx) {
+    if (!ctx) return TRW_OK;
+    (void)hipSetDevice(ctx->device);
+    for (hipEvent_t &e : ctx->timer)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TRW_OK;
+}
+
+static hipStream_t pick_stream(trw_context *ctx, void *stream) {
+    return stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+int trw_device_alloc(trw_context *ctx, uint64_t bytes, void **out) {
+    if (!ctx || !out) return fail(TRW_E_INVALID, "null argument");
+    *out = nullptr;
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TRW_HIP(hipMalloc(out, bytes ? bytes : 1), "hipMalloc");
+    return TRW_OK;
+}
+
+int trw_device_free(trw_context *ctx, void *ptr) {
+    if (!ctx) return fail(TRW_E_INVALID, "null argument");
+    if (!ptr) return TRW_OK;
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TRW_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+    TRW_HIP(hipFree(ptr), "hipFree");
+    return TRW_OK;
+}
+
+int trw_copy_to_device(trw_context *ctx, void *dst, const void *src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(TRW_E_INVALID, "null argument");
+    if (!bytes) return TRW_OK;
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TRW_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "copy to device");
+    TRW_HIP(hipStreamSynchronize(ctx->stream), "copy to device");
+    return TRW_OK;
+}
+
+int trw_copy_to_host(trw_context *ctx, void *dst, const void *src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(TRW_E_INVALID, "null argument");
+    if (!bytes) return TRW_OK;
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TRW_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "copy to host");
+    TRW_HIP(hipStreamSynchronize(ctx->stream), "copy to host");
+    return TRW_OK;
+}
+
+int trw_device_zero(trw_context *ctx, void *ptr, uint64_t bytes, void *stream) {
+    if (!ctx || (bytes && !ptr)) return fail(TRW_E_INVALID, "null argument");
+    if (bytes % 4) return fail(TRW_E_INVALID, "byte count must be a multiple of 4");
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TRW_HIP(trw::launch_fill_u32(static_cast<unsigned int *>(ptr), 0u, static_cast<int64_t>(bytes / 4),
+                                 pick_stream(ctx, stream)), "zero fill");
+    return TRW_OK;
+}
+
+int trw_synchronize(trw_context *ctx, void *stream) {
+    if (!ctx) return fail(TRW_E_INVALID, "null argument");
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TRW_HIP(hipStreamSynchronize(pick_stream(ctx, stream)), "synchronize");
+    return TRW_OK;
+}
+
+int trw_timer_begin(trw_context *ctx, void *stream) {
+    if (!ctx) return fail(TRW_E_INVALID, "null argument");
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    for (hipEvent_t &e : ctx->timer)
+        if (!e) TRW_HIP(hipEventCreate(&e), "hipEventCreate");
+    TRW_HIP(hipEventRecord(ctx->timer[0], pick_stream(ctx, stream)), "hipEventRecord");
+    return TRW_OK;
+}
+
+int trw_timer_end(trw_context *ctx, void *stream, double *elapsed_ms) {
+    if (!ctx || !elapsed_ms) return fail(TRW_E_INVALID, "null argument");
+    if (!ctx->timer[0]) return fail(TRW_E_STATE, "trw_timer_begin was not called");
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TRW_HIP(hipEventRecord(ctx->timer[1], pick_stream(ctx, stream)), "hipEventRecord");
+    TRW_HIP(hipEventSynchronize(ctx->timer[1]), "hipEventSynchronize");
+    float ms = 0.f;
+    TRW_HIP(hipEventElapsedTime(&ms, ctx->timer[0], ctx->timer[1]), "hipEventElapsedTime");
+    *elapsed_ms = ms;
+    return TRW_OK;
+}
+
+int trw_context_stream(trw_context *ctx, void **stream) {
+    if (!ctx || !stream) return fail(TRW_E_INVALID, "null argument");
+    *stream = reinterpret_cast<void *>(ctx->stream);
+    return TRW_OK;
+}
+
+static void graph_release(trw_graph *g) {
+    dfree(g->offset);
+    dfree(g->adj);
+    dfree(g->degree);
+    dfree(g->weight);
+    dfree(g->label);
+    dfree(g->lbl_sorted);
+    dfree(g->lbl_pos);
+    dfree(g->order);
+    dfree(g->alias);
+    dfree(g->prefix);
+    dfree(g->rs_max);
+    dfree(g->counters);
+    dfree(g->state);
+}
+
+int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *degree, const int32_t *adj,
+                     const double *edge_weight, const int32_t *edge_label, trw_graph **out) {
+    if (!ctx || !out || !degree || (ne > 0 && !adj)) return fail(TRW_E_INVALID, "null argument");
+    *out = nullptr;
+    if (nv <= 0 || ne < 0) return fail(TRW_E_INVALID, "num_vertices must be > 0 and num_edges >= 0");
+    std::vector<int64_t> offset(static_cast<size_t>(nv) + 1, 0);
+    int32_t maxd = 0;
+    for (int32_t v = 0; v < nv; ++v) {
+        if (degree[v] < 0) return fail(TRW_E_INVALID, "degree[%d] is negative", v);
+        offset[v + 1] = offset[v] + degree[v];
+        maxd = std::max(maxd, degree[v]);
+    }
+    if (offset[nv] != ne) return fail(TRW_E_INVALID, "degrees sum to %lld, num_edges is %lld",
+                                      static_cast<long long>(offset[nv]), static_cast<long long>(ne));
+    bool bad = false;
+    trw::parallel_for(ne, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i)
+            if (adj[i] < 0 || adj[i] >= nv) bad = true;
+    });
+    if (bad) return fail(TRW_E_INVALID, "adjacency holds a vertex id outside [0, num_vertices)");
+
+    // Vertices in descending degree order (by power-of-two class): the table
+    // builds start hubs' long serial chains first.
+    std::vector<int32_t> order(static_cast<size_t>(nv));
+    {
+        std::vector<int64_t> cls(34, 0);
+        auto klass = [](int32_t d) { return d == 0 ? 0 : 33 - __builtin_clz(static_cast<unsigned>(d)); };
+        for (int32_t v = 0; v < nv; ++v) cls[klass(degree[v])]++;
+        std::vector<int64_t> start(34, 0);
+        int64_t acc = 0;
+        for (int c = 33; c >= 0; --c) {
+            start[c] = acc;
+            acc += cls[c];
+        }
+        for (int32_t v = 0; v < nv; ++v) order[start[klass(degree[v])]++] = v;
+    }
+
+    std::unique_ptr<trw_graph> g(new trw_graph);
+    g->ctx = ctx;
+    g->nv = nv;
+    g->ne = ne;
+    g->max_degree = maxd;
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    struct Guard {
+        trw_graph *g;
+        ~Guard() {
+            if (g) graph_release(g);
+        }
+    } guard{g.get()};
+    const size_t V = static_cast<size_t>(nv), E = static_cast<size_t>(ne);
+    TRW_HIP(dmalloc(&g->offset, V + 1), "hipMalloc offset");
+    TRW_HIP(dmalloc(&g->degree, V), "hipMalloc degree");
+    TRW_HIP(dmalloc(&g->order, V), "hipMalloc order");
+    TRW_HIP(dmalloc(&g->adj, E + 16), "hipMalloc adj");
+    TRW_HIP(dmalloc(&g->counters, 8), "hipMalloc counters");
+    TRW_HIP(hipMemcpy(g->offset, offset.data(), sizeof(int64_t) * (V + 1), hipMemcpyHostToDevice), "copy offset");
+    TRW_HIP(hipMemcpy(g->degree, degree, sizeof(int32_t) * V, hipMemcpyHostToDevice), "copy degree");
+    TRW_HIP(hipMemcpy(g->order, order.data(), sizeof(int32_t) * V, hipMemcpyHostToDevice), "copy order");
+    if (E) TRW_HIP(hipMemcpy(g->adj, adj, sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy adj");
+    if (edge_weight) {
+        TRW_HIP(dmalloc(&g->weight, E + 16), "hipMalloc weight");
+        if (E) TRW_HIP(hipMemcpy(g->weight, edge_weight, sizeof(double) * E, hipMemcpyHostToDevice), "copy weight");
+    }
+    if (edge_label) {
+        std::vector<int32_t> ls(E), lp(E);
+        trw::build_label_index(nv, offset.data(), edge_label, ls.data(), lp.data());
+        TRW_HIP(dmalloc(&g->label, E + 16), "hipMalloc label");
+        TRW_HIP(dmalloc(&g->lbl_sorted, E + 16), "hipMalloc label index");
+        TRW_HIP(dmalloc(&g->lbl_pos, E + 16), "hipMalloc label index");
+        if (E) {
+            TRW_HIP(hipMemcpy(g->label, edge_label, sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy label");
+            TRW_HIP(hipMemcpy(g->lbl_sorted, ls.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy label index");
+            TRW_HIP(hipMemcpy(g->lbl_pos, lp.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy label index");
+        }
+    }
+    guard.g = nullptr;
+    *out = g.release();
+    return TRW_OK;
+}
+
+int trw_graph_free(trw_graph *g) {
+    if (!g) return TRW_OK;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+    graph_release(g);
+    delete g;
+    return TRW_OK;
+}
+
+int trw_graph_info(const trw_graph *g, int32_t *nv, int64_t *ne, int32_t *max_degree) {
+    if (!g) return fail(TRW_E_INVALID, "graph is null");
+    if (nv) *nv = g->nv;
+    if (ne) *ne = g->ne;
+    if (max_degree) *max_degree = g->max_degree;
+    return TRW_OK;
+}
+
+int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
+    if (!g) return fail(TRW_E_INVALID, "graph is null");
+    if (weights != TRW_WEIGHTS_EDGE && weights != TRW_WEIGHTS_UNIT) return fail(TRW_E_INVALID, "unknown weights %d", weights);
+    if (weights == TRW_WEIGHTS_EDGE && !g->weight) return fail(TRW_E_STATE, "graph was uploaded without edge weights");
+    if (sampling != TRW_SM_ALIAS && sampling != TRW_SM_ITS && sampling != TRW_SM_REJECTION)
+        return fail(TRW_E_INVALID, "static tables exist for ITS, alias and rejection sampling only");
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    hipStream_t st = g->ctx->stream;
+    const DevGraph dv = g->view();
+    const int32_t unit = weights == TRW_WEIGHTS_UNIT;
+    const size_t E = static_cast<size_t>(g->ne), V = static_cast<size_t>(g->nv);
+    hipEvent_t t0, t1;
+    TRW_HIP(hipEventCreate(&t0), "hipEventCreate");
+    TRW_HIP(hipEventCreate(&t1), "hipEventCreate");
+    struct Ev {
+        hipEvent_t a, b;
+        ~Ev() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } ev{t0, t1};
+    if (sampling == TRW_SM_ALIAS) {
+        if (!g->alias) TRW_HIP(dmalloc(&g->alias, E + 1), "hipMalloc alias table");
+        double *q = nullptr;
+        int32_t *lists = nullptr;
+        TRW_HIP(dmalloc(&q, E + 1), "hipMalloc alias scratch");
+        hipError_t e = dmalloc(&lists, E + 1);
+        if (e != hipSuccess) {
+            dfree(q);
+            return hip_fail(e, "hipMalloc alias scratch");
+        }
+        trw::launch_fill_u32(g->counters + 2, 0u, 1, st);
+        (void)hipEventRecord(t0, st);
+        e = trw::launch_alias_build(dv, g->weight, unit, g->order, q, lists, g->alias, g->counters + 2, st);
+        (void)hipEventRecord(t1, st);
+        hipError_t e2 = hipStreamSynchronize(st);
+        dfree(q);
+        dfree(lists);
+        if (e != hipSuccess) return hip_fail(e, "alias build launch");
+        if (e2 != hipSuccess) return hip_fail(e2, "alias build");
+        g->alias_weights = weights;
+    } else if (sampling == TRW_SM_ITS) {
+        if (!g->prefix) TRW_HIP(dmalloc(&g->prefix, E + 1), "hipMalloc prefix sums");
+        (void)hipEventRecord(t0, st);
+        hipError_t e = trw::launch_its_build(dv, g->weight, unit, g->order, g->prefix, st);
+        (void)hipEventRecord(t1, st);
+        if (e != hipSuccess) return hip_fail(e, "ITS build launch");
+        TRW_HIP(hipStreamSynchronize(st), "ITS build");
+        g->its_weights = weights;
+    } else {
+        if (!g->rs_max) TRW_HIP(dmalloc(&g->rs_max, V), "hipMalloc rejection maxima");
+        (void)hipEventRecord(t0, st);
+        hipError_t e = trw::launch_rs_build(dv, g->weight, unit, g->order, g->rs_max, st);
+        (void)hipEventRecord(t1, st);
+        if (e != hipSuccess) return hip_fail(e, "rejection build launch");
+        TRW_HIP(hipStreamSynchronize(st), "rejection build");
+        g->rs_weights = weights;
+    }
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess) g->last_build_ms = ms;
+    return TRW_OK;
+}
+
+int trw_last_build_ms(const trw_graph *g, double *ms) {
+    if (!g || !ms) return fail(TRW_E_INVALID, "null argument");
+    *ms = g->last_build_ms;
+    return TRW_OK;
+}
+
+template <class T>
+static int copy_table(const trw_graph *g, const T *src, int32_t built, size_t count, T *host, const char *what) {
+    if (!g || !host) return fail(TRW_E_INVALID, "null argument");
+    if (!src || built < 0) return fail(TRW_E_STATE, "%s has not been built", what);
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    TRW_HIP(hipStreamSynchronize(g->ctx->stream), "synchronize");
+    if (count) TRW_HIP(hipMemcpy(host, src, sizeof(T) * count, hipMemcpyDeviceToHost), what);
+    return TRW_OK;
+}
+
+int trw_copy_alias_table(const trw_graph *g, trw_alias_slot *host_out) {
+    static_assert(sizeof(trw_alias_slot) == sizeof(AliasSlot), "layout");
+    return copy_table(g, g ? g->alias : nullptr, g ? g->alias_weights : -1, g ? static_cast<size_t>(g->ne) : 0,
+                      reinterpret_cast<AliasSlot *>(host_out), "alias table");
+}
+
+int trw_copy_prefix_sum(const trw_graph *g, double *host_out) {
+    return copy_table(g, g ? g->prefix : nullptr, g ? g->its_weights : -1, g ? static_cast<size_t>(g->ne) : 0,
+                      host_out, "prefix-sum table");
+}
+
+int trw_copy_rejection_max(const trw_graph *g, double *host_out) {
+    return copy_table(g, g ? g->rs_max : nullptr, g ? g->rs_weights : -1, g ? static_cast<size_t>(g->nv) : 0,
+                      host_out, "rejection maxima");
+}
+
+// ---------------------------------------------------------------- walks
+
+// Maps an app + g_para to a device mode, building the static tables the mode
+// reads when they are missing.
+static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k) {
+    std::memset(&k, 0, sizeof(k));
+    if (p->app < TRW_APP_DEEPWALK || p->app > TRW_APP_GENERIC) return fail(TRW_E_INVALID, "unknown app %d", p->app);
+    if (p->execution < TRW_EXEC_UNIFORM || p->execution > TRW_EXEC_DYNAMIC)
+        return fail(TRW_E_INVALID, "unknown execution mode %d", p->execution);
+    if (p->sampling < TRW_SM_UNIFORM || p->sampling > TRW_SM_MAX_WEIGHT_REJECTION)
+        return fail(TRW_E_INVALID, "unknown sampling method %d", p->sampling);
+    const bool ppr = p->app == TRW_APP_PPR;
+    if (ppr) {
+        if (p->length < 1) return fail(TRW_E_INVALID, "PPR needs length >= 1");
+        if (!(p->stop_probability > 0.0 && p->stop_probability <= 1.0))
+            return fail(TRW_E_INVALID, "stop_probability must lie in (0, 1]");
+    } else if (p->length < 2) {
+        return fail(TRW_E_INVALID, "length must be >= 2 (a walk of length l makes l-1 moves)");
+    }
+    if (p->max_trials < 0) return fail(TRW_E_INVALID, "max_trials must be >= 0");
+    k.ppr = ppr;
+    k.length = p->length;
+    k.max_trials = p->max_trials;
+    k.stop_prob = p->stop_probability;
+
+    if (p->execution == TRW_EXEC_UNIFORM) {
+        k.mode = trw::kModeUniform;
+        return TRW_OK;
+    }
+    if (p->app == TRW_APP_NODE2VEC) {
+        if (p->execution != TRW_EXEC_DYNAMIC || p->sampling != TRW_SM_MAX_WEIGHT_REJECTION)
+            return fail(TRW_E_UNSUPPORTED, "Node2Vec runs in Dynamic mode with max-weight rejection sampling");
+        if (!(p->p > 0.0) || !(p->q > 0.0)) return fail(TRW_E_INVALID, "Node2Vec p and q must be > 0");
+        k.mode = trw::kModeNode2Vec;
+        k.n2v_p = 1.0 / p->p;  // node2vec.cpp:69
+        k.n2v_q = 1.0 / p->q;  // node2vec.cpp:76
+        k.n2v_max = std::max(1.0, std::max(k.n2v_q, k.n2v_p));  // node2vec.cpp:19
+        return TRW_OK;
+    }
+    if (p->app == TRW_APP_METAPATH) {
+        if (p->execution != TRW_EXEC_DYNAMIC || p->sampling != TRW_SM_ITS)
+            return fail(TRW_E_UNSUPPORTED, "MetaPath runs in Dynamic mode with inverse-transform sampling");
+        if (!g->label) return fail(TRW_E_STATE, "MetaPath needs edge labels");
+        if (p->schema_length <= 0 || p->schema_length > 16) return fail(TRW_E_INVALID, "schema length must be 1..16");
+        k.mode = trw::kModeMetaPath;
+        k.schema_length = p->schema_length;
+        for (int i = 0; i < p->schema_length; ++i) k.schema[i] = p->schema[i];
+        return TRW_OK;
+    }
+    // DeepWalk, GenericWalk (edge weights) and PPR (unit weights): walker-independent
+    // weights, so the Dynamic gather rebuilds exactly the Static tables every step.
+    if (p->sampling == TRW_SM_UNIFORM || p->sampling == TRW_SM_MAX_WEIGHT_REJECTION)
+        return fail(TRW_E_UNSUPPORTED, "sampling method %d is not defined for this app in mode %d", p->sampling,
+                    p->execution);
+    const int32_t wsrc = ppr ? TRW_WEIGHTS_UNIT : TRW_WEIGHTS_EDGE;
+    int32_t *built = p->sampling == TRW_SM_ALIAS ? &g->alias_weights
+                     : p->sampling == TRW_SM_ITS ? &g->its_weights
+                                                 : &g->rs_weights;
+    if (*built != wsrc) {
+        int rc = trw_build_tables(g, p->sampling, wsrc);
+        if (rc != TRW_OK) return rc;
+    }
+    k.mode = p->sampling == TRW_SM_ALIAS ? trw::kModeAlias : p->sampling == TRW_SM_ITS ? trw::kModeIts : trw::kModeRejection;
+    k.unit_weights = wsrc == TRW_WEIGHTS_UNIT;
+    return TRW_OK;
+}
+
+static int make_io(const trw_graph *g, const trw_walk_params *p, const trw_walk_io *io, const KindParams &a,
+                   const KindParams &b, BatchIO &out) {
+    if (!io) return fail(TRW_E_INVALID, "io is null");
+    if (io->n_walkers < 0) return fail(TRW_E_INVALID, "n_walkers must be >= 0");
+    if (io->n_walkers > 0 && !io->sources) return fail(TRW_E_INVALID, "sources is null");
+    if (io->n_walkers > (1 << 30)) return fail(TRW_E_INVALID, "at most 2^30 walkers per call");
+    const int32_t L = std::max(std::max(a.length, b.length), 1);
+    if (io->paths && io->path_stride < L) return fail(TRW_E_INVALID, "path_stride %d < length %d", io->path_stride, L);
+    if (p->rng != TRW_RNG_REPLAY && p->rng != TRW_RNG_PHILOX) return fail(TRW_E_INVALID, "unknown rng %d", p->rng);
+    const bool replay = p->rng == TRW_RNG_REPLAY;
+    if (replay) {
+        if (io->draws_per_walker < 0 || io->term_draws_per_walker < 0) return fail(TRW_E_INVALID, "negative draw count");
+        if (io->draws_per_walker > 0 && !io->draws) return fail(TRW_E_INVALID, "draws is null");
+        if ((a.ppr || b.ppr) && io->term_draws_per_walker > 0 && !io->term_draws)
+            return fail(TRW_E_INVALID, "term_draws is null");
+        if (io->draws_per_walker > 0xffffffffLL || io->term_draws_per_walker > 0xffffffffLL)
+            return fail(TRW_E_INVALID, "draws_per_walker must fit in 32 bits");
+    }
+    std::memset(&out, 0, sizeof(out));
+    out.sources = io->sources;
+    out.n = io->n_walkers;
+    out.path_stride = io->path_stride;
+    out.gid_base = static_cast<uint64_t>(io->walker_id_base);
+    out.draws = replay ? io->draws : nullptr;
+    out.K = replay ? io->draws_per_walker : 0;
+    out.tdraws = replay ? io->term_draws : nullptr;
+    out.KT = replay ? io->term_draws_per_walker : 0;
+    out.paths = io->paths;
+    out.lengths = io->lengths;
+    out.ends = io->ends;
+    out.status = io->status;
+    out.draws_used = io->draws_used;
+    out.counters = reinterpret_cast<unsigned long long *>(io->counters);
+    out.key0 = static_cast<uint32_t>(p->seed);
+    out.key1 = static_cast<uint32_t>(p->seed >> 32);
+    out.replay = replay;
+    (void)g;
+    return TRW_OK;
+}
+
+static int run_step_form(trw_graph *g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                         hipStream_t st) {
+    if (io.n == 0) return TRW_OK;
+    if (g->state_cap < io.n) {
+        dfree(g->state);
+        g->state_cap = 0;
+        TRW_HIP(dmalloc(&g->state, static_cast<size_t>(io.n)), "hipMalloc walker state");
+        g->state_cap = io.n;
+    }
+    const DevGraph dv = g->view();
+    TRW_HIP(trw::launch_walk_step_init(dv, a, b, split, io, g->state, st), "step-form init");
+    unsigned int alive = 1;
+    const int kChunk = 8;
+    while (alive) {
+        for (int i = 0; i < kChunk; ++i) {
+            TRW_HIP(trw::launch_fill_u32(g->counters + 1, 0u, 1, st), "step-form reset");
+            TRW_HIP(trw::launch_walk_step(dv, a, b, split, io, g->state, g->counters + 1, st), "step-form step");
+        }
+        TRW_HIP(hipMemcpyAsync(&alive, g->counters + 1, sizeof(alive), hipMemcpyDeviceToHost, st), "read alive");
+        TRW_HIP(hipStreamSynchronize(st), "step-form synchronize");
+    }
+    TRW_HIP(trw::launch_walk_step_finish(a, b, split, io, g->state, st), "step-form finish");
+    return TRW_OK;
+}
+
+int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream) {
+    if (!g || !params) return fail(TRW_E_INVALID, "null argument");
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    KindParams k;
+    int rc = resolve_kind(g, params, k);
+    if (rc != TRW_OK) return rc;
+    BatchIO bio;
+    rc = make_io(g, params, io, k, k, bio);
+    if (rc != TRW_OK) return rc;
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : g->ctx->stream;
+    int32_t form = params->form;
+    if (form == TRW_FORM_AUTO) form = (k.ppr || k.mode == trw::kModeMetaPath) ? TRW_FORM_QUEUE : TRW_FORM_FUSED;
+    const DevGraph dv = g->view();
+    switch (form) {
+    case TRW_FORM_FUSED:
+        if (k.ppr) return fail(TRW_E_UNSUPPORTED, "the fused form runs fixed-length walks; use QUEUE or STEP for PPR");
+        TRW_HIP(trw::launch_walk_fused(dv, k, bio, st), "fused walk launch");
+        return TRW_OK;
+    case TRW_FORM_QUEUE:
+        TRW_HIP(trw::launch_walk_queue(dv, k, k, bio.n, bio, g->counters, st), "queue walk launch");
+        return TRW_OK;
+    case TRW_FORM_STEP:
+        return run_step_form(g, k, k, bio.n, bio, st);
+    default:
+        return fail(TRW_E_INVALID, "unknown form %d", params->form);
+    }
+}
+
+int trw_walk_mix_async(trw_graph *g, const trw_walk_params *pa, const trw_walk_params *pb, int32_t split,
+                       const trw_walk_io *io, void *stream) {
+    if (!g || !pa || !pb || !io) return fail(TRW_E_INVALID, "null argument");
+    if (pa->rng != pb->rng || pa->seed != pb->seed) return fail(TRW_E_INVALID, "both kinds must share rng and seed");
+    if (split < 0 || split > io->n_walkers) return fail(TRW_E_INVALID, "split outside [0, n_walkers]");
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    KindParams a, b;
+    int rc = resolve_kind(g, pa, a);
+    if (rc != TRW_OK) return rc;
+    rc = resolve_kind(g, pb, b);
+    if (rc != TRW_OK) return rc;
+    BatchIO bio;
+    rc = make_io(g, pa, io, a, b, bio);
+    if (rc != TRW_OK) return rc;
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : g->ctx->stream;
+    const int32_t form = pa->form == TRW_FORM_AUTO ? TRW_FORM_QUEUE : pa->form;
+    if (form == TRW_FORM_QUEUE) {
+        TRW_HIP(trw::launch_walk_queue(g->view(), a, b, split, bio, g->counters, st), "queue walk launch");
+        return TRW_OK;
+    }
+    if (form == TRW_FORM_STEP) return run_step_form(g, a, b, split, bio, st);
+    return fail(TRW_E_UNSUPPORTED, "mixed batches run in the QUEUE or STEP form");
+}
+
+int trw_walk_host(trw_graph *g, const trw_walk_params *p, const int32_t *sources, int32_t n, const uint32_t *draws,
+                  int64_t K, const uint32_t *tdraws, int64_t KT, int32_t *paths, int32_t *lengths, int32_t *ends,
+                  int32_t *status, uint64_t *steps) {
+    if (!g || !p || (n > 0 && !sources)) return fail(TRW_E_INVALID, "null argument");
+    if (n < 0) return fail(TRW_E_INVALID, "n_walkers must be >= 0");
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    const size_t N = static_cast<size_t>(n);
+    const int32_t L = std::max(p->length, 1);
+    const bool replay = p->rng == TRW_RNG_REPLAY;
+    ScratchBuffers buf;
+    int32_t *d_src = nullptr, *d_paths = nullptr, *d_len = nullptr, *d_end = nullptr, *d_st = nullptr;
+    uint32_t *d_draws = nullptr, *d_tdraws = nullptr;
+    uint64_t *d_cnt = nullptr;
+    TRW_HIP(buf.get(&d_src, N), "hipMalloc");
+    TRW_HIP(buf.get(&d_cnt, 4), "hipMalloc");
+    if (paths) TRW_HIP(buf.get(&d_paths, N * static_cast<size_t>(L)), "hipMalloc");
+    if (lengths) TRW_HIP(buf.get(&d_len, N), "hipMalloc");
+    if (ends) TRW_HIP(buf.get(&d_end, N), "hipMalloc");
+    if (status) TRW_HIP(buf.get(&d_st, N), "hipMalloc");
+    hipStream_t st = g->ctx->stream;
+    if (N) TRW_HIP(hipMemcpyAsync(d_src, sources, sizeof(int32_t) * N, hipMemcpyHostToDevice, st), "copy sources");
+    if (replay && K > 0 && draws) {
+        TRW_HIP(buf.get(&d_draws, N * static_cast<size_t>(K)), "hipMalloc draws");
+        TRW_HIP(hipMemcpyAsync(d_draws, draws, sizeof(uint32_t) * N * static_cast<size_t>(K), hipMemcpyHostToDevice, st),
+                "copy draws");
+    }
+    if (replay && KT > 0 && tdraws) {
+        TRW_HIP(buf.get(&d_tdraws, N * static_cast<size_t>(KT)), "hipMalloc draws");
+        TRW_HIP(hipMemcpyAsync(d_tdraws, tdraws, sizeof(uint32_t) * N * static_cast<size_t>(KT), hipMemcpyHostToDevice,
+                               st), "copy draws");
+    }
+    TRW_HIP(hipMemsetAsync(d_cnt, 0, sizeof(uint64_t) * 4, st), "clear counters");
+    trw_walk_io io;
+    std::memset(&io, 0, sizeof(io));
+    io.sources = d_src;
+    io.n_walkers = n;
+    io.path_stride = L;
+    io.draws = d_draws;
+    io.draws_per_walker = K;
+    io.term_draws = d_tdraws;
+    io.term_draws_per_walker = KT;
+    io.paths = d_paths;
+    io.lengths = d_len;
+    io.ends = d_end;
+    io.status = d_st;
+    io.counters = d_cnt;
+    int rc = trw_walk_async(g, p, &io, st);
+    if (rc != TRW_OK) {
+        (void)hipStreamSynchronize(st);
+        return rc;
+    }
+    uint64_t cnt[4] = {0, 0, 0, 0};
+    TRW_HIP(hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, st), "copy counters");
+    if (paths) TRW_HIP(hipMemcpyAsync(paths, d_paths, sizeof(int32_t) * N * L, hipMemcpyDeviceToHost, st), "copy paths");
+    if (lengths) TRW_HIP(hipMemcpyAsync(lengths, d_len, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st), "copy lengths");
+    if (ends) TRW_HIP(hipMemcpyAsync(ends, d_end, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st), "copy ends");
+    if (status) TRW_HIP(hipMemcpyAsync(status, d_st, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st), "copy status");
+    TRW_HIP(hipStreamSynchronize(st), "walk");
+    if (steps) *steps = cnt[0];
+    return TRW_OK;
+}
+
+// ---------------------------------------------------------------- replay stream
+
+int trw_replay_fill(uint32_t seed, int64_t first_walker, int32_t n, int64_t K, uint32_t *out) {
+    if (n < 0 || K < 0 || first_walker < 0 || (n > 0 && K > 0 && !out)) return fail(TRW_E_INVALID, "bad argument");
+    if (n == 0 || K == 0) return TRW_OK;
+    trw::Sfmt19937 gen(seed);
+    gen.skip(static_cast<uint64_t>(first_walker) * static_cast<uint64_t>(K));
+    const int32_t kTileW = 64;
+    std::vector<uint32_t> chunk(static_cast<size_t>(kTileW) * static_cast<size_t>(K));
+    for (int32_t w0 = 0; w0 < n; w0 += kTileW) {
+        const int32_t tw = std::min(kTileW, n - w0);
+        gen.fill(chunk.data(), static_cast<int64_t>(tw) * K);
+        for (int64_t k = 0; k < K; ++k) {
+            uint32_t *row = out + k * n + w0;
+            for (int32_t i = 0; i < tw; ++i) row[i] = chunk[static_cast<size_t>(i) * K + k];
+        }
+    }
+    return TRW_OK;
+}
+
+// ---------------------------------------------------------------- walker sources
+
+int trw_sources_all(const int32_t *degree, int32_t nv, int32_t *out, int32_t *count) {
+    if (!degree || !out || !count || nv < 0) return fail(TRW_E_INVALID, "bad argument");
+    std::vector<int32_t> src;
+    src.reserve(static_cast<size_t>(nv));
+    for (int32_t v = 0; v < nv; ++v)
+        if (degree[v] != 0) src.push_back(v);
+    std::mt19937 rng(0);  // Utility::shuffle with debug_mode (utility.h:96-100)
+    std::shuffle(src.begin(), src.end(), rng);
+    std::copy(src.begin(), src.end(), out);
+    *count = static_cast<int32_t>(src.size());
+    return TRW_OK;
+}
+
+int trw_sources_single(const int32_t *degree, int32_t nv, int32_t n, int32_t *out) {
+    if (!degree || (!out && n > 0) || nv <= 0 || n < 0) return fail(TRW_E_INVALID, "bad argument");
+    int32_t u = 0, d = degree[0];
+    for (int32_t v = 1; v < nv; ++v)
+        if (degree[v] > d) {
+            d = degree[v];
+            u = v;
+        }
+    for (int32_t i = 0; i < n; ++i) out[i] = u;
+    return TRW_OK;
+}
+
+int trw_sources_random(const int32_t *degree, int32_t nv, int32_t n, uint32_t seed, int32_t *out) {
+    if (!degree || (!out && n > 0) || nv <= 0 || n < 0) return fail(TRW_E_INVALID, "bad argument");
+    bool any = false;
+    for (int32_t v = 0; v < nv && !any; ++v) any = degree[v] != 0;
+    if (!any && n > 0) return fail(TRW_E_INVALID, "every vertex is isolated");
+    trw::Sfmt19937 gen(seed);
+    for (int32_t i = 0; i < n;) {
+        const int32_t u = static_cast<int32_t>(gen.next() % static_cast<uint32_t>(nv));
+        if (degree[u] != 0) out[i++] = u;
+    }
+    return TRW_OK;
+}
+
+// ---------------------------------------------------------------- host graphs
+
+int trw_host_graph_chung_lu(int32_t nv, int64_t m, double gamma, double dmin, double dmax, uint64_t seed,
+                            int32_t with_weights, int32_t n_labels, trw_host_graph **out) {
+    if (!out) return fail(TRW_E_INVALID, "out is null");
+    *out = nullptr;
+    std::unique_ptr<trw_host_graph> h(new trw_host_graph);
+    std::string err;
+    if (!trw::chung_lu(nv, m, gamma, dmin, dmax, seed, with_weights != 0, n_labels, h->g, err))
+        return fail(TRW_E_INVALID, "%s", err.c_str());
+    *out = h.release();
+    return TRW_OK;
+}
+
+int trw_host_graph_load(const char *dir, int32_t with_weights, int32_t with_labels, trw_host_graph **out) {
+    if (!dir || !out) return fail(TRW_E_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<trw_host_graph> h(new trw_host_graph);
+    std::string err;
+    if (!trw::load_reference_csr(dir, with_weights != 0, with_labels != 0, h->g, err))
+        return fail(TRW_E_IO, "%s", err.c_str());
+    *out = h.release();
+    return TRW_OK;
+}
+
+int trw_host_graph_store(const trw_host_graph *h, const char *dir) {
+    if (!h || !dir) return fail(TRW_E_INVALID, "null argument");
+    std::string err;
+    if (!trw::store_reference_csr(h->g, dir, err)) return fail(TRW_E_IO, "%s", err.c_str());
+    return TRW_OK;
+}
+
+int trw_host_graph_arrays(const trw_host_graph *h, int32_t *nv, int64_t *ne, const int32_t **degree,
+                          const int32_t **adj, const double **w, const int32_t **label) {
+    if (!h) return fail(TRW_E_INVALID, "null argument");
+    if (nv) *nv = h->g.nv;
+    if (ne) *ne = h->g.ne;
+    if (degree) *degree = h->g.degree.data();
+    if (adj) *adj = h->g.adj.data();
+    if (w) *w = h->g.weight.empty() ? nullptr : h->g.weight.data();
+    if (label) *label = h->g.label.empty() ? nullptr : h->g.label.data();
+    return TRW_OK;
+}
+
+int trw_host_graph_free(trw_host_graph *h) {
+    delete h;
+    return TRW_OK;
+}

@@ -1,0 +1,119 @@
+// Internal types shared by the HIP kernels and the C-ABI host code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace trw {
+
+// AliasSlot, config/type_config.h:22-26.  16 bytes; read on the device as one
+// 128-bit load (the table is hipMalloc'ed, so every slot is 16-byte aligned).
+struct AliasSlot {
+    double prob;
+    int32_t first;
+    int32_t second;
+};
+static_assert(sizeof(AliasSlot) == 16, "AliasSlot must stay 16 bytes");
+
+// Per-walker outcome codes (same values as TRW_WALKER_* in thunderrw.h).
+enum WalkerStatus : int32_t {
+    kAlive = -1,
+    kDone = 0,
+    kGatherTerminated = 1,
+    kDeadEnd = 2,
+    kReplayExhausted = 3,
+    kTrialCap = 4,
+    kBadSource = 5,
+};
+
+// Move kernels.  One per (execution mode, sampler) pair the device provides.
+enum Mode : int32_t {
+    kModeUniform = 0,   // uniform_move            uniform_sampling.h:47-63
+    kModeAlias = 1,     // static_alias_move       alias_sampling.h:147-172
+    kModeIts = 2,       // static_its_move         inverse_transformation_sampling.h:259-286
+    kModeRejection = 3, // static_rs_move          rejection_sampling.h:334-366
+    kModeNode2Vec = 4,  // dynamic_max_weight_rs_move + Node2Vec::weight  rejection_sampling.h:368-402
+    kModeMetaPath = 5,  // dynamic_gather + dynamic_its_move + MetaPath::weight  creeper.h:178-296
+};
+
+// Read-only CSR view in HBM (util/graph/graph.h:51-83 field meanings).
+struct DevGraph {
+    int32_t nv;
+    int64_t ne;
+    const int64_t *offset;     // nv + 1
+    const int32_t *adj;        // ne, sorted per vertex
+    const int32_t *degree;     // nv
+    const double *weight;      // ne or null
+    const int32_t *label;      // ne or null
+    const AliasSlot *alias;    // ne (alias tables)
+    const double *prefix;      // ne (ITS prefix sums)
+    const double *rs_max;      // nv (rejection maxima)
+    const int32_t *lbl_sorted; // ne: labels of each vertex's segment, sorted (stable)
+    const int32_t *lbl_pos;    // ne: segment position of each entry of lbl_sorted
+};
+
+// Per walker kind launch parameters (app functor + g_para).
+struct KindParams {
+    int32_t mode;
+    int32_t ppr;            // terminate by stop probability (ppr.cpp:27-29) instead of length
+    int32_t length;         // g_para.length_
+    int32_t max_trials;     // 0 = no cap
+    int32_t unit_weights;   // rejection weights are 1.0 (TRW_WEIGHTS_UNIT tables)
+    int32_t schema_length;
+    double stop_prob;
+    double n2v_p, n2v_q, n2v_max;  // Node2Vec p_, q_, max_ (node2vec.cpp:18-20)
+    int32_t schema[16];
+};
+
+// Walker batch (device pointers).
+struct BatchIO {
+    const int32_t *sources;
+    int32_t n;
+    int32_t path_stride;
+    uint64_t gid_base;
+    const uint32_t *draws;
+    int64_t K;
+    const uint32_t *tdraws;
+    int64_t KT;
+    int32_t *paths;
+    int32_t *lengths;
+    int32_t *ends;
+    int32_t *status;
+    int32_t *draws_used;
+    unsigned long long *counters;  // [steps, gather-terminated, dead ends, errors]
+    uint32_t key0, key1;           // Philox key
+    int32_t replay;
+};
+
+// Walker state held in registers (fused, queue) or HBM (step form).
+struct WalkerState {
+    int32_t cur;
+    int32_t prev;
+    int32_t len;
+    int32_t st;
+    uint32_t k;   // replay move-draw cursor
+    uint32_t kt;  // replay termination-draw cursor
+};
+
+// Kernel launchers (walk.hip / tables.hip).  All asynchronous on `stream`.
+hipError_t launch_walk_fused(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream);
+hipError_t launch_walk_queue(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
+                             const BatchIO &io, unsigned int *queue_counter, hipStream_t stream);
+hipError_t launch_walk_step_init(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
+                                 const BatchIO &io, WalkerState *state, hipStream_t stream);
+hipError_t launch_walk_step(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
+                            const BatchIO &io, WalkerState *state, unsigned int *alive, hipStream_t stream);
+hipError_t launch_walk_step_finish(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                                   const WalkerState *state, hipStream_t stream);
+hipError_t launch_fill_u32(unsigned int *p, unsigned int value, int64_t n, hipStream_t stream);
+
+hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order,
+                              double *qbuf, int32_t *lists, AliasSlot *out, unsigned int *zero_count,
+                              hipStream_t stream);
+hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
+                            hipStream_t stream);
+hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
+                           hipStream_t stream);
+
+}  // namespace trw

@@ -1,0 +1,149 @@
+// Static table builds on the device: static_initialization (creeper.h:102-175).
+//
+// Every table is byte-identical to the reference's, so each vertex's FP64 work
+// runs in the reference's order on one lane: a left-to-right sum, IEEE division,
+// serial prefix sums.  Vertices are visited in descending degree order so the
+// long serial chains of hub vertices start first.
+#include "engine.h"
+
+namespace trw {
+namespace {
+
+constexpr int kBuildBlock = 128;
+
+__device__ __forceinline__ double weight_at(const double *w, int32_t unit, int64_t e) { return unit ? 1.0 : w[e]; }
+
+// alias_initialization (alias_sampling.h:12-54) without queues.
+//
+// The reference keeps two FIFO queues.  `large` is only ever popped at its
+// front, so it is the list of initial large indices in index order, consumed by
+// a cursor li.  `small` receives the initial small indices (index order) and then,
+// at its back, each large index at the moment it drops below 1 -- which happens
+// in `large` order.  So the small queue is always initial_smalls ++ larges[0..li)
+// and a second cursor si walks that concatenation.  lists[] holds the small
+// indices ascending from the segment start and the large indices from the
+// segment end backwards; q[] holds the normalised weights, and a large index's
+// residual is written back to q[] when it moves to the small side.
+__global__ __launch_bounds__(kBuildBlock) void alias_build_kernel(DevGraph g, const double *w, int32_t unit,
+                                                                   const int32_t *order, double *q, int32_t *lists,
+                                                                   AliasSlot *out, unsigned int *zero_count) {
+    const int32_t i = blockIdx.x * kBuildBlock + threadIdx.x;
+    if (i >= g.nv) return;
+    const int32_t v = order ? order[i] : i;
+    const int64_t off = g.offset[v];
+    const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
+    if (d == 0) return;
+
+    double sum = 0.0;  // std::accumulate(src, src + length, 0.0)
+    for (int32_t j = 0; j < d; ++j) sum = sum + weight_at(w, unit, off + j);
+    const double avg = sum / d;
+    AliasSlot *t = out + off;
+    if (avg == 0.0) {
+        for (int32_t j = 0; j < d; ++j) t[j] = AliasSlot{0.0, -1, -1};
+        atomicAdd(zero_count, 1u);
+        return;
+    }
+
+    double *qv = q + off;
+    int32_t *lst = lists + off;
+    const int32_t *nb = g.adj + off;
+    int32_t ns = 0, nl = 0;
+    for (int32_t j = 0; j < d; ++j) {
+        const double x = weight_at(w, unit, off + j) / avg;
+        qv[j] = x;
+        if (x < 1) lst[ns++] = j;
+        else lst[d - 1 - nl++] = j;
+    }
+
+    int32_t k = 0, si = 0, li = 0;
+    int32_t l = nl > 0 ? lst[d - 1] : 0;
+    double wl = nl > 0 ? qv[l] : 0.0;
+    while (si < ns + li && li < nl) {
+        const int32_t s = si < ns ? lst[si] : lst[d - 1 - (si - ns)];
+        const double ws = qv[s];
+        t[k++] = AliasSlot{ws, nb[s], nb[l]};
+        ++si;
+        wl = wl + ws - 1;
+        if (wl < 1) {
+            qv[l] = wl;
+            ++li;
+            if (li < nl) {
+                l = lst[d - 1 - li];
+                wl = qv[l];
+            }
+        }
+    }
+    for (; si < ns + li; ++si) {
+        const int32_t s = si < ns ? lst[si] : lst[d - 1 - (si - ns)];
+        t[k++] = AliasSlot{1.0, nb[s], nb[s]};
+    }
+    for (; li < nl; ++li) {
+        const int32_t x = lst[d - 1 - li];
+        t[k++] = AliasSlot{1.0, nb[x], nb[x]};
+    }
+}
+
+// its_initialization (inverse_transformation_sampling.h:19-24).
+__global__ __launch_bounds__(kBuildBlock) void its_build_kernel(DevGraph g, const double *w, int32_t unit,
+                                                                 const int32_t *order, double *out) {
+    const int32_t i = blockIdx.x * kBuildBlock + threadIdx.x;
+    if (i >= g.nv) return;
+    const int32_t v = order ? order[i] : i;
+    const int64_t off = g.offset[v];
+    const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
+    if (d == 0) return;
+    double acc = weight_at(w, unit, off);
+    out[off] = acc;
+    for (int32_t j = 1; j < d; ++j) {
+        acc = acc + weight_at(w, unit, off + j);
+        out[off + j] = acc;
+    }
+}
+
+// rs_initialization (rejection_sampling.h:12-14): *std::max_element.
+__global__ __launch_bounds__(kBuildBlock) void rs_build_kernel(DevGraph g, const double *w, int32_t unit,
+                                                                const int32_t *order, double *out) {
+    const int32_t i = blockIdx.x * kBuildBlock + threadIdx.x;
+    if (i >= g.nv) return;
+    const int32_t v = order ? order[i] : i;
+    const int64_t off = g.offset[v];
+    const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
+    if (d == 0) {
+        out[v] = 0.0;
+        return;
+    }
+    double m = weight_at(w, unit, off);
+    for (int32_t j = 1; j < d; ++j) {
+        const double x = weight_at(w, unit, off + j);
+        if (m < x) m = x;
+    }
+    out[v] = m;
+}
+
+inline unsigned grid_for(int64_t n) { return static_cast<unsigned>((n + kBuildBlock - 1) / kBuildBlock); }
+
+}  // namespace
+
+hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *qbuf,
+                              int32_t *lists, AliasSlot *out, unsigned int *zero_count, hipStream_t stream) {
+    if (g.nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(alias_build_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g, w, unit, order,
+                       qbuf, lists, out, zero_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
+                            hipStream_t stream) {
+    if (g.nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(its_build_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g, w, unit, order, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
+                           hipStream_t stream) {
+    if (g.nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rs_build_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g, w, unit, order, out);
+    return hipGetLastError();
+}
+
+}  // namespace trw

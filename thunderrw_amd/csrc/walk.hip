@@ -1,0 +1,636 @@
+// Walk kernels for gfx950: the Gather-Move-Update step engine.
+//
+// The reference hides DRAM latency on a CPU with a software-pipelined ring of
+// 64 walkers per thread (random_walk/uniform_sampling.h:11-45 and siblings).
+// Here latency is hidden by occupancy instead: one walker per lane, tens of
+// thousands of lanes in flight, every step a chain of dependent gathers.
+//
+// Three kernel forms share one per-step routine, advance():
+//   fused  - thread per walker, the whole fixed-length walk in one launch; path
+//            positions are staged in LDS and written as 64-byte row chunks.
+//   queue  - persistent: a lane whose walker finished takes the next walker id
+//            from a device counter (one atomic per wavefront per refill), the
+//            GPU analogue of the reference's ring refill (creeper.h:321-332).
+//   step   - the plain form: one launch per step over walker state kept in HBM.
+//
+// Exactness: the FP64 expressions are the reference's, evaluated in the same
+// order with contraction off (-ffp-contract=off); u % d is 32-bit unsigned, which
+// equals the reference's uint32 % int64 for 1 <= d < 2^31.
+#include "engine.h"
+
+namespace trw {
+namespace {
+
+constexpr int kFusedBlock = 256;
+constexpr int kTile = 16;          // path positions staged per flush (64-byte row chunks)
+constexpr int kQueueBlock = 256;
+constexpr int kStepBlock = 256;
+
+constexpr double kToReal1 = 1.0 / 4294967295.0;  // SFMT.h:193-207
+constexpr double kToReal2 = 1.0 / 4294967296.0;  // SFMT.h:214-228
+
+__device__ __forceinline__ double real1(uint32_t v) { return static_cast<double>(v) * kToReal1; }
+__device__ __forceinline__ double real2(uint32_t v) { return static_cast<double>(v) * kToReal2; }
+
+// Philox-4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                               uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+// Free-running stream: block (global walker id, step, trial).  Move draws are
+// words .x/.y of trial t's block; the PPR stop draw is word .x of trial 0xffffffff.
+__device__ __forceinline__ uint4 philox_block(const BatchIO &io, int32_t w, uint32_t step, uint32_t trial) {
+    const uint64_t gid = io.gid_base + static_cast<uint64_t>(w);
+    return philox4x32_10(static_cast<uint32_t>(gid), static_cast<uint32_t>(gid >> 32), step, trial, io.key0,
+                         io.key1);
+}
+
+__device__ __forceinline__ bool replay_take(const BatchIO &io, int32_t w, WalkerState &s, uint32_t &u) {
+    if (static_cast<int64_t>(s.k) >= io.K) {
+        s.st = kReplayExhausted;
+        return false;
+    }
+    u = io.draws[static_cast<int64_t>(s.k) * io.n + w];
+    s.k += 1;
+    return true;
+}
+
+template <bool REPLAY>
+__device__ __forceinline__ bool draw_one(const BatchIO &io, int32_t w, WalkerState &s, uint32_t step, uint32_t &u) {
+    if (REPLAY) return replay_take(io, w, s, u);
+    u = philox_block(io, w, step, 0u).x;
+    return true;
+}
+
+template <bool REPLAY>
+__device__ __forceinline__ bool draw_pair(const BatchIO &io, int32_t w, WalkerState &s, uint32_t step,
+                                          uint32_t trial, uint32_t &a, uint32_t &b) {
+    if (REPLAY) {
+        if (!replay_take(io, w, s, a)) return false;
+        return replay_take(io, w, s, b);
+    }
+    const uint4 x = philox_block(io, w, step, trial);
+    a = x.x;
+    b = x.y;
+    return true;
+}
+
+template <bool REPLAY>
+__device__ __forceinline__ bool draw_stop(const BatchIO &io, int32_t w, WalkerState &s, uint32_t step, uint32_t &u) {
+    if (REPLAY) {
+        if (static_cast<int64_t>(s.kt) >= io.KT) {
+            s.st = kReplayExhausted;
+            return false;
+        }
+        u = io.tdraws[static_cast<int64_t>(s.kt) * io.n + w];
+        s.kt += 1;
+        return true;
+    }
+    u = philox_block(io, w, step, 0xffffffffu).x;
+    return true;
+}
+
+// Utility::upper_bound_branchless (util/utility.h:18-27): the number of
+// entries <= target in a non-decreasing array.
+__device__ __forceinline__ int32_t count_le(const double *src, int32_t n, double target) {
+    const double *base = src;
+    while (n > 1) {
+        const int32_t half = n >> 1;
+        base = (base[half] <= target) ? base + half : base;
+        n -= half;
+    }
+    return static_cast<int32_t>((*base <= target) + (base - src));
+}
+
+// Graph::is_neighbor (util/graph/graph.h:180-199).
+__device__ __forceinline__ bool is_neighbor(const DevGraph &g, int32_t u, int32_t v) {
+    int32_t du = g.degree[u], dv = g.degree[v];
+    if (du < dv) {
+        const int32_t t = u;
+        u = v;
+        v = t;
+        dv = du;
+    }
+    const int32_t *nb = g.adj + g.offset[v];
+    int32_t b = 0, e = dv - 1;
+    while (b <= e) {
+        const int32_t m = b + ((e - b) >> 1);
+        const int32_t x = nb[m];
+        if (x == u) return true;
+        if (x > u) e = m - 1;
+        else b = m + 1;
+    }
+    return false;
+}
+
+__device__ __forceinline__ int32_t first_ge(const int32_t *a, int32_t n, int32_t x) {
+    int32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int32_t m = (lo + hi) >> 1;
+        if (a[m] < x) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int32_t first_gt(const int32_t *a, int32_t lo, int32_t n, int32_t x) {
+    int32_t hi = n;
+    while (lo < hi) {
+        const int32_t m = (lo + hi) >> 1;
+        if (a[m] <= x) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+// One Gather + Move + Update for walker w (creeper.h:298-334 for the update).
+// Returns true when the walker moved (and counted a step).  On return s.st is
+// kAlive or the walker's final status.
+template <int MODE, bool REPLAY>
+__device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams &P, const BatchIO &io, int32_t w,
+                                             WalkerState &s) {
+    const int32_t cur = s.cur;
+    const int64_t off = g.offset[cur];
+    const int32_t d = static_cast<int32_t>(g.offset[cur + 1] - off);
+    if (d <= 0) {
+        s.st = kDeadEnd;
+        return false;
+    }
+    const uint32_t step = static_cast<uint32_t>(s.len - 1);
+    const uint32_t ud = static_cast<uint32_t>(d);
+    int32_t next;
+
+    if (MODE == kModeUniform) {
+        // uniform_sampling.h:47-63
+        uint32_t u;
+        if (!draw_one<REPLAY>(io, w, s, step, u)) return false;
+        next = g.adj[off + (u % ud)];
+    } else if (MODE == kModeAlias) {
+        // alias_sampling.h:147-172: column u0 % d, then real2(u1) <= prob ? first : second.
+        uint32_t u0, u1;
+        if (!draw_pair<REPLAY>(io, w, s, step, 0u, u0, u1)) return false;
+        const uint4 raw = *reinterpret_cast<const uint4 *>(g.alias + off + (u0 % ud));
+        const double prob = __hiloint2double(static_cast<int>(raw.y), static_cast<int>(raw.x));
+        const int32_t first = static_cast<int32_t>(raw.z);
+        if (first < 0) {  // the vertex's weights sum to zero: terminate without consuming draws
+            if (REPLAY) s.k -= 2;
+            s.st = kGatherTerminated;
+            return false;
+        }
+        next = (real2(u1) <= prob) ? first : static_cast<int32_t>(raw.w);
+    } else if (MODE == kModeIts) {
+        // inverse_transformation_sampling.h:259-286
+        const double *P_ = g.prefix + off;
+        const double total = P_[d - 1];
+        if (total == 0.0) {
+            s.st = kGatherTerminated;
+            return false;
+        }
+        uint32_t u;
+        if (!draw_one<REPLAY>(io, w, s, step, u)) return false;
+        // The count is < d whenever the prefix sums are non-decreasing (non-negative
+        // weights); the clamp only keeps negative-weight inputs inside the segment.
+        next = g.adj[off + min(count_le(P_, d, real2(u) * total), d - 1)];
+    } else if (MODE == kModeRejection) {
+        // rejection_sampling.h:334-366: redraw (u % d, real2 * max) while r > w[pos].
+        const double m = g.rs_max[cur];
+        if (m == 0.0) {
+            s.st = kGatherTerminated;
+            return false;
+        }
+        uint32_t pos = 0;
+        for (uint32_t trial = 0;; ++trial) {
+            if (P.max_trials > 0 && trial >= static_cast<uint32_t>(P.max_trials)) {
+                s.st = kTrialCap;
+                return false;
+            }
+            uint32_t ua, ub;
+            if (!draw_pair<REPLAY>(io, w, s, step, trial, ua, ub)) return false;
+            pos = ua % ud;
+            const double r = real2(ub) * m;
+            const double wt = P.unit_weights ? 1.0 : g.weight[off + pos];
+            if (!(r > wt)) break;
+        }
+        next = g.adj[off + pos];
+    } else if (MODE == kModeNode2Vec) {
+        // rejection_sampling.h:368-402 with Node2Vec::weight (node2vec.cpp:24-38):
+        // accept iff real2(ub) * max_ <= weight(candidate).  The membership search
+        // runs only when the draw falls between the two possible weights.
+        const double lo = fmin(1.0, P.n2v_q), hi = fmax(1.0, P.n2v_q);
+        int32_t c = -1;
+        for (uint32_t trial = 0;; ++trial) {
+            if (P.max_trials > 0 && trial >= static_cast<uint32_t>(P.max_trials)) {
+                s.st = kTrialCap;
+                return false;
+            }
+            uint32_t ua, ub;
+            if (!draw_pair<REPLAY>(io, w, s, step, trial, ua, ub)) return false;
+            c = g.adj[off + (ua % ud)];
+            const double y = real2(ub) * P.n2v_max;
+            bool accept;
+            if (s.len == 1) accept = (y <= P.n2v_max);
+            else if (c == s.prev) accept = (y <= P.n2v_p);
+            else if (y <= lo) accept = true;
+            else if (y > hi) accept = false;
+            else accept = (y <= (is_neighbor(g, s.prev, c) ? 1.0 : P.n2v_q));
+            if (accept) break;
+        }
+        next = c;
+    } else {  // kModeMetaPath
+        // dynamic_gather (creeper.h:178-296) with MetaPath::weight (metapath.cpp:22-30)
+        // then dynamic_its_move: the prefix sums are match counts, so the pick is
+        // the (floor(real2(u) * K) + 1)-th neighbour whose label matches.
+        const int32_t target = P.schema[(s.len - 1) % P.schema_length];
+        const int32_t *ls = g.lbl_sorted + off;
+        const int32_t a = first_ge(ls, d, target);
+        const int32_t b = first_gt(ls, a, d, target);
+        const int32_t K = b - a;
+        if (K == 0) {
+            s.st = kGatherTerminated;
+            return false;
+        }
+        uint32_t u;
+        if (!draw_one<REPLAY>(io, w, s, step, u)) return false;
+        const double t = real2(u) * static_cast<double>(K);
+        next = g.adj[off + g.lbl_pos[off + a + static_cast<int32_t>(t)]];
+    }
+
+    s.prev = cur;
+    s.cur = next;
+    s.len += 1;
+    if (P.ppr) {
+        uint32_t ut;
+        if (!draw_stop<REPLAY>(io, w, s, step, ut)) return true;
+        if (real1(ut) <= P.stop_prob) s.st = kDone;  // ppr.cpp:27-29
+    } else if (s.len == P.length) {
+        s.st = kDone;                                 // deepwalk.cpp:23-25 and siblings
+    }
+    return true;
+}
+
+// MODE < 0: the walker kind's mode is read at run time (mixed batches).
+template <int MODE, bool REPLAY>
+__device__ __forceinline__ bool advance(const DevGraph &g, const KindParams &P, const BatchIO &io, int32_t w,
+                                        WalkerState &s) {
+    if (MODE >= 0) return advance_mode<MODE, REPLAY>(g, P, io, w, s);
+    switch (P.mode) {
+    case kModeUniform: return advance_mode<kModeUniform, REPLAY>(g, P, io, w, s);
+    case kModeAlias: return advance_mode<kModeAlias, REPLAY>(g, P, io, w, s);
+    case kModeIts: return advance_mode<kModeIts, REPLAY>(g, P, io, w, s);
+    case kModeRejection: return advance_mode<kModeRejection, REPLAY>(g, P, io, w, s);
+    case kModeNode2Vec: return advance_mode<kModeNode2Vec, REPLAY>(g, P, io, w, s);
+    default: return advance_mode<kModeMetaPath, REPLAY>(g, P, io, w, s);
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Adds per-lane counter values into io.counters with one atomic per wavefront.
+// Every lane of the wavefront must call it.
+__device__ __forceinline__ void flush_counters(const BatchIO &io, unsigned long long steps, unsigned long long gterm,
+                                               unsigned long long dead, unsigned long long err) {
+    steps = wave_sum(steps);
+    gterm = wave_sum(gterm);
+    dead = wave_sum(dead);
+    err = wave_sum(err);
+    if (io.counters != nullptr && (threadIdx.x & 63) == 0) {
+        if (steps) atomicAdd(io.counters + 0, steps);
+        if (gterm) atomicAdd(io.counters + 1, gterm);
+        if (dead) atomicAdd(io.counters + 2, dead);
+        if (err) atomicAdd(io.counters + 3, err);
+    }
+}
+
+__device__ __forceinline__ void classify(int32_t st, unsigned long long &gterm, unsigned long long &dead,
+                                         unsigned long long &err) {
+    gterm += (st == kGatherTerminated);
+    dead += (st == kDeadEnd);
+    err += (st == kReplayExhausted || st == kTrialCap || st == kBadSource);
+}
+
+__device__ __forceinline__ void write_outputs(const BatchIO &io, int32_t w, const WalkerState &s) {
+    if (io.lengths) io.lengths[w] = s.len;
+    if (io.ends) io.ends[w] = s.cur;
+    if (io.status) io.status[w] = s.st;
+    if (io.draws_used) io.draws_used[w] = static_cast<int32_t>(s.k);
+}
+
+__device__ __forceinline__ void init_walker(const DevGraph &g, const BatchIO &io, int32_t w, WalkerState &s) {
+    s.cur = io.sources[w];
+    s.prev = -1;
+    s.len = 1;
+    s.k = 0;
+    s.kt = 0;
+    s.st = (s.cur >= 0 && s.cur < g.nv) ? kAlive : kBadSource;
+}
+
+// ---------------------------------------------------------------- fused form
+template <int MODE, bool REPLAY>
+__global__ __launch_bounds__(kFusedBlock) void walk_fused_kernel(DevGraph g, KindParams P, BatchIO io) {
+    __shared__ int32_t tile[kFusedBlock * (kTile + 1)];
+    const int32_t tid = threadIdx.x;
+    const int32_t w0 = blockIdx.x * kFusedBlock;
+    const int32_t w = w0 + tid;
+    const bool valid = w < io.n;
+    WalkerState s = {0, -1, 1, kDone, 0u, 0u};
+    if (valid) init_walker(g, io, w, s);
+    const int32_t L = P.length;
+    const bool paths = io.paths != nullptr;
+    int32_t *row = tile + tid * (kTile + 1);
+    if (paths) row[0] = s.cur;
+    unsigned long long steps = 0;
+    int32_t t0 = 0;
+    for (int32_t pos = 1; pos < L; ++pos) {
+        int32_t v = -1;
+        if (s.st == kAlive && advance<MODE, REPLAY>(g, P, io, w, s)) {
+            v = s.cur;
+            ++steps;
+        }
+        if (paths) {
+            row[pos - t0] = v;
+            if (pos - t0 == kTile - 1 || pos == L - 1) {
+                const int32_t cnt = pos - t0 + 1;
+                __syncthreads();
+                for (int32_t i = tid; i < kFusedBlock * cnt; i += kFusedBlock) {
+                    const int32_t lw = i / cnt;
+                    const int32_t p = i - lw * cnt;
+                    if (w0 + lw < io.n)
+                        io.paths[static_cast<int64_t>(w0 + lw) * io.path_stride + t0 + p] = tile[lw * (kTile + 1) + p];
+                }
+                __syncthreads();
+                t0 = pos + 1;
+            }
+        }
+    }
+    unsigned long long gterm = 0, dead = 0, err = 0;
+    if (valid) {
+        if (s.st == kAlive) s.st = kDone;
+        write_outputs(io, w, s);
+        classify(s.st, gterm, dead, err);
+    }
+    flush_counters(io, steps, gterm, dead, err);
+}
+
+// ---------------------------------------------------------------- queue form
+template <int MODE, bool REPLAY>
+__global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, KindParams A, KindParams B,
+                                                                  int32_t split, BatchIO io, unsigned int *queue) {
+    const int lane = threadIdx.x & 63;
+    WalkerState s = {0, -1, 1, kDone, 0u, 0u};
+    int32_t w = -1;
+    bool kind_b = false;
+    bool drained = false;
+    unsigned long long steps = 0, gterm = 0, dead = 0, err = 0;
+    const unsigned int n = static_cast<unsigned int>(io.n);
+
+    for (;;) {
+        const bool need = (s.st != kAlive) && !drained;
+        const unsigned long long need_mask = __ballot(need);
+        if (need_mask) {
+            const int cnt = __popcll(need_mask);
+            const int leader = __ffsll(static_cast<long long>(need_mask)) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(queue, static_cast<unsigned int>(cnt));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const unsigned int idx = base + static_cast<unsigned int>(__popcll(need_mask & ((1ull << lane) - 1ull)));
+                if (idx < n) {
+                    w = static_cast<int32_t>(idx);
+                    kind_b = w >= split;
+                    init_walker(g, io, w, s);
+                    const KindParams &P = kind_b ? B : A;
+                    if (io.paths) {
+                        int32_t *rowp = io.paths + static_cast<int64_t>(w) * io.path_stride;
+                        rowp[0] = s.cur;
+                    }
+                    if (s.st != kAlive) {  // bad source: finished before the first step
+                        if (io.paths) {
+                            const int32_t Lp = P.length > 0 ? P.length : 1;
+                            for (int32_t p = 1; p < Lp; ++p) io.paths[static_cast<int64_t>(w) * io.path_stride + p] = -1;
+                        }
+                        write_outputs(io, w, s);
+                        classify(s.st, gterm, dead, err);
+                    }
+                }
+            }
+            if (base + static_cast<unsigned int>(cnt) >= n) drained = true;
+        }
+        const bool active = s.st == kAlive;
+        if (__ballot(active) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            const KindParams &P = kind_b ? B : A;
+            const bool moved = kind_b ? advance<MODE, REPLAY>(g, B, io, w, s) : advance<MODE, REPLAY>(g, A, io, w, s);
+            if (moved) {
+                ++steps;
+                if (io.paths && s.len - 1 < P.length) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
+            }
+            if (s.st != kAlive) {
+                if (io.paths)
+                    for (int32_t p = s.len; p < P.length; ++p) io.paths[static_cast<int64_t>(w) * io.path_stride + p] = -1;
+                write_outputs(io, w, s);
+                classify(s.st, gterm, dead, err);
+            }
+        }
+    }
+    flush_counters(io, steps, gterm, dead, err);
+}
+
+// ----------------------------------------------------------------- step form
+template <int MODE, bool REPLAY>
+__global__ __launch_bounds__(kStepBlock) void walk_step_init_kernel(DevGraph g, KindParams A, KindParams B,
+                                                                     int32_t split, BatchIO io, WalkerState *state) {
+    const int32_t w = blockIdx.x * kStepBlock + threadIdx.x;
+    if (w >= io.n) return;
+    WalkerState s;
+    init_walker(g, io, w, s);
+    state[w] = s;
+    if (io.paths) {
+        const KindParams &P = w >= split ? B : A;
+        const int32_t Lp = P.length > 0 ? P.length : 1;
+        int32_t *rowp = io.paths + static_cast<int64_t>(w) * io.path_stride;
+        rowp[0] = s.cur;
+        for (int32_t p = 1; p < Lp; ++p) rowp[p] = -1;
+    }
+}
+
+template <int MODE, bool REPLAY>
+__global__ __launch_bounds__(kStepBlock) void walk_step_kernel(DevGraph g, KindParams A, KindParams B, int32_t split,
+                                                                BatchIO io, WalkerState *state, unsigned int *alive) {
+    const int32_t w = blockIdx.x * kStepBlock + threadIdx.x;
+    unsigned long long steps = 0;
+    bool still = false;
+    if (w < io.n) {
+        WalkerState s = state[w];
+        if (s.st == kAlive) {
+            const bool kind_b = w >= split;
+            const KindParams &P = kind_b ? B : A;
+            const bool moved = kind_b ? advance<MODE, REPLAY>(g, B, io, w, s) : advance<MODE, REPLAY>(g, A, io, w, s);
+            if (moved) {
+                steps = 1;
+                if (io.paths && s.len - 1 < P.length) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
+            }
+            still = s.st == kAlive;
+            state[w] = s;
+        }
+    }
+    const unsigned long long live = __ballot(still);
+    if ((threadIdx.x & 63) == 0 && live) atomicAdd(alive, static_cast<unsigned int>(__popcll(live)));
+    steps = wave_sum(steps);
+    if (io.counters != nullptr && (threadIdx.x & 63) == 0 && steps) atomicAdd(io.counters, steps);
+}
+
+__global__ __launch_bounds__(kStepBlock) void walk_step_finish_kernel(BatchIO io, const WalkerState *state) {
+    const int32_t w = blockIdx.x * kStepBlock + threadIdx.x;
+    unsigned long long gterm = 0, dead = 0, err = 0;
+    if (w < io.n) {
+        WalkerState s = state[w];
+        if (s.st == kAlive) s.st = kDone;
+        write_outputs(io, w, s);
+        classify(s.st, gterm, dead, err);
+    }
+    flush_counters(io, 0, gterm, dead, err);
+}
+
+__global__ void fill_u32_kernel(unsigned int *p, unsigned int value, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+inline unsigned grid_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
+
+template <bool REPLAY>
+hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream) {
+    const dim3 grid(grid_for(io.n, kFusedBlock)), block(kFusedBlock);
+    switch (p.mode) {
+    case kModeUniform: hipLaunchKernelGGL((walk_fused_kernel<kModeUniform, REPLAY>), grid, block, 0, stream, g, p, io); break;
+    case kModeAlias: hipLaunchKernelGGL((walk_fused_kernel<kModeAlias, REPLAY>), grid, block, 0, stream, g, p, io); break;
+    case kModeIts: hipLaunchKernelGGL((walk_fused_kernel<kModeIts, REPLAY>), grid, block, 0, stream, g, p, io); break;
+    case kModeRejection: hipLaunchKernelGGL((walk_fused_kernel<kModeRejection, REPLAY>), grid, block, 0, stream, g, p, io); break;
+    case kModeNode2Vec: hipLaunchKernelGGL((walk_fused_kernel<kModeNode2Vec, REPLAY>), grid, block, 0, stream, g, p, io); break;
+    default: hipLaunchKernelGGL((walk_fused_kernel<kModeMetaPath, REPLAY>), grid, block, 0, stream, g, p, io); break;
+    }
+    return hipGetLastError();
+}
+
+int queue_grid(int device_cus) { return device_cus * 8; }
+
+template <int MODE, bool REPLAY>
+void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                  unsigned int *q, hipStream_t stream, unsigned grid) {
+    hipLaunchKernelGGL((walk_queue_kernel<MODE, REPLAY>), dim3(grid), dim3(kQueueBlock), 0, stream, g, a, b, split, io, q);
+}
+
+template <bool REPLAY>
+void queue_dispatch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                    unsigned int *q, hipStream_t stream, unsigned grid) {
+    const bool single = (split >= io.n) || (a.mode == b.mode);
+    if (!single) {
+        queue_launch<-1, REPLAY>(g, a, b, split, io, q, stream, grid);
+        return;
+    }
+    switch (split >= io.n ? a.mode : b.mode) {
+    case kModeUniform: queue_launch<kModeUniform, REPLAY>(g, a, b, split, io, q, stream, grid); break;
+    case kModeAlias: queue_launch<kModeAlias, REPLAY>(g, a, b, split, io, q, stream, grid); break;
+    case kModeIts: queue_launch<kModeIts, REPLAY>(g, a, b, split, io, q, stream, grid); break;
+    case kModeRejection: queue_launch<kModeRejection, REPLAY>(g, a, b, split, io, q, stream, grid); break;
+    case kModeNode2Vec: queue_launch<kModeNode2Vec, REPLAY>(g, a, b, split, io, q, stream, grid); break;
+    default: queue_launch<kModeMetaPath, REPLAY>(g, a, b, split, io, q, stream, grid); break;
+    }
+}
+
+template <bool REPLAY>
+void step_dispatch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                   WalkerState *state, unsigned int *alive, hipStream_t stream) {
+    const dim3 grid(grid_for(io.n, kStepBlock)), block(kStepBlock);
+    const bool single = (split >= io.n) || (a.mode == b.mode);
+    if (!single) {
+        hipLaunchKernelGGL((walk_step_kernel<-1, REPLAY>), grid, block, 0, stream, g, a, b, split, io, state, alive);
+        return;
+    }
+    switch (split >= io.n ? a.mode : b.mode) {
+    case kModeUniform: hipLaunchKernelGGL((walk_step_kernel<kModeUniform, REPLAY>), grid, block, 0, stream, g, a, b, split, io, state, alive); break;
+    case kModeAlias: hipLaunchKernelGGL((walk_step_kernel<kModeAlias, REPLAY>), grid, block, 0, stream, g, a, b, split, io, state, alive); break;
+    case kModeIts: hipLaunchKernelGGL((walk_step_kernel<kModeIts, REPLAY>), grid, block, 0, stream, g, a, b, split, io, state, alive); break;
+    case kModeRejection: hipLaunchKernelGGL((walk_step_kernel<kModeRejection, REPLAY>), grid, block, 0, stream, g, a, b, split, io, state, alive); break;
+    case kModeNode2Vec: hipLaunchKernelGGL((walk_step_kernel<kModeNode2Vec, REPLAY>), grid, block, 0, stream, g, a, b, split, io, state, alive); break;
+    default: hipLaunchKernelGGL((walk_step_kernel<kModeMetaPath, REPLAY>), grid, block, 0, stream, g, a, b, split, io, state, alive); break;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_walk_fused(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream) {
+    if (io.n <= 0) return hipSuccess;
+    return io.replay ? fused_dispatch<true>(g, p, io, stream) : fused_dispatch<false>(g, p, io, stream);
+}
+
+hipError_t launch_walk_queue(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
+                             const BatchIO &io, unsigned int *queue_counter, hipStream_t stream) {
+    if (io.n <= 0) return hipSuccess;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
+    }
+    const int64_t needed = grid_for(io.n, kQueueBlock);
+    const unsigned grid = static_cast<unsigned>(needed < queue_grid(cus) ? needed : queue_grid(cus));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, queue_counter, 0u, int64_t(1));
+    if (io.replay) queue_dispatch<true>(g, a, b, split, io, queue_counter, stream, grid);
+    else queue_dispatch<false>(g, a, b, split, io, queue_counter, stream, grid);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk_step_init(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
+                                 const BatchIO &io, WalkerState *state, hipStream_t stream) {
+    if (io.n <= 0) return hipSuccess;
+    const dim3 grid(grid_for(io.n, kStepBlock)), block(kStepBlock);
+    if (io.replay) hipLaunchKernelGGL((walk_step_init_kernel<-1, true>), grid, block, 0, stream, g, a, b, split, io, state);
+    else hipLaunchKernelGGL((walk_step_init_kernel<-1, false>), grid, block, 0, stream, g, a, b, split, io, state);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk_step(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
+                            const BatchIO &io, WalkerState *state, unsigned int *alive, hipStream_t stream) {
+    if (io.n <= 0) return hipSuccess;
+    if (io.replay) step_dispatch<true>(g, a, b, split, io, state, alive, stream);
+    else step_dispatch<false>(g, a, b, split, io, state, alive, stream);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk_step_finish(const KindParams &, const KindParams &, int32_t, const BatchIO &io,
+                                   const WalkerState *state, hipStream_t stream) {
+    if (io.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(walk_step_finish_kernel, dim3(grid_for(io.n, kStepBlock)), dim3(kStepBlock), 0, stream, io,
+                       state);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(unsigned int *p, unsigned int value, int64_t n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, value, n);
+    return hipGetLastError();
+}
+
+}  // namespace trw

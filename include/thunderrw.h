@@ -139,6 +139,9 @@ int trw_device_free(trw_context *ctx, void *ptr);
 int trw_copy_to_device(trw_context *ctx, void *dst, const void *src, uint64_t bytes);
 int trw_copy_to_host(trw_context *ctx, void *dst, const void *src, uint64_t bytes);
 int trw_device_zero(trw_context *ctx, void *ptr, uint64_t bytes, void *stream);
+/* Streaming copy kernel (16-byte vectors; bytes % 16 == 0): the known-size reference
+ * kernel the HBM byte counters are calibrated against. */
+int trw_device_copy(trw_context *ctx, void *dst, const void *src, uint64_t bytes, void *stream);
 int trw_synchronize(trw_context *ctx, void *stream);
 int trw_timer_begin(trw_context *ctx, void *stream);
 int trw_timer_end(trw_context *ctx, void *stream, double *elapsed_ms);

@@ -217,6 +217,15 @@ int trw_device_zero(trw_context *ctx, void *ptr, uint64_t bytes, void *stream) {
     return TRW_OK;
 }
 
+int trw_device_copy(trw_context *ctx, void *dst, const void *src, uint64_t bytes, void *stream) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(TRW_E_INVALID, "null argument");
+    if (bytes % 16) return fail(TRW_E_INVALID, "byte count must be a multiple of 16");
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    TRW_HIP(trw::launch_copy_u128(static_cast<uint4 *>(dst), static_cast<const uint4 *>(src),
+                                  static_cast<int64_t>(bytes / 16), pick_stream(ctx, stream)), "copy kernel");
+    return TRW_OK;
+}
+
 int trw_synchronize(trw_context *ctx, void *stream) {
     if (!ctx) return fail(TRW_E_INVALID, "null argument");
     TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");

@@ -107,6 +107,7 @@ hipError_t launch_walk_step(const DevGraph &g, const KindParams &a, const KindPa
 hipError_t launch_walk_step_finish(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                                    const WalkerState *state, hipStream_t stream);
 hipError_t launch_fill_u32(unsigned int *p, unsigned int value, int64_t n, hipStream_t stream);
+hipError_t launch_copy_u128(uint4 *dst, const uint4 *src, int64_t n, hipStream_t stream);
 
 hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order,
                               double *qbuf, int32_t *lists, AliasSlot *out, unsigned int *zero_count,

@@ -18,6 +18,8 @@
 // equals the reference's uint32 % int64 for 1 <= d < 2^31.
 #include "engine.h"
 
+#include <algorithm>
+
 namespace trw {
 namespace {
 
@@ -518,6 +520,12 @@ __global__ void fill_u32_kernel(unsigned int *p, unsigned int value, int64_t n) 
     if (i < n) p[i] = value;
 }
 
+__global__ void copy_u128_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 inline unsigned grid_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
 
 template <bool REPLAY>
@@ -624,6 +632,13 @@ hipError_t launch_walk_step_finish(const KindParams &, const KindParams &, int32
     if (io.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(walk_step_finish_kernel, dim3(grid_for(io.n, kStepBlock)), dim3(kStepBlock), 0, stream, io,
                        state);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_u128(uint4 *dst, const uint4 *src, int64_t n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>(grid_for(n, 256), 256 * 64);
+    hipLaunchKernelGGL(copy_u128_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, dst, src, n);
     return hipGetLastError();
 }
 

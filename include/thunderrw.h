@@ -95,6 +95,8 @@ typedef struct {
     uint64_t seed;          /* Philox key (PHILOX) */
     int32_t form;           /* TRW_FORM_* */
     int32_t max_trials;     /* rejection trials per step before TRW_WALKER_TRIAL_CAP; 0 = no cap */
+    int32_t walkers_per_thread; /* FUSED uniform/alias: walkers interleaved per lane (1, 2, 4); 0 = tuned default */
+    int32_t reserved;       /* must be 0 */
 } trw_walk_params;
 
 /* Walker batch: all pointers are DEVICE pointers (nullable where noted). */
@@ -145,6 +147,12 @@ int trw_device_copy(trw_context *ctx, void *dst, const void *src, uint64_t bytes
 int trw_synchronize(trw_context *ctx, void *stream);
 int trw_timer_begin(trw_context *ctx, void *stream);
 int trw_timer_end(trw_context *ctx, void *stream, double *elapsed_ms);
+
+/* Diagnostic (no reference counterpart): dependent random 16-byte gathers per second
+ * over a table of table_bytes, `chains` (1, 2, 4, 8) independent chains per lane at full
+ * occupancy, `steps` gathers per chain -- the ceiling for a Move that is one such gather. */
+int trw_probe_random_gather(trw_context *ctx, uint64_t table_bytes, int32_t chains, int32_t steps,
+                            double *gathers_per_s);
 
 /* ---- graph ----
  * Graph::load_csr (util/graph/graph.cpp:45-101) from host arrays: degree[nv]

@@ -35,7 +35,8 @@ class WalkParams(C.Structure):
     _fields_ = [("app", C.c_int32), ("execution", C.c_int32), ("sampling", C.c_int32), ("length", C.c_int32),
                 ("stop_probability", C.c_double), ("p", C.c_double), ("q", C.c_double),
                 ("schema", C.c_int32 * 16), ("schema_length", C.c_int32), ("rng", C.c_int32),
-                ("seed", C.c_uint64), ("form", C.c_int32), ("max_trials", C.c_int32)]
+                ("seed", C.c_uint64), ("form", C.c_int32), ("max_trials", C.c_int32),
+                ("walkers_per_thread", C.c_int32), ("reserved", C.c_int32)]
 
 
 class WalkIO(C.Structure):
@@ -59,6 +60,7 @@ _SIGS = {
     "trw_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "trw_device_zero": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "trw_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "trw_probe_random_gather": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_int32, C.c_void_p]),
     "trw_synchronize": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_timer_begin": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_timer_end": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

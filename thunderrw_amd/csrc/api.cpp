@@ -95,6 +95,9 @@ struct trw_graph {
     int32_t *lbl_sorted = nullptr;
     int32_t *lbl_pos = nullptr;
     int32_t *order = nullptr;
+    uint2 *vrec = nullptr;
+    uint4 *erec = nullptr;
+    uint4 *arec = nullptr;
     AliasSlot *alias = nullptr;
     double *prefix = nullptr;
     double *rs_max = nullptr;
@@ -118,6 +121,9 @@ struct trw_graph {
         g.rs_max = rs_max;
         g.lbl_sorted = lbl_sorted;
         g.lbl_pos = lbl_pos;
+        g.vrec = vrec;
+        g.erec = erec;
+        g.arec = arec;
         return g;
     }
 };
@@ -226,6 +232,16 @@ int trw_device_copy(trw_context *ctx, void *dst, const void *src, uint64_t bytes
     return TRW_OK;
 }
 
+int trw_probe_random_gather(trw_context *ctx, uint64_t table_bytes, int32_t chains, int32_t steps,
+                            double *gathers_per_s) {
+    if (!ctx || !gathers_per_s || steps <= 0) return fail(TRW_E_INVALID, "bad argument");
+    TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+    int cus = 0;
+    TRW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "device attribute");
+    TRW_HIP(trw::run_gather_probe(table_bytes, chains, steps, cus, ctx->stream, gathers_per_s), "gather probe");
+    return TRW_OK;
+}
+
 int trw_synchronize(trw_context *ctx, void *stream) {
     if (!ctx) return fail(TRW_E_INVALID, "null argument");
     TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
@@ -269,6 +285,9 @@ static void graph_release(trw_graph *g) {
     dfree(g->lbl_sorted);
     dfree(g->lbl_pos);
     dfree(g->order);
+    dfree(g->vrec);
+    dfree(g->erec);
+    dfree(g->arec);
     dfree(g->alias);
     dfree(g->prefix);
     dfree(g->rs_max);
@@ -335,6 +354,18 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
     TRW_HIP(hipMemcpy(g->degree, degree, sizeof(int32_t) * V, hipMemcpyHostToDevice), "copy degree");
     TRW_HIP(hipMemcpy(g->order, order.data(), sizeof(int32_t) * V, hipMemcpyHostToDevice), "copy order");
     if (E) TRW_HIP(hipMemcpy(g->adj, adj, sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy adj");
+    if (ne < (int64_t(1) << 32)) {
+        // Vertex records {first edge, degree}: one 8-byte gather per step instead of two offsets.
+        std::vector<uint2> rec(V);
+        for (size_t v = 0; v < V; ++v) rec[v] = make_uint2(static_cast<unsigned>(offset[v]), static_cast<unsigned>(degree[v]));
+        TRW_HIP(dmalloc(&g->vrec, V), "hipMalloc vertex records");
+        TRW_HIP(hipMemcpy(g->vrec, rec.data(), sizeof(uint2) * V, hipMemcpyHostToDevice), "copy vertex records");
+        // Neighbour records {adj[e], its first edge, its degree}: a Move's read of the
+        // chosen entry also yields the next vertex's range (one gather per step).
+        TRW_HIP(dmalloc(&g->erec, E + 1), "hipMalloc neighbour records");
+        TRW_HIP(trw::launch_edge_records(g->view(), g->erec, ctx->stream), "neighbour records");
+        TRW_HIP(hipStreamSynchronize(ctx->stream), "neighbour records");
+    }
     if (edge_weight) {
         TRW_HIP(dmalloc(&g->weight, E + 16), "hipMalloc weight");
         if (E) TRW_HIP(hipMemcpy(g->weight, edge_weight, sizeof(double) * E, hipMemcpyHostToDevice), "copy weight");
@@ -414,6 +445,12 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
         if (e != hipSuccess) return hip_fail(e, "alias build launch");
         if (e2 != hipSuccess) return hip_fail(e2, "alias build");
         g->alias_weights = weights;
+        if (g->vrec) {
+            // Alias walk records: the slot plus both candidates' ranges (32 bytes per slot).
+            if (!g->arec) TRW_HIP(dmalloc(&g->arec, 2 * E + 2), "hipMalloc alias walk records");
+            TRW_HIP(trw::launch_alias_records(g->view(), g->arec, st), "alias walk records");
+            TRW_HIP(hipStreamSynchronize(st), "alias walk records");
+        }
     } else if (sampling == TRW_SM_ITS) {
         if (!g->prefix) TRW_HIP(dmalloc(&g->prefix, E + 1), "hipMalloc prefix sums");
         (void)hipEventRecord(t0, st);
@@ -488,6 +525,11 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k) {
         return fail(TRW_E_INVALID, "length must be >= 2 (a walk of length l makes l-1 moves)");
     }
     if (p->max_trials < 0) return fail(TRW_E_INVALID, "max_trials must be >= 0");
+    if (p->walkers_per_thread != 0 && p->walkers_per_thread != 1 && p->walkers_per_thread != 2 &&
+        p->walkers_per_thread != 4)
+        return fail(TRW_E_INVALID, "walkers_per_thread must be 0, 1, 2 or 4");
+    if (p->reserved != 0) return fail(TRW_E_INVALID, "reserved must be 0");
+    k.walkers_per_thread = p->walkers_per_thread;
     k.ppr = ppr;
     k.length = p->length;
     k.max_trials = p->max_trials;

@@ -51,6 +51,10 @@ struct DevGraph {
     const double *rs_max;      // nv (rejection maxima)
     const int32_t *lbl_sorted; // ne: labels of each vertex's segment, sorted (stable)
     const int32_t *lbl_pos;    // ne: segment position of each entry of lbl_sorted
+    const uint2 *vrec;         // nv: {first edge, degree} when num_edges < 2^32, else null
+    const uint4 *erec;         // ne: neighbour records {adj[e], first edge of adj[e], degree of adj[e], 0}
+    const uint4 *arec;         // 2*ne: alias walk records {prob lo, prob hi, first, second}
+                               //       {first's edge, first's degree, second's edge, second's degree}
 };
 
 // Per walker kind launch parameters (app functor + g_para).
@@ -59,6 +63,7 @@ struct KindParams {
     int32_t ppr;            // terminate by stop probability (ppr.cpp:27-29) instead of length
     int32_t length;         // g_para.length_
     int32_t max_trials;     // 0 = no cap
+    int32_t walkers_per_thread;  // fused uniform/alias: 1, 2 or 4
     int32_t unit_weights;   // rejection weights are 1.0 (TRW_WEIGHTS_UNIT tables)
     int32_t schema_length;
     double stop_prob;
@@ -92,9 +97,14 @@ struct WalkerState {
     int32_t prev;
     int32_t len;
     int32_t st;
-    uint32_t k;   // replay move-draw cursor
-    uint32_t kt;  // replay termination-draw cursor
+    uint32_t k;     // replay move-draw cursor
+    uint32_t kt;    // replay termination-draw cursor
+    uint32_t coff;  // current vertex: first edge and degree (cdeg == kNoRange: not known yet)
+    uint32_t cdeg;
+    uint32_t poff;  // previous vertex: first edge and degree (Node2Vec membership test)
+    uint32_t pdeg;
 };
+constexpr uint32_t kNoRange = 0xffffffffu;
 
 // Kernel launchers (walk.hip / tables.hip).  All asynchronous on `stream`.
 hipError_t launch_walk_fused(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream);
@@ -109,9 +119,14 @@ hipError_t launch_walk_step_finish(const KindParams &a, const KindParams &b, int
 hipError_t launch_fill_u32(unsigned int *p, unsigned int value, int64_t n, hipStream_t stream);
 hipError_t launch_copy_u128(uint4 *dst, const uint4 *src, int64_t n, hipStream_t stream);
 
+hipError_t run_gather_probe(uint64_t table_bytes, int32_t chains, int32_t steps, int32_t cus, hipStream_t stream,
+                            double *loads_per_s);
+
 hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order,
                               double *qbuf, int32_t *lists, AliasSlot *out, unsigned int *zero_count,
                               hipStream_t stream);
+hipError_t launch_edge_records(const DevGraph &g, uint4 *erec, hipStream_t stream);
+hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stream);
 hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
                             hipStream_t stream);
 hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,

@@ -120,6 +120,35 @@ __global__ __launch_bounds__(kBuildBlock) void rs_build_kernel(DevGraph g, const
     out[v] = m;
 }
 
+// Neighbour records: a Move that lands on adj[e] also learns where that
+// vertex's adjacency starts and how long it is, so the next step needs no
+// separate vertex gather.
+__global__ void edge_records_kernel(const int32_t *adj, const uint2 *vrec, int64_t ne, uint4 *erec) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t v = adj[e];
+        const uint2 r = vrec[v];
+        erec[e] = make_uint4(static_cast<uint32_t>(v), r.x, r.y, 0u);
+    }
+}
+
+// Alias walk records: the reference AliasSlot plus both candidates' ranges.
+__global__ void alias_records_kernel(const AliasSlot *alias, const uint2 *vrec, int64_t ne, uint4 *arec) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const AliasSlot a = alias[e];
+        const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(a.prob));
+        arec[2 * e] = make_uint4(static_cast<uint32_t>(bits), static_cast<uint32_t>(bits >> 32),
+                                 static_cast<uint32_t>(a.first), static_cast<uint32_t>(a.second));
+        if (a.first < 0) {
+            arec[2 * e + 1] = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            const uint2 f = vrec[a.first], s = vrec[a.second];
+            arec[2 * e + 1] = make_uint4(f.x, f.y, s.x, s.y);
+        }
+    }
+}
+
 inline unsigned grid_for(int64_t n) { return static_cast<unsigned>((n + kBuildBlock - 1) / kBuildBlock); }
 
 }  // namespace
@@ -129,6 +158,18 @@ hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, 
     if (g.nv <= 0) return hipSuccess;
     hipLaunchKernelGGL(alias_build_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g, w, unit, order,
                        qbuf, lists, out, zero_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_records(const DevGraph &g, uint4 *erec, hipStream_t stream) {
+    if (g.ne <= 0) return hipSuccess;
+    hipLaunchKernelGGL(edge_records_kernel, dim3(8192), dim3(256), 0, stream, g.adj, g.vrec, g.ne, erec);
+    return hipGetLastError();
+}
+
+hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stream) {
+    if (g.ne <= 0) return hipSuccess;
+    hipLaunchKernelGGL(alias_records_kernel, dim3(8192), dim3(256), 0, stream, g.alias, g.vrec, g.ne, arec);
     return hipGetLastError();
 }
 

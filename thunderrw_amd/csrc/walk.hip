@@ -54,8 +54,10 @@ __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
     return make_uint4(c0, c1, c2, c3);
 }
 
-// Free-running stream: block (global walker id, step, trial).  Move draws are
-// words .x/.y of trial t's block; the PPR stop draw is word .x of trial 0xffffffff.
+// Free-running stream: one Philox block per (global walker id, step, trial).
+// A step's move draws are words .x/.y of its trial's block; the PPR stop draw is
+// word .w of the step's trial-0 block.  Callers hand the trial-0 block in, so it
+// can be computed a step ahead, while the previous step's gather is in flight.
 __device__ __forceinline__ uint4 philox_block(const BatchIO &io, int32_t w, uint32_t step, uint32_t trial) {
     const uint64_t gid = io.gid_base + static_cast<uint64_t>(w);
     return philox4x32_10(static_cast<uint32_t>(gid), static_cast<uint32_t>(gid >> 32), step, trial, io.key0,
@@ -73,27 +75,32 @@ __device__ __forceinline__ bool replay_take(const BatchIO &io, int32_t w, Walker
 }
 
 template <bool REPLAY>
-__device__ __forceinline__ bool draw_one(const BatchIO &io, int32_t w, WalkerState &s, uint32_t step, uint32_t &u) {
+__device__ __forceinline__ bool draw_one(const BatchIO &io, int32_t w, WalkerState &s, const uint4 &blk, uint32_t &u) {
     if (REPLAY) return replay_take(io, w, s, u);
-    u = philox_block(io, w, step, 0u).x;
+    u = blk.x;
     return true;
 }
 
 template <bool REPLAY>
-__device__ __forceinline__ bool draw_pair(const BatchIO &io, int32_t w, WalkerState &s, uint32_t step,
-                                          uint32_t trial, uint32_t &a, uint32_t &b) {
+__device__ __forceinline__ bool draw_pair(const BatchIO &io, int32_t w, WalkerState &s, uint32_t step, uint32_t trial,
+                                          const uint4 &blk, uint32_t &a, uint32_t &b) {
     if (REPLAY) {
         if (!replay_take(io, w, s, a)) return false;
         return replay_take(io, w, s, b);
     }
-    const uint4 x = philox_block(io, w, step, trial);
-    a = x.x;
-    b = x.y;
+    if (trial == 0) {
+        a = blk.x;
+        b = blk.y;
+    } else {
+        const uint4 x = philox_block(io, w, step, trial);
+        a = x.x;
+        b = x.y;
+    }
     return true;
 }
 
 template <bool REPLAY>
-__device__ __forceinline__ bool draw_stop(const BatchIO &io, int32_t w, WalkerState &s, uint32_t step, uint32_t &u) {
+__device__ __forceinline__ bool draw_stop(const BatchIO &io, int32_t w, WalkerState &s, const uint4 &blk, uint32_t &u) {
     if (REPLAY) {
         if (static_cast<int64_t>(s.kt) >= io.KT) {
             s.st = kReplayExhausted;
@@ -103,7 +110,7 @@ __device__ __forceinline__ bool draw_stop(const BatchIO &io, int32_t w, WalkerSt
         s.kt += 1;
         return true;
     }
-    u = philox_block(io, w, step, 0xffffffffu).x;
+    u = blk.w;
     return true;
 }
 
@@ -119,17 +126,21 @@ __device__ __forceinline__ int32_t count_le(const double *src, int32_t n, double
     return static_cast<int32_t>((*base <= target) + (base - src));
 }
 
-// Graph::is_neighbor (util/graph/graph.h:180-199).
-__device__ __forceinline__ bool is_neighbor(const DevGraph &g, int32_t u, int32_t v) {
-    int32_t du = g.degree[u], dv = g.degree[v];
-    if (du < dv) {
+// Graph::is_neighbor (util/graph/graph.h:180-199) with both endpoints' ranges
+// known: binary search of the larger-degree endpoint in the smaller-degree
+// endpoint's sorted adjacency.
+__device__ __forceinline__ bool is_neighbor(const DevGraph &g, int32_t u, int64_t uoff, int32_t udeg, int32_t v,
+                                            int64_t voff, int32_t vdeg) {
+    if (udeg < vdeg) {
         const int32_t t = u;
         u = v;
         v = t;
-        dv = du;
+        voff = uoff;
+        vdeg = udeg;
     }
-    const int32_t *nb = g.adj + g.offset[v];
-    int32_t b = 0, e = dv - 1;
+    (void)v;
+    const int32_t *nb = g.adj + voff;
+    int32_t b = 0, e = vdeg - 1;
     while (b <= e) {
         const int32_t m = b + ((e - b) >> 1);
         const int32_t x = nb[m];
@@ -160,15 +171,92 @@ __device__ __forceinline__ int32_t first_gt(const int32_t *a, int32_t lo, int32_
     return lo;
 }
 
-// One Gather + Move + Update for walker w (creeper.h:298-334 for the update).
-// Returns true when the walker moved (and counted a step).  On return s.st is
-// kAlive or the walker's final status.
+// First edge and degree of v from the graph: one 8-byte vertex record when the
+// graph has them (num_edges < 2^32), else the two CSR offsets.
+__device__ __forceinline__ void vertex_range(const DevGraph &g, int32_t v, int64_t &off, int32_t &d) {
+    if (g.vrec) {
+        const uint2 r = g.vrec[v];
+        off = static_cast<int64_t>(r.x);
+        d = static_cast<int32_t>(r.y);
+    } else {
+        off = g.offset[v];
+        d = static_cast<int32_t>(g.offset[v + 1] - off);
+    }
+}
+
+// Range of a vertex whose record may already be held by the walker.
+__device__ __forceinline__ void known_range(const DevGraph &g, int32_t v, uint32_t roff, uint32_t rdeg, int64_t &off,
+                                            int32_t &d) {
+    if (rdeg != kNoRange) {
+        off = static_cast<int64_t>(roff);
+        d = static_cast<int32_t>(rdeg);
+    } else {
+        vertex_range(g, v, off, d);
+    }
+}
+
+// Adjacency entry e; with neighbour records the same 16-byte read also yields
+// the neighbour's range, which the walker carries into its next step.
+__device__ __forceinline__ int32_t read_neighbor(const DevGraph &g, int64_t e, uint32_t &noff, uint32_t &ndeg) {
+    if (g.erec) {
+        const uint4 r = g.erec[e];
+        noff = r.y;
+        ndeg = r.z;
+        return static_cast<int32_t>(r.x);
+    }
+    noff = 0u;
+    ndeg = kNoRange;
+    return g.adj[e];
+}
+
+// Alias slot e (alias_sampling.h:147-172 pick).  Returns false for a vertex
+// whose weights sum to zero.
+__device__ __forceinline__ bool read_alias(const DevGraph &g, int64_t e, uint32_t u1, int32_t &next, uint32_t &noff,
+                                           uint32_t &ndeg) {
+    if (g.arec) {
+        const uint4 a = g.arec[2 * e];
+        const uint4 b = g.arec[2 * e + 1];
+        if (static_cast<int32_t>(a.z) < 0) return false;
+        const bool take_first = real2(u1) <= __hiloint2double(static_cast<int>(a.y), static_cast<int>(a.x));
+        next = static_cast<int32_t>(take_first ? a.z : a.w);
+        noff = take_first ? b.x : b.z;
+        ndeg = take_first ? b.y : b.w;
+        return true;
+    }
+    const uint4 raw = *reinterpret_cast<const uint4 *>(g.alias + e);
+    if (static_cast<int32_t>(raw.z) < 0) return false;
+    const double prob = __hiloint2double(static_cast<int>(raw.y), static_cast<int>(raw.x));
+    next = (real2(u1) <= prob) ? static_cast<int32_t>(raw.z) : static_cast<int32_t>(raw.w);
+    noff = 0u;
+    ndeg = kNoRange;
+    return true;
+}
+
+// Update (creeper.h:298-334) after a move from `cur` (range off/d) to `next`.
+__device__ __forceinline__ void finish_move(const DevGraph &g, const KindParams &P, WalkerState &s, int32_t cur,
+                                            int64_t off, int32_t d, int32_t next, uint32_t noff, uint32_t ndeg) {
+    s.prev = cur;
+    s.poff = static_cast<uint32_t>(off);
+    s.pdeg = g.vrec ? static_cast<uint32_t>(d) : kNoRange;
+    s.cur = next;
+    s.coff = noff;
+    s.cdeg = ndeg;
+    s.len += 1;
+    (void)P;
+}
+
+// One Gather + Move + Update for walker w.  Returns true when the walker moved
+// (and counted a step).  On return s.st is kAlive or the walker's final status.
+//
+// blk holds this step's trial-0 Philox block on entry; after the step's gather
+// has been issued the next step's block is computed into it (free-running only).
 template <int MODE, bool REPLAY>
 __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams &P, const BatchIO &io, int32_t w,
-                                             WalkerState &s) {
+                                             WalkerState &s, uint4 &blk) {
     const int32_t cur = s.cur;
-    const int64_t off = g.offset[cur];
-    const int32_t d = static_cast<int32_t>(g.offset[cur + 1] - off);
+    int64_t off;
+    int32_t d;
+    known_range(g, cur, s.coff, s.cdeg, off, d);
     if (d <= 0) {
         s.st = kDeadEnd;
         return false;
@@ -176,25 +264,26 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     const uint32_t step = static_cast<uint32_t>(s.len - 1);
     const uint32_t ud = static_cast<uint32_t>(d);
     int32_t next;
+    uint32_t noff = 0u, ndeg = kNoRange;
+    uint4 nblk = blk;
 
     if (MODE == kModeUniform) {
         // uniform_sampling.h:47-63
         uint32_t u;
-        if (!draw_one<REPLAY>(io, w, s, step, u)) return false;
-        next = g.adj[off + (u % ud)];
+        if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
+        next = read_neighbor(g, off + (u % ud), noff, ndeg);
+        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
     } else if (MODE == kModeAlias) {
         // alias_sampling.h:147-172: column u0 % d, then real2(u1) <= prob ? first : second.
         uint32_t u0, u1;
-        if (!draw_pair<REPLAY>(io, w, s, step, 0u, u0, u1)) return false;
-        const uint4 raw = *reinterpret_cast<const uint4 *>(g.alias + off + (u0 % ud));
-        const double prob = __hiloint2double(static_cast<int>(raw.y), static_cast<int>(raw.x));
-        const int32_t first = static_cast<int32_t>(raw.z);
-        if (first < 0) {  // the vertex's weights sum to zero: terminate without consuming draws
-            if (REPLAY) s.k -= 2;
+        if (!draw_pair<REPLAY>(io, w, s, step, 0u, blk, u0, u1)) return false;
+        const bool ok = read_alias(g, off + (u0 % ud), u1, next, noff, ndeg);
+        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
+        if (!ok) {
+            if (REPLAY) s.k -= 2;  // the vertex's weights sum to zero: terminate without consuming draws
             s.st = kGatherTerminated;
             return false;
         }
-        next = (real2(u1) <= prob) ? first : static_cast<int32_t>(raw.w);
     } else if (MODE == kModeIts) {
         // inverse_transformation_sampling.h:259-286
         const double *P_ = g.prefix + off;
@@ -204,10 +293,12 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             return false;
         }
         uint32_t u;
-        if (!draw_one<REPLAY>(io, w, s, step, u)) return false;
+        if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
+        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
         // The count is < d whenever the prefix sums are non-decreasing (non-negative
         // weights); the clamp only keeps negative-weight inputs inside the segment.
-        next = g.adj[off + min(count_le(P_, d, real2(u) * total), d - 1)];
+        const int32_t pos = min(count_le(P_, d, real2(u) * total), d - 1);
+        next = read_neighbor(g, off + pos, noff, ndeg);
     } else if (MODE == kModeRejection) {
         // rejection_sampling.h:334-366: redraw (u % d, real2 * max) while r > w[pos].
         const double m = g.rs_max[cur];
@@ -222,13 +313,14 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
                 return false;
             }
             uint32_t ua, ub;
-            if (!draw_pair<REPLAY>(io, w, s, step, trial, ua, ub)) return false;
+            if (!draw_pair<REPLAY>(io, w, s, step, trial, blk, ua, ub)) return false;
             pos = ua % ud;
             const double r = real2(ub) * m;
             const double wt = P.unit_weights ? 1.0 : g.weight[off + pos];
             if (!(r > wt)) break;
         }
-        next = g.adj[off + pos];
+        next = read_neighbor(g, off + pos, noff, ndeg);
+        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
     } else if (MODE == kModeNode2Vec) {
         // rejection_sampling.h:368-402 with Node2Vec::weight (node2vec.cpp:24-38):
         // accept iff real2(ub) * max_ <= weight(candidate).  The membership search
@@ -241,18 +333,25 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
                 return false;
             }
             uint32_t ua, ub;
-            if (!draw_pair<REPLAY>(io, w, s, step, trial, ua, ub)) return false;
-            c = g.adj[off + (ua % ud)];
+            if (!draw_pair<REPLAY>(io, w, s, step, trial, blk, ua, ub)) return false;
+            c = read_neighbor(g, off + (ua % ud), noff, ndeg);
             const double y = real2(ub) * P.n2v_max;
             bool accept;
             if (s.len == 1) accept = (y <= P.n2v_max);
             else if (c == s.prev) accept = (y <= P.n2v_p);
             else if (y <= lo) accept = true;
             else if (y > hi) accept = false;
-            else accept = (y <= (is_neighbor(g, s.prev, c) ? 1.0 : P.n2v_q));
+            else {
+                int64_t poff, coff;
+                int32_t pd, cd;
+                known_range(g, s.prev, s.poff, s.pdeg, poff, pd);
+                known_range(g, c, noff, ndeg, coff, cd);
+                accept = (y <= (is_neighbor(g, s.prev, poff, pd, c, coff, cd) ? 1.0 : P.n2v_q));
+            }
             if (accept) break;
         }
         next = c;
+        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
     } else {  // kModeMetaPath
         // dynamic_gather (creeper.h:178-296) with MetaPath::weight (metapath.cpp:22-30)
         // then dynamic_its_move: the prefix sums are match counts, so the pick is
@@ -267,17 +366,18 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             return false;
         }
         uint32_t u;
-        if (!draw_one<REPLAY>(io, w, s, step, u)) return false;
+        if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
         const double t = real2(u) * static_cast<double>(K);
-        next = g.adj[off + g.lbl_pos[off + a + static_cast<int32_t>(t)]];
+        next = read_neighbor(g, off + g.lbl_pos[off + a + static_cast<int32_t>(t)], noff, ndeg);
+        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
     }
 
-    s.prev = cur;
-    s.cur = next;
-    s.len += 1;
+    finish_move(g, P, s, cur, off, d, next, noff, ndeg);
+    const uint4 this_blk = blk;
+    blk = nblk;
     if (P.ppr) {
         uint32_t ut;
-        if (!draw_stop<REPLAY>(io, w, s, step, ut)) return true;
+        if (!draw_stop<REPLAY>(io, w, s, this_blk, ut)) return true;
         if (real1(ut) <= P.stop_prob) s.st = kDone;  // ppr.cpp:27-29
     } else if (s.len == P.length) {
         s.st = kDone;                                 // deepwalk.cpp:23-25 and siblings
@@ -288,16 +388,23 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
 // MODE < 0: the walker kind's mode is read at run time (mixed batches).
 template <int MODE, bool REPLAY>
 __device__ __forceinline__ bool advance(const DevGraph &g, const KindParams &P, const BatchIO &io, int32_t w,
-                                        WalkerState &s) {
-    if (MODE >= 0) return advance_mode<MODE, REPLAY>(g, P, io, w, s);
+                                        WalkerState &s, uint4 &blk) {
+    if (MODE >= 0) return advance_mode<MODE, REPLAY>(g, P, io, w, s, blk);
     switch (P.mode) {
-    case kModeUniform: return advance_mode<kModeUniform, REPLAY>(g, P, io, w, s);
-    case kModeAlias: return advance_mode<kModeAlias, REPLAY>(g, P, io, w, s);
-    case kModeIts: return advance_mode<kModeIts, REPLAY>(g, P, io, w, s);
-    case kModeRejection: return advance_mode<kModeRejection, REPLAY>(g, P, io, w, s);
-    case kModeNode2Vec: return advance_mode<kModeNode2Vec, REPLAY>(g, P, io, w, s);
-    default: return advance_mode<kModeMetaPath, REPLAY>(g, P, io, w, s);
+    case kModeUniform: return advance_mode<kModeUniform, REPLAY>(g, P, io, w, s, blk);
+    case kModeAlias: return advance_mode<kModeAlias, REPLAY>(g, P, io, w, s, blk);
+    case kModeIts: return advance_mode<kModeIts, REPLAY>(g, P, io, w, s, blk);
+    case kModeRejection: return advance_mode<kModeRejection, REPLAY>(g, P, io, w, s, blk);
+    case kModeNode2Vec: return advance_mode<kModeNode2Vec, REPLAY>(g, P, io, w, s, blk);
+    default: return advance_mode<kModeMetaPath, REPLAY>(g, P, io, w, s, blk);
     }
+}
+
+// Trial-0 Philox block of the walker's next step (free-running; unused in replay).
+template <bool REPLAY>
+__device__ __forceinline__ uint4 first_block(const BatchIO &io, int32_t w, const WalkerState &s) {
+    if (REPLAY) return make_uint4(0u, 0u, 0u, 0u);
+    return philox_block(io, w, static_cast<uint32_t>(s.len - 1), 0u);
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
@@ -336,49 +443,76 @@ __device__ __forceinline__ void write_outputs(const BatchIO &io, int32_t w, cons
     if (io.draws_used) io.draws_used[w] = static_cast<int32_t>(s.k);
 }
 
+__device__ __forceinline__ WalkerState empty_walker() {
+    return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange};
+}
+
 __device__ __forceinline__ void init_walker(const DevGraph &g, const BatchIO &io, int32_t w, WalkerState &s) {
+    s = empty_walker();
     s.cur = io.sources[w];
-    s.prev = -1;
-    s.len = 1;
-    s.k = 0;
-    s.kt = 0;
     s.st = (s.cur >= 0 && s.cur < g.nv) ? kAlive : kBadSource;
 }
 
 // ---------------------------------------------------------------- fused form
+//
+// Path output.  Each wavefront stages T consecutive path positions of its own
+// walkers in LDS and then writes them as contiguous row chunks (T * 4 bytes per
+// walker row, 64 bytes for T = 16).  Only lanes of one wavefront exchange data
+// through its region, so a wavefront-scope fence suffices: no workgroup barrier
+// makes fast wavefronts wait for slow ones.
+template <int WPT, int T>
+struct WaveTile {
+    int32_t *region;  // WPT * 64 rows of T + 1 ints
+    int32_t lane;
+
+    __device__ __forceinline__ void put(int j, int32_t p, int32_t v) { region[(j * 64 + lane) * (T + 1) + p] = v; }
+
+    // Writes positions [t0, t0 + cnt) of every walker of this wavefront; walker
+    // (j, lane) is first_walker + j * stride_j + lane.
+    __device__ __forceinline__ void flush(const BatchIO &io, int32_t first_walker, int32_t stride_j, int32_t t0,
+                                          int32_t cnt) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int32_t i = lane; i < WPT * 64 * cnt; i += 64) {
+            const int32_t lw = i / cnt;
+            const int32_t p = i - lw * cnt;
+            const int32_t gw = first_walker + (lw >> 6) * stride_j + (lw & 63);
+            if (gw < io.n) io.paths[static_cast<int64_t>(gw) * io.path_stride + t0 + p] = region[lw * (T + 1) + p];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
+// Thread per walker, the whole fixed-length walk in one launch (any Move).
 template <int MODE, bool REPLAY>
 __global__ __launch_bounds__(kFusedBlock) void walk_fused_kernel(DevGraph g, KindParams P, BatchIO io) {
     __shared__ int32_t tile[kFusedBlock * (kTile + 1)];
     const int32_t tid = threadIdx.x;
-    const int32_t w0 = blockIdx.x * kFusedBlock;
-    const int32_t w = w0 + tid;
+    const int32_t w = blockIdx.x * kFusedBlock + tid;
+    const int32_t wave_first = w - (tid & 63);
+    WaveTile<1, kTile> out{tile + (tid >> 6) * 64 * (kTile + 1), tid & 63};
     const bool valid = w < io.n;
-    WalkerState s = {0, -1, 1, kDone, 0u, 0u};
+    WalkerState s = empty_walker();
     if (valid) init_walker(g, io, w, s);
+    uint4 blk = first_block<REPLAY>(io, w, s);
     const int32_t L = P.length;
     const bool paths = io.paths != nullptr;
-    int32_t *row = tile + tid * (kTile + 1);
-    if (paths) row[0] = s.cur;
+    if (paths) out.put(0, 0, s.cur);
     unsigned long long steps = 0;
     int32_t t0 = 0;
     for (int32_t pos = 1; pos < L; ++pos) {
         int32_t v = -1;
-        if (s.st == kAlive && advance<MODE, REPLAY>(g, P, io, w, s)) {
+        if (s.st == kAlive && advance<MODE, REPLAY>(g, P, io, w, s, blk)) {
             v = s.cur;
             ++steps;
         }
         if (paths) {
-            row[pos - t0] = v;
+            out.put(0, pos - t0, v);
             if (pos - t0 == kTile - 1 || pos == L - 1) {
-                const int32_t cnt = pos - t0 + 1;
-                __syncthreads();
-                for (int32_t i = tid; i < kFusedBlock * cnt; i += kFusedBlock) {
-                    const int32_t lw = i / cnt;
-                    const int32_t p = i - lw * cnt;
-                    if (w0 + lw < io.n)
-                        io.paths[static_cast<int64_t>(w0 + lw) * io.path_stride + t0 + p] = tile[lw * (kTile + 1) + p];
-                }
-                __syncthreads();
+                out.flush(io, wave_first, 0, t0, pos - t0 + 1);
                 t0 = pos + 1;
             }
         }
@@ -392,12 +526,139 @@ __global__ __launch_bounds__(kFusedBlock) void walk_fused_kernel(DevGraph g, Kin
     flush_counters(io, steps, gterm, dead, err);
 }
 
+// Fused form, uniform and alias Moves, WPT walkers interleaved per lane.
+//
+// With neighbour / alias walk records a step is one dependent gather (the chosen
+// entry, which also carries the next vertex's range).  Issuing the WPT walkers'
+// gathers back to back keeps WPT independent requests in flight per lane, on top
+// of the eight wavefronts per SIMD that occupancy provides.  Walker (t, j) of a
+// block is block_base + j * blockDim + t, so every per-j access is coalesced.
+template <int MODE, bool REPLAY, int WPT>
+__global__ __launch_bounds__(kFusedBlock) void walk_fused_multi_kernel(DevGraph g, KindParams P, BatchIO io) {
+    constexpr int T = kTile / WPT;  // staged positions per flush: 16 KiB of LDS per block for any WPT
+    __shared__ int32_t tile[WPT * kFusedBlock * (T + 1)];
+    const int32_t tid = threadIdx.x;
+    const int32_t wbase = blockIdx.x * (kFusedBlock * WPT);
+    WaveTile<WPT, T> out{tile + (tid >> 6) * (WPT * 64 * (T + 1)), tid & 63};
+    const int32_t wave_first = wbase + (tid & ~63);
+    const bool paths = io.paths != nullptr;
+    const int32_t L = P.length;
+    WalkerState s[WPT];
+    uint4 blk[WPT];
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+        const int32_t w = wbase + j * kFusedBlock + tid;
+        s[j] = empty_walker();
+        if (w < io.n) init_walker(g, io, w, s[j]);
+        blk[j] = first_block<REPLAY>(io, w, s[j]);
+        if (paths) out.put(j, 0, s[j].cur);
+    }
+    unsigned long long steps = 0;
+    int32_t t0 = 0;
+    for (int32_t pos = 1; pos < L; ++pos) {
+        int64_t off[WPT];
+        int32_t d[WPT];
+        uint32_t u0[WPT], u1[WPT];
+        bool go[WPT];
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            go[j] = s[j].st == kAlive;
+            off[j] = 0;
+            d[j] = 1;
+            if (go[j]) known_range(g, s[j].cur, s[j].coff, s[j].cdeg, off[j], d[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            const int32_t w = wbase + j * kFusedBlock + tid;
+            const uint32_t step = static_cast<uint32_t>(s[j].len - 1);
+            u0[j] = u1[j] = 0;
+            if (!go[j]) continue;
+            if (d[j] <= 0) {
+                s[j].st = kDeadEnd;
+                go[j] = false;
+                continue;
+            }
+            if (MODE == kModeUniform) go[j] = draw_one<REPLAY>(io, w, s[j], blk[j], u0[j]);
+            else go[j] = draw_pair<REPLAY>(io, w, s[j], step, 0u, blk[j], u0[j], u1[j]);
+        }
+        // Issue every walker's gather, then compute next steps' Philox blocks in
+        // the gathers' shadow, then consume.
+        uint4 ra[WPT], rb[WPT];
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            if (!go[j]) continue;
+            const int64_t e = off[j] + (u0[j] % static_cast<uint32_t>(d[j]));
+            if (MODE == kModeUniform) {
+                ra[j] = g.erec[e];
+            } else {
+                ra[j] = g.arec[2 * e];
+                rb[j] = g.arec[2 * e + 1];
+            }
+        }
+        // The tile holds positions [t0, pos); when full, write it out now, behind
+        // the gathers just issued, so the stores overlap them instead of delaying
+        // the next step's wait.
+        if (paths && pos - t0 == T) {
+            out.flush(io, wave_first, kFusedBlock, t0, T);
+            t0 = pos;
+        }
+        if (!REPLAY) {
+#pragma unroll
+            for (int j = 0; j < WPT; ++j)
+                if (go[j]) blk[j] = philox_block(io, wbase + j * kFusedBlock + tid, static_cast<uint32_t>(s[j].len), 0u);
+        }
+        int32_t next[WPT];
+#pragma unroll
+        for (int j = 0; j < WPT; ++j) {
+            next[j] = -1;
+            if (!go[j]) continue;
+            uint32_t noff, ndeg;
+            if (MODE == kModeUniform) {
+                next[j] = static_cast<int32_t>(ra[j].x);
+                noff = ra[j].y;
+                ndeg = ra[j].z;
+            } else {
+                if (static_cast<int32_t>(ra[j].z) < 0) {  // the vertex's weights sum to zero
+                    if (REPLAY) s[j].k -= 2;
+                    s[j].st = kGatherTerminated;
+                    continue;
+                }
+                const bool take_first =
+                    real2(u1[j]) <= __hiloint2double(static_cast<int>(ra[j].y), static_cast<int>(ra[j].x));
+                next[j] = static_cast<int32_t>(take_first ? ra[j].z : ra[j].w);
+                noff = take_first ? rb[j].x : rb[j].z;
+                ndeg = take_first ? rb[j].y : rb[j].w;
+            }
+            finish_move(g, P, s[j], s[j].cur, off[j], d[j], next[j], noff, ndeg);
+            ++steps;
+            if (s[j].len == L) s[j].st = kDone;
+        }
+        if (paths) {
+#pragma unroll
+            for (int j = 0; j < WPT; ++j) out.put(j, pos - t0, next[j]);
+        }
+    }
+    if (paths) out.flush(io, wave_first, kFusedBlock, t0, L - t0);
+    unsigned long long gterm = 0, dead = 0, err = 0;
+#pragma unroll
+    for (int j = 0; j < WPT; ++j) {
+        const int32_t w = wbase + j * kFusedBlock + tid;
+        if (w < io.n) {
+            if (s[j].st == kAlive) s[j].st = kDone;
+            write_outputs(io, w, s[j]);
+            classify(s[j].st, gterm, dead, err);
+        }
+    }
+    flush_counters(io, steps, gterm, dead, err);
+}
+
 // ---------------------------------------------------------------- queue form
 template <int MODE, bool REPLAY>
 __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, KindParams A, KindParams B,
                                                                   int32_t split, BatchIO io, unsigned int *queue) {
     const int lane = threadIdx.x & 63;
-    WalkerState s = {0, -1, 1, kDone, 0u, 0u};
+    WalkerState s = empty_walker();
+    uint4 blk = make_uint4(0u, 0u, 0u, 0u);
     int32_t w = -1;
     bool kind_b = false;
     bool drained = false;
@@ -419,6 +680,7 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                     w = static_cast<int32_t>(idx);
                     kind_b = w >= split;
                     init_walker(g, io, w, s);
+                    blk = first_block<REPLAY>(io, w, s);
                     const KindParams &P = kind_b ? B : A;
                     if (io.paths) {
                         int32_t *rowp = io.paths + static_cast<int64_t>(w) * io.path_stride;
@@ -443,7 +705,8 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
         }
         if (active) {
             const KindParams &P = kind_b ? B : A;
-            const bool moved = kind_b ? advance<MODE, REPLAY>(g, B, io, w, s) : advance<MODE, REPLAY>(g, A, io, w, s);
+            const bool moved =
+                kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
             if (moved) {
                 ++steps;
                 if (io.paths && s.len - 1 < P.length) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
@@ -488,7 +751,9 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_kernel(DevGraph g, KindP
         if (s.st == kAlive) {
             const bool kind_b = w >= split;
             const KindParams &P = kind_b ? B : A;
-            const bool moved = kind_b ? advance<MODE, REPLAY>(g, B, io, w, s) : advance<MODE, REPLAY>(g, A, io, w, s);
+            uint4 blk = first_block<REPLAY>(io, w, s);
+            const bool moved =
+                kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
             if (moved) {
                 steps = 1;
                 if (io.paths && s.len - 1 < P.length) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
@@ -528,9 +793,34 @@ __global__ void copy_u128_kernel(uint4 *__restrict__ dst, const uint4 *__restric
 
 inline unsigned grid_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
 
+template <int MODE, bool REPLAY>
+void fused_multi_launch(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream) {
+    const dim3 block(kFusedBlock);
+    switch (p.walkers_per_thread == 0 ? 1 : p.walkers_per_thread) {
+    case 4:
+        hipLaunchKernelGGL((walk_fused_multi_kernel<MODE, REPLAY, 4>), dim3(grid_for(io.n, 4 * kFusedBlock)), block, 0,
+                           stream, g, p, io);
+        break;
+    case 2:
+        hipLaunchKernelGGL((walk_fused_multi_kernel<MODE, REPLAY, 2>), dim3(grid_for(io.n, 2 * kFusedBlock)), block, 0,
+                           stream, g, p, io);
+        break;
+    default:
+        hipLaunchKernelGGL((walk_fused_multi_kernel<MODE, REPLAY, 1>), dim3(grid_for(io.n, kFusedBlock)), block, 0,
+                           stream, g, p, io);
+        break;
+    }
+}
+
 template <bool REPLAY>
 hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream) {
     const dim3 grid(grid_for(io.n, kFusedBlock)), block(kFusedBlock);
+    const bool records = (p.mode == kModeUniform) ? g.erec != nullptr : g.arec != nullptr;
+    if (records && !p.ppr && (p.mode == kModeUniform || p.mode == kModeAlias)) {
+        if (p.mode == kModeUniform) fused_multi_launch<kModeUniform, REPLAY>(g, p, io, stream);
+        else fused_multi_launch<kModeAlias, REPLAY>(g, p, io, stream);
+        return hipGetLastError();
+    }
     switch (p.mode) {
     case kModeUniform: hipLaunchKernelGGL((walk_fused_kernel<kModeUniform, REPLAY>), grid, block, 0, stream, g, p, io); break;
     case kModeAlias: hipLaunchKernelGGL((walk_fused_kernel<kModeAlias, REPLAY>), grid, block, 0, stream, g, p, io); break;

@@ -176,7 +176,7 @@ class DeviceGraph:
 
 
 def make_params(app, execution=None, sampling=None, length=None, stop_probability=0.2, p=2.0, q=0.5, schema=None,
-                rng="philox", seed=0, form="auto", max_trials=0):
+                rng="philox", seed=0, form="auto", max_trials=0, walkers_per_thread=0):
     """g_para + app functor arguments.  Unspecified fields take the app's reference defaults."""
     name = app if isinstance(app, str) else {v: k for k, v in A.APPS.items()}[app]
     de, ds, dl = APP_DEFAULTS[name]
@@ -197,6 +197,7 @@ def make_params(app, execution=None, sampling=None, length=None, stop_probabilit
     prm.seed = int(seed)
     prm.form = _enum(_FORM, form)
     prm.max_trials = int(max_trials)
+    prm.walkers_per_thread = int(walkers_per_thread)
     return prm
 
 

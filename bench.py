@@ -20,6 +20,7 @@ binary (oracle/_ref/deepwalk.out, built from the reference sources) on all
 host cores on the same graph.
 """
 import argparse
+import ctypes
 import json
 import os
 import re
@@ -225,6 +226,15 @@ def main():
         details["its_steps_per_s"] = per_launch / (ms_i / 1e3 / args.steps)
         log("fused ITS (philox): %.3e steps/s" % details["its_steps_per_s"])
 
+    if not args.quick:
+        # Measured ceiling for this access pattern: dependent random 16-byte gathers at full
+        # occupancy over a table the size of the alias walk records (and a 32 MiB cache-resident one).
+        for label, nbytes in (("probe_gathers_per_s_4GiB", 4 << 30), ("probe_gathers_per_s_32MiB", 32 << 20)):
+            v = ctypes.c_double()
+            T._abi.check(T.lib().trw_probe_random_gather(ctx.h, nbytes, 4, 64, ctypes.byref(v)))
+            details[label] = v.value
+        log("random-gather ceiling: %.3e gathers/s (4 GiB table)" % details["probe_gathers_per_s_4GiB"])
+
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20
         a = T.DeviceBuffer(ctx, nbytes)
@@ -257,7 +267,7 @@ def main():
                                "FP64 U[1,5) weights, length 80, one walker per non-isolated vertex, philox"
                                % args.scale,
                    "vertices": nv, "directed_edges": int(len(adj)), "walkers": n, "walk_length": length,
-                   "kernel": "walk_fused_kernel<alias, philox>"},
+                   "kernel": "walk_fused_multi_kernel<alias, philox, 1 walker per lane>"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,

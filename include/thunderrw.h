@@ -88,7 +88,8 @@ typedef struct {
     int32_t sampling;       /* TRW_SM_*, -sm */
     int32_t length;         /* g_para.length_ (-l; PPR uses 1, ppr.cpp:51) */
     double stop_probability;/* PPR -sp (ppr.cpp:46-49) */
-    double p, q;            /* Node2Vec -p, -q as given; inverted as node2vec.cpp:69,76 */
+    double p, q;            /* Node2Vec p_ and q_: return and in-out weights, i.e. 1/p and 1/q of
+                               the command line (node2vec.cpp:64-76) */
     int32_t schema[16];     /* MetaPath -s (metapath.cpp:69-83) */
     int32_t schema_length;
     int32_t rng;            /* TRW_RNG_* */

@@ -59,6 +59,40 @@ def build(verbose=False):
     return LIB
 
 
+DROPIN = os.path.join(HERE, "dropin")
+DROPIN_APPS = ["deepwalk", "ppr", "node2vec", "metapath", "genericwalk"]
+
+
+def build_dropin(reference_dir, verbose=False):
+    """Compiles the reference's own app sources, unchanged, against dropin/creeper_main.h and this
+    library, into dropin/bin/<app> (a checkout of the reference is needed; the binaries are not
+    committed)."""
+    apps_dir = os.path.join(reference_dir, "random_walk", "apps")
+    if not os.path.isdir(apps_dir):
+        return []
+    out_dir = os.path.join(DROPIN, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+
+    def one(app):
+        exe = os.path.join(out_dir, app)
+        cmd = [cxx, "-std=c++17", "-O2", "-I", DROPIN, "-I", os.path.join(ROOT, "include"),
+               os.path.join(apps_dir, app + ".cpp"), "-o", exe, "-L", HERE, "-lthunderrw",
+               "-Wl,-rpath,$ORIGIN/../.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("drop-in build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+        return exe
+
+    with cf.ThreadPoolExecutor(max_workers=len(DROPIN_APPS)) as ex:
+        exes = list(ex.map(one, DROPIN_APPS))
+    if verbose:
+        print("built drop-in apps:", " ".join(DROPIN_APPS))
+    return exes
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    if len(sys.argv) > 1:
+        build_dropin(sys.argv[1], verbose=True)
     sys.exit(0)

@@ -544,8 +544,8 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k) {
             return fail(TRW_E_UNSUPPORTED, "Node2Vec runs in Dynamic mode with max-weight rejection sampling");
         if (!(p->p > 0.0) || !(p->q > 0.0)) return fail(TRW_E_INVALID, "Node2Vec p and q must be > 0");
         k.mode = trw::kModeNode2Vec;
-        k.n2v_p = 1.0 / p->p;  // node2vec.cpp:69
-        k.n2v_q = 1.0 / p->q;  // node2vec.cpp:76
+        k.n2v_p = p->p;  // Node2Vec::p_ (node2vec.cpp:18, after the inversion at :69)
+        k.n2v_q = p->q;  // Node2Vec::q_ (:76)
         k.n2v_max = std::max(1.0, std::max(k.n2v_q, k.n2v_p));  // node2vec.cpp:19
         return TRW_OK;
     }

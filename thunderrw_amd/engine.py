@@ -177,7 +177,11 @@ class DeviceGraph:
 
 def make_params(app, execution=None, sampling=None, length=None, stop_probability=0.2, p=2.0, q=0.5, schema=None,
                 rng="philox", seed=0, form="auto", max_trials=0, walkers_per_thread=0):
-    """g_para + app functor arguments.  Unspecified fields take the app's reference defaults."""
+    """g_para + app functor arguments.  Unspecified fields take the app's reference defaults.
+
+    p and q are Node2Vec's command-line values; the functor's weights are 1/p and 1/q
+    (node2vec.cpp:64-76), which is what the C ABI takes.
+    """
     name = app if isinstance(app, str) else {v: k for k, v in A.APPS.items()}[app]
     de, ds, dl = APP_DEFAULTS[name]
     prm = A.WalkParams()
@@ -186,7 +190,7 @@ def make_params(app, execution=None, sampling=None, length=None, stop_probabilit
     prm.sampling = _enum(_SM, sampling if sampling is not None else ds)
     prm.length = dl if length is None else int(length)
     prm.stop_probability = stop_probability
-    prm.p, prm.q = p, q
+    prm.p, prm.q = 1.0 / p, 1.0 / q
     sch = list(schema or [])
     if len(sch) > 16:
         raise ValueError("schema longer than 16")

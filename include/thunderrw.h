@@ -97,8 +97,14 @@ typedef struct {
     int32_t form;           /* TRW_FORM_* */
     int32_t max_trials;     /* rejection trials per step before TRW_WALKER_TRIAL_CAP; 0 = no cap */
     int32_t walkers_per_thread; /* FUSED uniform/alias: walkers interleaved per lane (1, 2, 4); 0 = tuned default */
-    int32_t reserved;       /* must be 0 */
+    int32_t weights;        /* TRW_WALK_WEIGHTS_*: the weights Static/Dynamic sampling uses */
 } trw_walk_params;
+
+/* trw_walk_params.weights.  APP is the reference: F::weight of the app (edge weights for
+ * DeepWalk/GenericWalk, 1.0 for PPR).  EDGE and UNIT override it, e.g. PPR over edge weights. */
+#define TRW_WALK_WEIGHTS_APP 0
+#define TRW_WALK_WEIGHTS_EDGE 1
+#define TRW_WALK_WEIGHTS_UNIT 2
 
 /* Walker batch: all pointers are DEVICE pointers (nullable where noted). */
 typedef struct {
@@ -116,6 +122,7 @@ typedef struct {
     int32_t *status;               /* nullable: TRW_WALKER_* */
     int32_t *draws_used;           /* nullable: REPLAY move draws consumed */
     uint64_t *counters;            /* nullable: [0] steps [1] gather-terminated [2] dead ends [3] errors; accumulated */
+    const uint64_t *walker_ids;    /* nullable: per-walker global ids keying Philox (else walker_id_base + index) */
 } trw_walk_io;
 
 typedef struct trw_context trw_context;

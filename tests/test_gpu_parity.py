@@ -244,6 +244,56 @@ def test_mixed_batch_queue_equals_step_and_single_kinds(gpu_ctx):
         dg.close()
 
 
+def test_walker_ids_key_the_free_running_stream(gpu_ctx):
+    """A walker's Philox stream depends only on its global id: a permuted batch with
+    per-walker ids gives each walker the walk it has in the ordered batch."""
+    g = graphs.chung_lu(20000, 120000, 2.1, 2, 3000, 43)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        live = np.nonzero(g.degree > 0)[0]
+        rng = np.random.default_rng(8)
+        n = 50000
+        src = rng.choice(live, n).astype(np.int32)
+        perm = rng.permutation(n)
+        base = 1 << 40
+        for app, sm, form in [("deepwalk", "alias", "fused"), ("ppr", "alias", "queue"), ("node2vec", "max_weight_rejection", "queue")]:
+            prm = T.make_params(app, execution="static" if app != "node2vec" else "dynamic", sampling=sm, length=30,
+                                rng="philox", seed=21, form=form, stop_probability=0.2)
+            ordered = T.walk(dg, prm, src, walker_id_base=base)
+            shuffled = T.walk(dg, prm, src[perm], walker_ids=base + perm.astype(np.uint64))
+            for key in ("paths", "lengths", "ends", "status"):
+                assert np.array_equal(shuffled[key], ordered[key][perm]), (app, key)
+    finally:
+        dg.close()
+
+
+def test_weights_selector_overrides_the_app_weights(gpu_ctx):
+    """PPR samples unit weights by default (its functor's weight() is 1.0); weights="edge"
+    makes the same walk sample the edge weights.  Both equal the oracle's replay walks over
+    alias tables built from those weights."""
+    g = graphs.chung_lu(5000, 30000, 2.1, 2, 600, 44)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        src = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        K, KT = 256, 128
+        draws, tdraws = T.replay_fill(31, len(src), K), T.replay_fill(32, len(src), KT)
+        d, td = draws.T.copy(), tdraws.T.copy()
+        out = {}
+        for sel, w in [("app", np.ones(g.ne)), ("edge", np.asarray(g.weight, np.float64))]:
+            prm = T.make_params("ppr", execution="static", sampling="alias", rng="replay", weights=sel,
+                                stop_probability=0.2)
+            r = T.walk(dg, prm, src, draws, tdraws)
+            alias, _ = ob.alias_build(g, w)
+            ref = ob.walk(ob.Graph(g, alias), "ppr", 1, 2, 1, src, d, K, td, KT, stop_prob=0.2)
+            for key in ("lengths", "ends", "status", "draws_used"):
+                assert np.array_equal(r[key], ref[key]), (sel, key)
+            assert np.all(r["status"] == 0)
+            out[sel] = r["ends"]
+        assert not np.array_equal(out["app"], out["edge"])
+    finally:
+        dg.close()
+
+
 def test_full_size_deepwalk_alias_graph(gpu_ctx):
     """2^22-vertex Chung-Lu graph: device tables equal the oracle's, replay paths of a
     walker subset equal the oracle's, and every free-running step follows an edge."""

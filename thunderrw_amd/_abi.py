@@ -14,6 +14,7 @@ APP_DEEPWALK, APP_PPR, APP_NODE2VEC, APP_METAPATH, APP_GENERIC = 0, 1, 2, 3, 4
 RNG_REPLAY, RNG_PHILOX = 0, 1
 FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP = 0, 1, 2, 3
 WEIGHTS_EDGE, WEIGHTS_UNIT = 0, 1
+WALK_WEIGHTS_APP, WALK_WEIGHTS_EDGE, WALK_WEIGHTS_UNIT = 0, 1, 2
 WALKER_DONE, WALKER_GATHER_TERMINATED, WALKER_DEAD_END, WALKER_REPLAY_EXHAUSTED, WALKER_TRIAL_CAP, \
     WALKER_BAD_SOURCE = 0, 1, 2, 3, 4, 5
 
@@ -36,7 +37,7 @@ class WalkParams(C.Structure):
                 ("stop_probability", C.c_double), ("p", C.c_double), ("q", C.c_double),
                 ("schema", C.c_int32 * 16), ("schema_length", C.c_int32), ("rng", C.c_int32),
                 ("seed", C.c_uint64), ("form", C.c_int32), ("max_trials", C.c_int32),
-                ("walkers_per_thread", C.c_int32), ("reserved", C.c_int32)]
+                ("walkers_per_thread", C.c_int32), ("weights", C.c_int32)]
 
 
 class WalkIO(C.Structure):
@@ -44,7 +45,7 @@ class WalkIO(C.Structure):
                 ("walker_id_base", C.c_int64), ("draws", C.c_void_p), ("draws_per_walker", C.c_int64),
                 ("term_draws", C.c_void_p), ("term_draws_per_walker", C.c_int64), ("paths", C.c_void_p),
                 ("lengths", C.c_void_p), ("ends", C.c_void_p), ("status", C.c_void_p), ("draws_used", C.c_void_p),
-                ("counters", C.c_void_p)]
+                ("counters", C.c_void_p), ("walker_ids", C.c_void_p)]
 
 
 _SIGS = {

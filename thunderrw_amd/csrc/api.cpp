@@ -14,6 +14,7 @@
 
 #include "engine.h"
 #include "hostgraph.h"
+#include "internal.h"
 #include "sfmt.h"
 
 using trw::AliasSlot;
@@ -76,61 +77,21 @@ void dfree(T *&p) {
 
 }  // namespace
 
-struct trw_context {
-    int32_t device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t timer[2] = {nullptr, nullptr};
-};
-
-struct trw_graph {
-    trw_context *ctx = nullptr;
-    int32_t nv = 0;
-    int64_t ne = 0;
-    int32_t max_degree = 0;
-    int64_t *offset = nullptr;
-    int32_t *adj = nullptr;
-    int32_t *degree = nullptr;
-    double *weight = nullptr;
-    int32_t *label = nullptr;
-    int32_t *lbl_sorted = nullptr;
-    int32_t *lbl_pos = nullptr;
-    int32_t *order = nullptr;
-    uint2 *vrec = nullptr;
-    uint4 *erec = nullptr;
-    uint4 *arec = nullptr;
-    AliasSlot *alias = nullptr;
-    double *prefix = nullptr;
-    double *rs_max = nullptr;
-    int32_t alias_weights = -1, its_weights = -1, rs_weights = -1;  // TRW_WEIGHTS_* the table was built from
-    unsigned int *counters = nullptr;                                 // [0] queue head [1] alive [2] zero-sum count
-    WalkerState *state = nullptr;
-    int64_t state_cap = 0;
-    double last_build_ms = 0.0;
-
-    DevGraph view() const {
-        DevGraph g;
-        g.nv = nv;
-        g.ne = ne;
-        g.offset = offset;
-        g.adj = adj;
-        g.degree = degree;
-        g.weight = weight;
-        g.label = label;
-        g.alias = alias;
-        g.prefix = prefix;
-        g.rs_max = rs_max;
-        g.lbl_sorted = lbl_sorted;
-        g.lbl_pos = lbl_pos;
-        g.vrec = vrec;
-        g.erec = erec;
-        g.arec = arec;
-        return g;
-    }
-};
-
 struct trw_host_graph {
     trw::HostGraph g;
 };
+
+int trw::set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int trw::set_hip_error(hipError_t e, const char *what) { return hip_fail(e, what); }
 
 // The entry points take their C linkage from the declarations in thunderrw.h.
 
@@ -509,7 +470,7 @@ int trw_copy_rejection_max(const trw_graph *g, double *host_out) {
 
 // Maps an app + g_para to a device mode, building the static tables the mode
 // reads when they are missing.
-static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k) {
+static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, bool allow_build = true) {
     std::memset(&k, 0, sizeof(k));
     if (p->app < TRW_APP_DEEPWALK || p->app > TRW_APP_GENERIC) return fail(TRW_E_INVALID, "unknown app %d", p->app);
     if (p->execution < TRW_EXEC_UNIFORM || p->execution > TRW_EXEC_DYNAMIC)
@@ -528,7 +489,8 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k) {
     if (p->walkers_per_thread != 0 && p->walkers_per_thread != 1 && p->walkers_per_thread != 2 &&
         p->walkers_per_thread != 4)
         return fail(TRW_E_INVALID, "walkers_per_thread must be 0, 1, 2 or 4");
-    if (p->reserved != 0) return fail(TRW_E_INVALID, "reserved must be 0");
+    if (p->weights < TRW_WALK_WEIGHTS_APP || p->weights > TRW_WALK_WEIGHTS_UNIT)
+        return fail(TRW_E_INVALID, "unknown weights selector %d", p->weights);
     k.walkers_per_thread = p->walkers_per_thread;
     k.ppr = ppr;
     k.length = p->length;
@@ -564,11 +526,16 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k) {
     if (p->sampling == TRW_SM_UNIFORM || p->sampling == TRW_SM_MAX_WEIGHT_REJECTION)
         return fail(TRW_E_UNSUPPORTED, "sampling method %d is not defined for this app in mode %d", p->sampling,
                     p->execution);
-    const int32_t wsrc = ppr ? TRW_WEIGHTS_UNIT : TRW_WEIGHTS_EDGE;
+    const int32_t wsrc = p->weights == TRW_WALK_WEIGHTS_EDGE   ? TRW_WEIGHTS_EDGE
+                         : p->weights == TRW_WALK_WEIGHTS_UNIT ? TRW_WEIGHTS_UNIT
+                         : ppr                                  ? TRW_WEIGHTS_UNIT
+                                                                : TRW_WEIGHTS_EDGE;
+    if (wsrc == TRW_WEIGHTS_EDGE && !g->weight) return fail(TRW_E_STATE, "graph was uploaded without edge weights");
     int32_t *built = p->sampling == TRW_SM_ALIAS ? &g->alias_weights
                      : p->sampling == TRW_SM_ITS ? &g->its_weights
                                                  : &g->rs_weights;
     if (*built != wsrc) {
+        if (!allow_build) return fail(TRW_E_STATE, "the tables this walk reads have not been built (trw_build_tables)");
         int rc = trw_build_tables(g, p->sampling, wsrc);
         if (rc != TRW_OK) return rc;
     }
@@ -600,6 +567,7 @@ static int make_io(const trw_graph *g, const trw_walk_params *p, const trw_walk_
     out.n = io->n_walkers;
     out.path_stride = io->path_stride;
     out.gid_base = static_cast<uint64_t>(io->walker_id_base);
+    out.walker_ids = io->walker_ids;
     out.draws = replay ? io->draws : nullptr;
     out.K = replay ? io->draws_per_walker : 0;
     out.tdraws = replay ? io->term_draws : nullptr;
@@ -642,16 +610,17 @@ static int run_step_form(trw_graph *g, const KindParams &a, const KindParams &b,
     return TRW_OK;
 }
 
-int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream) {
+int trw::walk_launch(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, hipStream_t stream,
+                     unsigned int *queue_head, bool allow_table_build) {
     if (!g || !params) return fail(TRW_E_INVALID, "null argument");
     TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
     KindParams k;
-    int rc = resolve_kind(g, params, k);
+    int rc = resolve_kind(g, params, k, allow_table_build);
     if (rc != TRW_OK) return rc;
     BatchIO bio;
     rc = make_io(g, params, io, k, k, bio);
     if (rc != TRW_OK) return rc;
-    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : g->ctx->stream;
+    hipStream_t st = stream ? stream : g->ctx->stream;
     int32_t form = params->form;
     if (form == TRW_FORM_AUTO) form = (k.ppr || k.mode == trw::kModeMetaPath) ? TRW_FORM_QUEUE : TRW_FORM_FUSED;
     const DevGraph dv = g->view();
@@ -661,13 +630,18 @@ int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_i
         TRW_HIP(trw::launch_walk_fused(dv, k, bio, st), "fused walk launch");
         return TRW_OK;
     case TRW_FORM_QUEUE:
-        TRW_HIP(trw::launch_walk_queue(dv, k, k, bio.n, bio, g->counters, st), "queue walk launch");
+        TRW_HIP(trw::launch_walk_queue(dv, k, k, bio.n, bio, queue_head ? queue_head : g->counters, st),
+                "queue walk launch");
         return TRW_OK;
     case TRW_FORM_STEP:
         return run_step_form(g, k, k, bio.n, bio, st);
     default:
         return fail(TRW_E_INVALID, "unknown form %d", params->form);
     }
+}
+
+int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream) {
+    return trw::walk_launch(g, params, io, reinterpret_cast<hipStream_t>(stream), nullptr, true);
 }
 
 int trw_walk_mix_async(trw_graph *g, const trw_walk_params *pa, const trw_walk_params *pb, int32_t split,

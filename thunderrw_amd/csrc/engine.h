@@ -64,7 +64,7 @@ struct KindParams {
     int32_t length;         // g_para.length_
     int32_t max_trials;     // 0 = no cap
     int32_t walkers_per_thread;  // fused uniform/alias: 1, 2 or 4
-    int32_t unit_weights;   // rejection weights are 1.0 (TRW_WEIGHTS_UNIT tables)
+    int32_t unit_weights;   // rejection weights are 1.0 (unit-weight tables)
     int32_t schema_length;
     double stop_prob;
     double n2v_p, n2v_q, n2v_max;  // Node2Vec p_, q_, max_ (node2vec.cpp:18-20)
@@ -77,6 +77,7 @@ struct BatchIO {
     int32_t n;
     int32_t path_stride;
     uint64_t gid_base;
+    const uint64_t *walker_ids;  // per-walker global ids (null: gid_base + index)
     const uint32_t *draws;
     int64_t K;
     const uint32_t *tdraws;
@@ -103,6 +104,8 @@ struct WalkerState {
     uint32_t cdeg;
     uint32_t poff;  // previous vertex: first edge and degree (Node2Vec membership test)
     uint32_t pdeg;
+    uint32_t gid_lo;  // global walker id: the Philox counter's walker words
+    uint32_t gid_hi;
 };
 constexpr uint32_t kNoRange = 0xffffffffu;
 

@@ -1,0 +1,73 @@
+// Library-internal declarations shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/thunderrw.h"
+#include "engine.h"
+
+struct trw_context {
+    int32_t device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t timer[2] = {nullptr, nullptr};
+};
+
+struct trw_graph {
+    trw_context *ctx = nullptr;
+    int32_t nv = 0;
+    int64_t ne = 0;
+    int32_t max_degree = 0;
+    int64_t *offset = nullptr;
+    int32_t *adj = nullptr;
+    int32_t *degree = nullptr;
+    double *weight = nullptr;
+    int32_t *label = nullptr;
+    int32_t *lbl_sorted = nullptr;
+    int32_t *lbl_pos = nullptr;
+    int32_t *order = nullptr;
+    uint2 *vrec = nullptr;
+    uint4 *erec = nullptr;
+    uint4 *arec = nullptr;
+    trw::AliasSlot *alias = nullptr;
+    double *prefix = nullptr;
+    double *rs_max = nullptr;
+    int32_t alias_weights = -1, its_weights = -1, rs_weights = -1;  // TRW_WEIGHTS_* the table was built from
+    unsigned int *counters = nullptr;                                 // [0] queue head [1] alive [2] zero-sum count
+    trw::WalkerState *state = nullptr;
+    int64_t state_cap = 0;
+    double last_build_ms = 0.0;
+
+    trw::DevGraph view() const {
+        trw::DevGraph g;
+        g.nv = nv;
+        g.ne = ne;
+        g.offset = offset;
+        g.adj = adj;
+        g.degree = degree;
+        g.weight = weight;
+        g.label = label;
+        g.alias = alias;
+        g.prefix = prefix;
+        g.rs_max = rs_max;
+        g.lbl_sorted = lbl_sorted;
+        g.lbl_pos = lbl_pos;
+        g.vrec = vrec;
+        g.erec = erec;
+        g.arec = arec;
+        return g;
+    }
+};
+
+
+namespace trw {
+
+// Records an error message for trw_last_error() and returns code.
+int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int set_hip_error(hipError_t e, const char *what);
+
+// trw_walk_async with an explicit work-queue head (one per concurrent QUEUE-form
+// launch; null = the graph's own) and control over building missing tables.
+int walk_launch(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, hipStream_t stream,
+                unsigned int *queue_head, bool allow_table_build);
+
+}  // namespace trw

@@ -58,10 +58,8 @@ __device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_
 // A step's move draws are words .x/.y of its trial's block; the PPR stop draw is
 // word .w of the step's trial-0 block.  Callers hand the trial-0 block in, so it
 // can be computed a step ahead, while the previous step's gather is in flight.
-__device__ __forceinline__ uint4 philox_block(const BatchIO &io, int32_t w, uint32_t step, uint32_t trial) {
-    const uint64_t gid = io.gid_base + static_cast<uint64_t>(w);
-    return philox4x32_10(static_cast<uint32_t>(gid), static_cast<uint32_t>(gid >> 32), step, trial, io.key0,
-                         io.key1);
+__device__ __forceinline__ uint4 philox_block(const BatchIO &io, const WalkerState &s, uint32_t step, uint32_t trial) {
+    return philox4x32_10(s.gid_lo, s.gid_hi, step, trial, io.key0, io.key1);
 }
 
 __device__ __forceinline__ bool replay_take(const BatchIO &io, int32_t w, WalkerState &s, uint32_t &u) {
@@ -92,7 +90,7 @@ __device__ __forceinline__ bool draw_pair(const BatchIO &io, int32_t w, WalkerSt
         a = blk.x;
         b = blk.y;
     } else {
-        const uint4 x = philox_block(io, w, step, trial);
+        const uint4 x = philox_block(io, s, step, trial);
         a = x.x;
         b = x.y;
     }
@@ -272,13 +270,13 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         uint32_t u;
         if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
         next = read_neighbor(g, off + (u % ud), noff, ndeg);
-        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
+        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
     } else if (MODE == kModeAlias) {
         // alias_sampling.h:147-172: column u0 % d, then real2(u1) <= prob ? first : second.
         uint32_t u0, u1;
         if (!draw_pair<REPLAY>(io, w, s, step, 0u, blk, u0, u1)) return false;
         const bool ok = read_alias(g, off + (u0 % ud), u1, next, noff, ndeg);
-        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
+        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
         if (!ok) {
             if (REPLAY) s.k -= 2;  // the vertex's weights sum to zero: terminate without consuming draws
             s.st = kGatherTerminated;
@@ -294,7 +292,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         }
         uint32_t u;
         if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
-        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
+        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
         // The count is < d whenever the prefix sums are non-decreasing (non-negative
         // weights); the clamp only keeps negative-weight inputs inside the segment.
         const int32_t pos = min(count_le(P_, d, real2(u) * total), d - 1);
@@ -320,7 +318,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             if (!(r > wt)) break;
         }
         next = read_neighbor(g, off + pos, noff, ndeg);
-        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
+        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
     } else if (MODE == kModeNode2Vec) {
         // rejection_sampling.h:368-402 with Node2Vec::weight (node2vec.cpp:24-38):
         // accept iff real2(ub) * max_ <= weight(candidate).  The membership search
@@ -351,7 +349,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             if (accept) break;
         }
         next = c;
-        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
+        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
     } else {  // kModeMetaPath
         // dynamic_gather (creeper.h:178-296) with MetaPath::weight (metapath.cpp:22-30)
         // then dynamic_its_move: the prefix sums are match counts, so the pick is
@@ -369,7 +367,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
         const double t = real2(u) * static_cast<double>(K);
         next = read_neighbor(g, off + g.lbl_pos[off + a + static_cast<int32_t>(t)], noff, ndeg);
-        if (!REPLAY) nblk = philox_block(io, w, step + 1, 0u);
+        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
     }
 
     finish_move(g, P, s, cur, off, d, next, noff, ndeg);
@@ -404,7 +402,7 @@ __device__ __forceinline__ bool advance(const DevGraph &g, const KindParams &P, 
 template <bool REPLAY>
 __device__ __forceinline__ uint4 first_block(const BatchIO &io, int32_t w, const WalkerState &s) {
     if (REPLAY) return make_uint4(0u, 0u, 0u, 0u);
-    return philox_block(io, w, static_cast<uint32_t>(s.len - 1), 0u);
+    return philox_block(io, s, static_cast<uint32_t>(s.len - 1), 0u);
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
@@ -444,11 +442,14 @@ __device__ __forceinline__ void write_outputs(const BatchIO &io, int32_t w, cons
 }
 
 __device__ __forceinline__ WalkerState empty_walker() {
-    return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange};
+    return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange, 0u, 0u};
 }
 
 __device__ __forceinline__ void init_walker(const DevGraph &g, const BatchIO &io, int32_t w, WalkerState &s) {
     s = empty_walker();
+    const uint64_t gid = io.walker_ids ? io.walker_ids[w] : io.gid_base + static_cast<uint64_t>(w);
+    s.gid_lo = static_cast<uint32_t>(gid);
+    s.gid_hi = static_cast<uint32_t>(gid >> 32);
     s.cur = io.sources[w];
     s.st = (s.cur >= 0 && s.cur < g.nv) ? kAlive : kBadSource;
 }
@@ -605,7 +606,7 @@ __global__ __launch_bounds__(kFusedBlock) void walk_fused_multi_kernel(DevGraph 
         if (!REPLAY) {
 #pragma unroll
             for (int j = 0; j < WPT; ++j)
-                if (go[j]) blk[j] = philox_block(io, wbase + j * kFusedBlock + tid, static_cast<uint32_t>(s[j].len), 0u);
+                if (go[j]) blk[j] = philox_block(io, s[j], static_cast<uint32_t>(s[j].len), 0u);
         }
         int32_t next[WPT];
 #pragma unroll

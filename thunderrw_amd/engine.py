@@ -176,7 +176,7 @@ class DeviceGraph:
 
 
 def make_params(app, execution=None, sampling=None, length=None, stop_probability=0.2, p=2.0, q=0.5, schema=None,
-                rng="philox", seed=0, form="auto", max_trials=0, walkers_per_thread=0):
+                rng="philox", seed=0, form="auto", max_trials=0, walkers_per_thread=0, weights="app"):
     """g_para + app functor arguments.  Unspecified fields take the app's reference defaults.
 
     p and q are Node2Vec's command-line values; the functor's weights are 1/p and 1/q
@@ -202,12 +202,13 @@ def make_params(app, execution=None, sampling=None, length=None, stop_probabilit
     prm.form = _enum(_FORM, form)
     prm.max_trials = int(max_trials)
     prm.walkers_per_thread = int(walkers_per_thread)
+    prm.weights = {"app": A.WALK_WEIGHTS_APP, "edge": A.WALK_WEIGHTS_EDGE, "unit": A.WALK_WEIGHTS_UNIT}[weights]
     return prm
 
 
 def make_io(sources, n_walkers, path_stride=0, walker_id_base=0, draws=None, draws_per_walker=0, term_draws=None,
             term_draws_per_walker=0, paths=None, lengths=None, ends=None, status=None, draws_used=None,
-            counters=None):
+            counters=None, walker_ids=None):
     io = A.WalkIO()
     io.sources = _ptr(sources)
     io.n_walkers = int(n_walkers)
@@ -219,6 +220,7 @@ def make_io(sources, n_walkers, path_stride=0, walker_id_base=0, draws=None, dra
     io.term_draws_per_walker = int(term_draws_per_walker)
     io.paths, io.lengths, io.ends = _ptr(paths), _ptr(lengths), _ptr(ends)
     io.status, io.draws_used, io.counters = _ptr(status), _ptr(draws_used), _ptr(counters)
+    io.walker_ids = _ptr(walker_ids)
     return io
 
 
@@ -323,11 +325,12 @@ class DeviceTimer:
 
 
 def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, walker_id_base=0, stream=None,
-         mix=None):
+         mix=None, walker_ids=None):
     """Runs one batch in HBM buffers and returns numpy results.
 
     draws / term_draws: replay streams in the [k][walker] layout, or None (Philox).
     mix: optional (params_b, split) for a two-kind batch.
+    walker_ids: optional per-walker global ids keying the Philox stream (uint64).
     """
     ctx = graph.ctx
     src_h = np.ascontiguousarray(sources, dtype=np.int32)
@@ -353,8 +356,9 @@ def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, w
         status = up(np.full(m, -9, np.int32))
         used = up(np.zeros(m, np.int32))
         counters = up(np.zeros(4, np.uint64))
+        ids = None if walker_ids is None else up(np.ascontiguousarray(walker_ids, dtype=np.uint64))
         io = make_io(src, n, L, walker_id_base, d_draws, K, d_tdraws, KT, paths, lengths, ends, status, used,
-                     counters)
+                     counters, walker_ids=ids)
         if mix is None:
             walk_async(graph, params, io, stream)
         else:

@@ -241,6 +241,111 @@ int trw_host_graph_arrays(const trw_host_graph *h, int32_t *num_vertices, int64_
                           const int32_t **edge_label);
 int trw_host_graph_free(trw_host_graph *h);
 
+/* ---- serving: continuous batching of walk queries ----
+ * No counterpart in the reference, whose compute<F> (creeper.h:689-831) is one
+ * blocking batch over a fixed walker vector.  A server keeps one uploaded graph
+ * (tables built beforehand with trw_build_tables) resident and runs a dispatcher
+ * thread: queries that arrive while others are on the device are packed, in arrival
+ * order, into batches of walkers; each batch runs on one of `streams` slots (own HIP
+ * stream, device buffers and pinned staging), so batches overlap on the device.  A
+ * batch holds at most max_batch_walkers walkers and path_bytes_per_batch bytes of
+ * path rows (walkers x length x 4 B); a query larger than that is split over several
+ * batches.  Admission defers a query until a slot frees up, and submit rejects one
+ * with TRW_E_BUSY when max_pending_queries are already waiting.
+ *
+ * Walker i of a query starts at sources[i % n_sources] and draws from the Philox
+ * stream of (params.seed, walker_id_base + i), so a query's result does not depend
+ * on what else is in flight: alone or in a mix, it is the same. */
+#define TRW_E_BUSY (-7)    /* admission: the pending queue is full */
+#define TRW_E_TIMEOUT (-8) /* trw_server_wait: not finished within the timeout */
+
+#define TRW_RESULT_PPR 0   /* end-vertex counts (PPR estimate = count / n_walkers) */
+#define TRW_RESULT_PATHS 1 /* full paths, n_walkers x max(length, 1) */
+
+#define TRW_QUERY_PENDING 0
+#define TRW_QUERY_RUNNING 1
+#define TRW_QUERY_DONE 2
+#define TRW_QUERY_FAILED 3
+
+typedef struct trw_server trw_server;
+
+typedef struct {
+    int32_t streams;              /* slots (batches in flight), 1..16; 0 = 4 */
+    int32_t max_batch_walkers;    /* walker capacity of one batch; 0 = 1 << 20 */
+    int64_t path_bytes_per_batch; /* path-row capacity of one batch; 0 = 64 MiB */
+    int32_t max_pending_queries;  /* queries waiting for admission before submit says BUSY; 0 = 1 << 16 */
+    int32_t completion_threads;   /* host threads that copy results out and build PPR counts; 0 = 4 */
+    int32_t max_queries_per_batch;/* 0 = 1024 */
+    int32_t reserved;
+} trw_server_config;
+
+typedef struct {
+    const int32_t *sources;   /* host array, copied at submit */
+    int32_t n_sources;
+    int32_t n_walkers;        /* walker i starts at sources[i % n_sources] */
+    uint64_t walker_id_base;  /* Philox id of walker 0; give queries disjoint id ranges */
+    trw_walk_params params;   /* rng must be TRW_RNG_PHILOX; form AUTO, FUSED or QUEUE */
+    int32_t result;           /* TRW_RESULT_* */
+    int32_t reserved;
+} trw_query;
+
+typedef struct {
+    int32_t state;            /* TRW_QUERY_* */
+    int32_t n_walkers;
+    uint64_t steps;           /* sum over walkers of (length - 1) */
+    uint64_t errors;          /* walkers that ended with status >= TRW_WALKER_REPLAY_EXHAUSTED */
+    int64_t ppr_entries;      /* distinct end vertices (TRW_RESULT_PPR) */
+    double queue_ms;          /* arrival -> first walkers launched */
+    double latency_ms;        /* arrival -> result ready */
+} trw_query_info;
+
+typedef struct {
+    uint64_t queries_done;
+    uint64_t walkers;
+    uint64_t steps;
+    uint64_t batches;
+    uint64_t rejected;        /* submits refused with TRW_E_BUSY */
+    double mean_batch_walkers;
+    double mean_batch_queries;
+} trw_server_stats;
+
+typedef struct {
+    double wall_ms;           /* first scheduled arrival -> last result */
+    double offered_qps;
+    double achieved_qps;
+    uint64_t steps;
+    double steps_per_s;
+    double p50_ms, p90_ms, p99_ms, p999_ms, mean_ms, max_ms;
+    uint64_t batches;
+    double mean_batch_queries;
+    uint64_t rejected;
+} trw_open_loop_report;
+
+int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server **out);
+/* Stops the dispatcher after the queries in flight finish and frees every result. */
+int trw_server_destroy(trw_server *s);
+/* Non-blocking; the query's arrival time is the time of the call. */
+int trw_server_submit(trw_server *s, const trw_query *q, uint64_t *query_id);
+/* Blocks until the query is done or failed (timeout_ms < 0: no limit). */
+int trw_server_wait(trw_server *s, uint64_t query_id, double timeout_ms, trw_query_info *info);
+int trw_server_query_info(trw_server *s, uint64_t query_id, trw_query_info *info);
+/* TRW_RESULT_PPR: the distinct end vertices ascending and their walker counts
+ * (capacity >= info.ppr_entries). */
+int trw_server_ppr_result(trw_server *s, uint64_t query_id, int32_t *vertices, uint32_t *counts, int64_t capacity);
+/* Per-walker outputs of a done query (nullable): paths (TRW_RESULT_PATHS only,
+ * n_walkers x max(length,1), -1 padded), lengths, ends, status. */
+int trw_server_walk_result(trw_server *s, uint64_t query_id, int32_t *paths, int32_t *lengths, int32_t *ends,
+                           int32_t *status);
+int trw_server_release(trw_server *s, uint64_t query_id);
+int trw_server_stats_get(trw_server *s, trw_server_stats *out);
+/* Open-loop load: submits queries[0..n) at Poisson arrival times of rate qps (seeded),
+ * waits for all, and reports latency percentiles measured from each scheduled arrival
+ * (a submit that runs late still counts from its schedule).  ids and latency_ms
+ * (nullable, n entries) receive each query's id and latency; results stay held until
+ * trw_server_release. */
+int trw_server_run_open_loop(trw_server *s, const trw_query *queries, int32_t n, double qps, uint64_t seed,
+                             uint64_t *ids, double *latency_ms, trw_open_loop_report *report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,39 +32,39 @@ def test_library_exports_every_declared_function():
     assert sorted(A.exported_names()) == names
 
 
+STRUCTS = {"trw_walk_params": A.WalkParams, "trw_walk_io": A.WalkIO, "trw_alias_slot": A.AliasSlot,
+           "trw_server_config": A.ServerConfig, "trw_query": A.Query, "trw_query_info": A.QueryInfo,
+           "trw_server_stats": A.ServerStats, "trw_open_loop_report": A.OpenLoopReport}
+
+
 def test_struct_layouts_match_the_header():
-    src = r"""
-#include <stddef.h>
-#include <stdio.h>
-#include "thunderrw.h"
-#define O(t, f) printf("%s.%s %zu\n", #t, #f, offsetof(t, f))
-int main(void) {
-  printf("params %zu io %zu slot %zu\n", sizeof(trw_walk_params), sizeof(trw_walk_io), sizeof(trw_alias_slot));
-  O(trw_walk_params, stop_probability); O(trw_walk_params, schema); O(trw_walk_params, schema_length);
-  O(trw_walk_params, rng); O(trw_walk_params, seed); O(trw_walk_params, form); O(trw_walk_params, max_trials);
-  O(trw_walk_io, n_walkers); O(trw_walk_io, walker_id_base); O(trw_walk_io, draws_per_walker);
-  O(trw_walk_io, term_draws); O(trw_walk_io, paths); O(trw_walk_io, counters);
-  return 0;
-}
-"""
+    """Every field offset and struct size of the ctypes mirror equals the C compiler's."""
+    lines = []
+    for cname, cls in STRUCTS.items():
+        lines.append('  printf("%s.sizeof %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in cls._fields_:
+            lines.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    src = "#include <stddef.h>\n#include <stdio.h>\n#include \"thunderrw.h\"\nint main(void) {\n%s\n  return 0;\n}\n" % (
+        "\n".join(lines))
     with tempfile.TemporaryDirectory() as tmp:
         c = os.path.join(tmp, "layout.c")
         open(c, "w").write(src)
         exe = os.path.join(tmp, "layout")
         subprocess.run(["gcc", "-I", os.path.dirname(HEADER), "-o", exe, c], check=True)
         out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
-    sizes = out[0].split()
     import ctypes
-    assert int(sizes[1]) == ctypes.sizeof(A.WalkParams)
-    assert int(sizes[3]) == ctypes.sizeof(A.WalkIO)
-    assert int(sizes[5]) == ctypes.sizeof(A.AliasSlot) == 16
-    for line in out[1:]:
+    seen = 0
+    for line in out:
         if not line:
             continue
-        name, off = line.split()
+        name, val = line.split()
         t, f = name.split(".")
-        cls = A.WalkParams if t == "trw_walk_params" else A.WalkIO
-        assert getattr(cls, f).offset == int(off), name
+        cls = STRUCTS[t]
+        want = ctypes.sizeof(cls) if f == "sizeof" else getattr(cls, f).offset
+        assert want == int(val), name
+        seen += 1
+    assert seen == sum(len(c._fields_) + 1 for c in STRUCTS.values())
+    assert ctypes.sizeof(A.AliasSlot) == 16
 
 
 def test_replay_fill_is_the_chunked_sfmt_stream():

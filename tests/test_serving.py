@@ -1,0 +1,157 @@
+"""Query server (trw_server_*): continuous batching over a resident graph.
+
+Results are keyed by (seed, walker id), so each query's outcome is exactly the
+same served alone, served in a mix of queries of other sizes and kinds, split
+over several batches, or walked directly with trw_walk_async; PPR estimates agree
+with the oracle's FP64 power iteration within the sampling bound."""
+
+import numpy as np
+import pytest
+
+import graphs
+import oracle_bind as ob
+import thunderrw_amd as T
+from thunderrw_amd import _abi as A
+
+pytestmark = pytest.mark.gpu
+
+PPR = dict(execution="static", sampling="alias", weights="edge", stop_probability=0.2, rng="philox", seed=5)
+
+
+@pytest.fixture(scope="module")
+def served(gpu_ctx):
+    g = graphs.chung_lu(40000, 240000, 2.1, 2, 4000, 81)
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight)
+    dg.build_tables("alias", "edge")
+    yield g, dg
+    dg.close()
+
+
+def make_queries(g, n, seed):
+    rng = np.random.default_rng(seed)
+    live = np.nonzero(g.degree > 0)[0]
+    out = []
+    for i in range(n):
+        k = int(np.exp(rng.uniform(0, np.log(64))))
+        srcs = rng.choice(live, k).astype(np.int32)
+        walkers = int(np.exp(rng.uniform(np.log(500), np.log(30000))))
+        kind = "paths" if i % 5 == 3 else "ppr"
+        out.append((kind, srcs, walkers, (i + 1) << 32))
+    return out
+
+
+def params_for(kind):
+    if kind == "ppr":
+        return T.make_params("ppr", **PPR)
+    return T.make_params("deepwalk", execution="static", sampling="alias", length=24, rng="philox", seed=5)
+
+
+def collect(srv, qid, kind):
+    info = srv.wait(qid)
+    assert info["state"] == A.QUERY_DONE and info["errors"] == 0
+    if kind == "ppr":
+        v, c = srv.ppr(qid)
+        res = dict(v=v, c=c, steps=info["steps"])
+    else:
+        res = srv.walk_result(qid, length=24)
+        res["steps"] = info["steps"]
+    srv.release(qid)
+    return res
+
+
+def same(a, b):
+    return all(np.array_equal(a[k], b[k]) if isinstance(a[k], np.ndarray) else a[k] == b[k] for k in a if a[k] is not None)
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix(served):
+    g, dg = served
+    qs = make_queries(g, 30, 1)
+    alone = []
+    with T.Server(dg, streams=1) as srv:
+        for kind, srcs, n, base in qs:
+            qid = srv.submit(params_for(kind), srcs, n, base, result=kind)
+            alone.append(collect(srv, qid, kind))
+    # Small batches split the large queries; four slots overlap batches on the device.
+    with T.Server(dg, streams=4, max_batch_walkers=16384, path_bytes_per_batch=16384 * 24 * 4 // 2,
+                  max_queries_per_batch=8) as srv:
+        ids = [srv.submit(params_for(kind), srcs, n, base, result=kind) for kind, srcs, n, base in qs]
+        mixed = [collect(srv, qid, q[0]) for qid, q in zip(ids, qs)]
+        st = srv.stats()
+    total = sum(n for _, _, n, _ in qs)
+    assert max(n for _, _, n, _ in qs) > 16384  # at least one query was split
+    assert st["batches"] >= total / 16384 and st["mean_batch_queries"] > 1.5
+    for i, (a, b) in enumerate(zip(alone, mixed)):
+        assert same(a, b), i
+    # A direct walk of the same walkers (sources[i % |S|], ids base + i) gives the same result.
+    for (kind, srcs, n, base), a in zip(qs[:6], alone[:6]):
+        src = srcs[np.arange(n) % len(srcs)]
+        r = T.walk(dg, params_for(kind), src, walker_ids=base + np.arange(n, dtype=np.uint64),
+                   want_paths=kind != "ppr")
+        if kind == "ppr":
+            v, c = np.unique(r["ends"], return_counts=True)
+            assert np.array_equal(v, a["v"]) and np.array_equal(c, a["c"])
+        else:
+            assert np.array_equal(r["paths"], a["paths"])
+        assert r["steps"] == a["steps"]
+
+
+def test_served_ppr_matches_power_iteration(served):
+    g, dg = served
+    live = np.nonzero(g.degree > 0)[0]
+    sources = live[[3, 50, 700, 9000, 20000]].astype(np.int32)
+    n = 1 << 21
+    with T.Server(dg, streams=2, max_batch_walkers=1 << 19) as srv:
+        qid = srv.submit(params_for("ppr"), sources, n, 1 << 40)
+        info = srv.wait(qid)
+        v, c = srv.ppr(qid)
+    assert c.sum() == n
+    est = np.zeros(g.nv)
+    est[v] = c / n
+    pi, _ = ob.ppr_power(g, 0.2, sources, weighted=True)
+    bound = np.sum(np.sqrt(pi * (1 - pi) / n))
+    assert np.abs(est - pi).sum() < 1.5 * bound
+    assert abs(info["steps"] / n - 1 / 0.2) < 0.05
+
+
+def test_admission_defers_and_rejects(served):
+    g, dg = served
+    live = np.nonzero(g.degree > 0)[0].astype(np.int32)
+    with T.Server(dg, streams=1, max_batch_walkers=4096, max_pending_queries=4) as srv:
+        ids, busy = [], 0
+        for i in range(64):
+            try:
+                ids.append(srv.submit(params_for("ppr"), live[i:i + 3], 200000, (i + 1) << 32))
+            except T.TrwError as e:
+                assert e.code == A.E_BUSY
+                busy += 1
+        assert busy > 0 and len(ids) >= 4
+        for qid in ids:
+            assert srv.wait(qid)["state"] == A.QUERY_DONE
+        assert srv.stats()["rejected"] == busy
+    with T.Server(dg, streams=1, path_bytes_per_batch=64) as srv:
+        with pytest.raises(T.TrwError):  # one 24-vertex path row does not fit 64 bytes
+            srv.submit(params_for("paths"), live[:2], 10, result="paths")
+        with pytest.raises(T.TrwError):  # replay streams cannot be served
+            srv.submit(T.make_params("ppr", rng="replay"), live[:2], 10)
+        with pytest.raises(T.TrwError):  # sources are checked at submit
+            srv.submit(params_for("ppr"), np.array([g.nv], np.int32), 10)
+
+
+def test_open_loop_reports_latency_percentiles(served):
+    g, dg = served
+    qs = make_queries(g, 300, 2)
+    with T.Server(dg, streams=4) as srv:
+        structs = [srv.query(params_for(kind), srcs, n, base, result=kind) for kind, srcs, n, base in qs]
+        rep, ids, lat = srv.run_open_loop(structs, qps=3000.0, seed=7)
+        assert rep["rejected"] == 0
+        assert np.all(lat > 0)
+        assert rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
+        assert abs(rep["p99_ms"] - np.quantile(lat, 0.99, method="inverted_cdf")) < 1e-9
+        total = 0
+        for qid in ids:
+            total += srv.info(int(qid))["steps"]
+            srv.release(int(qid))
+        assert rep["steps"] == total > 0
+        print("open loop: %.0f qps offered, %.0f achieved, p50 %.3f ms, p99 %.3f ms, %.3g steps/s, "
+              "%.1f queries per batch" % (rep["offered_qps"], rep["achieved_qps"], rep["p50_ms"], rep["p99_ms"],
+                                          rep["steps_per_s"], rep["mean_batch_queries"]))

@@ -6,7 +6,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libthunderrw.so")
 
 OK = 0
-E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_STATE, E_IO = -1, -2, -3, -4, -5, -6
+E_INVALID, E_HIP, E_NOMEM, E_UNSUPPORTED, E_STATE, E_IO, E_BUSY, E_TIMEOUT = -1, -2, -3, -4, -5, -6, -7, -8
 
 EXEC_UNIFORM, EXEC_STATIC, EXEC_DYNAMIC = 0, 1, 2
 SM_UNIFORM, SM_ITS, SM_ALIAS, SM_REJECTION, SM_MAX_WEIGHT_REJECTION = 0, 1, 2, 3, 4
@@ -15,6 +15,8 @@ RNG_REPLAY, RNG_PHILOX = 0, 1
 FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP = 0, 1, 2, 3
 WEIGHTS_EDGE, WEIGHTS_UNIT = 0, 1
 WALK_WEIGHTS_APP, WALK_WEIGHTS_EDGE, WALK_WEIGHTS_UNIT = 0, 1, 2
+RESULT_PPR, RESULT_PATHS = 0, 1
+QUERY_PENDING, QUERY_RUNNING, QUERY_DONE, QUERY_FAILED = 0, 1, 2, 3
 WALKER_DONE, WALKER_GATHER_TERMINATED, WALKER_DEAD_END, WALKER_REPLAY_EXHAUSTED, WALKER_TRIAL_CAP, \
     WALKER_BAD_SOURCE = 0, 1, 2, 3, 4, 5
 
@@ -46,6 +48,36 @@ class WalkIO(C.Structure):
                 ("term_draws", C.c_void_p), ("term_draws_per_walker", C.c_int64), ("paths", C.c_void_p),
                 ("lengths", C.c_void_p), ("ends", C.c_void_p), ("status", C.c_void_p), ("draws_used", C.c_void_p),
                 ("counters", C.c_void_p), ("walker_ids", C.c_void_p)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("streams", C.c_int32), ("max_batch_walkers", C.c_int32), ("path_bytes_per_batch", C.c_int64),
+                ("max_pending_queries", C.c_int32), ("completion_threads", C.c_int32),
+                ("max_queries_per_batch", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Query(C.Structure):
+    _fields_ = [("sources", C.c_void_p), ("n_sources", C.c_int32), ("n_walkers", C.c_int32),
+                ("walker_id_base", C.c_uint64), ("params", WalkParams), ("result", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class QueryInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("n_walkers", C.c_int32), ("steps", C.c_uint64), ("errors", C.c_uint64),
+                ("ppr_entries", C.c_int64), ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queries_done", C.c_uint64), ("walkers", C.c_uint64), ("steps", C.c_uint64),
+                ("batches", C.c_uint64), ("rejected", C.c_uint64), ("mean_batch_walkers", C.c_double),
+                ("mean_batch_queries", C.c_double)]
+
+
+class OpenLoopReport(C.Structure):
+    _fields_ = [("wall_ms", C.c_double), ("offered_qps", C.c_double), ("achieved_qps", C.c_double),
+                ("steps", C.c_uint64), ("steps_per_s", C.c_double), ("p50_ms", C.c_double), ("p90_ms", C.c_double),
+                ("p99_ms", C.c_double), ("p999_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("batches", C.c_uint64), ("mean_batch_queries", C.c_double), ("rejected", C.c_uint64)]
 
 
 _SIGS = {
@@ -89,6 +121,17 @@ _SIGS = {
     "trw_host_graph_arrays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]),
     "trw_host_graph_free": (C.c_int, [C.c_void_p]),
+    "trw_server_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "trw_server_destroy": (C.c_int, [C.c_void_p]),
+    "trw_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "trw_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.c_void_p]),
+    "trw_server_query_info": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "trw_server_ppr_result": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int64]),
+    "trw_server_walk_result": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "trw_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "trw_server_stats_get": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_server_run_open_loop": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_uint64, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]),
 }
 
 _LIB = None

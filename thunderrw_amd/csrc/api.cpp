@@ -640,6 +640,20 @@ int trw::walk_launch(trw_graph *g, const trw_walk_params *params, const trw_walk
     }
 }
 
+int trw::walk_check(trw_graph *g, const trw_walk_params *params) {
+    if (!g || !params) return fail(TRW_E_INVALID, "null argument");
+    KindParams k;
+    int rc = resolve_kind(g, params, k, false);
+    if (rc != TRW_OK) return rc;
+    if (params->rng != TRW_RNG_REPLAY && params->rng != TRW_RNG_PHILOX)
+        return fail(TRW_E_INVALID, "unknown rng %d", params->rng);
+    if (params->form < TRW_FORM_AUTO || params->form > TRW_FORM_STEP)
+        return fail(TRW_E_INVALID, "unknown form %d", params->form);
+    if (params->form == TRW_FORM_FUSED && k.ppr)
+        return fail(TRW_E_UNSUPPORTED, "the fused form runs fixed-length walks; use QUEUE or STEP for PPR");
+    return TRW_OK;
+}
+
 int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream) {
     return trw::walk_launch(g, params, io, reinterpret_cast<hipStream_t>(stream), nullptr, true);
 }

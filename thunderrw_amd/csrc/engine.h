@@ -120,6 +120,21 @@ hipError_t launch_walk_step(const DevGraph &g, const KindParams &a, const KindPa
 hipError_t launch_walk_step_finish(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                                    const WalkerState *state, hipStream_t stream);
 hipError_t launch_fill_u32(unsigned int *p, unsigned int value, int64_t n, hipStream_t stream);
+
+// One query chunk of a serving batch: walkers [batch_offset, batch_offset + count)
+// of the batch are walkers [first_walker, first_walker + count) of the query, whose
+// source set is pool[pool_offset, pool_offset + n_sources).
+struct ServeSegment {
+    int32_t batch_offset;
+    int32_t count;
+    int32_t pool_offset;
+    int32_t n_sources;
+    int32_t first_walker;
+    int32_t pad;
+    uint64_t id_base;
+};
+hipError_t launch_serve_expand(const ServeSegment *segs, int32_t n_segs, const int32_t *pool, int32_t n,
+                               int32_t *sources, uint64_t *ids, hipStream_t stream);
 hipError_t launch_copy_u128(uint4 *dst, const uint4 *src, int64_t n, hipStream_t stream);
 
 hipError_t run_gather_probe(uint64_t table_bytes, int32_t chains, int32_t steps, int32_t cus, hipStream_t stream,

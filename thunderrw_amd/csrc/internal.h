@@ -70,4 +70,8 @@ int set_hip_error(hipError_t e, const char *what);
 int walk_launch(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, hipStream_t stream,
                 unsigned int *queue_head, bool allow_table_build);
 
+// Validates a walk's parameters against the graph without launching or building
+// anything (the tables it reads must already be built).
+int walk_check(trw_graph *g, const trw_walk_params *params);
+
 }  // namespace trw

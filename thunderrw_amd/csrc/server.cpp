@@ -1,0 +1,820 @@
+// Query server: continuous batching of walk queries over one resident graph.
+//
+// The reference has no serving path; its compute<F> (random_walk/creeper.h:689-831)
+// runs one blocking batch.  Here a dispatcher thread owns the device side:
+//   * submit() appends a query (copied) to a FIFO of pending queries;
+//   * whenever a slot is free, the dispatcher packs pending queries that share the
+//     head query's walk parameters, in arrival order, into one batch (up to the
+//     slot's walker, path-byte, query and source-pool capacities; a query that does
+//     not fit whole is split and its remainder stays at the head);
+//   * a batch crosses PCIe as a segment table plus the queries' source sets, is
+//     expanded on the device into per-walker start vertices and Philox ids, walks
+//     on the slot's own stream, and its per-walker outputs come back into the
+//     slot's pinned staging;
+//   * the dispatcher polls the slots' completion events; a finished slot is handed
+//     to a completion thread, which copies each segment into its query, builds the
+//     PPR end-vertex counts of queries that are complete, and frees the slot.
+// Batches on different slots overlap on the device.  Each walker's random stream is
+// keyed by (seed, walker_id_base + index in its query), so a result does not depend
+// on how queries were batched.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.h"
+#include "internal.h"
+
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+constexpr int32_t kPoolInts = 1 << 22;  // source-set pool of one batch (16 MiB)
+
+struct Query {
+    uint64_t id = 0;
+    trw_walk_params params{};
+    int32_t result = TRW_RESULT_PPR;
+    int32_t path_len = 1;
+    int32_t n_walkers = 0;
+    uint64_t id_base = 0;
+    std::vector<int32_t> sources;
+    // Owned by the dispatcher.
+    int32_t next_walker = 0;
+    // Completion threads write disjoint walker ranges; the one that completes the
+    // query builds its counts.
+    std::atomic<int32_t> done_walkers{0};
+    std::atomic<uint64_t> steps{0}, errors{0};
+    std::vector<int32_t> ends, lengths, status, paths;
+    std::vector<int32_t> ppr_vertices;
+    std::vector<uint32_t> ppr_counts;
+    // Guarded by the server mutex.
+    Clock::time_point arrival, dispatched, finished;
+    bool was_dispatched = false;
+    int state = TRW_QUERY_PENDING;
+    int error_code = TRW_OK;
+    std::string error;
+};
+
+bool same_walk(const Query &a, const Query &b) {
+    const trw_walk_params &x = a.params, &y = b.params;
+    if (a.result != b.result) return false;
+    if (x.app != y.app || x.execution != y.execution || x.sampling != y.sampling || x.length != y.length ||
+        x.rng != y.rng || x.seed != y.seed || x.form != y.form || x.max_trials != y.max_trials ||
+        x.walkers_per_thread != y.walkers_per_thread || x.weights != y.weights ||
+        x.schema_length != y.schema_length)
+        return false;
+    if (std::memcmp(&x.stop_probability, &y.stop_probability, sizeof(double)) != 0 ||
+        std::memcmp(&x.p, &y.p, sizeof(double)) != 0 || std::memcmp(&x.q, &y.q, sizeof(double)) != 0)
+        return false;
+    return std::equal(x.schema, x.schema + std::max(0, std::min(x.schema_length, 16)), y.schema);
+}
+
+// LSD radix sort of vertex ids in 11-bit digits.
+void radix_sort(std::vector<uint32_t> &a, std::vector<uint32_t> &tmp, int bits) {
+    tmp.resize(a.size());
+    for (int shift = 0; shift < bits; shift += 11) {
+        uint32_t count[2048] = {0};
+        for (uint32_t x : a) ++count[(x >> shift) & 2047u];
+        uint32_t sum = 0;
+        for (uint32_t &c : count) {
+            const uint32_t t = c;
+            c = sum;
+            sum += t;
+        }
+        for (uint32_t x : a) tmp[count[(x >> shift) & 2047u]++] = x;
+        a.swap(tmp);
+    }
+}
+
+struct Segment {
+    std::shared_ptr<Query> q;
+    int32_t first = 0;
+    int32_t count = 0;
+    int32_t batch_offset = 0;
+};
+
+enum class SlotState { Free, InFlight, Draining };
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    trw::ServeSegment *d_segs = nullptr, *h_segs = nullptr;
+    int32_t *d_pool = nullptr, *h_pool = nullptr;
+    int32_t *d_sources = nullptr;
+    uint64_t *d_ids = nullptr;
+    int32_t *d_ends = nullptr, *d_lengths = nullptr, *d_status = nullptr, *d_paths = nullptr;
+    uint64_t *d_counters = nullptr;
+    unsigned int *d_queue_head = nullptr;
+    int32_t *h_ends = nullptr, *h_lengths = nullptr, *h_status = nullptr, *h_paths = nullptr;
+    std::vector<Segment> segs;
+    int32_t n = 0;
+    int32_t path_stride = 0;
+    int32_t pool_used = 0;
+    SlotState state = SlotState::Free;
+};
+
+class WorkerPool {
+  public:
+    explicit WorkerPool(int n) {
+        for (int i = 0; i < n; ++i) threads_.emplace_back([this] { run(); });
+    }
+    ~WorkerPool() { stop(); }
+    void push(std::function<void()> f) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            tasks_.push_back(std::move(f));
+        }
+        cv_.notify_one();
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stopping_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_)
+            if (t.joinable()) t.join();
+        threads_.clear();
+    }
+
+  private:
+    void run() {
+        for (;;) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return stopping_ || !tasks_.empty(); });
+                if (tasks_.empty()) return;
+                f = std::move(tasks_.front());
+                tasks_.pop_front();
+            }
+            f();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> tasks_;
+    std::vector<std::thread> threads_;
+    bool stopping_ = false;
+};
+
+}  // namespace
+
+struct trw_server {
+    trw_graph *g = nullptr;
+    int32_t device = 0;
+    int32_t max_walkers = 0;
+    int64_t path_ints = 0;
+    int32_t max_pending = 0;
+    int32_t max_segments = 0;
+    int vertex_bits = 1;
+    std::vector<Slot> slots;
+
+    std::mutex mu;
+    std::condition_variable cv;       // dispatcher: new work, a slot freed, stop
+    std::condition_variable cv_done;  // waiters: a query finished
+    std::deque<std::shared_ptr<Query>> pending;
+    std::unordered_map<uint64_t, std::shared_ptr<Query>> queries;
+    uint64_t next_id = 1;
+    bool stopping = false;
+    trw_server_stats stats{};
+    uint64_t batch_walkers = 0, batch_queries = 0;
+
+    std::thread dispatcher;
+    std::unique_ptr<WorkerPool> workers;
+
+    int submit(const trw_query *q, Clock::time_point arrival, uint64_t *id_out);
+    void dispatch_loop();
+    bool form_batch(Slot &sl, std::unique_lock<std::mutex> &lk);
+    void launch(Slot &sl);
+    void drain(Slot *sl, hipError_t err, std::string msg);
+    void finalize(const std::shared_ptr<Query> &q);
+    void free_slots();
+};
+
+// ---------------------------------------------------------------- dispatcher
+
+bool trw_server::form_batch(Slot &sl, std::unique_lock<std::mutex> &) {
+    // Called with mu held and a non-empty pending queue.
+    const std::shared_ptr<Query> head = pending.front();
+    const int32_t L = head->result == TRW_RESULT_PATHS ? head->path_len : 0;
+    int64_t room_paths = L > 0 ? path_ints / L : INT64_MAX;
+    int32_t n = 0, pool = 0;
+    sl.segs.clear();
+    const auto now = Clock::now();
+    for (auto it = pending.begin(); it != pending.end();) {
+        const std::shared_ptr<Query> &q = *it;
+        if (q != head && !same_walk(*q, *head)) {
+            ++it;
+            continue;
+        }
+        if (static_cast<int32_t>(sl.segs.size()) == max_segments) break;
+        if (pool + static_cast<int32_t>(q->sources.size()) > kPoolInts) break;
+        const int64_t room = std::min<int64_t>(max_walkers - n, room_paths - n);
+        if (room <= 0) break;
+        const int32_t take = static_cast<int32_t>(std::min<int64_t>(q->n_walkers - q->next_walker, room));
+        Segment s;
+        s.q = q;
+        s.first = q->next_walker;
+        s.count = take;
+        s.batch_offset = n;
+        sl.segs.push_back(s);
+        n += take;
+        pool += static_cast<int32_t>(q->sources.size());
+        q->next_walker += take;
+        if (!q->was_dispatched) {
+            q->was_dispatched = true;
+            q->dispatched = now;
+            q->state = TRW_QUERY_RUNNING;
+        }
+        if (q->next_walker == q->n_walkers) it = pending.erase(it);
+        else break;  // the slot is full; the rest of this query waits at the head
+    }
+    sl.n = n;
+    sl.path_stride = L;
+    sl.pool_used = pool;
+    ++stats.batches;
+    batch_walkers += static_cast<uint64_t>(n);
+    batch_queries += sl.segs.size();
+    return !sl.segs.empty();
+}
+
+void trw_server::launch(Slot &sl) {
+    // Runs without mu: the slot is InFlight and owned by the dispatcher until its
+    // completion event fires.
+    int32_t pool = 0;
+    for (size_t i = 0; i < sl.segs.size(); ++i) {
+        const Segment &s = sl.segs[i];
+        trw::ServeSegment d;
+        d.batch_offset = s.batch_offset;
+        d.count = s.count;
+        d.pool_offset = pool;
+        d.n_sources = static_cast<int32_t>(s.q->sources.size());
+        d.first_walker = s.first;
+        d.pad = 0;
+        d.id_base = s.q->id_base;
+        sl.h_segs[i] = d;
+        std::memcpy(sl.h_pool + pool, s.q->sources.data(), sizeof(int32_t) * s.q->sources.size());
+        pool += d.n_sources;
+    }
+    const trw_walk_params params = sl.segs.front().q->params;
+    const size_t N = static_cast<size_t>(sl.n);
+    hipStream_t st = sl.stream;
+    std::string msg;
+    hipError_t e = hipMemcpyAsync(sl.d_segs, sl.h_segs, sizeof(trw::ServeSegment) * sl.segs.size(),
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(sl.d_pool, sl.h_pool, sizeof(int32_t) * static_cast<size_t>(pool), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = trw::launch_serve_expand(sl.d_segs, static_cast<int32_t>(sl.segs.size()), sl.d_pool, sl.n, sl.d_sources,
+                                     sl.d_ids, st);
+    if (e == hipSuccess) e = trw::launch_fill_u32(reinterpret_cast<unsigned int *>(sl.d_counters), 0u, 8, st);
+    if (e == hipSuccess) {
+        trw_walk_io io;
+        std::memset(&io, 0, sizeof(io));
+        io.sources = sl.d_sources;
+        io.n_walkers = sl.n;
+        io.path_stride = sl.path_stride;
+        io.walker_ids = sl.d_ids;
+        io.paths = sl.path_stride > 0 ? sl.d_paths : nullptr;
+        io.lengths = sl.d_lengths;
+        io.ends = sl.d_ends;
+        io.status = sl.d_status;
+        io.counters = sl.d_counters;
+        const int rc = trw::walk_launch(g, &params, &io, st, sl.d_queue_head, false);
+        if (rc != TRW_OK) {
+            e = hipErrorLaunchFailure;
+            msg = trw_last_error();
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(sl.h_ends, sl.d_ends, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(sl.h_lengths, sl.d_lengths, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(sl.h_status, sl.d_status, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && sl.path_stride > 0)
+        e = hipMemcpyAsync(sl.h_paths, sl.d_paths, sizeof(int32_t) * N * static_cast<size_t>(sl.path_stride),
+                           hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(sl.done, st);
+    if (e != hipSuccess) {
+        // Let whatever was queued finish before the staging buffers are reused.
+        (void)hipStreamSynchronize(st);
+        if (msg.empty()) msg = std::string("serving batch launch: ") + hipGetErrorString(e);
+        std::lock_guard<std::mutex> lk(mu);
+        sl.state = SlotState::Draining;
+        Slot *p = &sl;
+        workers->push([this, p, e, msg] { drain(p, e, msg); });
+    }
+}
+
+void trw_server::dispatch_loop() {
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        bool progressed = false;
+        bool in_flight = false;
+        for (Slot &sl : slots) {
+            if (sl.state != SlotState::InFlight) continue;
+            const hipError_t e = hipEventQuery(sl.done);
+            if (e == hipErrorNotReady) {
+                in_flight = true;
+                continue;
+            }
+            sl.state = SlotState::Draining;
+            Slot *p = &sl;
+            std::string msg = e == hipSuccess ? std::string() : std::string("serving batch: ") + hipGetErrorString(e);
+            workers->push([this, p, e, msg] { drain(p, e, msg); });
+            progressed = true;
+        }
+        while (!pending.empty()) {
+            Slot *free = nullptr;
+            for (Slot &sl : slots)
+                if (sl.state == SlotState::Free) {
+                    free = &sl;
+                    break;
+                }
+            if (!free) break;
+            if (!form_batch(*free, lk)) break;
+            free->state = SlotState::InFlight;
+            in_flight = true;
+            progressed = true;
+            lk.unlock();
+            launch(*free);
+            lk.lock();
+        }
+        if (stopping && pending.empty()) {
+            bool idle = true;
+            for (const Slot &sl : slots) idle = idle && sl.state == SlotState::Free;
+            if (idle) return;
+        }
+        if (progressed) continue;
+        if (in_flight) {
+            // Poll the completion events; yielding keeps the latency of a finished
+            // batch to a few microseconds without a HIP callback thread.
+            lk.unlock();
+            std::this_thread::yield();
+            lk.lock();
+        } else {
+            cv.wait(lk);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- completion
+
+void trw_server::drain(Slot *sl, hipError_t err, std::string msg) {
+    const bool ok = err == hipSuccess && msg.empty();
+    for (const Segment &s : sl->segs) {
+        Query &q = *s.q;
+        if (ok) {
+            const size_t o = static_cast<size_t>(s.batch_offset), f = static_cast<size_t>(s.first),
+                         c = static_cast<size_t>(s.count);
+            std::memcpy(q.ends.data() + f, sl->h_ends + o, sizeof(int32_t) * c);
+            std::memcpy(q.lengths.data() + f, sl->h_lengths + o, sizeof(int32_t) * c);
+            std::memcpy(q.status.data() + f, sl->h_status + o, sizeof(int32_t) * c);
+            if (sl->path_stride > 0) {
+                const size_t L = static_cast<size_t>(sl->path_stride);
+                std::memcpy(q.paths.data() + f * L, sl->h_paths + o * L, sizeof(int32_t) * c * L);
+            }
+            uint64_t steps = 0, errors = 0;
+            for (size_t i = 0; i < c; ++i) {
+                steps += static_cast<uint64_t>(std::max(sl->h_lengths[o + i] - 1, 0));
+                errors += sl->h_status[o + i] >= TRW_WALKER_REPLAY_EXHAUSTED;
+            }
+            q.steps += steps;
+            q.errors += errors;
+        } else {
+            std::lock_guard<std::mutex> lk(mu);
+            if (q.error_code == TRW_OK) {
+                q.error_code = TRW_E_HIP;
+                q.error = msg;
+            }
+        }
+        if (q.done_walkers.fetch_add(s.count) + s.count == q.n_walkers) finalize(s.q);
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        sl->segs.clear();
+        sl->state = SlotState::Free;
+    }
+    cv.notify_one();
+}
+
+void trw_server::finalize(const std::shared_ptr<Query> &qp) {
+    Query &q = *qp;
+    bool failed;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        failed = q.error_code != TRW_OK;
+    }
+    if (!failed && q.result == TRW_RESULT_PPR) {
+        std::vector<uint32_t> v, tmp;
+        v.reserve(q.ends.size());
+        for (size_t i = 0; i < q.ends.size(); ++i)
+            if (q.status[i] < TRW_WALKER_REPLAY_EXHAUSTED) v.push_back(static_cast<uint32_t>(q.ends[i]));
+        radix_sort(v, tmp, vertex_bits);
+        for (size_t i = 0; i < v.size();) {
+            size_t j = i + 1;
+            while (j < v.size() && v[j] == v[i]) ++j;
+            q.ppr_vertices.push_back(static_cast<int32_t>(v[i]));
+            q.ppr_counts.push_back(static_cast<uint32_t>(j - i));
+            i = j;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        q.finished = Clock::now();
+        q.state = failed ? TRW_QUERY_FAILED : TRW_QUERY_DONE;
+        ++stats.queries_done;
+        stats.walkers += static_cast<uint64_t>(q.n_walkers);
+        stats.steps += q.steps.load();
+    }
+    cv_done.notify_all();
+}
+
+// ---------------------------------------------------------------- submit
+
+int trw_server::submit(const trw_query *q, Clock::time_point arrival, uint64_t *id_out) {
+    if (!q) return trw::set_error(TRW_E_INVALID, "query is null");
+    if (q->n_walkers < 0) return trw::set_error(TRW_E_INVALID, "n_walkers must be >= 0");
+    if (q->n_walkers > 0 && (q->n_sources <= 0 || !q->sources))
+        return trw::set_error(TRW_E_INVALID, "a query with walkers needs at least one source");
+    if (q->n_sources > kPoolInts)
+        return trw::set_error(TRW_E_INVALID, "at most %d sources per query", kPoolInts);
+    if (q->result != TRW_RESULT_PPR && q->result != TRW_RESULT_PATHS)
+        return trw::set_error(TRW_E_INVALID, "unknown result kind %d", q->result);
+    if (q->params.rng != TRW_RNG_PHILOX)
+        return trw::set_error(TRW_E_UNSUPPORTED, "served queries draw from the Philox stream (TRW_RNG_PHILOX)");
+    if (q->params.form == TRW_FORM_STEP)
+        return trw::set_error(TRW_E_UNSUPPORTED, "the server runs the FUSED or QUEUE form");
+    int rc = trw::walk_check(g, &q->params);
+    if (rc != TRW_OK) return rc;
+    const int32_t L = std::max(q->params.length, 1);
+    if (q->result == TRW_RESULT_PATHS && L > path_ints)
+        return trw::set_error(TRW_E_INVALID, "one path row (%d ints) exceeds the batch path capacity", L);
+    for (int32_t i = 0; i < q->n_sources; ++i)
+        if (q->sources[i] < 0 || q->sources[i] >= g->nv)
+            return trw::set_error(TRW_E_INVALID, "source %d outside [0, %d)", q->sources[i], g->nv);
+
+    auto qp = std::make_shared<Query>();
+    qp->params = q->params;
+    qp->result = q->result;
+    qp->path_len = L;
+    qp->n_walkers = q->n_walkers;
+    qp->id_base = q->walker_id_base;
+    qp->sources.assign(q->sources, q->sources + std::max(q->n_sources, 0));
+    const size_t N = static_cast<size_t>(q->n_walkers);
+    qp->ends.resize(N);
+    qp->lengths.resize(N);
+    qp->status.resize(N);
+    if (q->result == TRW_RESULT_PATHS) qp->paths.resize(N * static_cast<size_t>(L));
+    qp->arrival = arrival;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (stopping) return trw::set_error(TRW_E_STATE, "the server is shutting down");
+        if (static_cast<int64_t>(pending.size()) >= max_pending) {
+            ++stats.rejected;
+            return trw::set_error(TRW_E_BUSY, "%d queries already wait for admission", max_pending);
+        }
+        qp->id = next_id++;
+        queries[qp->id] = qp;
+        if (N == 0) {
+            qp->state = TRW_QUERY_DONE;
+            qp->dispatched = qp->finished = Clock::now();
+            ++stats.queries_done;
+        } else {
+            pending.push_back(qp);
+        }
+        if (id_out) *id_out = qp->id;
+    }
+    cv.notify_one();
+    cv_done.notify_all();
+    return TRW_OK;
+}
+
+void trw_server::free_slots() {
+    for (Slot &sl : slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        (void)hipFree(sl.d_segs);
+        (void)hipFree(sl.d_pool);
+        (void)hipFree(sl.d_sources);
+        (void)hipFree(sl.d_ids);
+        (void)hipFree(sl.d_ends);
+        (void)hipFree(sl.d_lengths);
+        (void)hipFree(sl.d_status);
+        (void)hipFree(sl.d_paths);
+        (void)hipFree(sl.d_counters);
+        (void)hipFree(sl.d_queue_head);
+        (void)hipHostFree(sl.h_segs);
+        (void)hipHostFree(sl.h_pool);
+        (void)hipHostFree(sl.h_ends);
+        (void)hipHostFree(sl.h_lengths);
+        (void)hipHostFree(sl.h_status);
+        (void)hipHostFree(sl.h_paths);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        sl = Slot();
+    }
+}
+
+namespace {
+
+template <class T>
+hipError_t dev(T **p, size_t count) {
+    return hipMalloc(reinterpret_cast<void **>(p), sizeof(T) * std::max<size_t>(count, 1));
+}
+
+template <class T>
+hipError_t pinned(T **p, size_t count) {
+    return hipHostMalloc(reinterpret_cast<void **>(p), sizeof(T) * std::max<size_t>(count, 1), hipHostMallocDefault);
+}
+
+std::shared_ptr<Query> find_query(trw_server *s, uint64_t id) {
+    auto it = s->queries.find(id);
+    return it == s->queries.end() ? nullptr : it->second;
+}
+
+void fill_info(const Query &q, trw_query_info *info) {
+    std::memset(info, 0, sizeof(*info));
+    info->state = q.state;
+    info->n_walkers = q.n_walkers;
+    const bool finished = q.state == TRW_QUERY_DONE || q.state == TRW_QUERY_FAILED;
+    info->steps = finished ? q.steps.load() : 0;
+    info->errors = finished ? q.errors.load() : 0;
+    info->ppr_entries = finished ? static_cast<int64_t>(q.ppr_vertices.size()) : 0;
+    info->queue_ms = q.was_dispatched || finished ? ms_between(q.arrival, q.dispatched) : -1.0;
+    info->latency_ms = finished ? ms_between(q.arrival, q.finished) : -1.0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+
+int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server **out) {
+    if (!g || !out) return trw::set_error(TRW_E_INVALID, "null argument");
+    *out = nullptr;
+    trw_server_config c;
+    std::memset(&c, 0, sizeof(c));
+    if (config) c = *config;
+    if (c.streams == 0) c.streams = 4;
+    if (c.max_batch_walkers == 0) c.max_batch_walkers = 1 << 20;
+    if (c.path_bytes_per_batch == 0) c.path_bytes_per_batch = 64ll << 20;
+    if (c.max_pending_queries == 0) c.max_pending_queries = 1 << 16;
+    if (c.completion_threads == 0) c.completion_threads = 4;
+    if (c.max_queries_per_batch == 0) c.max_queries_per_batch = 1024;
+    if (c.streams < 1 || c.streams > 16) return trw::set_error(TRW_E_INVALID, "streams must be in [1, 16]");
+    if (c.max_batch_walkers < 1 || c.max_batch_walkers > (1 << 28))
+        return trw::set_error(TRW_E_INVALID, "max_batch_walkers must be in [1, 2^28]");
+    if (c.path_bytes_per_batch < 4) return trw::set_error(TRW_E_INVALID, "path_bytes_per_batch must be >= 4");
+    if (c.max_pending_queries < 1 || c.completion_threads < 1 || c.completion_threads > 64 ||
+        c.max_queries_per_batch < 1 || c.max_queries_per_batch > (1 << 20))
+        return trw::set_error(TRW_E_INVALID, "bad queue, thread or per-batch query limit");
+
+    auto s = std::make_unique<trw_server>();
+    s->g = g;
+    s->device = g->ctx->device;
+    s->max_walkers = c.max_batch_walkers;
+    s->path_ints = c.path_bytes_per_batch / 4;
+    s->max_pending = c.max_pending_queries;
+    s->max_segments = c.max_queries_per_batch;
+    while (s->vertex_bits < 31 && (int64_t(1) << s->vertex_bits) < g->nv) ++s->vertex_bits;
+    hipError_t e = hipSetDevice(s->device);
+    if (e != hipSuccess) return trw::set_hip_error(e, "hipSetDevice");
+    s->slots.resize(static_cast<size_t>(c.streams));
+    const size_t W = static_cast<size_t>(c.max_batch_walkers), P = static_cast<size_t>(s->path_ints),
+                 S = static_cast<size_t>(c.max_queries_per_batch);
+    for (Slot &sl : s->slots) {
+        // Default stream priority: every stream of the library runs at the same one.
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = dev(&sl.d_segs, S);
+        if (e == hipSuccess) e = dev(&sl.d_pool, kPoolInts);
+        if (e == hipSuccess) e = dev(&sl.d_sources, W);
+        if (e == hipSuccess) e = dev(&sl.d_ids, W);
+        if (e == hipSuccess) e = dev(&sl.d_ends, W);
+        if (e == hipSuccess) e = dev(&sl.d_lengths, W);
+        if (e == hipSuccess) e = dev(&sl.d_status, W);
+        if (e == hipSuccess) e = dev(&sl.d_paths, P);
+        if (e == hipSuccess) e = dev(&sl.d_counters, 4);
+        if (e == hipSuccess) e = dev(&sl.d_queue_head, 4);
+        if (e == hipSuccess) e = pinned(&sl.h_segs, S);
+        if (e == hipSuccess) e = pinned(&sl.h_pool, kPoolInts);
+        if (e == hipSuccess) e = pinned(&sl.h_ends, W);
+        if (e == hipSuccess) e = pinned(&sl.h_lengths, W);
+        if (e == hipSuccess) e = pinned(&sl.h_status, W);
+        if (e == hipSuccess) e = pinned(&sl.h_paths, P);
+    }
+    if (e != hipSuccess) {
+        s->free_slots();
+        return trw::set_hip_error(e, "allocating the server's slots");
+    }
+    s->workers = std::make_unique<WorkerPool>(c.completion_threads);
+    trw_server *raw = s.get();
+    s->dispatcher = std::thread([raw] { raw->dispatch_loop(); });
+    *out = s.release();
+    return TRW_OK;
+}
+
+int trw_server_destroy(trw_server *s) {
+    if (!s) return TRW_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv.notify_all();
+    if (s->dispatcher.joinable()) s->dispatcher.join();
+    s->workers->stop();
+    (void)hipSetDevice(s->device);
+    s->free_slots();
+    delete s;
+    return TRW_OK;
+}
+
+int trw_server_submit(trw_server *s, const trw_query *q, uint64_t *query_id) {
+    if (!s) return trw::set_error(TRW_E_INVALID, "server is null");
+    return s->submit(q, Clock::now(), query_id);
+}
+
+int trw_server_wait(trw_server *s, uint64_t query_id, double timeout_ms, trw_query_info *info) {
+    if (!s) return trw::set_error(TRW_E_INVALID, "server is null");
+    std::unique_lock<std::mutex> lk(s->mu);
+    std::shared_ptr<Query> q = find_query(s, query_id);
+    if (!q) return trw::set_error(TRW_E_INVALID, "unknown query %llu", static_cast<unsigned long long>(query_id));
+    auto finished = [&] { return q->state == TRW_QUERY_DONE || q->state == TRW_QUERY_FAILED; };
+    if (timeout_ms < 0) {
+        s->cv_done.wait(lk, finished);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), finished)) {
+        if (info) fill_info(*q, info);
+        return trw::set_error(TRW_E_TIMEOUT, "query %llu not finished", static_cast<unsigned long long>(query_id));
+    }
+    if (info) fill_info(*q, info);
+    if (q->state == TRW_QUERY_FAILED) return trw::set_error(q->error_code, "%s", q->error.c_str());
+    return TRW_OK;
+}
+
+int trw_server_query_info(trw_server *s, uint64_t query_id, trw_query_info *info) {
+    if (!s || !info) return trw::set_error(TRW_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::shared_ptr<Query> q = find_query(s, query_id);
+    if (!q) return trw::set_error(TRW_E_INVALID, "unknown query %llu", static_cast<unsigned long long>(query_id));
+    fill_info(*q, info);
+    return TRW_OK;
+}
+
+int trw_server_ppr_result(trw_server *s, uint64_t query_id, int32_t *vertices, uint32_t *counts, int64_t capacity) {
+    if (!s) return trw::set_error(TRW_E_INVALID, "server is null");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::shared_ptr<Query> q = find_query(s, query_id);
+    if (!q) return trw::set_error(TRW_E_INVALID, "unknown query %llu", static_cast<unsigned long long>(query_id));
+    if (q->state != TRW_QUERY_DONE) return trw::set_error(TRW_E_STATE, "query is not done");
+    if (q->result != TRW_RESULT_PPR) return trw::set_error(TRW_E_STATE, "query was not a PPR query");
+    const int64_t n = static_cast<int64_t>(q->ppr_vertices.size());
+    if (capacity < n) return trw::set_error(TRW_E_INVALID, "capacity %lld < %lld entries", (long long)capacity, (long long)n);
+    if (n && (!vertices || !counts)) return trw::set_error(TRW_E_INVALID, "null output");
+    std::copy(q->ppr_vertices.begin(), q->ppr_vertices.end(), vertices);
+    std::copy(q->ppr_counts.begin(), q->ppr_counts.end(), counts);
+    return TRW_OK;
+}
+
+int trw_server_walk_result(trw_server *s, uint64_t query_id, int32_t *paths, int32_t *lengths, int32_t *ends,
+                           int32_t *status) {
+    if (!s) return trw::set_error(TRW_E_INVALID, "server is null");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::shared_ptr<Query> q = find_query(s, query_id);
+    if (!q) return trw::set_error(TRW_E_INVALID, "unknown query %llu", static_cast<unsigned long long>(query_id));
+    if (q->state != TRW_QUERY_DONE) return trw::set_error(TRW_E_STATE, "query is not done");
+    if (paths && q->result != TRW_RESULT_PATHS) return trw::set_error(TRW_E_STATE, "query kept no paths");
+    if (paths) std::copy(q->paths.begin(), q->paths.end(), paths);
+    if (lengths) std::copy(q->lengths.begin(), q->lengths.end(), lengths);
+    if (ends) std::copy(q->ends.begin(), q->ends.end(), ends);
+    if (status) std::copy(q->status.begin(), q->status.end(), status);
+    return TRW_OK;
+}
+
+int trw_server_release(trw_server *s, uint64_t query_id) {
+    if (!s) return trw::set_error(TRW_E_INVALID, "server is null");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->queries.find(query_id);
+    if (it == s->queries.end())
+        return trw::set_error(TRW_E_INVALID, "unknown query %llu", static_cast<unsigned long long>(query_id));
+    if (it->second->state != TRW_QUERY_DONE && it->second->state != TRW_QUERY_FAILED)
+        return trw::set_error(TRW_E_STATE, "query is still pending or running");
+    s->queries.erase(it);
+    return TRW_OK;
+}
+
+int trw_server_stats_get(trw_server *s, trw_server_stats *out) {
+    if (!s || !out) return trw::set_error(TRW_E_INVALID, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->mean_batch_walkers = s->stats.batches ? double(s->batch_walkers) / double(s->stats.batches) : 0.0;
+    out->mean_batch_queries = s->stats.batches ? double(s->batch_queries) / double(s->stats.batches) : 0.0;
+    return TRW_OK;
+}
+
+int trw_server_run_open_loop(trw_server *s, const trw_query *queries, int32_t n, double qps, uint64_t seed,
+                             uint64_t *ids, double *latency_ms, trw_open_loop_report *report) {
+    if (!s || (n > 0 && !queries) || !report) return trw::set_error(TRW_E_INVALID, "null argument");
+    if (n < 0 || !(qps > 0.0)) return trw::set_error(TRW_E_INVALID, "need n >= 0 and qps > 0");
+    std::memset(report, 0, sizeof(*report));
+    trw_server_stats before;
+    trw_server_stats_get(s, &before);
+    uint64_t batch_queries_before;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        batch_queries_before = s->batch_queries;
+    }
+    std::mt19937_64 rng(seed);
+    std::exponential_distribution<double> gap(qps);
+    std::vector<Clock::time_point> when(static_cast<size_t>(n));
+    const Clock::time_point t0 = Clock::now() + std::chrono::milliseconds(2);
+    double t = 0.0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (i) t += gap(rng);
+        when[static_cast<size_t>(i)] = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(t));
+    }
+    std::vector<uint64_t> id(static_cast<size_t>(n), 0);
+    int first_error = TRW_OK;
+    std::string first_message;
+    for (int32_t i = 0; i < n; ++i) {
+        const Clock::time_point at = when[static_cast<size_t>(i)];
+        std::this_thread::sleep_until(at - std::chrono::microseconds(100));
+        while (Clock::now() < at) {
+        }
+        const int rc = s->submit(&queries[i], at, &id[static_cast<size_t>(i)]);
+        if (rc == TRW_E_BUSY) {
+            ++report->rejected;
+        } else if (rc != TRW_OK && first_error == TRW_OK) {
+            first_error = rc;
+            first_message = trw_last_error();
+        }
+    }
+    std::vector<double> lat;
+    lat.reserve(static_cast<size_t>(n));
+    Clock::time_point last = t0;
+    uint64_t steps = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const uint64_t qid = id[static_cast<size_t>(i)];
+        if (ids) ids[i] = qid;
+        if (latency_ms) latency_ms[i] = -1.0;
+        if (!qid) continue;
+        trw_query_info info;
+        const int rc = trw_server_wait(s, qid, -1.0, &info);
+        if (rc != TRW_OK && first_error == TRW_OK) {
+            first_error = rc;
+            first_message = trw_last_error();
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        std::shared_ptr<Query> q = find_query(s, qid);
+        if (!q) continue;
+        last = std::max(last, q->finished);
+        steps += q->steps.load();
+        lat.push_back(info.latency_ms);
+        if (latency_ms) latency_ms[i] = info.latency_ms;
+    }
+    trw_server_stats after;
+    trw_server_stats_get(s, &after);
+    report->wall_ms = ms_between(t0, last);
+    report->offered_qps = qps;
+    report->achieved_qps = report->wall_ms > 0 ? 1e3 * double(lat.size()) / report->wall_ms : 0.0;
+    report->steps = steps;
+    report->steps_per_s = report->wall_ms > 0 ? 1e3 * double(steps) / report->wall_ms : 0.0;
+    report->batches = after.batches - before.batches;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        report->mean_batch_queries =
+            report->batches ? double(s->batch_queries - batch_queries_before) / double(report->batches) : 0.0;
+    }
+    if (!lat.empty()) {
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double p) {
+            const size_t k = static_cast<size_t>(std::min<double>(lat.size() - 1, std::ceil(p * lat.size()) - 1));
+            return lat[k];
+        };
+        report->p50_ms = pct(0.50);
+        report->p90_ms = pct(0.90);
+        report->p99_ms = pct(0.99);
+        report->p999_ms = pct(0.999);
+        report->max_ms = lat.back();
+        double sum = 0;
+        for (double x : lat) sum += x;
+        report->mean_ms = sum / double(lat.size());
+    }
+    if (first_error != TRW_OK) return trw::set_error(first_error, "%s", first_message.c_str());
+    return TRW_OK;
+}

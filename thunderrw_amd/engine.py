@@ -372,3 +372,100 @@ def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, w
     finally:
         for b in bufs:
             b.free()
+
+
+def _struct_dict(st):
+    return {name: getattr(st, name) for name, _ in st._fields_}
+
+
+class Server:
+    """Continuous-batching query server over one DeviceGraph (trw_server_*).
+
+    Build the tables the queries sample from before creating it.  Queries draw
+    from the Philox stream: give each one a disjoint walker-id range (the default
+    is query_number << 32) and its result is the same alone or in a mix."""
+
+    def __init__(self, graph, streams=4, max_batch_walkers=1 << 20, path_bytes_per_batch=64 << 20,
+                 max_pending_queries=1 << 16, completion_threads=4, max_queries_per_batch=1024):
+        self.graph = graph
+        cfg = A.ServerConfig(streams, max_batch_walkers, path_bytes_per_batch, max_pending_queries, completion_threads,
+                             max_queries_per_batch, 0)
+        self.h = C.c_void_p()
+        check(lib().trw_server_create(graph.h, C.byref(cfg), C.byref(self.h)))
+        self._count = 0
+
+    def close(self):
+        if self.h:
+            check(lib().trw_server_destroy(self.h))
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def query(self, params, sources, n_walkers, walker_id_base=None, result="ppr"):
+        """A trw_query struct (keeps a reference to its source array)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        if walker_id_base is None:
+            self._count += 1
+            walker_id_base = self._count << 32
+        q = A.Query(_ptr(src), len(src), int(n_walkers), int(walker_id_base), params,
+                    A.RESULT_PPR if result == "ppr" else A.RESULT_PATHS, 0)
+        q._src = src
+        return q
+
+    def submit(self, params, sources, n_walkers, walker_id_base=None, result="ppr"):
+        q = self.query(params, sources, n_walkers, walker_id_base, result)
+        qid = C.c_uint64()
+        check(lib().trw_server_submit(self.h, C.byref(q), C.byref(qid)))
+        return qid.value
+
+    def wait(self, qid, timeout_ms=-1.0):
+        info = A.QueryInfo()
+        check(lib().trw_server_wait(self.h, qid, float(timeout_ms), C.byref(info)))
+        return _struct_dict(info)
+
+    def info(self, qid):
+        info = A.QueryInfo()
+        check(lib().trw_server_query_info(self.h, qid, C.byref(info)))
+        return _struct_dict(info)
+
+    def ppr(self, qid):
+        """(vertices ascending, walker counts) of a done PPR query."""
+        n = self.info(qid)["ppr_entries"]
+        v = np.empty(max(n, 1), np.int32)
+        c = np.empty(max(n, 1), np.uint32)
+        check(lib().trw_server_ppr_result(self.h, qid, _ptr(v), _ptr(c), n))
+        return v[:n], c[:n]
+
+    def walk_result(self, qid, length=None):
+        info = self.info(qid)
+        n = info["n_walkers"]
+        m = max(n, 1)
+        lengths, ends, status = np.empty(m, np.int32), np.empty(m, np.int32), np.empty(m, np.int32)
+        paths = None if length is None else np.empty((m, max(length, 1)), np.int32)
+        check(lib().trw_server_walk_result(self.h, qid, _ptr(paths), _ptr(lengths), _ptr(ends), _ptr(status)))
+        return dict(paths=None if paths is None else paths[:n], lengths=lengths[:n], ends=ends[:n],
+                    status=status[:n])
+
+    def release(self, qid):
+        check(lib().trw_server_release(self.h, qid))
+
+    def stats(self):
+        st = A.ServerStats()
+        check(lib().trw_server_stats_get(self.h, C.byref(st)))
+        return _struct_dict(st)
+
+    def run_open_loop(self, queries, qps, seed=0):
+        """Submits trw_query structs at Poisson arrivals of rate qps; returns
+        (report dict, query ids, latencies in ms)."""
+        n = len(queries)
+        arr = (A.Query * max(n, 1))(*queries)
+        ids = np.zeros(max(n, 1), np.uint64)
+        lat = np.zeros(max(n, 1), np.float64)
+        rep = A.OpenLoopReport()
+        check(lib().trw_server_run_open_loop(self.h, arr, n, float(qps), int(seed), _ptr(ids), _ptr(lat),
+                                             C.byref(rep)))
+        return _struct_dict(rep), ids[:n], lat[:n]

@@ -63,6 +63,9 @@ def parse_args():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline measurement only")
+    ap.add_argument("--qps", type=float, default=4000.0, help="offered load of the PPR serving run")
+    ap.add_argument("--queries", type=int, default=4000, help="queries in the PPR serving run")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also launch the copy kernel over this many MiB (byte-counter calibration)")
     return ap.parse_args()
@@ -155,6 +158,82 @@ def reference_cpu_baseline(hg, length):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def mix_forms(ctx, dg, deg, args):
+    """Uneven PPR + MetaPath mix: 2^23 walkers from uniform random non-isolated sources
+    (generate_random_source), walkers [0, 2^22) PPR with termination 0.15, the rest
+    MetaPath on schema 0,1,2,3,4, both logging up to 80 path entries.  The same batch
+    runs in the persistent work-queue form, the compacted hipGraph step form and the
+    plain per-step-launch form; the forms give identical walks."""
+    n = 1 << 23
+    split = n // 2
+    src = T.sources_random(deg, n, 4)
+    batch = Batch(ctx, src, 80)
+    out = {"walkers": n, "split": split}
+    for form in ("queue", "graph", "step"):
+        a = T.make_params("ppr", length=80, stop_probability=0.15, rng="philox", seed=4, form=form)
+        b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=4, form=form)
+        io = batch.io()
+        T.walk_mix_async(dg, a, b, split, io)
+        T.synchronize(ctx)
+        batch.reset()
+        reps = max(1, min(args.steps, 3))
+        timer = T.DeviceTimer(ctx)
+        timer.begin()
+        for _ in range(reps):
+            T.walk_mix_async(dg, a, b, split, io)
+        ms = timer.end()
+        T.synchronize(ctx)
+        steps = batch.steps() // reps
+        out[form + "_steps_per_s"] = steps / (ms / 1e3 / reps)
+        out[form + "_ms"] = ms / reps
+        out["steps_per_batch"] = steps
+        log("mix %s form: %.2f ms, %.3e steps/s" % (form, ms / reps, out[form + "_steps_per_s"]))
+    out["batched_vs_plain"] = out["queue_steps_per_s"] / out["step_steps_per_s"]
+    out["graph_vs_plain"] = out["graph_steps_per_s"] / out["step_steps_per_s"]
+    batch.free()
+    return out
+
+
+def serving(ctx, args):
+    """PPR query serving on a 2^24-vertex Chung-Lu graph (exponent 2.1, average degree 16,
+    dmin 2, dmax 65536, FP64 U[1,5) weights, weighted alias tables resident).  Poisson
+    arrivals at a fixed rate; each query has |S| ~ logU[1, 4096] sources drawn uniformly
+    from the non-isolated vertices and N ~ logU[1e3, 1e5] walkers; termination 0.2."""
+    t0 = time.time()
+    hg = T.HostGraph.chung_lu(1 << 24, 8 << 24, 2.1, 2.0, 65536.0, 5, True, 0)
+    deg, adj, w, _ = hg.arrays()
+    log("serving graph: %d directed edges (%.1fs)" % (len(adj), time.time() - t0))
+    dg = T.DeviceGraph(ctx, deg, adj, w)
+    live = np.nonzero(deg > 0)[0].astype(np.int32)
+    del deg, adj, w  # views into the host graph, which is freed next
+    hg.close()
+    dg.build_tables("alias", "edge")
+    rng = np.random.default_rng(5)
+    prm = T.make_params("ppr", execution="static", sampling="alias", weights="edge", stop_probability=0.2,
+                        rng="philox", seed=5)
+    out = {"graph": "chung-lu 2^24 V, avg deg 16, gamma 2.1, dmin 2, dmax 65536, FP64 U[1,5) weights",
+           "queries": args.queries, "offered_qps": args.qps, "alias_table_build_ms": round(dg.last_build_ms(), 3)}
+    with T.Server(dg, streams=4, completion_threads=8) as srv:
+        qs = []
+        for i in range(args.queries):
+            k = int(np.exp(rng.uniform(0.0, np.log(4096.0))))
+            nw = int(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
+            qs.append(srv.query(prm, rng.choice(live, k), nw, (i + 1) << 32))
+        # Warm-up at the same rate (first-touch of pinned staging, kernel loads).
+        srv.run_open_loop(qs[:200], args.qps, seed=1)
+        rep, ids, lat = srv.run_open_loop(qs, args.qps, seed=2)
+        st = srv.stats()
+        for q in ids:
+            srv.release(int(q))
+    dg.close()
+    out.update({k: rep[k] for k in ("achieved_qps", "p50_ms", "p90_ms", "p99_ms", "p999_ms", "mean_ms", "max_ms",
+                                    "steps_per_s", "steps", "batches", "mean_batch_queries", "rejected")})
+    out["mean_batch_walkers"] = st["mean_batch_walkers"]
+    log("serving at %.0f qps: p50 %.3f ms, p99 %.3f ms, %.3e steps/s, %.1f queries per batch"
+        % (args.qps, rep["p50_ms"], rep["p99_ms"], rep["steps_per_s"], rep["mean_batch_queries"]))
+    return out
+
+
 def load_traffic():
     try:
         with open(TRAFFIC_FILE) as f:
@@ -173,12 +252,13 @@ def main():
     nv = 1 << args.scale
     length = 80
     t0 = time.time()
-    hg = T.HostGraph.chung_lu(nv, 8 * nv, 2.1, 2.0, 65536.0, args.seed, True, 0)
-    deg, adj, w, _ = hg.arrays()
+    # Labels U{0..4} ride along for the PPR + MetaPath mix; they do not change the structure.
+    hg = T.HostGraph.chung_lu(nv, 8 * nv, 2.1, 2.0, 65536.0, args.seed, True, 5)
+    deg, adj, w, lab = hg.arrays()
     log("graph: %d vertices, %d directed edges, max degree %d (%.1fs)" % (nv, len(adj), deg.max(), time.time() - t0))
 
     ctx = T.Context(0)
-    dg = T.DeviceGraph(ctx, deg, adj, w)
+    dg = T.DeviceGraph(ctx, deg, adj, w, lab)
     dg.build_tables("alias", "edge")
     alias_build_ms = dg.last_build_ms()
     log("alias tables built on the device in %.1f ms" % alias_build_ms)
@@ -235,6 +315,13 @@ def main():
             details[label] = v.value
         log("random-gather ceiling: %.3e gathers/s (4 GiB table)" % details["probe_gathers_per_s_4GiB"])
 
+    if not args.quick:
+        details["mix_ppr_metapath"] = mix_forms(ctx, dg, deg, args)
+    batch.free()
+    dg.close()
+    if not args.quick and not args.no_serving:
+        details["ppr_serving"] = serving(ctx, args)
+
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20
         a = T.DeviceBuffer(ctx, nbytes)
@@ -273,8 +360,6 @@ def main():
         "cpu_baseline": cpu,
         "details": details,
     }
-    batch.free()
-    dg.close()
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -60,6 +60,8 @@ extern "C" {
 #define TRW_FORM_FUSED 1 /* thread per walker, whole walk in one launch, LDS-staged path tiles */
 #define TRW_FORM_QUEUE 2 /* persistent kernel, lanes refill from a device work queue */
 #define TRW_FORM_STEP 3  /* plain form: one launch per step over walker state in HBM */
+#define TRW_FORM_GRAPH 4 /* per-step launches over a compacted list of live walkers, replayed
+                            from a hipGraph that captures 8 steps */
 
 /* Per-walker outcome (status[] output). */
 #define TRW_WALKER_DONE 0              /* update() ended it (length reached / PPR stop draw) */
@@ -195,7 +197,7 @@ int trw_last_build_ms(const trw_graph *g, double *ms);
 int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream);
 
 /* Two walker kinds in one batch: walkers [0, split) run params_a, [split, n) run
- * params_b (the uneven PPR + MetaPath mix).  form: QUEUE or STEP. */
+ * params_b (the uneven PPR + MetaPath mix).  form: QUEUE, STEP or GRAPH. */
 int trw_walk_mix_async(trw_graph *g, const trw_walk_params *params_a, const trw_walk_params *params_b,
                        int32_t split, const trw_walk_io *io, void *stream);
 

@@ -30,7 +30,7 @@ def case_params(case, form="auto", rng="replay", seed=0):
 
 
 def forms_for(app):
-    return ["queue", "step"] if app == "ppr" else ["fused", "queue", "step"]
+    return ["queue", "step", "graph"] if app == "ppr" else ["fused", "queue", "step", "graph"]
 
 
 @pytest.mark.parametrize("gname", ["small", "hub"])
@@ -225,12 +225,14 @@ def test_mixed_batch_queue_equals_step_and_single_kinds(gpu_ctx):
         src = rng.choice(live, n).astype(np.int32)
         split = n // 2
         res = {}
-        for form in ("queue", "step"):
+        for form in ("queue", "step", "graph"):
             a = T.make_params("ppr", length=80, stop_probability=0.15, rng="philox", seed=5, form=form)
             b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=5, form=form)
             res[form] = T.walk(dg, a, src, mix=(b, split))
         for key in ("paths", "lengths", "ends", "status"):
             assert np.array_equal(res["queue"][key], res["step"][key]), key
+            assert np.array_equal(res["queue"][key], res["graph"][key]), key
+        assert res["queue"]["steps"] == res["step"]["steps"] == res["graph"]["steps"]
         # Each kind alone (keyed by the same global walker ids) gives the same walks as in the mix.
         a = T.make_params("ppr", length=80, stop_probability=0.15, rng="philox", seed=5)
         b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=5)

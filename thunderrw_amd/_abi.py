@@ -12,7 +12,7 @@ EXEC_UNIFORM, EXEC_STATIC, EXEC_DYNAMIC = 0, 1, 2
 SM_UNIFORM, SM_ITS, SM_ALIAS, SM_REJECTION, SM_MAX_WEIGHT_REJECTION = 0, 1, 2, 3, 4
 APP_DEEPWALK, APP_PPR, APP_NODE2VEC, APP_METAPATH, APP_GENERIC = 0, 1, 2, 3, 4
 RNG_REPLAY, RNG_PHILOX = 0, 1
-FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP = 0, 1, 2, 3
+FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP, FORM_GRAPH = 0, 1, 2, 3, 4
 WEIGHTS_EDGE, WEIGHTS_UNIT = 0, 1
 WALK_WEIGHTS_APP, WALK_WEIGHTS_EDGE, WALK_WEIGHTS_UNIT = 0, 1, 2
 RESULT_PPR, RESULT_PATHS = 0, 1

@@ -254,6 +254,7 @@ static void graph_release(trw_graph *g) {
     dfree(g->rs_max);
     dfree(g->counters);
     dfree(g->state);
+    dfree(g->active);
 }
 
 int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *degree, const int32_t *adj,
@@ -585,15 +586,22 @@ static int make_io(const trw_graph *g, const trw_walk_params *p, const trw_walk_
     return TRW_OK;
 }
 
+static int ensure_state(trw_graph *g, int64_t n) {
+    if (g->state_cap >= n) return TRW_OK;
+    dfree(g->state);
+    dfree(g->active);
+    g->state_cap = 0;
+    TRW_HIP(dmalloc(&g->state, static_cast<size_t>(n)), "hipMalloc walker state");
+    TRW_HIP(dmalloc(&g->active, 2 * static_cast<size_t>(n)), "hipMalloc live-walker lists");
+    g->state_cap = n;
+    return TRW_OK;
+}
+
 static int run_step_form(trw_graph *g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                          hipStream_t st) {
     if (io.n == 0) return TRW_OK;
-    if (g->state_cap < io.n) {
-        dfree(g->state);
-        g->state_cap = 0;
-        TRW_HIP(dmalloc(&g->state, static_cast<size_t>(io.n)), "hipMalloc walker state");
-        g->state_cap = io.n;
-    }
+    int rc = ensure_state(g, io.n);
+    if (rc != TRW_OK) return rc;
     const DevGraph dv = g->view();
     TRW_HIP(trw::launch_walk_step_init(dv, a, b, split, io, g->state, st), "step-form init");
     unsigned int alive = 1;
@@ -607,6 +615,56 @@ static int run_step_form(trw_graph *g, const KindParams &a, const KindParams &b,
         TRW_HIP(hipStreamSynchronize(st), "step-form synchronize");
     }
     TRW_HIP(trw::launch_walk_step_finish(a, b, split, io, g->state, st), "step-form finish");
+    return TRW_OK;
+}
+
+namespace {
+struct GraphExec {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphExec() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+}  // namespace
+
+// Live-walker list in ping-pong buffers: step k reads list[k % 2] and appends the
+// survivors to list[(k + 1) % 2].  A chunk of an even number of steps (counter
+// resets are fill kernels, so the chunk holds kernel nodes only) is captured once
+// and replayed until the list is empty; after each replay the live list is list[0].
+static int run_graph_form(trw_graph *g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                          hipStream_t st) {
+    if (io.n == 0) return TRW_OK;
+    int rc = ensure_state(g, io.n);
+    if (rc != TRW_OK) return rc;
+    const DevGraph dv = g->view();
+    int32_t *list[2] = {g->active, g->active + g->state_cap};
+    unsigned int *count[2] = {g->counters + 4, g->counters + 5};
+    TRW_HIP(trw::launch_walk_step_init(dv, a, b, split, io, g->state, st), "graph-form init");
+    TRW_HIP(trw::launch_fill_u32(count[0], 0u, 1, st), "graph-form reset");
+    TRW_HIP(trw::launch_walk_list_init(g->state, io.n, list[0], count[0], st), "graph-form list");
+    const int kChunk = 8;
+    GraphExec ge;
+    TRW_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < kChunk && e == hipSuccess; ++k) {
+        const int i = k & 1, o = i ^ 1;
+        e = trw::launch_fill_u32(count[o], 0u, 1, st);
+        if (e == hipSuccess)
+            e = trw::launch_walk_step_compact(dv, a, b, split, io, g->state, list[i], count[i], list[o], count[o], st);
+    }
+    const hipError_t ec = hipStreamEndCapture(st, &ge.graph);
+    TRW_HIP(e, "capturing the step chunk");
+    TRW_HIP(ec, "end capture");
+    TRW_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0), "instantiate the step chunk");
+    unsigned int live = 1;
+    while (live) {
+        TRW_HIP(hipGraphLaunch(ge.exec, st), "replay the step chunk");
+        TRW_HIP(hipMemcpyAsync(&live, count[0], sizeof(live), hipMemcpyDeviceToHost, st), "read live count");
+        TRW_HIP(hipStreamSynchronize(st), "graph-form synchronize");
+    }
+    TRW_HIP(trw::launch_walk_step_finish(a, b, split, io, g->state, st), "graph-form finish");
     return TRW_OK;
 }
 
@@ -635,6 +693,8 @@ int trw::walk_launch(trw_graph *g, const trw_walk_params *params, const trw_walk
         return TRW_OK;
     case TRW_FORM_STEP:
         return run_step_form(g, k, k, bio.n, bio, st);
+    case TRW_FORM_GRAPH:
+        return run_graph_form(g, k, k, bio.n, bio, st);
     default:
         return fail(TRW_E_INVALID, "unknown form %d", params->form);
     }
@@ -647,7 +707,7 @@ int trw::walk_check(trw_graph *g, const trw_walk_params *params) {
     if (rc != TRW_OK) return rc;
     if (params->rng != TRW_RNG_REPLAY && params->rng != TRW_RNG_PHILOX)
         return fail(TRW_E_INVALID, "unknown rng %d", params->rng);
-    if (params->form < TRW_FORM_AUTO || params->form > TRW_FORM_STEP)
+    if (params->form < TRW_FORM_AUTO || params->form > TRW_FORM_GRAPH)
         return fail(TRW_E_INVALID, "unknown form %d", params->form);
     if (params->form == TRW_FORM_FUSED && k.ppr)
         return fail(TRW_E_UNSUPPORTED, "the fused form runs fixed-length walks; use QUEUE or STEP for PPR");
@@ -679,7 +739,8 @@ int trw_walk_mix_async(trw_graph *g, const trw_walk_params *pa, const trw_walk_p
         return TRW_OK;
     }
     if (form == TRW_FORM_STEP) return run_step_form(g, a, b, split, bio, st);
-    return fail(TRW_E_UNSUPPORTED, "mixed batches run in the QUEUE or STEP form");
+    if (form == TRW_FORM_GRAPH) return run_graph_form(g, a, b, split, bio, st);
+    return fail(TRW_E_UNSUPPORTED, "mixed batches run in the QUEUE, STEP or GRAPH form");
 }
 
 int trw_walk_host(trw_graph *g, const trw_walk_params *p, const int32_t *sources, int32_t n, const uint32_t *draws,

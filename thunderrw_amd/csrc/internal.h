@@ -34,6 +34,7 @@ struct trw_graph {
     int32_t alias_weights = -1, its_weights = -1, rs_weights = -1;  // TRW_WEIGHTS_* the table was built from
     unsigned int *counters = nullptr;                                 // [0] queue head [1] alive [2] zero-sum count
     trw::WalkerState *state = nullptr;
+    int32_t *active = nullptr;  // two live-walker lists of state_cap entries (GRAPH form)
     int64_t state_cap = 0;
     double last_build_ms = 0.0;
 

@@ -769,6 +769,65 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_kernel(DevGraph g, KindP
     if (io.counters != nullptr && (threadIdx.x & 63) == 0 && steps) atomicAdd(io.counters, steps);
 }
 
+// Compacted step form: each launch advances only the walkers on the active list
+// list_in[0, *count_in) by one step and appends the ones still alive to list_out
+// (one atomic per wavefront), so a step's cost follows the live walkers rather than
+// the batch.  The grid is fixed (grid-stride over the list), which lets a chunk of
+// these launches be captured once in a hipGraph and replayed.
+__device__ __forceinline__ void append_survivor(bool keep, int32_t w, int32_t *list_out, unsigned int *count_out) {
+    const unsigned long long mask = __ballot(keep);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(count_out, static_cast<unsigned int>(__popcll(mask)));
+    base = __shfl(base, leader);
+    if (keep) list_out[base + __popcll(mask & ((1ull << lane) - 1ull))] = w;
+}
+
+__global__ __launch_bounds__(kStepBlock) void walk_list_init_kernel(const WalkerState *state, int32_t n,
+                                                                     int32_t *list, unsigned int *count) {
+    const int32_t w = blockIdx.x * kStepBlock + threadIdx.x;
+    const bool keep = w < n && state[w].st == kAlive;
+    append_survivor(keep, w, list, count);
+}
+
+template <int MODE, bool REPLAY>
+__global__ __launch_bounds__(kStepBlock) void walk_step_compact_kernel(DevGraph g, KindParams A, KindParams B,
+                                                                        int32_t split, BatchIO io, WalkerState *state,
+                                                                        const int32_t *list_in,
+                                                                        const unsigned int *count_in,
+                                                                        int32_t *list_out, unsigned int *count_out) {
+    const uint32_t count = *count_in;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t stride = gridDim.x * kStepBlock;
+    unsigned long long steps = 0;
+    // Wave-uniform trip count: every lane of a wavefront reaches each ballot.
+    for (uint32_t base = blockIdx.x * kStepBlock + (threadIdx.x - lane); base < count; base += stride) {
+        const uint32_t i = base + lane;
+        bool keep = false;
+        int32_t w = 0;
+        if (i < count) {
+            w = list_in[i];
+            WalkerState s = state[w];
+            const bool kind_b = w >= split;
+            const KindParams &P = kind_b ? B : A;
+            uint4 blk = first_block<REPLAY>(io, w, s);
+            const bool moved =
+                kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
+            if (moved) {
+                ++steps;
+                if (io.paths && s.len - 1 < P.length) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
+            }
+            keep = s.st == kAlive;
+            state[w] = s;
+        }
+        append_survivor(keep, w, list_out, count_out);
+    }
+    steps = wave_sum(steps);
+    if (io.counters != nullptr && lane == 0 && steps) atomicAdd(io.counters, steps);
+}
+
 __global__ __launch_bounds__(kStepBlock) void walk_step_finish_kernel(BatchIO io, const WalkerState *state) {
     const int32_t w = blockIdx.x * kStepBlock + threadIdx.x;
     unsigned long long gterm = 0, dead = 0, err = 0;
@@ -915,6 +974,59 @@ hipError_t launch_walk_step(const DevGraph &g, const KindParams &a, const KindPa
     if (io.n <= 0) return hipSuccess;
     if (io.replay) step_dispatch<true>(g, a, b, split, io, state, alive, stream);
     else step_dispatch<false>(g, a, b, split, io, state, alive, stream);
+    return hipGetLastError();
+}
+
+template <int MODE, bool REPLAY>
+void compact_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                    WalkerState *state, const int32_t *list_in, const unsigned int *count_in, int32_t *list_out,
+                    unsigned int *count_out, unsigned grid, hipStream_t stream) {
+    hipLaunchKernelGGL((walk_step_compact_kernel<MODE, REPLAY>), dim3(grid), dim3(kStepBlock), 0, stream, g, a, b, split,
+                       io, state, list_in, count_in, list_out, count_out);
+}
+
+template <bool REPLAY>
+void compact_dispatch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                      WalkerState *state, const int32_t *list_in, const unsigned int *count_in, int32_t *list_out,
+                      unsigned int *count_out, unsigned grid, hipStream_t stream) {
+    const bool single = (split >= io.n) || (a.mode == b.mode);
+    if (!single) {
+        compact_launch<-1, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream);
+        return;
+    }
+    switch (split >= io.n ? a.mode : b.mode) {
+    case kModeUniform: compact_launch<kModeUniform, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
+    case kModeAlias: compact_launch<kModeAlias, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
+    case kModeIts: compact_launch<kModeIts, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
+    case kModeRejection: compact_launch<kModeRejection, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
+    case kModeNode2Vec: compact_launch<kModeNode2Vec, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
+    default: compact_launch<kModeMetaPath, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
+    }
+}
+
+hipError_t launch_walk_list_init(const WalkerState *state, int32_t n, int32_t *list, unsigned int *count,
+                                 hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(walk_list_init_kernel, dim3(grid_for(n, kStepBlock)), dim3(kStepBlock), 0, stream, state, n,
+                       list, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk_step_compact(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
+                                    const BatchIO &io, WalkerState *state, const int32_t *list_in,
+                                    const unsigned int *count_in, int32_t *list_out, unsigned int *count_out,
+                                    hipStream_t stream) {
+    if (io.n <= 0) return hipSuccess;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
+    }
+    // Full occupancy (8 wavefronts of 256-lane blocks per CU) or the batch, whichever is smaller.
+    const int64_t needed = grid_for(io.n, kStepBlock);
+    const unsigned grid = static_cast<unsigned>(needed < int64_t(cus) * 8 ? needed : int64_t(cus) * 8);
+    if (io.replay) compact_dispatch<true>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream);
+    else compact_dispatch<false>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream);
     return hipGetLastError();
 }
 

@@ -19,7 +19,8 @@ _EXEC = {"uniform": A.EXEC_UNIFORM, "static": A.EXEC_STATIC, "dynamic": A.EXEC_D
 _SM = {"uniform": A.SM_UNIFORM, "its": A.SM_ITS, "alias": A.SM_ALIAS, "rejection": A.SM_REJECTION,
        "max_weight_rejection": A.SM_MAX_WEIGHT_REJECTION}
 _RNG = {"replay": A.RNG_REPLAY, "philox": A.RNG_PHILOX}
-_FORM = {"auto": A.FORM_AUTO, "fused": A.FORM_FUSED, "queue": A.FORM_QUEUE, "step": A.FORM_STEP}
+_FORM = {"auto": A.FORM_AUTO, "fused": A.FORM_FUSED, "queue": A.FORM_QUEUE, "step": A.FORM_STEP,
+         "graph": A.FORM_GRAPH}
 
 ALIAS_DTYPE = np.dtype([("alias_value", "<f8"), ("first", "<i4"), ("second", "<i4")])
 

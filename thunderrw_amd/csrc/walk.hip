@@ -656,7 +656,8 @@ __global__ __launch_bounds__(kFusedBlock) void walk_fused_multi_kernel(DevGraph 
 // ---------------------------------------------------------------- queue form
 template <int MODE, bool REPLAY>
 __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, KindParams A, KindParams B,
-                                                                  int32_t split, BatchIO io, unsigned int *queue) {
+                                                                  int32_t split, BatchIO io, unsigned int *queue,
+                                                                  unsigned int chunk) {
     const int lane = threadIdx.x & 63;
     WalkerState s = empty_walker();
     uint4 blk = make_uint4(0u, 0u, 0u, 0u);
@@ -665,18 +666,40 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
     bool drained = false;
     unsigned long long steps = 0, gterm = 0, dead = 0, err = 0;
     const unsigned int n = static_cast<unsigned int>(io.n);
+    // The wavefront takes walker indices from the shared queue `chunk` at a time and
+    // hands them to its idle lanes from [qnext, qend): one returning atomic per chunk
+    // rather than per refill keeps the single queue head off the critical path.
+    unsigned int qnext = 0, qend = 0;
+    bool queue_empty = false;
 
     for (;;) {
         const bool need = (s.st != kAlive) && !drained;
         const unsigned long long need_mask = __ballot(need);
         if (need_mask) {
-            const int cnt = __popcll(need_mask);
-            const int leader = __ffsll(static_cast<long long>(need_mask)) - 1;
-            unsigned int base = 0;
-            if (lane == leader) base = atomicAdd(queue, static_cast<unsigned int>(cnt));
-            base = __shfl(base, leader, 64);
+            const unsigned int cnt = static_cast<unsigned int>(__popcll(need_mask));
+            const unsigned int avail = qend - qnext;
+            unsigned int fresh = n;  // first index of a newly taken chunk (n = none)
+            if (avail < cnt && !queue_empty) {
+                const int leader = __ffsll(static_cast<long long>(need_mask)) - 1;
+                unsigned int base = 0;
+                if (lane == leader) base = atomicAdd(queue, chunk);
+                base = __shfl(base, leader, 64);
+                if (base >= n) queue_empty = true;
+                else fresh = base;
+            }
+            const unsigned int rank = static_cast<unsigned int>(__popcll(need_mask & ((1ull << lane) - 1ull)));
+            unsigned int idx = n;
+            if (rank < avail) idx = qnext + rank;
+            else if (fresh < n && fresh + (rank - avail) < min(fresh + chunk, n)) idx = fresh + (rank - avail);
+            if (fresh < n) {
+                const unsigned int used = cnt > avail ? cnt - avail : 0u;
+                qend = min(fresh + chunk, n);
+                qnext = min(fresh + used, qend);
+            } else {
+                qnext += min(cnt, avail);
+            }
+            if (queue_empty && qnext >= qend) drained = true;
             if (need) {
-                const unsigned int idx = base + static_cast<unsigned int>(__popcll(need_mask & ((1ull << lane) - 1ull)));
                 if (idx < n) {
                     w = static_cast<int32_t>(idx);
                     kind_b = w >= split;
@@ -697,7 +720,6 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                     }
                 }
             }
-            if (base + static_cast<unsigned int>(cnt) >= n) drained = true;
         }
         const bool active = s.st == kAlive;
         if (__ballot(active) == 0) {
@@ -745,7 +767,6 @@ template <int MODE, bool REPLAY>
 __global__ __launch_bounds__(kStepBlock) void walk_step_kernel(DevGraph g, KindParams A, KindParams B, int32_t split,
                                                                 BatchIO io, WalkerState *state, unsigned int *alive) {
     const int32_t w = blockIdx.x * kStepBlock + threadIdx.x;
-    unsigned long long steps = 0;
     bool still = false;
     if (w < io.n) {
         WalkerState s = state[w];
@@ -755,18 +776,15 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_kernel(DevGraph g, KindP
             uint4 blk = first_block<REPLAY>(io, w, s);
             const bool moved =
                 kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
-            if (moved) {
-                steps = 1;
-                if (io.paths && s.len - 1 < P.length) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
-            }
+            if (moved && io.paths && s.len - 1 < P.length)
+                io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
             still = s.st == kAlive;
             state[w] = s;
         }
     }
-    const unsigned long long live = __ballot(still);
-    if ((threadIdx.x & 63) == 0 && live) atomicAdd(alive, static_cast<unsigned int>(__popcll(live)));
-    steps = wave_sum(steps);
-    if (io.counters != nullptr && (threadIdx.x & 63) == 0 && steps) atomicAdd(io.counters, steps);
+    // Any live walker sets the flag; a plain store, so the launch does not serialise on it.
+    // Steps are counted once, from the final lengths, by the finish kernel.
+    if (__ballot(still) && (threadIdx.x & 63) == 0) *alive = 1u;
 }
 
 // Compacted step form: each launch advances only the walkers on the active list
@@ -774,22 +792,34 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_kernel(DevGraph g, KindP
 // (one atomic per wavefront), so a step's cost follows the live walkers rather than
 // the batch.  The grid is fixed (grid-stride over the list), which lets a chunk of
 // these launches be captured once in a hipGraph and replayed.
-__device__ __forceinline__ void append_survivor(bool keep, int32_t w, int32_t *list_out, unsigned int *count_out) {
+// Appends the lanes with keep set to list_out: one returning atomic per block (the
+// block's wavefront counts are combined in LDS first), so the shared counter is not
+// the bottleneck.  Every thread of the block must call it.
+__device__ __forceinline__ void block_append(bool keep, int32_t w, int32_t *list_out, unsigned int *count_out) {
+    constexpr int kWaves = kStepBlock / 64;
+    __shared__ unsigned int wave_count[kWaves];
+    __shared__ unsigned int block_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long mask = __ballot(keep);
-    if (!mask) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    unsigned int base = 0;
-    if (lane == leader) base = atomicAdd(count_out, static_cast<unsigned int>(__popcll(mask)));
-    base = __shfl(base, leader);
-    if (keep) list_out[base + __popcll(mask & ((1ull << lane) - 1ull))] = w;
+    if (lane == 0) wave_count[wave] = static_cast<unsigned int>(__popcll(mask));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int total = 0;
+        for (int j = 0; j < kWaves; ++j) total += wave_count[j];
+        block_base = total ? atomicAdd(count_out, total) : 0u;
+    }
+    __syncthreads();
+    unsigned int offset = block_base;
+    for (int j = 0; j < wave; ++j) offset += wave_count[j];
+    if (keep) list_out[offset + __popcll(mask & ((1ull << lane) - 1ull))] = w;
+    __syncthreads();  // wave_count and block_base are reused by the next call
 }
 
 __global__ __launch_bounds__(kStepBlock) void walk_list_init_kernel(const WalkerState *state, int32_t n,
                                                                      int32_t *list, unsigned int *count) {
     const int32_t w = blockIdx.x * kStepBlock + threadIdx.x;
     const bool keep = w < n && state[w].st == kAlive;
-    append_survivor(keep, w, list, count);
+    block_append(keep, w, list, count);
 }
 
 template <int MODE, bool REPLAY>
@@ -799,12 +829,10 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_compact_kernel(DevGraph 
                                                                         const unsigned int *count_in,
                                                                         int32_t *list_out, unsigned int *count_out) {
     const uint32_t count = *count_in;
-    const uint32_t lane = threadIdx.x & 63;
     const uint32_t stride = gridDim.x * kStepBlock;
-    unsigned long long steps = 0;
-    // Wave-uniform trip count: every lane of a wavefront reaches each ballot.
-    for (uint32_t base = blockIdx.x * kStepBlock + (threadIdx.x - lane); base < count; base += stride) {
-        const uint32_t i = base + lane;
+    // Block-uniform trip count: every thread of the block reaches each block_append.
+    for (uint32_t base = blockIdx.x * kStepBlock; base < count; base += stride) {
+        const uint32_t i = base + threadIdx.x;
         bool keep = false;
         int32_t w = 0;
         if (i < count) {
@@ -815,29 +843,46 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_compact_kernel(DevGraph 
             uint4 blk = first_block<REPLAY>(io, w, s);
             const bool moved =
                 kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
-            if (moved) {
-                ++steps;
-                if (io.paths && s.len - 1 < P.length) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
-            }
+            if (moved && io.paths && s.len - 1 < P.length)
+                io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
             keep = s.st == kAlive;
             state[w] = s;
         }
-        append_survivor(keep, w, list_out, count_out);
+        block_append(keep, w, list_out, count_out);
     }
-    steps = wave_sum(steps);
-    if (io.counters != nullptr && lane == 0 && steps) atomicAdd(io.counters, steps);
 }
 
+// Every move adds one to a walker's length, so a batch's steps are the sum of
+// (length - 1); the counters are reduced over the block before one atomic each.
 __global__ __launch_bounds__(kStepBlock) void walk_step_finish_kernel(BatchIO io, const WalkerState *state) {
+    constexpr int kWaves = kStepBlock / 64;
+    __shared__ unsigned long long part[4][kWaves];
     const int32_t w = blockIdx.x * kStepBlock + threadIdx.x;
-    unsigned long long gterm = 0, dead = 0, err = 0;
+    unsigned long long steps = 0, gterm = 0, dead = 0, err = 0;
     if (w < io.n) {
         WalkerState s = state[w];
         if (s.st == kAlive) s.st = kDone;
         write_outputs(io, w, s);
         classify(s.st, gterm, dead, err);
+        steps = static_cast<unsigned long long>(s.len > 1 ? s.len - 1 : 0);
     }
-    flush_counters(io, 0, gterm, dead, err);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    steps = wave_sum(steps);
+    gterm = wave_sum(gterm);
+    dead = wave_sum(dead);
+    err = wave_sum(err);
+    if (lane == 0) {
+        part[0][wave] = steps;
+        part[1][wave] = gterm;
+        part[2][wave] = dead;
+        part[3][wave] = err;
+    }
+    __syncthreads();
+    if (io.counters != nullptr && threadIdx.x < 4) {
+        unsigned long long v = 0;
+        for (int j = 0; j < kWaves; ++j) v += part[threadIdx.x][j];
+        if (v) atomicAdd(io.counters + threadIdx.x, v);
+    }
 }
 
 __global__ void fill_u32_kernel(unsigned int *p, unsigned int value, int64_t n) {
@@ -897,7 +942,12 @@ int queue_grid(int device_cus) { return device_cus * 8; }
 template <int MODE, bool REPLAY>
 void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                   unsigned int *q, hipStream_t stream, unsigned grid) {
-    hipLaunchKernelGGL((walk_queue_kernel<MODE, REPLAY>), dim3(grid), dim3(kQueueBlock), 0, stream, g, a, b, split, io, q);
+    // About eight chunks per wavefront of the grid, between 64 and 1024 walkers each.
+    const int64_t waves = static_cast<int64_t>(grid) * (kQueueBlock / 64);
+    int64_t chunk = static_cast<int64_t>(io.n) / (waves * 8);
+    chunk = chunk < 64 ? 64 : (chunk > 1024 ? 1024 : chunk);
+    hipLaunchKernelGGL((walk_queue_kernel<MODE, REPLAY>), dim3(grid), dim3(kQueueBlock), 0, stream, g, a, b, split, io, q,
+                       static_cast<unsigned int>(chunk));
 }
 
 template <bool REPLAY>

@@ -125,6 +125,31 @@ def test_random_graphs_match_oracle(gpu_ctx, seed):
             dg.close()
 
 
+@pytest.mark.parametrize("n_labels,schema", [(8, "7,0,3,9,7,-1,5"), (12, "11,0,3,7,9")])
+def test_metapath_label_alphabets_match_oracle(gpu_ctx, n_labels, schema):
+    """Labels in [0, 8) take the label-start records (schema labels outside the alphabet
+    find no match); wider alphabets take the binary searches.  Both equal the oracle."""
+    base = graphs.chung_lu(6000, 30000, 2.0, 2, 2500, 35)
+    rng = np.random.default_rng(n_labels)
+    g = graphs.CSR(base.degree, base.adj, base.weight, rng.integers(0, n_labels, base.ne))
+    dg = dgraph(gpu_ctx, g)
+    try:
+        live = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        src = np.concatenate([live, np.full(64, int(np.argmax(g.degree)), np.int32)]).astype(np.int32)
+        case = {"app": "metapath", "execution": 2, "sampling": 1, "length": 40, "extra": {"-s": schema}}
+        K, KT = 200, 1
+        ref = oracle_walk(g, case, src, K, KT, 36, 37)
+        assert ref["steps"] > len(src)
+        draws, tdraws = T.replay_fill(36, len(src), K), T.replay_fill(37, len(src), KT)
+        for form in forms_for("metapath"):
+            r = T.walk(dg, case_params(case, form), src, draws, tdraws)
+            for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                assert np.array_equal(r[key], ref[key]), (form, key)
+            assert r["steps"] == ref["steps"], form
+    finally:
+        dg.close()
+
+
 def test_replay_exhaustion_and_bad_sources_are_reported(gpu_ctx):
     g, src = load_golden_graph("small")
     dg = dgraph(gpu_ctx, g)

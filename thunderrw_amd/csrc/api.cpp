@@ -245,6 +245,8 @@ static void graph_release(trw_graph *g) {
     dfree(g->label);
     dfree(g->lbl_sorted);
     dfree(g->lbl_pos);
+    dfree(g->lstart);
+    dfree(g->lerec);
     dfree(g->order);
     dfree(g->vrec);
     dfree(g->erec);
@@ -342,6 +344,16 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
             TRW_HIP(hipMemcpy(g->label, edge_label, sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy label");
             TRW_HIP(hipMemcpy(g->lbl_sorted, ls.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy label index");
             TRW_HIP(hipMemcpy(g->lbl_pos, lp.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy label index");
+        }
+        // Small label alphabets (all labels in [0, 8)): per-vertex label starts and
+        // label-ordered neighbour records make a MetaPath move two gathers.
+        bool small = g->erec != nullptr;
+        for (size_t e = 0; e < E && small; ++e) small = edge_label[e] >= 0 && edge_label[e] < 8;
+        if (small) {
+            TRW_HIP(dmalloc(&g->lstart, 8 * V + 8), "hipMalloc label starts");
+            TRW_HIP(dmalloc(&g->lerec, E + 1), "hipMalloc label records");
+            TRW_HIP(trw::launch_label_records(g->view(), g->lstart, g->lerec, ctx->stream), "label records");
+            TRW_HIP(hipStreamSynchronize(ctx->stream), "label records");
         }
     }
     guard.g = nullptr;

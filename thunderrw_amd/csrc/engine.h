@@ -55,6 +55,10 @@ struct DevGraph {
     const uint4 *erec;         // ne: neighbour records {adj[e], first edge of adj[e], degree of adj[e], 0}
     const uint4 *arec;         // 2*ne: alias walk records {prob lo, prob hi, first, second}
                                //       {first's edge, first's degree, second's edge, second's degree}
+    const uint32_t *lstart;    // 8*nv when every label is in [0, 8): lstart[8v + k] = neighbours of v
+                               //       with label < k (the start of label k in label order), else null
+    const uint4 *lerec;        // ne with lstart: neighbour records in label order,
+                               //       lerec[off(v) + j] = erec[off(v) + lbl_pos[off(v) + j]]
 };
 
 // Per walker kind launch parameters (app functor + g_para).
@@ -154,6 +158,7 @@ hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, 
                               hipStream_t stream);
 hipError_t launch_edge_records(const DevGraph &g, uint4 *erec, hipStream_t stream);
 hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stream);
+hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lerec, hipStream_t stream);
 hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
                             hipStream_t stream);
 hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,

@@ -28,6 +28,8 @@ struct trw_graph {
     uint2 *vrec = nullptr;
     uint4 *erec = nullptr;
     uint4 *arec = nullptr;
+    uint32_t *lstart = nullptr;
+    uint4 *lerec = nullptr;
     trw::AliasSlot *alias = nullptr;
     double *prefix = nullptr;
     double *rs_max = nullptr;
@@ -55,6 +57,8 @@ struct trw_graph {
         g.vrec = vrec;
         g.erec = erec;
         g.arec = arec;
+        g.lstart = lstart;
+        g.lerec = lerec;
         return g;
     }
 };

@@ -132,6 +132,42 @@ __global__ void edge_records_kernel(const int32_t *adj, const uint2 *vrec, int64
     }
 }
 
+// Label starts: for each vertex the number of neighbours with label < k, k = 0..7,
+// by binary search of its label-sorted segment.
+__global__ void label_start_kernel(const int64_t *offset, const int32_t *lbl_sorted, int32_t nv, uint32_t *lstart) {
+    const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int64_t off = offset[v];
+    const int32_t d = static_cast<int32_t>(offset[v + 1] - off);
+    const int32_t *ls = lbl_sorted + off;
+    for (int32_t k = 0; k < 8; ++k) {
+        int32_t lo = 0, hi = d;
+        while (lo < hi) {
+            const int32_t m = (lo + hi) >> 1;
+            if (ls[m] < k) lo = m + 1;
+            else hi = m;
+        }
+        lstart[8 * static_cast<int64_t>(v) + k] = static_cast<uint32_t>(lo);
+    }
+}
+
+// Neighbour records in label order: entry j of v's label-sorted segment is the
+// neighbour record of segment position lbl_pos[off + j].
+__global__ void label_records_kernel(const int64_t *offset, int32_t nv, const int32_t *lbl_pos, const uint4 *erec,
+                                     int64_t ne, uint4 *lerec) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        // The vertex whose segment holds e: the last v with offset[v] <= e.
+        int32_t lo = 0, hi = nv;
+        while (hi - lo > 1) {
+            const int32_t m = (lo + hi) >> 1;
+            if (offset[m] <= e) lo = m;
+            else hi = m;
+        }
+        lerec[e] = erec[offset[lo] + lbl_pos[e]];
+    }
+}
+
 // Alias walk records: the reference AliasSlot plus both candidates' ranges.
 __global__ void alias_records_kernel(const AliasSlot *alias, const uint2 *vrec, int64_t ne, uint4 *arec) {
     for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
@@ -170,6 +206,16 @@ hipError_t launch_edge_records(const DevGraph &g, uint4 *erec, hipStream_t strea
 hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stream) {
     if (g.ne <= 0) return hipSuccess;
     hipLaunchKernelGGL(alias_records_kernel, dim3(8192), dim3(256), 0, stream, g.alias, g.vrec, g.ne, arec);
+    return hipGetLastError();
+}
+
+hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lerec, hipStream_t stream) {
+    if (g.nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(label_start_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.lbl_sorted,
+                       g.nv, lstart);
+    if (g.ne > 0)
+        hipLaunchKernelGGL(label_records_kernel, dim3(8192), dim3(256), 0, stream, g.offset, g.nv, g.lbl_pos, g.erec,
+                           g.ne, lerec);
     return hipGetLastError();
 }
 

@@ -355,18 +355,37 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         // then dynamic_its_move: the prefix sums are match counts, so the pick is
         // the (floor(real2(u) * K) + 1)-th neighbour whose label matches.
         const int32_t target = P.schema[(s.len - 1) % P.schema_length];
-        const int32_t *ls = g.lbl_sorted + off;
-        const int32_t a = first_ge(ls, d, target);
-        const int32_t b = first_gt(ls, a, d, target);
-        const int32_t K = b - a;
+        int32_t a, K;
+        if (g.lerec) {
+            // Matches of `target` are [lstart[target], lstart[target + 1]) in label order.
+            uint32_t lo = 0u, hi = 0u;
+            if (target >= 0 && target < 8) {
+                const uint32_t *L = g.lstart + 8 * static_cast<int64_t>(cur);
+                lo = L[target];
+                hi = target < 7 ? L[target + 1] : ud;
+            }
+            a = static_cast<int32_t>(lo);
+            K = static_cast<int32_t>(hi - lo);
+        } else {
+            const int32_t *ls = g.lbl_sorted + off;
+            a = first_ge(ls, d, target);
+            K = first_gt(ls, a, d, target) - a;
+        }
         if (K == 0) {
             s.st = kGatherTerminated;
             return false;
         }
         uint32_t u;
         if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
-        const double t = real2(u) * static_cast<double>(K);
-        next = read_neighbor(g, off + g.lbl_pos[off + a + static_cast<int32_t>(t)], noff, ndeg);
+        const int32_t t = static_cast<int32_t>(real2(u) * static_cast<double>(K));
+        if (g.lerec) {
+            const uint4 r = g.lerec[off + a + t];
+            next = static_cast<int32_t>(r.x);
+            noff = r.y;
+            ndeg = r.z;
+        } else {
+            next = read_neighbor(g, off + g.lbl_pos[off + a + t], noff, ndeg);
+        }
         if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
     }
 
@@ -711,10 +730,6 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                         rowp[0] = s.cur;
                     }
                     if (s.st != kAlive) {  // bad source: finished before the first step
-                        if (io.paths) {
-                            const int32_t Lp = P.length > 0 ? P.length : 1;
-                            for (int32_t p = 1; p < Lp; ++p) io.paths[static_cast<int64_t>(w) * io.path_stride + p] = -1;
-                        }
                         write_outputs(io, w, s);
                         classify(s.st, gterm, dead, err);
                     }
@@ -735,8 +750,6 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                 if (io.paths && s.len - 1 < P.length) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
             }
             if (s.st != kAlive) {
-                if (io.paths)
-                    for (int32_t p = s.len; p < P.length; ++p) io.paths[static_cast<int64_t>(w) * io.path_stride + p] = -1;
                 write_outputs(io, w, s);
                 classify(s.st, gterm, dead, err);
             }
@@ -754,13 +767,7 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_init_kernel(DevGraph g, 
     WalkerState s;
     init_walker(g, io, w, s);
     state[w] = s;
-    if (io.paths) {
-        const KindParams &P = w >= split ? B : A;
-        const int32_t Lp = P.length > 0 ? P.length : 1;
-        int32_t *rowp = io.paths + static_cast<int64_t>(w) * io.path_stride;
-        rowp[0] = s.cur;
-        for (int32_t p = 1; p < Lp; ++p) rowp[p] = -1;
-    }
+    if (io.paths) io.paths[static_cast<int64_t>(w) * io.path_stride] = s.cur;
 }
 
 template <int MODE, bool REPLAY>
@@ -885,6 +892,19 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_finish_kernel(BatchIO io
     }
 }
 
+// -1 in entries [0, length) of every path row (length of the walker's kind), written
+// row-contiguously before the QUEUE, STEP and GRAPH forms run, so that their walkers
+// write only the vertices they visit and never pad rows one entry at a time.
+__global__ void fill_paths_kernel(int32_t *paths, int32_t n, int32_t stride, int32_t split, int32_t La, int32_t Lb) {
+    const int32_t Lmax = La > Lb ? La : Lb;
+    const int64_t total = static_cast<int64_t>(n) * Lmax;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t row = static_cast<int32_t>(i / Lmax), col = static_cast<int32_t>(i - static_cast<int64_t>(row) * Lmax);
+        if (col < (row >= split ? Lb : La)) paths[static_cast<int64_t>(row) * stride + col] = -1;
+    }
+}
+
 __global__ void fill_u32_kernel(unsigned int *p, unsigned int value, int64_t n) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = value;
@@ -938,6 +958,15 @@ hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO 
 }
 
 int queue_grid(int device_cus) { return device_cus * 8; }
+
+void fill_paths(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io, hipStream_t stream) {
+    if (!io.paths || io.n <= 0) return;
+    const int32_t La = a.length > 0 ? a.length : 1, Lb = b.length > 0 ? b.length : 1;
+    const int64_t total = static_cast<int64_t>(io.n) * (La > Lb ? La : Lb);
+    const int64_t blocks = std::min<int64_t>(grid_for(total, 256), 256 * 64);
+    hipLaunchKernelGGL(fill_paths_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, io.paths, io.n,
+                       io.path_stride, split, La, Lb);
+}
 
 template <int MODE, bool REPLAY>
 void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
@@ -1005,6 +1034,7 @@ hipError_t launch_walk_queue(const DevGraph &g, const KindParams &a, const KindP
     const int64_t needed = grid_for(io.n, kQueueBlock);
     const unsigned grid = static_cast<unsigned>(needed < queue_grid(cus) ? needed : queue_grid(cus));
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, queue_counter, 0u, int64_t(1));
+    fill_paths(a, b, split, io, stream);
     if (io.replay) queue_dispatch<true>(g, a, b, split, io, queue_counter, stream, grid);
     else queue_dispatch<false>(g, a, b, split, io, queue_counter, stream, grid);
     return hipGetLastError();
@@ -1013,6 +1043,7 @@ hipError_t launch_walk_queue(const DevGraph &g, const KindParams &a, const KindP
 hipError_t launch_walk_step_init(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
                                  const BatchIO &io, WalkerState *state, hipStream_t stream) {
     if (io.n <= 0) return hipSuccess;
+    fill_paths(a, b, split, io, stream);
     const dim3 grid(grid_for(io.n, kStepBlock)), block(kStepBlock);
     if (io.replay) hipLaunchKernelGGL((walk_step_init_kernel<-1, true>), grid, block, 0, stream, g, a, b, split, io, state);
     else hipLaunchKernelGGL((walk_step_init_kernel<-1, false>), grid, block, 0, stream, g, a, b, split, io, state);

@@ -160,8 +160,9 @@ def reference_cpu_baseline(hg, length):
 
 def mix_forms(ctx, dg, deg, args):
     """Uneven PPR + MetaPath mix: 2^23 walkers from uniform random non-isolated sources
-    (generate_random_source), walkers [0, 2^22) PPR with termination 0.15, the rest
-    MetaPath on schema 0,1,2,3,4, both logging up to 80 path entries.  The same batch
+    (generate_random_source), walkers [0, 2^22) PPR with termination 0.15 and length 1
+    (g_para.length_ = 1, ppr.cpp:51: the path row holds the source), the rest MetaPath on
+    schema 0,1,2,3,4 with length 80 (metapath.cpp:48), paths logged.  The same batch
     runs in the persistent work-queue form, the compacted hipGraph step form and the
     plain per-step-launch form; the forms give identical walks."""
     n = 1 << 23
@@ -170,7 +171,7 @@ def mix_forms(ctx, dg, deg, args):
     batch = Batch(ctx, src, 80)
     out = {"walkers": n, "split": split}
     for form in ("queue", "graph", "step"):
-        a = T.make_params("ppr", length=80, stop_probability=0.15, rng="philox", seed=4, form=form)
+        a = T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form=form)
         b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=4, form=form)
         io = batch.io()
         T.walk_mix_async(dg, a, b, split, io)

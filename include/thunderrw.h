@@ -190,6 +190,17 @@ int trw_copy_rejection_max(const trw_graph *g, double *host_out);
 /* Device time of the last trw_build_tables call in milliseconds. */
 int trw_last_build_ms(const trw_graph *g, double *ms);
 
+/* ---- per-graph tuning ----
+ * Launch choices that change speed, never results (no reference counterpart; the
+ * reference fixes its ring sizes at compile time, types.h:17-31).  trw_autotune picks
+ * them by measurement; 0 means the built-in default. */
+#define TRW_TUNE_WALKERS_PER_THREAD 0  /* FUSED uniform/alias walkers per lane when params say 0: 1, 2, 4 */
+#define TRW_TUNE_ITS_TREE_MIN_DEGREE 1 /* ITS: vertices of at least this degree search their 128-byte-node
+                                          tree, the rest binary-search their prefix sums */
+#define TRW_TUNE_QUEUE_CHUNK 2         /* QUEUE: walker indices a wavefront takes per atomic (64..4096) */
+int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value);
+int trw_graph_get_tuning(const trw_graph *g, int32_t knob, int64_t *value);
+
 /* ---- walks ----
  * compute<F>(graph, walkers, f) (creeper.h:689-831) for a built-in app, on a
  * caller stream (hipStream_t as void*; NULL = the context stream).  Asynchronous:

@@ -150,6 +150,50 @@ def test_metapath_label_alphabets_match_oracle(gpu_ctx, n_labels, schema):
         dg.close()
 
 
+def test_tuning_changes_no_walk(gpu_ctx):
+    """ITS with the search tree on every vertex, on hubs only, and off; the queue form with
+    several chunk sizes; the fused form with 1, 2 and 4 walkers per lane: identical replay
+    walks, equal to the oracle's.  Weights with a negative entry build no tree."""
+    g = graphs.hub_graph(5000, 4500, 20000, 71)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        live = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        src = np.concatenate([live, np.full(256, int(np.argmax(g.degree)), np.int32)]).astype(np.int32)
+        K = 4 * 40 + 64
+        draws = T.replay_fill(72, len(src), K)
+        cases = [({"app": "deepwalk", "execution": 1, "sampling": 1, "length": 40, "extra": {}},
+                  [dict(its_tree_min_degree=v) for v in (0, 64, 1 << 30)], ["fused", "queue"]),
+                 ({"app": "deepwalk", "execution": 1, "sampling": 2, "length": 40, "extra": {}},
+                  [dict(walkers_per_thread=v) for v in (1, 2, 4)], ["fused"]),
+                 ({"app": "metapath", "execution": 2, "sampling": 1, "length": 40, "extra": {"-s": "0,1,2,3,4"}},
+                  [dict(queue_chunk=v) for v in (64, 256, 4096)], ["queue"])]
+        for case, knobs, forms in cases:
+            ref = oracle_walk(g, case, src, K, 1, 72, 73)
+            for kn in knobs:
+                dg.set_tuning(**kn)
+                for form in forms:
+                    r = T.walk(dg, case_params(case, form), src, draws, T.replay_fill(73, len(src), 1))
+                    for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                        assert np.array_equal(r[key], ref[key]), (case["app"], kn, form, key)
+            dg.set_tuning(walkers_per_thread=0, its_tree_min_degree=0, queue_chunk=0)
+    finally:
+        dg.close()
+    # Negative weights: no tree; the walk still equals the oracle's (binary search, clamped pick).
+    w = np.array(g.weight)
+    w[::7] *= -0.25
+    gn = graphs.CSR(g.degree, g.adj, w, g.label)
+    dg = dgraph(gpu_ctx, gn)
+    try:
+        case = {"app": "deepwalk", "execution": 1, "sampling": 1, "length": 20, "extra": {}}
+        src = np.nonzero(gn.degree > 0)[0].astype(np.int32)
+        ref = oracle_walk(gn, case, src, 100, 1, 74, 75)
+        r = T.walk(dg, case_params(case, "fused"), src, T.replay_fill(74, len(src), 100))
+        for key in ("paths", "lengths", "status"):
+            assert np.array_equal(r[key], ref[key]), key
+    finally:
+        dg.close()
+
+
 def test_replay_exhaustion_and_bad_sources_are_reported(gpu_ctx):
     g, src = load_golden_graph("small")
     dg = dgraph(gpu_ctx, g)

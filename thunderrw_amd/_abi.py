@@ -15,6 +15,7 @@ RNG_REPLAY, RNG_PHILOX = 0, 1
 FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP, FORM_GRAPH = 0, 1, 2, 3, 4
 WEIGHTS_EDGE, WEIGHTS_UNIT = 0, 1
 WALK_WEIGHTS_APP, WALK_WEIGHTS_EDGE, WALK_WEIGHTS_UNIT = 0, 1, 2
+TUNE_WALKERS_PER_THREAD, TUNE_ITS_TREE_MIN_DEGREE, TUNE_QUEUE_CHUNK = 0, 1, 2
 RESULT_PPR, RESULT_PATHS = 0, 1
 QUERY_PENDING, QUERY_RUNNING, QUERY_DONE, QUERY_FAILED = 0, 1, 2, 3
 WALKER_DONE, WALKER_GATHER_TERMINATED, WALKER_DEAD_END, WALKER_REPLAY_EXHAUSTED, WALKER_TRIAL_CAP, \
@@ -106,6 +107,8 @@ _SIGS = {
     "trw_copy_prefix_sum": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_copy_rejection_max": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_last_build_ms": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_graph_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
+    "trw_graph_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "trw_walk_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "trw_walk_mix_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "trw_walk_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -247,6 +248,8 @@ static void graph_release(trw_graph *g) {
     dfree(g->lbl_pos);
     dfree(g->lstart);
     dfree(g->lerec);
+    dfree(g->its_idx);
+    dfree(g->its_root);
     dfree(g->order);
     dfree(g->vrec);
     dfree(g->erec);
@@ -331,6 +334,8 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
         TRW_HIP(hipStreamSynchronize(ctx->stream), "neighbour records");
     }
     if (edge_weight) {
+        for (size_t e = 0; e < E && g->weights_nonneg; ++e)
+            g->weights_nonneg = edge_weight[e] >= 0.0 && edge_weight[e] < HUGE_VAL;  // NaN fails both
         TRW_HIP(dmalloc(&g->weight, E + 16), "hipMalloc weight");
         if (E) TRW_HIP(hipMemcpy(g->weight, edge_weight, sizeof(double) * E, hipMemcpyHostToDevice), "copy weight");
     }
@@ -375,6 +380,34 @@ int trw_graph_info(const trw_graph *g, int32_t *nv, int64_t *ne, int32_t *max_de
     if (nv) *nv = g->nv;
     if (ne) *ne = g->ne;
     if (max_degree) *max_degree = g->max_degree;
+    return TRW_OK;
+}
+
+// Search trees over every vertex's prefix sums (16-key, 128-byte nodes; see its_levels).
+// The tree's upper bound equals the binary search's only for non-decreasing sums, so
+// weights with a negative or NaN entry get none.
+static int build_its_trees(trw_graph *g, bool monotone) {
+    dfree(g->its_idx);
+    if (!monotone || g->nv == 0) return TRW_OK;
+    hipStream_t st = g->ctx->stream;
+    const size_t V = static_cast<size_t>(g->nv);
+    if (!g->its_root) TRW_HIP(dmalloc(&g->its_root, V), "hipMalloc ITS tree roots");
+    TRW_HIP(trw::launch_its_tree_sizes(g->view(), g->its_root, st), "ITS tree sizes");
+    std::vector<uint32_t> root(V);
+    TRW_HIP(hipMemcpyAsync(root.data(), g->its_root, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, st), "ITS tree sizes");
+    TRW_HIP(hipStreamSynchronize(st), "ITS tree sizes");
+    uint64_t total = 0;
+    for (size_t v = 0; v < V; ++v) {
+        const uint32_t c = root[v];
+        root[v] = c ? static_cast<uint32_t>(total) : trw::kNoRange;
+        total += c;
+    }
+    if (total == 0 || total >= trw::kNoRange) return TRW_OK;  // nothing to index, or roots overflow 32 bits
+    TRW_HIP(hipMemcpyAsync(g->its_root, root.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice, st), "ITS tree roots");
+    TRW_HIP(dmalloc(&g->its_idx, static_cast<size_t>(total) * trw::kItsFan), "hipMalloc ITS trees");
+    TRW_HIP(trw::launch_its_tree_fill(g->view(), g->prefix, g->its_root, g->its_idx, st), "ITS tree fill");
+    TRW_HIP(trw::launch_its_tree_roots_to_records(g->view(), g->its_root, g->erec, st), "ITS tree roots");
+    TRW_HIP(hipStreamSynchronize(st), "ITS trees");  // the host copy of the roots must outlive the upload
     return TRW_OK;
 }
 
@@ -426,11 +459,14 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
             TRW_HIP(hipStreamSynchronize(st), "alias walk records");
         }
     } else if (sampling == TRW_SM_ITS) {
-        if (!g->prefix) TRW_HIP(dmalloc(&g->prefix, E + 1), "hipMalloc prefix sums");
+        // Padded so that a search may read the whole aligned 16-entry chunk holding any entry.
+        if (!g->prefix) TRW_HIP(dmalloc(&g->prefix, E + 2 * trw::kItsFan), "hipMalloc prefix sums");
         (void)hipEventRecord(t0, st);
         hipError_t e = trw::launch_its_build(dv, g->weight, unit, g->order, g->prefix, st);
-        (void)hipEventRecord(t1, st);
         if (e != hipSuccess) return hip_fail(e, "ITS build launch");
+        int rc = build_its_trees(g, unit || g->weights_nonneg);
+        (void)hipEventRecord(t1, st);
+        if (rc != TRW_OK) return rc;
         TRW_HIP(hipStreamSynchronize(st), "ITS build");
         g->its_weights = weights;
     } else {
@@ -444,6 +480,32 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
     }
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess) g->last_build_ms = ms;
+    return TRW_OK;
+}
+
+int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value) {
+    if (!g) return fail(TRW_E_INVALID, "graph is null");
+    switch (knob) {
+    case TRW_TUNE_WALKERS_PER_THREAD:
+        if (value != 0 && value != 1 && value != 2 && value != 4) return fail(TRW_E_INVALID, "walkers per lane: 0, 1, 2 or 4");
+        break;
+    case TRW_TUNE_ITS_TREE_MIN_DEGREE:
+        if (value < 0 || value > INT32_MAX) return fail(TRW_E_INVALID, "degree threshold outside [0, 2^31)");
+        break;
+    case TRW_TUNE_QUEUE_CHUNK:
+        if (value < 0 || value > 4096) return fail(TRW_E_INVALID, "queue chunk outside [0, 4096]");
+        break;
+    default:
+        return fail(TRW_E_INVALID, "unknown tuning knob %d", knob);
+    }
+    g->tune[knob] = value;
+    return TRW_OK;
+}
+
+int trw_graph_get_tuning(const trw_graph *g, int32_t knob, int64_t *value) {
+    if (!g || !value) return fail(TRW_E_INVALID, "null argument");
+    if (knob < 0 || knob >= 3) return fail(TRW_E_INVALID, "unknown tuning knob %d", knob);
+    *value = g->tune[knob];
     return TRW_OK;
 }
 
@@ -504,7 +566,9 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
         return fail(TRW_E_INVALID, "walkers_per_thread must be 0, 1, 2 or 4");
     if (p->weights < TRW_WALK_WEIGHTS_APP || p->weights > TRW_WALK_WEIGHTS_UNIT)
         return fail(TRW_E_INVALID, "unknown weights selector %d", p->weights);
-    k.walkers_per_thread = p->walkers_per_thread;
+    k.walkers_per_thread = p->walkers_per_thread ? p->walkers_per_thread : static_cast<int32_t>(g->tune[TRW_TUNE_WALKERS_PER_THREAD]);
+    k.its_tree_min = static_cast<int32_t>(g->tune[TRW_TUNE_ITS_TREE_MIN_DEGREE]);
+    k.queue_chunk = static_cast<int32_t>(g->tune[TRW_TUNE_QUEUE_CHUNK]);
     k.ppr = ppr;
     k.length = p->length;
     k.max_trials = p->max_trials;

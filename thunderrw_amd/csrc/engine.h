@@ -59,7 +59,40 @@ struct DevGraph {
                                //       with label < k (the start of label k in label order), else null
     const uint4 *lerec;        // ne with lstart: neighbour records in label order,
                                //       lerec[off(v) + j] = erec[off(v) + lbl_pos[off(v) + j]]
+    const double *its_idx;     // ITS search trees (16-key nodes), or null
+    const uint32_t *its_root;  // nv: first node of v's tree in its_idx, kNoRange if v has none
 };
+
+// ITS search tree of one vertex.  Its prefix sums P[off, off + d) split into leaf
+// blocks at the 16-element (128-byte) boundaries of the global prefix array; level 1
+// holds the last prefix value of every leaf block, and each higher level the last key
+// of every 16-key node below, up to a single root node.  Nodes are 16 doubles
+// (+inf padded), stored root level first.  The search makes the comparisons
+// `key <= target` of an upper bound, so for non-decreasing sums it returns exactly
+// what the branchless binary search returns.
+constexpr int kItsFan = 16;
+constexpr int kItsMaxLevels = 8;
+
+__host__ __device__ inline int32_t its_leaf_blocks(int64_t off, int32_t d) {
+    if (d <= 0) return 0;
+    const int64_t first_end = ((off >> 4) + 1) << 4;
+    const int64_t end = off + d;
+    if (end <= first_end) return 1;
+    return 1 + static_cast<int32_t>((end - first_end + 15) >> 4);
+}
+
+// Node counts per level (level 1 first); returns the number of levels (0 for one leaf block).
+__host__ __device__ inline int32_t its_levels(int32_t nb, int64_t *nodes) {
+    if (nb <= 1) return 0;
+    int32_t L = 0;
+    int64_t keys = nb;
+    do {
+        nodes[L] = (keys + kItsFan - 1) / kItsFan;
+        keys = nodes[L];
+        ++L;
+    } while (nodes[L - 1] > 1 && L < kItsMaxLevels);
+    return L;
+}
 
 // Per walker kind launch parameters (app functor + g_para).
 struct KindParams {
@@ -73,6 +106,8 @@ struct KindParams {
     double stop_prob;
     double n2v_p, n2v_q, n2v_max;  // Node2Vec p_, q_, max_ (node2vec.cpp:18-20)
     int32_t schema[16];
+    int32_t its_tree_min;   // ITS: vertices of at least this degree search their tree (when built)
+    int32_t queue_chunk;    // QUEUE: walker indices a wavefront takes per atomic (0 = from the batch size)
 };
 
 // Walker batch (device pointers).
@@ -110,6 +145,7 @@ struct WalkerState {
     uint32_t pdeg;
     uint32_t gid_lo;  // global walker id: the Philox counter's walker words
     uint32_t gid_hi;
+    uint32_t cidx;    // current vertex: root of its ITS search tree (kNoRange: not known / none)
 };
 constexpr uint32_t kNoRange = 0xffffffffu;
 
@@ -159,6 +195,12 @@ hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, 
 hipError_t launch_edge_records(const DevGraph &g, uint4 *erec, hipStream_t stream);
 hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stream);
 hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lerec, hipStream_t stream);
+// ITS search trees: node counts per vertex into root (then the caller turns them into
+// offsets), the trees themselves, and the roots copied into the neighbour records' .w.
+hipError_t launch_its_tree_sizes(const DevGraph &g, uint32_t *root, hipStream_t stream);
+hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint32_t *root, double *idx,
+                                hipStream_t stream);
+hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *root, uint4 *erec, hipStream_t stream);
 hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
                             hipStream_t stream);
 hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,

@@ -30,6 +30,10 @@ struct trw_graph {
     uint4 *arec = nullptr;
     uint32_t *lstart = nullptr;
     uint4 *lerec = nullptr;
+    double *its_idx = nullptr;     // ITS search trees (null: not built, or weights not all >= 0)
+    uint32_t *its_root = nullptr;  // nv tree roots
+    bool weights_nonneg = true;    // every edge weight is finite and >= 0
+    int64_t tune[3] = {0, 0, 0};   // TRW_TUNE_* knobs
     trw::AliasSlot *alias = nullptr;
     double *prefix = nullptr;
     double *rs_max = nullptr;
@@ -59,6 +63,8 @@ struct trw_graph {
         g.arec = arec;
         g.lstart = lstart;
         g.lerec = lerec;
+        g.its_idx = its_idx;
+        g.its_root = its_root;
         return g;
     }
 };

@@ -168,6 +168,69 @@ __global__ void label_records_kernel(const int64_t *offset, int32_t nv, const in
     }
 }
 
+// ITS search trees.  One thread per vertex; hubs write a few thousand keys once.
+__global__ void its_tree_sizes_kernel(const int64_t *offset, int32_t nv, uint32_t *count) {
+    const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int64_t off = offset[v];
+    int64_t nodes[kItsMaxLevels];
+    const int32_t L = its_levels(its_leaf_blocks(off, static_cast<int32_t>(offset[v + 1] - off)), nodes);
+    int64_t total = 0;
+    for (int32_t k = 0; k < L; ++k) total += nodes[k];
+    count[v] = static_cast<uint32_t>(total);
+}
+
+__global__ void its_tree_fill_kernel(const int64_t *offset, int32_t nv, const double *P, const uint32_t *root,
+                                     double *idx) {
+    const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= nv || root[v] == kNoRange) return;
+    const int64_t off = offset[v];
+    const int32_t d = static_cast<int32_t>(offset[v + 1] - off);
+    const int32_t nb = its_leaf_blocks(off, d);
+    int64_t nodes[kItsMaxLevels];
+    const int32_t L = its_levels(nb, nodes);
+    // Level k (0 = level 1) starts after the levels above it.
+    int64_t base[kItsMaxLevels];
+    int64_t acc = 0;
+    for (int32_t k = L - 1; k >= 0; --k) {
+        base[k] = acc;
+        acc += nodes[k];
+    }
+    double *tree = idx + static_cast<int64_t>(root[v]) * kItsFan;
+    const int64_t first_end = ((off >> 4) + 1) << 4;
+    // Level 1: the last prefix value of every leaf block.
+    double *l1 = tree + base[0] * kItsFan;
+    for (int64_t i = 0; i < nodes[0] * kItsFan; ++i) {
+        double key = __longlong_as_double(0x7ff0000000000000LL);  // +inf padding
+        if (i < nb) {
+            const int64_t end = (i == 0) ? first_end : first_end + kItsFan * i;
+            key = P[(end < off + d ? end : off + d) - 1];
+        }
+        l1[i] = key;
+    }
+    // Higher levels: the last key of every node below.
+    int64_t keys_below = nb;
+    for (int32_t k = 1; k < L; ++k) {
+        const double *below = tree + base[k - 1] * kItsFan;
+        double *lk = tree + base[k] * kItsFan;
+        for (int64_t i = 0; i < nodes[k] * kItsFan; ++i) {
+            double key = __longlong_as_double(0x7ff0000000000000LL);
+            if (i < nodes[k - 1]) {
+                const int64_t last = (i + 1) * kItsFan - 1;
+                key = below[last < keys_below ? last : keys_below - 1];
+            }
+            lk[i] = key;
+        }
+        keys_below = nodes[k - 1];
+    }
+}
+
+__global__ void its_tree_roots_kernel(const int32_t *adj, int64_t ne, const uint32_t *root, uint4 *erec) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        erec[e].w = root[adj[e]];
+}
+
 // Alias walk records: the reference AliasSlot plus both candidates' ranges.
 __global__ void alias_records_kernel(const AliasSlot *alias, const uint2 *vrec, int64_t ne, uint4 *arec) {
     for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
@@ -216,6 +279,26 @@ hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lere
     if (g.ne > 0)
         hipLaunchKernelGGL(label_records_kernel, dim3(8192), dim3(256), 0, stream, g.offset, g.nv, g.lbl_pos, g.erec,
                            g.ne, lerec);
+    return hipGetLastError();
+}
+
+hipError_t launch_its_tree_sizes(const DevGraph &g, uint32_t *root, hipStream_t stream) {
+    if (g.nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(its_tree_sizes_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv, root);
+    return hipGetLastError();
+}
+
+hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint32_t *root, double *idx,
+                                hipStream_t stream) {
+    if (g.nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(its_tree_fill_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv, prefix,
+                       root, idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *root, uint4 *erec, hipStream_t stream) {
+    if (g.ne <= 0 || !erec) return hipSuccess;
+    hipLaunchKernelGGL(its_tree_roots_kernel, dim3(8192), dim3(256), 0, stream, g.adj, g.ne, root, erec);
     return hipGetLastError();
 }
 

@@ -207,6 +207,65 @@ __device__ __forceinline__ int32_t read_neighbor(const DevGraph &g, int64_t e, u
     return g.adj[e];
 }
 
+// As read_neighbor, also returning the neighbour's ITS tree root (record word .w).
+__device__ __forceinline__ int32_t read_neighbor_its(const DevGraph &g, int64_t e, uint32_t &noff, uint32_t &ndeg,
+                                                     uint32_t &nidx) {
+    if (g.erec) {
+        const uint4 r = g.erec[e];
+        noff = r.y;
+        ndeg = r.z;
+        nidx = g.its_idx ? r.w : kNoRange;
+        return static_cast<int32_t>(r.x);
+    }
+    noff = 0u;
+    ndeg = kNoRange;
+    nidx = kNoRange;
+    return g.adj[e];
+}
+
+// Number of the 16 keys of a 128-byte tree node that are <= target.
+__device__ __forceinline__ int32_t node_count_le(const double *node, double target) {
+    const double2 *n2 = reinterpret_cast<const double2 *>(node);
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < kItsFan / 2; ++i) {
+        const double2 k = n2[i];
+        c += (k.x <= target) + (k.y <= target);
+    }
+    return c;
+}
+
+// Upper bound of target in the prefix sums P[off, off + d) through the vertex's search
+// tree (its root node has already been read into `root_node`): the count of entries
+// <= target, d when the target is at least the total.
+__device__ __forceinline__ int32_t its_tree_count(const DevGraph &g, int64_t off, int32_t d, uint32_t root,
+                                                  const double *root_node, int32_t L, const int64_t *nodes,
+                                                  int32_t root_keys, double target) {
+    int64_t node = node_count_le(root_node, target);
+    if (node >= root_keys) return d;
+    int64_t base = 1;  // levels below the root start after its single node
+    for (int32_t k = L - 2; k >= 0; --k) {
+        const double *N = g.its_idx + (static_cast<int64_t>(root) + base + node) * kItsFan;
+        node = node * kItsFan + node_count_le(N, target);
+        base += nodes[k];
+    }
+    // Leaf block `node`: one aligned 16-entry chunk of the prefix array.
+    const int64_t first_end = ((off >> 4) + 1) << 4;
+    const int64_t start = node == 0 ? off : first_end + kItsFan * (node - 1);
+    const int64_t lim = node == 0 ? first_end : start + kItsFan;
+    const int64_t end = lim < off + d ? lim : off + d;
+    const int64_t chunk = (start >> 4) << 4;
+    const double2 *C = reinterpret_cast<const double2 *>(g.prefix + chunk);
+    int32_t cnt = 0;
+#pragma unroll
+    for (int i = 0; i < kItsFan / 2; ++i) {
+        const double2 v = C[i];
+        const int64_t j = chunk + 2 * i;
+        cnt += (j >= start && j < end && v.x <= target) + (j + 1 >= start && j + 1 < end && v.y <= target);
+    }
+    return static_cast<int32_t>(start - off) + cnt;
+}
+
 // Alias slot e (alias_sampling.h:147-172 pick).  Returns false for a vertex
 // whose weights sum to zero.
 __device__ __forceinline__ bool read_alias(const DevGraph &g, int64_t e, uint32_t u1, int32_t &next, uint32_t &noff,
@@ -232,13 +291,15 @@ __device__ __forceinline__ bool read_alias(const DevGraph &g, int64_t e, uint32_
 
 // Update (creeper.h:298-334) after a move from `cur` (range off/d) to `next`.
 __device__ __forceinline__ void finish_move(const DevGraph &g, const KindParams &P, WalkerState &s, int32_t cur,
-                                            int64_t off, int32_t d, int32_t next, uint32_t noff, uint32_t ndeg) {
+                                            int64_t off, int32_t d, int32_t next, uint32_t noff, uint32_t ndeg,
+                                            uint32_t nidx = kNoRange) {
     s.prev = cur;
     s.poff = static_cast<uint32_t>(off);
     s.pdeg = g.vrec ? static_cast<uint32_t>(d) : kNoRange;
     s.cur = next;
     s.coff = noff;
     s.cdeg = ndeg;
+    s.cidx = nidx;
     s.len += 1;
     (void)P;
 }
@@ -262,7 +323,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     const uint32_t step = static_cast<uint32_t>(s.len - 1);
     const uint32_t ud = static_cast<uint32_t>(d);
     int32_t next;
-    uint32_t noff = 0u, ndeg = kNoRange;
+    uint32_t noff = 0u, ndeg = kNoRange, nidx = kNoRange;
     uint4 nblk = blk;
 
     if (MODE == kModeUniform) {
@@ -284,19 +345,40 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         }
     } else if (MODE == kModeIts) {
         // inverse_transformation_sampling.h:259-286
-        const double *P_ = g.prefix + off;
-        const double total = P_[d - 1];
-        if (total == 0.0) {
-            s.st = kGatherTerminated;
-            return false;
+        uint32_t root = kNoRange;
+        if (g.its_idx && d >= P.its_tree_min) root = s.cidx != kNoRange ? s.cidx : g.its_root[cur];
+        int32_t pos;
+        if (root != kNoRange) {
+            // Search tree: the root's last key is the total, P[d - 1].
+            int64_t nodes[kItsMaxLevels];
+            const int32_t nb = its_leaf_blocks(off, d);
+            const int32_t L = its_levels(nb, nodes);
+            const int32_t root_keys = L == 1 ? nb : static_cast<int32_t>(nodes[L - 2]);
+            const double *R = g.its_idx + static_cast<int64_t>(root) * kItsFan;
+            const double total = R[root_keys - 1];
+            if (total == 0.0) {
+                s.st = kGatherTerminated;
+                return false;
+            }
+            uint32_t u;
+            if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
+            if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+            pos = min(its_tree_count(g, off, d, root, R, L, nodes, root_keys, real2(u) * total), d - 1);
+        } else {
+            const double *P_ = g.prefix + off;
+            const double total = P_[d - 1];
+            if (total == 0.0) {
+                s.st = kGatherTerminated;
+                return false;
+            }
+            uint32_t u;
+            if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
+            if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+            // The count is < d whenever the prefix sums are non-decreasing (non-negative
+            // weights); the clamp only keeps negative-weight inputs inside the segment.
+            pos = min(count_le(P_, d, real2(u) * total), d - 1);
         }
-        uint32_t u;
-        if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
-        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
-        // The count is < d whenever the prefix sums are non-decreasing (non-negative
-        // weights); the clamp only keeps negative-weight inputs inside the segment.
-        const int32_t pos = min(count_le(P_, d, real2(u) * total), d - 1);
-        next = read_neighbor(g, off + pos, noff, ndeg);
+        next = read_neighbor_its(g, off + pos, noff, ndeg, nidx);
     } else if (MODE == kModeRejection) {
         // rejection_sampling.h:334-366: redraw (u % d, real2 * max) while r > w[pos].
         const double m = g.rs_max[cur];
@@ -389,7 +471,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
     }
 
-    finish_move(g, P, s, cur, off, d, next, noff, ndeg);
+    finish_move(g, P, s, cur, off, d, next, noff, ndeg, nidx);
     const uint4 this_blk = blk;
     blk = nblk;
     if (P.ppr) {
@@ -461,7 +543,7 @@ __device__ __forceinline__ void write_outputs(const BatchIO &io, int32_t w, cons
 }
 
 __device__ __forceinline__ WalkerState empty_walker() {
-    return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange, 0u, 0u};
+    return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange, 0u, 0u, kNoRange};
 }
 
 __device__ __forceinline__ void init_walker(const DevGraph &g, const BatchIO &io, int32_t w, WalkerState &s) {
@@ -971,10 +1053,11 @@ void fill_paths(const KindParams &a, const KindParams &b, int32_t split, const B
 template <int MODE, bool REPLAY>
 void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                   unsigned int *q, hipStream_t stream, unsigned grid) {
-    // About eight chunks per wavefront of the grid, between 64 and 1024 walkers each.
+    // About eight chunks per wavefront of the grid, between 64 and 1024 walkers each,
+    // unless the graph's tuning fixes it.
     const int64_t waves = static_cast<int64_t>(grid) * (kQueueBlock / 64);
-    int64_t chunk = static_cast<int64_t>(io.n) / (waves * 8);
-    chunk = chunk < 64 ? 64 : (chunk > 1024 ? 1024 : chunk);
+    int64_t chunk = a.queue_chunk > 0 ? a.queue_chunk : static_cast<int64_t>(io.n) / (waves * 8);
+    chunk = chunk < 64 ? 64 : (chunk > 4096 ? 4096 : chunk);
     hipLaunchKernelGGL((walk_queue_kernel<MODE, REPLAY>), dim3(grid), dim3(kQueueBlock), 0, stream, g, a, b, split, io, q,
                        static_cast<unsigned int>(chunk));
 }

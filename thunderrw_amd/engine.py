@@ -155,6 +155,22 @@ class DeviceGraph:
         check(lib().trw_last_build_ms(self.h, C.byref(ms)))
         return ms.value
 
+    TUNING = {"walkers_per_thread": A.TUNE_WALKERS_PER_THREAD, "its_tree_min_degree": A.TUNE_ITS_TREE_MIN_DEGREE,
+              "queue_chunk": A.TUNE_QUEUE_CHUNK}
+
+    def set_tuning(self, **knobs):
+        """Launch choices that change speed, never results (trw_graph_set_tuning)."""
+        for k, v in knobs.items():
+            check(lib().trw_graph_set_tuning(self.h, self.TUNING[k], int(v)))
+
+    def tuning(self):
+        out = {}
+        for k, knob in self.TUNING.items():
+            v = C.c_int64()
+            check(lib().trw_graph_get_tuning(self.h, knob, C.byref(v)))
+            out[k] = v.value
+        return out
+
     def alias_table(self):
         out = np.zeros(self.num_edges, dtype=ALIAS_DTYPE)
         check(lib().trw_copy_alias_table(self.h, _ptr(out)))

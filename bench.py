@@ -63,8 +63,8 @@ def parse_args():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline measurement only")
-    ap.add_argument("--qps", type=float, default=4000.0, help="offered load of the PPR serving run")
-    ap.add_argument("--queries", type=int, default=4000, help="queries in the PPR serving run")
+    ap.add_argument("--qps", type=float, default=10000.0, help="offered load of the PPR serving run")
+    ap.add_argument("--queries", type=int, default=10000, help="queries in the PPR serving run")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also launch the copy kernel over this many MiB (byte-counter calibration)")
@@ -214,7 +214,7 @@ def serving(ctx, args):
                         rng="philox", seed=5)
     out = {"graph": "chung-lu 2^24 V, avg deg 16, gamma 2.1, dmin 2, dmax 65536, FP64 U[1,5) weights",
            "queries": args.queries, "offered_qps": args.qps, "alias_table_build_ms": round(dg.last_build_ms(), 3)}
-    with T.Server(dg, streams=4, completion_threads=8) as srv:
+    with T.Server(dg, streams=4, completion_threads=4) as srv:
         qs = []
         for i in range(args.queries):
             k = int(np.exp(rng.uniform(0.0, np.log(4096.0))))

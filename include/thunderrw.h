@@ -201,6 +201,22 @@ int trw_last_build_ms(const trw_graph *g, double *ms);
 int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value);
 int trw_graph_get_tuning(const trw_graph *g, int32_t knob, int64_t *value);
 
+/* Autotuner: for the walk `params` describes (rng must be PHILOX), times a sample of
+ * sample_walkers walkers from random non-isolated sources (seeded) for each candidate
+ * of every knob that applies -- walkers per lane for FUSED uniform/alias, the ITS tree
+ * degree threshold, the QUEUE chunk -- one knob at a time, and keeps the fastest on the
+ * graph.  Walks are identical under every setting, so tuning never changes results. */
+typedef struct {
+    int32_t knobs;                 /* entries used below */
+    int32_t knob[3];               /* TRW_TUNE_* */
+    int64_t chosen[3];
+    int32_t candidates[3];         /* values tried per knob */
+    double default_steps_per_s;    /* all knobs at their defaults */
+    double tuned_steps_per_s;
+} trw_tune_report;
+int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_walkers, uint64_t seed,
+                 int32_t with_paths, trw_tune_report *report);
+
 /* ---- walks ----
  * compute<F>(graph, walkers, f) (creeper.h:689-831) for a built-in app, on a
  * caller stream (hipStream_t as void*; NULL = the context stream).  Asynchronous:
@@ -345,8 +361,9 @@ int trw_server_query_info(trw_server *s, uint64_t query_id, trw_query_info *info
 /* TRW_RESULT_PPR: the distinct end vertices ascending and their walker counts
  * (capacity >= info.ppr_entries). */
 int trw_server_ppr_result(trw_server *s, uint64_t query_id, int32_t *vertices, uint32_t *counts, int64_t capacity);
-/* Per-walker outputs of a done query (nullable): paths (TRW_RESULT_PATHS only,
- * n_walkers x max(length,1), -1 padded), lengths, ends, status. */
+/* Per-walker outputs of a done TRW_RESULT_PATHS query (nullable): paths (n_walkers x
+ * max(length,1), -1 padded), lengths, ends, status.  A PPR query's batch is reduced on
+ * the device to end-vertex counts, so it keeps only those (trw_server_ppr_result). */
 int trw_server_walk_result(trw_server *s, uint64_t query_id, int32_t *paths, int32_t *lengths, int32_t *ends,
                            int32_t *status);
 int trw_server_release(trw_server *s, uint64_t query_id);

@@ -34,7 +34,8 @@ def test_library_exports_every_declared_function():
 
 STRUCTS = {"trw_walk_params": A.WalkParams, "trw_walk_io": A.WalkIO, "trw_alias_slot": A.AliasSlot,
            "trw_server_config": A.ServerConfig, "trw_query": A.Query, "trw_query_info": A.QueryInfo,
-           "trw_server_stats": A.ServerStats, "trw_open_loop_report": A.OpenLoopReport}
+           "trw_server_stats": A.ServerStats, "trw_open_loop_report": A.OpenLoopReport,
+           "trw_tune_report": A.TuneReport}
 
 
 def test_struct_layouts_match_the_header():

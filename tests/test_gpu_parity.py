@@ -194,6 +194,26 @@ def test_tuning_changes_no_walk(gpu_ctx):
         dg.close()
 
 
+def test_autotune_picks_knobs_and_keeps_walks(gpu_ctx):
+    g = graphs.chung_lu(30000, 200000, 2.1, 2, 4000, 77)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        src = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        for app, kw, knob in [("deepwalk", dict(execution="static", sampling="its", length=30), "its_tree_min_degree"),
+                              ("deepwalk", dict(execution="static", sampling="alias", length=30), "walkers_per_thread"),
+                              ("ppr", dict(stop_probability=0.2), "queue_chunk")]:
+            prm = T.make_params(app, rng="philox", seed=3, **kw)
+            before = T.walk(dg, prm, src)
+            rep = dg.autotune(prm, sample_walkers=1 << 16, seed=1)
+            assert knob in rep["chosen"] and rep["tuned_steps_per_s"] >= rep["default_steps_per_s"] > 0
+            assert dg.tuning()[knob] == rep["chosen"][knob]
+            after = T.walk(dg, prm, src)
+            for key in ("paths", "lengths", "ends", "status"):
+                assert np.array_equal(before[key], after[key]), (app, key)
+    finally:
+        dg.close()
+
+
 def test_replay_exhaustion_and_bad_sources_are_reported(gpu_ctx):
     g, src = load_golden_graph("small")
     dg = dgraph(gpu_ctx, g)

@@ -51,6 +51,11 @@ class WalkIO(C.Structure):
                 ("counters", C.c_void_p), ("walker_ids", C.c_void_p)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("knobs", C.c_int32), ("knob", C.c_int32 * 3), ("chosen", C.c_int64 * 3),
+                ("candidates", C.c_int32 * 3), ("default_steps_per_s", C.c_double), ("tuned_steps_per_s", C.c_double)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("streams", C.c_int32), ("max_batch_walkers", C.c_int32), ("path_bytes_per_batch", C.c_int64),
                 ("max_pending_queries", C.c_int32), ("completion_threads", C.c_int32),
@@ -109,6 +114,7 @@ _SIGS = {
     "trw_last_build_ms": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_graph_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "trw_graph_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "trw_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_int32, C.c_void_p]),
     "trw_walk_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "trw_walk_mix_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "trw_walk_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,

@@ -20,7 +20,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libthunderrw.so")
 ARCH = os.environ.get("TRW_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["walk.hip", "tables.hip", "probe.hip", "serve.hip", "api.cpp", "server.cpp", "sfmt.cpp", "hostgraph.cpp"]
+SOURCES = ["walk.hip", "tables.hip", "probe.hip", "serve.hip", "api.cpp", "server.cpp", "autotune.cpp", "sfmt.cpp",
+           "hostgraph.cpp"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"), "-Wall"]
 HIP_ONLY = ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument"]

@@ -184,6 +184,14 @@ struct ServeSegment {
 };
 hipError_t launch_serve_expand(const ServeSegment *segs, int32_t n_segs, const int32_t *pool, int32_t n,
                                int32_t *sources, uint64_t *ids, hipStream_t stream);
+// PPR batch aggregation on the device: sorted (segment << 32 | end vertex) runs with
+// walker counts (end vertex 0xffffffff: walkers that ended in error) and per-segment steps.
+hipError_t serve_ppr_temp_bytes(int32_t max_walkers, int32_t max_segments, size_t *bytes);
+hipError_t launch_serve_ppr_reduce(const ServeSegment *segs, const int32_t *seg_offsets, int32_t n_segs, int32_t n,
+                                   const int32_t *ends, const int32_t *lengths, const int32_t *status,
+                                   uint64_t *keys_in, uint64_t *keys_out, uint64_t *walker_steps, uint64_t *run_keys,
+                                   uint32_t *run_counts, uint32_t *num_runs, uint64_t *seg_steps, void *temp,
+                                   size_t temp_bytes, hipStream_t stream);
 hipError_t launch_copy_u128(uint4 *dst, const uint4 *src, int64_t n, hipStream_t stream);
 
 hipError_t run_gather_probe(uint64_t table_bytes, int32_t chains, int32_t steps, int32_t cus, hipStream_t stream,

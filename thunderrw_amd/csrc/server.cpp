@@ -11,9 +11,12 @@
 //     expanded on the device into per-walker start vertices and Philox ids, walks
 //     on the slot's own stream, and its per-walker outputs come back into the
 //     slot's pinned staging;
+//   * a PPR batch is reduced on the device to each segment's sorted end vertices
+//     with walker counts (radix sort + run-length encoding) and its step count, so
+//     only those runs come back; a walk batch returns its per-walker outputs;
 //   * the dispatcher polls the slots' completion events; a finished slot is handed
-//     to a completion thread, which copies each segment into its query, builds the
-//     PPR end-vertex counts of queries that are complete, and frees the slot.
+//     to a completion thread, which hands each segment's results to its query
+//     (merging the runs of a query split over several batches) and frees the slot.
 // Batches on different slots overlap on the device.  Each walker's random stream is
 // keyed by (seed, walker_id_base + index in its query), so a result does not depend
 // on how queries were batched.
@@ -61,7 +64,8 @@ struct Query {
     // query builds its counts.
     std::atomic<int32_t> done_walkers{0};
     std::atomic<uint64_t> steps{0}, errors{0};
-    std::vector<int32_t> ends, lengths, status, paths;
+    std::vector<int32_t> ends, lengths, status, paths;  // TRW_RESULT_PATHS queries
+    std::mutex merge_mu;                                // PPR runs of a query split over batches
     std::vector<int32_t> ppr_vertices;
     std::vector<uint32_t> ppr_counts;
     // Guarded by the server mutex.
@@ -86,21 +90,27 @@ bool same_walk(const Query &a, const Query &b) {
     return std::equal(x.schema, x.schema + std::max(0, std::min(x.schema_length, 16)), y.schema);
 }
 
-// LSD radix sort of vertex ids in 11-bit digits.
-void radix_sort(std::vector<uint32_t> &a, std::vector<uint32_t> &tmp, int bits) {
-    tmp.resize(a.size());
-    for (int shift = 0; shift < bits; shift += 11) {
-        uint32_t count[2048] = {0};
-        for (uint32_t x : a) ++count[(x >> shift) & 2047u];
-        uint32_t sum = 0;
-        for (uint32_t &c : count) {
-            const uint32_t t = c;
-            c = sum;
-            sum += t;
+// Adds the ascending (vertex, count) runs b into the ascending runs a.
+void merge_runs(std::vector<int32_t> &av, std::vector<uint32_t> &ac, const int32_t *bv, const uint32_t *bc, size_t nb) {
+    std::vector<int32_t> v;
+    std::vector<uint32_t> c;
+    v.reserve(av.size() + nb);
+    c.reserve(av.size() + nb);
+    size_t i = 0, j = 0;
+    while (i < av.size() || j < nb) {
+        if (j == nb || (i < av.size() && av[i] < bv[j])) {
+            v.push_back(av[i]);
+            c.push_back(ac[i++]);
+        } else if (i == av.size() || bv[j] < av[i]) {
+            v.push_back(bv[j]);
+            c.push_back(bc[j++]);
+        } else {
+            v.push_back(av[i]);
+            c.push_back(ac[i++] + bc[j++]);
         }
-        for (uint32_t x : a) tmp[count[(x >> shift) & 2047u]++] = x;
-        a.swap(tmp);
     }
+    av.swap(v);
+    ac.swap(c);
 }
 
 struct Segment {
@@ -123,10 +133,19 @@ struct Slot {
     uint64_t *d_counters = nullptr;
     unsigned int *d_queue_head = nullptr;
     int32_t *h_ends = nullptr, *h_lengths = nullptr, *h_status = nullptr, *h_paths = nullptr;
+    // PPR aggregation (d_ids doubles as the unsorted keys once the walk has read them).
+    int32_t *d_seg_off = nullptr, *h_seg_off = nullptr;
+    uint64_t *d_keys_out = nullptr, *d_wsteps = nullptr, *d_run_keys = nullptr, *d_seg_steps = nullptr;
+    uint32_t *d_run_counts = nullptr, *d_num_runs = nullptr;
+    void *d_temp = nullptr;
+    size_t temp_bytes = 0;
+    uint64_t *h_run_keys = nullptr, *h_seg_steps = nullptr;
+    uint32_t *h_run_counts = nullptr, *h_num_runs = nullptr;
     std::vector<Segment> segs;
     int32_t n = 0;
     int32_t path_stride = 0;
     int32_t pool_used = 0;
+    bool ppr = false;  // the batch's queries want end-vertex counts
     SlotState state = SlotState::Free;
 };
 
@@ -184,7 +203,6 @@ struct trw_server {
     int64_t path_ints = 0;
     int32_t max_pending = 0;
     int32_t max_segments = 0;
-    int vertex_bits = 1;
     std::vector<Slot> slots;
 
     std::mutex mu;
@@ -250,6 +268,7 @@ bool trw_server::form_batch(Slot &sl, std::unique_lock<std::mutex> &) {
     sl.n = n;
     sl.path_stride = L;
     sl.pool_used = pool;
+    sl.ppr = head->result == TRW_RESULT_PPR;
     ++stats.batches;
     batch_walkers += static_cast<uint64_t>(n);
     batch_queries += sl.segs.size();
@@ -271,9 +290,12 @@ void trw_server::launch(Slot &sl) {
         d.pad = 0;
         d.id_base = s.q->id_base;
         sl.h_segs[i] = d;
+        sl.h_seg_off[i] = s.batch_offset;
         std::memcpy(sl.h_pool + pool, s.q->sources.data(), sizeof(int32_t) * s.q->sources.size());
         pool += d.n_sources;
     }
+    const int32_t nseg = static_cast<int32_t>(sl.segs.size());
+    sl.h_seg_off[nseg] = sl.n;
     const trw_walk_params params = sl.segs.front().q->params;
     const size_t N = static_cast<size_t>(sl.n);
     hipStream_t st = sl.stream;
@@ -282,6 +304,9 @@ void trw_server::launch(Slot &sl) {
                                   hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(sl.d_pool, sl.h_pool, sizeof(int32_t) * static_cast<size_t>(pool), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && sl.ppr)
+        e = hipMemcpyAsync(sl.d_seg_off, sl.h_seg_off, sizeof(int32_t) * static_cast<size_t>(nseg + 1),
+                           hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = trw::launch_serve_expand(sl.d_segs, static_cast<int32_t>(sl.segs.size()), sl.d_pool, sl.n, sl.d_sources,
                                      sl.d_ids, st);
@@ -304,12 +329,31 @@ void trw_server::launch(Slot &sl) {
             msg = trw_last_error();
         }
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(sl.h_ends, sl.d_ends, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(sl.h_lengths, sl.d_lengths, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(sl.h_status, sl.d_status, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && sl.path_stride > 0)
-        e = hipMemcpyAsync(sl.h_paths, sl.d_paths, sizeof(int32_t) * N * static_cast<size_t>(sl.path_stride),
-                           hipMemcpyDeviceToHost, st);
+    if (sl.ppr) {
+        if (e == hipSuccess)
+            e = trw::launch_serve_ppr_reduce(sl.d_segs, sl.d_seg_off, nseg, sl.n, sl.d_ends, sl.d_lengths, sl.d_status,
+                                             sl.d_ids, sl.d_keys_out, sl.d_wsteps, sl.d_run_keys, sl.d_run_counts,
+                                             sl.d_num_runs, sl.d_seg_steps, sl.d_temp, sl.temp_bytes, st);
+        // Runs are at most one per walker; the copy is bounded by the batch, the drain reads num_runs.
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.h_num_runs, sl.d_num_runs, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.h_run_keys, sl.d_run_keys, sizeof(uint64_t) * N, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.h_run_counts, sl.d_run_counts, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.h_seg_steps, sl.d_seg_steps, sizeof(uint64_t) * static_cast<size_t>(nseg),
+                               hipMemcpyDeviceToHost, st);
+    } else {
+        if (e == hipSuccess) e = hipMemcpyAsync(sl.h_ends, sl.d_ends, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.h_lengths, sl.d_lengths, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(sl.h_status, sl.d_status, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && sl.path_stride > 0)
+            e = hipMemcpyAsync(sl.h_paths, sl.d_paths, sizeof(int32_t) * N * static_cast<size_t>(sl.path_stride),
+                               hipMemcpyDeviceToHost, st);
+    }
     if (e == hipSuccess) e = hipEventRecord(sl.done, st);
     if (e != hipSuccess) {
         // Let whatever was queued finish before the staging buffers are reused.
@@ -379,9 +423,36 @@ void trw_server::dispatch_loop() {
 
 void trw_server::drain(Slot *sl, hipError_t err, std::string msg) {
     const bool ok = err == hipSuccess && msg.empty();
-    for (const Segment &s : sl->segs) {
+    const uint32_t runs = ok && sl->ppr ? *sl->h_num_runs : 0u;
+    uint32_t r = 0;  // runs are sorted by segment, then vertex
+    for (size_t j = 0; j < sl->segs.size(); ++j) {
+        const Segment &s = sl->segs[j];
         Query &q = *s.q;
-        if (ok) {
+        if (ok && sl->ppr) {
+            std::vector<int32_t> v;
+            std::vector<uint32_t> c;
+            uint64_t errors = 0;
+            for (; r < runs && (sl->h_run_keys[r] >> 32) == j; ++r) {
+                const uint32_t vert = static_cast<uint32_t>(sl->h_run_keys[r]);
+                if (vert == 0xffffffffu) {
+                    errors += sl->h_run_counts[r];
+                } else {
+                    v.push_back(static_cast<int32_t>(vert));
+                    c.push_back(sl->h_run_counts[r]);
+                }
+            }
+            {
+                std::lock_guard<std::mutex> lk(q.merge_mu);
+                if (q.ppr_vertices.empty()) {
+                    q.ppr_vertices.swap(v);
+                    q.ppr_counts.swap(c);
+                } else {
+                    merge_runs(q.ppr_vertices, q.ppr_counts, v.data(), c.data(), v.size());
+                }
+            }
+            q.steps += sl->h_seg_steps[j];
+            q.errors += errors;
+        } else if (ok) {
             const size_t o = static_cast<size_t>(s.batch_offset), f = static_cast<size_t>(s.first),
                          c = static_cast<size_t>(s.count);
             std::memcpy(q.ends.data() + f, sl->h_ends + o, sizeof(int32_t) * c);
@@ -421,20 +492,6 @@ void trw_server::finalize(const std::shared_ptr<Query> &qp) {
     {
         std::lock_guard<std::mutex> lk(mu);
         failed = q.error_code != TRW_OK;
-    }
-    if (!failed && q.result == TRW_RESULT_PPR) {
-        std::vector<uint32_t> v, tmp;
-        v.reserve(q.ends.size());
-        for (size_t i = 0; i < q.ends.size(); ++i)
-            if (q.status[i] < TRW_WALKER_REPLAY_EXHAUSTED) v.push_back(static_cast<uint32_t>(q.ends[i]));
-        radix_sort(v, tmp, vertex_bits);
-        for (size_t i = 0; i < v.size();) {
-            size_t j = i + 1;
-            while (j < v.size() && v[j] == v[i]) ++j;
-            q.ppr_vertices.push_back(static_cast<int32_t>(v[i]));
-            q.ppr_counts.push_back(static_cast<uint32_t>(j - i));
-            i = j;
-        }
     }
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -479,10 +536,12 @@ int trw_server::submit(const trw_query *q, Clock::time_point arrival, uint64_t *
     qp->id_base = q->walker_id_base;
     qp->sources.assign(q->sources, q->sources + std::max(q->n_sources, 0));
     const size_t N = static_cast<size_t>(q->n_walkers);
-    qp->ends.resize(N);
-    qp->lengths.resize(N);
-    qp->status.resize(N);
-    if (q->result == TRW_RESULT_PATHS) qp->paths.resize(N * static_cast<size_t>(L));
+    if (q->result == TRW_RESULT_PATHS) {
+        qp->ends.resize(N);
+        qp->lengths.resize(N);
+        qp->status.resize(N);
+        qp->paths.resize(N * static_cast<size_t>(L));
+    }
     qp->arrival = arrival;
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -526,6 +585,19 @@ void trw_server::free_slots() {
         (void)hipHostFree(sl.h_lengths);
         (void)hipHostFree(sl.h_status);
         (void)hipHostFree(sl.h_paths);
+        (void)hipFree(sl.d_seg_off);
+        (void)hipFree(sl.d_keys_out);
+        (void)hipFree(sl.d_wsteps);
+        (void)hipFree(sl.d_run_keys);
+        (void)hipFree(sl.d_seg_steps);
+        (void)hipFree(sl.d_run_counts);
+        (void)hipFree(sl.d_num_runs);
+        (void)hipFree(sl.d_temp);
+        (void)hipHostFree(sl.h_seg_off);
+        (void)hipHostFree(sl.h_run_keys);
+        (void)hipHostFree(sl.h_seg_steps);
+        (void)hipHostFree(sl.h_run_counts);
+        (void)hipHostFree(sl.h_num_runs);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         sl = Slot();
@@ -592,7 +664,6 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
     s->path_ints = c.path_bytes_per_batch / 4;
     s->max_pending = c.max_pending_queries;
     s->max_segments = c.max_queries_per_batch;
-    while (s->vertex_bits < 31 && (int64_t(1) << s->vertex_bits) < g->nv) ++s->vertex_bits;
     hipError_t e = hipSetDevice(s->device);
     if (e != hipSuccess) return trw::set_hip_error(e, "hipSetDevice");
     s->slots.resize(static_cast<size_t>(c.streams));
@@ -618,6 +689,20 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
         if (e == hipSuccess) e = pinned(&sl.h_lengths, W);
         if (e == hipSuccess) e = pinned(&sl.h_status, W);
         if (e == hipSuccess) e = pinned(&sl.h_paths, P);
+        if (e == hipSuccess) e = dev(&sl.d_seg_off, S + 1);
+        if (e == hipSuccess) e = dev(&sl.d_keys_out, W);
+        if (e == hipSuccess) e = dev(&sl.d_wsteps, W);
+        if (e == hipSuccess) e = dev(&sl.d_run_keys, W);
+        if (e == hipSuccess) e = dev(&sl.d_seg_steps, S);
+        if (e == hipSuccess) e = dev(&sl.d_run_counts, W);
+        if (e == hipSuccess) e = dev(&sl.d_num_runs, 1);
+        if (e == hipSuccess) e = trw::serve_ppr_temp_bytes(c.max_batch_walkers, c.max_queries_per_batch, &sl.temp_bytes);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_temp, std::max<size_t>(sl.temp_bytes, 256));
+        if (e == hipSuccess) e = pinned(&sl.h_seg_off, S + 1);
+        if (e == hipSuccess) e = pinned(&sl.h_run_keys, W);
+        if (e == hipSuccess) e = pinned(&sl.h_seg_steps, S);
+        if (e == hipSuccess) e = pinned(&sl.h_run_counts, W);
+        if (e == hipSuccess) e = pinned(&sl.h_num_runs, 1);
     }
     if (e != hipSuccess) {
         s->free_slots();
@@ -698,7 +783,8 @@ int trw_server_walk_result(trw_server *s, uint64_t query_id, int32_t *paths, int
     std::shared_ptr<Query> q = find_query(s, query_id);
     if (!q) return trw::set_error(TRW_E_INVALID, "unknown query %llu", static_cast<unsigned long long>(query_id));
     if (q->state != TRW_QUERY_DONE) return trw::set_error(TRW_E_STATE, "query is not done");
-    if (paths && q->result != TRW_RESULT_PATHS) return trw::set_error(TRW_E_STATE, "query kept no paths");
+    if (q->result != TRW_RESULT_PATHS)
+        return trw::set_error(TRW_E_STATE, "a PPR query keeps its end-vertex counts only (trw_server_ppr_result)");
     if (paths) std::copy(q->paths.begin(), q->paths.end(), paths);
     if (lengths) std::copy(q->lengths.begin(), q->lengths.end(), lengths);
     if (ends) std::copy(q->ends.begin(), q->ends.end(), ends);

@@ -1053,10 +1053,10 @@ void fill_paths(const KindParams &a, const KindParams &b, int32_t split, const B
 template <int MODE, bool REPLAY>
 void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                   unsigned int *q, hipStream_t stream, unsigned grid) {
-    // About eight chunks per wavefront of the grid, between 64 and 1024 walkers each,
+    // About two chunks per wavefront of the grid, between 64 and 4096 walkers each,
     // unless the graph's tuning fixes it.
     const int64_t waves = static_cast<int64_t>(grid) * (kQueueBlock / 64);
-    int64_t chunk = a.queue_chunk > 0 ? a.queue_chunk : static_cast<int64_t>(io.n) / (waves * 8);
+    int64_t chunk = a.queue_chunk > 0 ? a.queue_chunk : static_cast<int64_t>(io.n) / (waves * 2);
     chunk = chunk < 64 ? 64 : (chunk > 4096 ? 4096 : chunk);
     hipLaunchKernelGGL((walk_queue_kernel<MODE, REPLAY>), dim3(grid), dim3(kQueueBlock), 0, stream, g, a, b, split, io, q,
                        static_cast<unsigned int>(chunk));

@@ -163,6 +163,16 @@ class DeviceGraph:
         for k, v in knobs.items():
             check(lib().trw_graph_set_tuning(self.h, self.TUNING[k], int(v)))
 
+    def autotune(self, params, sample_walkers=1 << 20, seed=0, with_paths=True):
+        """trw_autotune: times the applicable knobs' candidates on a sample batch and keeps the
+        fastest on the graph; returns {"chosen": {knob: value}, "default_steps_per_s", "tuned_steps_per_s"}."""
+        rep = A.TuneReport()
+        check(lib().trw_autotune(self.h, C.byref(params), int(sample_walkers), int(seed), int(bool(with_paths)),
+                                 C.byref(rep)))
+        names = {v: k for k, v in self.TUNING.items()}
+        return {"chosen": {names[rep.knob[i]]: rep.chosen[i] for i in range(rep.knobs)},
+                "default_steps_per_s": rep.default_steps_per_s, "tuned_steps_per_s": rep.tuned_steps_per_s}
+
     def tuning(self):
         out = {}
         for k, knob in self.TUNING.items():

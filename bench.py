@@ -66,6 +66,8 @@ def parse_args():
     ap.add_argument("--qps", type=float, default=10000.0, help="offered load of the PPR serving run")
     ap.add_argument("--queries", type=int, default=10000, help="queries in the PPR serving run")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--only", choices=["its", "mix", "serving"],
+                    help="run one secondary measurement alone and print its figures (for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also launch the copy kernel over this many MiB (byte-counter calibration)")
     return ap.parse_args()
@@ -158,6 +160,23 @@ def reference_cpu_baseline(hg, length):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def its_walk(ctx, dg, batch, length, per_launch, args):
+    """DeepWalk with static ITS sampling on the same graph and walkers (prefix sums and
+    search trees built on the device, tree threshold autotuned)."""
+    out = {}
+    dg.build_tables("its", "edge")
+    out["its_table_build_ms"] = round(dg.last_build_ms(), 3)
+    prm_i = T.make_params("deepwalk", execution="static", sampling="its", length=length, rng="philox", seed=7,
+                          form="fused")
+    tune = dg.autotune(prm_i, sample_walkers=1 << 21, seed=7)
+    out["its_autotune"] = tune["chosen"]
+    batch.reset()
+    ms_i = time_launches(ctx, dg, prm_i, batch.io(), 1, args.steps)
+    out["its_steps_per_s"] = per_launch / (ms_i / 1e3 / args.steps)
+    log("fused ITS (philox, tuned %s): %.3e steps/s" % (tune["chosen"], out["its_steps_per_s"]))
+    return out
+
+
 def mix_forms(ctx, dg, deg, args):
     """Uneven PPR + MetaPath mix: 2^23 walkers from uniform random non-isolated sources
     (generate_random_source), walkers [0, 2^22) PPR with termination 0.15 and length 1
@@ -170,6 +189,9 @@ def mix_forms(ctx, dg, deg, args):
     src = T.sources_random(deg, n, 4)
     batch = Batch(ctx, src, 80)
     out = {"walkers": n, "split": split}
+    tune = dg.autotune(T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form="queue"),
+                       sample_walkers=1 << 21, seed=4, with_paths=False)
+    out["queue_autotune"] = tune["chosen"]
     for form in ("queue", "graph", "step"):
         a = T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form=form)
         b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=4, form=form)
@@ -250,6 +272,11 @@ def main():
         print("bench.py measures one GPU; this build has no multi-GPU path (walkers would shard by "
               "contiguous ranges with the graph replicated, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.only == "serving":
+        ctx = T.Context(0)
+        print(json.dumps({"ppr_serving": serving(ctx, args)}), flush=True)
+        ctx.close()
+        return
     nv = 1 << args.scale
     length = 80
     t0 = time.time()
@@ -264,9 +291,20 @@ def main():
     alias_build_ms = dg.last_build_ms()
     log("alias tables built on the device in %.1f ms" % alias_build_ms)
 
+    if args.only == "mix":
+        print(json.dumps({"mix_ppr_metapath": mix_forms(ctx, dg, deg, args)}), flush=True)
+        dg.close()
+        ctx.close()
+        return
     src = T.sources_all(deg)
     n = len(src)
     batch = Batch(ctx, src, length)
+    if args.only == "its":
+        print(json.dumps(its_walk(ctx, dg, batch, length, (length - 1) * n, args)), flush=True)
+        batch.free()
+        dg.close()
+        ctx.close()
+        return
     prm = T.make_params("deepwalk", execution="static", sampling="alias", length=length, rng="philox", seed=2024,
                         form="fused")
     ms = time_launches(ctx, dg, prm, batch.io(), args.warmup, args.steps)
@@ -299,13 +337,7 @@ def main():
         details["alias_replay_roofline_frac"] = v_r * B_STEP["alias_replay"] / 1e9 / HBM_PEAK_GBS
         log("fused alias (replay): %.3e steps/s" % v_r)
 
-        dg.build_tables("its", "edge")
-        details["its_table_build_ms"] = round(dg.last_build_ms(), 3)
-        prm_i = T.make_params("deepwalk", execution="static", sampling="its", length=length, rng="philox", seed=7,
-                              form="fused")
-        ms_i = time_launches(ctx, dg, prm_i, batch.io(), 1, args.steps)
-        details["its_steps_per_s"] = per_launch / (ms_i / 1e3 / args.steps)
-        log("fused ITS (philox): %.3e steps/s" % details["its_steps_per_s"])
+        details.update(its_walk(ctx, dg, batch, length, per_launch, args))
 
     if not args.quick:
         # Measured ceiling for this access pattern: dependent random 16-byte gathers at full

@@ -975,15 +975,29 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_finish_kernel(BatchIO io
 }
 
 // -1 in entries [0, length) of every path row (length of the walker's kind), written
-// row-contiguously before the QUEUE, STEP and GRAPH forms run, so that their walkers
-// write only the vertices they visit and never pad rows one entry at a time.
-__global__ void fill_paths_kernel(int32_t *paths, int32_t n, int32_t stride, int32_t split, int32_t La, int32_t Lb) {
-    const int32_t Lmax = La > Lb ? La : Lb;
-    const int64_t total = static_cast<int64_t>(n) * Lmax;
+// before the QUEUE, STEP and GRAPH forms run, so that their walkers write only the
+// vertices they visit and never pad rows one entry at a time.  Rows whose length equals
+// the stride form one contiguous range, filled with 16-byte stores; other rows take a
+// thread per (row, entry).
+__global__ void fill_range_kernel(int32_t *p, int64_t n) {
+    const int64_t head = (-(reinterpret_cast<uintptr_t>(p) >> 2)) & 3;  // entries before 16-byte alignment
+    const int64_t h = head < n ? head : n;
+    const int64_t vec = (n - h) >> 2;
+    int4 *pv = reinterpret_cast<int4 *>(p + h);
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (int64_t i = t; i < vec; i += stride) pv[i] = make_int4(-1, -1, -1, -1);
+    if (t < h) p[t] = -1;
+    const int64_t tail = h + 4 * vec;
+    if (t < n - tail) p[tail + t] = -1;
+}
+
+__global__ void fill_rows_kernel(int32_t *p, int64_t rows, int32_t stride, int32_t L) {
+    const int64_t total = rows * L;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-        const int32_t row = static_cast<int32_t>(i / Lmax), col = static_cast<int32_t>(i - static_cast<int64_t>(row) * Lmax);
-        if (col < (row >= split ? Lb : La)) paths[static_cast<int64_t>(row) * stride + col] = -1;
+        const int64_t row = L == 1 ? i : i / L;
+        p[row * stride + (i - row * L)] = -1;
     }
 }
 
@@ -1041,13 +1055,23 @@ hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO 
 
 int queue_grid(int device_cus) { return device_cus * 8; }
 
+void fill_path_rows(int32_t *paths, int64_t rows, int32_t stride, int32_t L, hipStream_t stream) {
+    if (rows <= 0) return;
+    const int64_t total = rows * L;
+    const int64_t blocks = std::min<int64_t>(grid_for(L == stride ? (total + 3) / 4 : total, 256), 256 * 64);
+    if (L == stride)
+        hipLaunchKernelGGL(fill_range_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, paths, total);
+    else
+        hipLaunchKernelGGL(fill_rows_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, paths, rows,
+                           stride, L);
+}
+
 void fill_paths(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io, hipStream_t stream) {
     if (!io.paths || io.n <= 0) return;
     const int32_t La = a.length > 0 ? a.length : 1, Lb = b.length > 0 ? b.length : 1;
-    const int64_t total = static_cast<int64_t>(io.n) * (La > Lb ? La : Lb);
-    const int64_t blocks = std::min<int64_t>(grid_for(total, 256), 256 * 64);
-    hipLaunchKernelGGL(fill_paths_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, io.paths, io.n,
-                       io.path_stride, split, La, Lb);
+    const int32_t sa = split < io.n ? split : io.n;
+    fill_path_rows(io.paths, sa, io.path_stride, La, stream);
+    fill_path_rows(io.paths + static_cast<int64_t>(sa) * io.path_stride, io.n - sa, io.path_stride, Lb, stream);
 }
 
 template <int MODE, bool REPLAY>

@@ -397,6 +397,15 @@ def test_full_size_deepwalk_alias_graph(gpu_ctx):
         table = dg.alias_table()
         ref_table, _ = ob.alias_build(g, w)
         assert np.array_equal(table.view(np.uint8), ref_table.view(np.uint8))
+        # The ITS and rejection tables too (hubs of degree >= 1024 take the wavefront builds).
+        dg.build_tables("its", "edge")
+        assert np.array_equal(dg.prefix_sum().view(np.uint64), ob.its_build(g, w).view(np.uint64))
+        dg.build_tables("rejection", "edge")
+        assert np.array_equal(dg.rejection_max().view(np.uint64), ob.rs_build(g, w).view(np.uint64))
+        dg.build_tables("alias", "unit")
+        ref_unit, _ = ob.alias_build(g, np.ones(len(adj)))
+        assert np.array_equal(dg.alias_table().view(np.uint8), ref_unit.view(np.uint8))
+        dg.build_tables("alias", "edge")
 
         src = T.sources_all(deg)
         sub = src[:: max(1, len(src) // 65536)][:65536].astype(np.int32)

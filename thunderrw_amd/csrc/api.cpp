@@ -286,6 +286,7 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
     // Vertices in descending degree order (by power-of-two class): the table
     // builds start hubs' long serial chains first.
     std::vector<int32_t> order(static_cast<size_t>(nv));
+    int64_t hubs = 0;
     {
         std::vector<int64_t> cls(34, 0);
         auto klass = [](int32_t d) { return d == 0 ? 0 : 33 - __builtin_clz(static_cast<unsigned>(d)); };
@@ -297,6 +298,7 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
             acc += cls[c];
         }
         for (int32_t v = 0; v < nv; ++v) order[start[klass(degree[v])]++] = v;
+        for (int c = klass(trw::kWaveBuildMin); c <= 33; ++c) hubs += cls[c];
     }
 
     std::unique_ptr<trw_graph> g(new trw_graph);
@@ -304,6 +306,7 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
     g->nv = nv;
     g->ne = ne;
     g->max_degree = maxd;
+    g->build_hubs = static_cast<int32_t>(hubs);
     TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     struct Guard {
         trw_graph *g;
@@ -434,21 +437,15 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
     } ev{t0, t1};
     if (sampling == TRW_SM_ALIAS) {
         if (!g->alias) TRW_HIP(dmalloc(&g->alias, E + 1), "hipMalloc alias table");
-        double *q = nullptr;
-        int32_t *lists = nullptr;
-        TRW_HIP(dmalloc(&q, E + 1), "hipMalloc alias scratch");
-        hipError_t e = dmalloc(&lists, E + 1);
-        if (e != hipSuccess) {
-            dfree(q);
-            return hip_fail(e, "hipMalloc alias scratch");
-        }
+        trw::AliasEntry *scratch = nullptr;
+        TRW_HIP(dmalloc(&scratch, E + 1), "hipMalloc alias scratch");
         trw::launch_fill_u32(g->counters + 2, 0u, 1, st);
         (void)hipEventRecord(t0, st);
-        e = trw::launch_alias_build(dv, g->weight, unit, g->order, q, lists, g->alias, g->counters + 2, st);
+        hipError_t e = trw::launch_alias_build(dv, g->weight, unit, g->order, g->build_hubs, scratch, g->alias,
+                                              g->counters + 2, st);
         (void)hipEventRecord(t1, st);
         hipError_t e2 = hipStreamSynchronize(st);
-        dfree(q);
-        dfree(lists);
+        dfree(scratch);
         if (e != hipSuccess) return hip_fail(e, "alias build launch");
         if (e2 != hipSuccess) return hip_fail(e2, "alias build");
         g->alias_weights = weights;
@@ -462,7 +459,7 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
         // Padded so that a search may read the whole aligned 16-entry chunk holding any entry.
         if (!g->prefix) TRW_HIP(dmalloc(&g->prefix, E + 2 * trw::kItsFan), "hipMalloc prefix sums");
         (void)hipEventRecord(t0, st);
-        hipError_t e = trw::launch_its_build(dv, g->weight, unit, g->order, g->prefix, st);
+        hipError_t e = trw::launch_its_build(dv, g->weight, unit, g->order, g->build_hubs, g->prefix, st);
         if (e != hipSuccess) return hip_fail(e, "ITS build launch");
         int rc = build_its_trees(g, unit || g->weights_nonneg);
         (void)hipEventRecord(t1, st);
@@ -472,7 +469,7 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
     } else {
         if (!g->rs_max) TRW_HIP(dmalloc(&g->rs_max, V), "hipMalloc rejection maxima");
         (void)hipEventRecord(t0, st);
-        hipError_t e = trw::launch_rs_build(dv, g->weight, unit, g->order, g->rs_max, st);
+        hipError_t e = trw::launch_rs_build(dv, g->weight, unit, g->order, g->build_hubs, g->rs_max, st);
         (void)hipEventRecord(t1, st);
         if (e != hipSuccess) return hip_fail(e, "rejection build launch");
         TRW_HIP(hipStreamSynchronize(st), "rejection build");

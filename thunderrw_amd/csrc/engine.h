@@ -197,9 +197,17 @@ hipError_t launch_copy_u128(uint4 *dst, const uint4 *src, int64_t n, hipStream_t
 hipError_t run_gather_probe(uint64_t table_bytes, int32_t chains, int32_t steps, int32_t cus, hipStream_t stream,
                             double *loads_per_s);
 
-hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order,
-                              double *qbuf, int32_t *lists, AliasSlot *out, unsigned int *zero_count,
-                              hipStream_t stream);
+// Scratch entry of the alias build: an index's normalised weight (or residual) and neighbour id.
+struct AliasEntry {
+    double q;
+    int32_t nbr;
+    int32_t pad;
+};
+// Table builds.  order lists the vertices by descending degree class; its first `hubs`
+// entries (degree >= kWaveBuildMin) are built by a wavefront each, the rest by a lane each.
+constexpr int32_t kWaveBuildMin = 1024;
+hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
+                              AliasEntry *scratch, AliasSlot *out, unsigned int *zero_count, hipStream_t stream);
 hipError_t launch_edge_records(const DevGraph &g, uint4 *erec, hipStream_t stream);
 hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stream);
 hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lerec, hipStream_t stream);
@@ -209,9 +217,9 @@ hipError_t launch_its_tree_sizes(const DevGraph &g, uint32_t *root, hipStream_t 
 hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint32_t *root, double *idx,
                                 hipStream_t stream);
 hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *root, uint4 *erec, hipStream_t stream);
-hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
-                            hipStream_t stream);
-hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
-                           hipStream_t stream);
+hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
+                            double *out, hipStream_t stream);
+hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
+                           double *out, hipStream_t stream);
 
 }  // namespace trw

@@ -33,6 +33,7 @@ struct trw_graph {
     double *its_idx = nullptr;     // ITS search trees (null: not built, or weights not all >= 0)
     uint32_t *its_root = nullptr;  // nv tree roots
     bool weights_nonneg = true;    // every edge weight is finite and >= 0
+    int32_t build_hubs = 0;        // vertices of degree >= kWaveBuildMin (a prefix of order)
     int64_t tune[3] = {0, 0, 0};   // TRW_TUNE_* knobs
     trw::AliasSlot *alias = nullptr;
     double *prefix = nullptr;

@@ -10,6 +10,8 @@ namespace trw {
 namespace {
 
 constexpr int kBuildBlock = 128;
+constexpr int kAliasChunk = 16;      // alias build: entries loaded per batch of the serial loops
+constexpr int kAliasLargeChunk = 8;  // alias build: large entries held ahead
 
 __device__ __forceinline__ double weight_at(const double *w, int32_t unit, int64_t e) { return unit ? 1.0 : w[e]; }
 
@@ -20,22 +22,37 @@ __device__ __forceinline__ double weight_at(const double *w, int32_t unit, int64
 // a cursor li.  `small` receives the initial small indices (index order) and then,
 // at its back, each large index at the moment it drops below 1 -- which happens
 // in `large` order.  So the small queue is always initial_smalls ++ larges[0..li)
-// and a second cursor si walks that concatenation.  lists[] holds the small
-// indices ascending from the segment start and the large indices from the
-// segment end backwards; q[] holds the normalised weights, and a large index's
-// residual is written back to q[] when it moves to the small side.
+// and a second cursor si walks that concatenation.  The scratch entries hold each
+// index's normalised weight and neighbour id: smalls ascending from the segment
+// start, larges from the segment end backwards, and a large entry's weight is
+// replaced by its residual when it moves to the small side.  Every load address in
+// the pairing loop then depends on the cursors only, never on a loaded value.
 __global__ __launch_bounds__(kBuildBlock) void alias_build_kernel(DevGraph g, const double *w, int32_t unit,
-                                                                   const int32_t *order, double *q, int32_t *lists,
-                                                                   AliasSlot *out, unsigned int *zero_count) {
-    const int32_t i = blockIdx.x * kBuildBlock + threadIdx.x;
+                                                                   const int32_t *order, int32_t first,
+                                                                   AliasEntry *scratch, AliasSlot *out,
+                                                                   unsigned int *zero_count) {
+    const int32_t i = first + blockIdx.x * kBuildBlock + threadIdx.x;
     if (i >= g.nv) return;
     const int32_t v = order ? order[i] : i;
     const int64_t off = g.offset[v];
     const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
     if (d == 0) return;
 
-    double sum = 0.0;  // std::accumulate(src, src + length, 0.0)
-    for (int32_t j = 0; j < d; ++j) sum = sum + weight_at(w, unit, off + j);
+    double sum = 0.0;  // std::accumulate(src, src + length, 0.0): left to right
+    if (unit) {
+        for (int32_t j = 0; j < d; ++j) sum = sum + 1.0;
+    } else {
+        const double *wv = w + off;
+        int32_t j = 0;
+        for (; j + kAliasChunk <= d; j += kAliasChunk) {  // a chunk of loads in flight, added in order
+            double x[kAliasChunk];
+#pragma unroll
+            for (int c = 0; c < kAliasChunk; ++c) x[c] = wv[j + c];
+#pragma unroll
+            for (int c = 0; c < kAliasChunk; ++c) sum = sum + x[c];
+        }
+        for (; j < d; ++j) sum = sum + wv[j];
+    }
     const double avg = sum / d;
     AliasSlot *t = out + off;
     if (avg == 0.0) {
@@ -44,49 +61,135 @@ __global__ __launch_bounds__(kBuildBlock) void alias_build_kernel(DevGraph g, co
         return;
     }
 
-    double *qv = q + off;
-    int32_t *lst = lists + off;
+    AliasEntry *E = scratch + off;
     const int32_t *nb = g.adj + off;
     int32_t ns = 0, nl = 0;
-    for (int32_t j = 0; j < d; ++j) {
-        const double x = weight_at(w, unit, off + j) / avg;
-        qv[j] = x;
-        if (x < 1) lst[ns++] = j;
-        else lst[d - 1 - nl++] = j;
+    for (int32_t j0 = 0; j0 < d; j0 += kAliasChunk) {  // loads a chunk ahead, classifies in order
+        double wc[kAliasChunk];
+        int32_t nc[kAliasChunk];
+#pragma unroll
+        for (int c = 0; c < kAliasChunk; ++c) {  // unconditional (clamped) loads issue back to back
+            const int32_t j = min(j0 + c, d - 1);
+            wc[c] = weight_at(w, unit, off + j);
+            nc[c] = nb[j];
+        }
+#pragma unroll
+        for (int c = 0; c < kAliasChunk; ++c) {
+            if (j0 + c >= d) break;
+            const double x = wc[c] / avg;
+            const AliasEntry en{x, nc[c], 0};
+            if (x < 1) E[ns++] = en;
+            else E[d - 1 - nl++] = en;
+        }
     }
 
     int32_t k = 0, si = 0, li = 0;
-    int32_t l = nl > 0 ? lst[d - 1] : 0;
-    double wl = nl > 0 ? qv[l] : 0.0;
+    // Larges are consumed in order from the segment end; a chunk of their entries is
+    // held ahead (their weights change only once they are consumed).
+    AliasEntry lbuf[kAliasLargeChunk];
+    int32_t lbase = 0;  // large index of lbuf[0]
+#pragma unroll
+    for (int c = 0; c < kAliasLargeChunk; ++c) lbuf[c] = E[max(d - 1 - c, 0)];
+    int32_t lpos = d - 1;
+    double wl = nl > 0 ? lbuf[0].q : 0.0;
+    int32_t lnb = nl > 0 ? lbuf[0].nbr : 0;
+    double drop_q = 0.0;  // residual and neighbour of the large that dropped last
+    int32_t drop_nbr = 0;
     while (si < ns + li && li < nl) {
-        const int32_t s = si < ns ? lst[si] : lst[d - 1 - (si - ns)];
-        const double ws = qv[s];
-        t[k++] = AliasSlot{ws, nb[s], nb[l]};
-        ++si;
-        wl = wl + ws - 1;
-        if (wl < 1) {
-            qv[l] = wl;
-            ++li;
-            if (li < nl) {
-                l = lst[d - 1 - li];
-                wl = qv[l];
+        if (si >= ns && si - ns == li - 1) {
+            // The only pending small is the large that just dropped: once the initial
+            // smalls are used up this is the common case, and its entry is in registers.
+            t[k++] = AliasSlot{drop_q, drop_nbr, lnb};
+            ++si;
+            wl = wl + drop_q - 1;
+            if (wl < 1) {
+                E[lpos].q = wl;
+                drop_q = wl;
+                drop_nbr = lnb;
+                ++li;
+                if (li < nl) {
+                    if (li - lbase == kAliasLargeChunk) {
+                        lbase = li;
+#pragma unroll
+                        for (int c2 = 0; c2 < kAliasLargeChunk; ++c2) lbuf[c2] = E[max(d - 1 - (lbase + c2), 0)];
+                    }
+                    lpos = d - 1 - li;
+                    AliasEntry le = lbuf[0];
+#pragma unroll
+                    for (int c2 = 1; c2 < kAliasLargeChunk; ++c2)
+                        if (li - lbase == c2) le = lbuf[c2];
+                    wl = le.q;
+                    lnb = le.nbr;
+                }
+            }
+            continue;
+        }
+        // The next small entries that are already final (initial smalls, and larges that
+        // have dropped with their residual written), a chunk at a time so that their
+        // loads overlap; the pairing itself stays strictly sequential.
+        const int32_t avail = min(kAliasChunk, ns + li - si);
+        AliasEntry buf[kAliasChunk];
+#pragma unroll
+        for (int c = 0; c < kAliasChunk; ++c) {  // clamped to the segment; entries past avail are unused
+            const int32_t p = si + c < ns ? si + c : d - 1 - (si + c - ns);
+            buf[c] = E[max(0, min(p, d - 1))];
+        }
+#pragma unroll
+        for (int c = 0; c < kAliasChunk; ++c) {
+            if (c >= avail || li >= nl) break;
+            const AliasEntry se = buf[c];
+            t[k++] = AliasSlot{se.q, se.nbr, lnb};
+            ++si;
+            wl = wl + se.q - 1;
+            if (wl < 1) {
+                E[lpos].q = wl;
+                drop_q = wl;
+                drop_nbr = lnb;
+                ++li;
+                if (li < nl) {
+                    if (li - lbase == kAliasLargeChunk) {
+                        lbase = li;
+#pragma unroll
+                        for (int c2 = 0; c2 < kAliasLargeChunk; ++c2) lbuf[c2] = E[max(d - 1 - (lbase + c2), 0)];
+                    }
+                    lpos = d - 1 - li;
+                    AliasEntry le = lbuf[0];
+#pragma unroll
+                    for (int c2 = 1; c2 < kAliasLargeChunk; ++c2)
+                        if (li - lbase == c2) le = lbuf[c2];
+                    wl = le.q;
+                    lnb = le.nbr;
+                }
             }
         }
     }
-    for (; si < ns + li; ++si) {
-        const int32_t s = si < ns ? lst[si] : lst[d - 1 - (si - ns)];
-        t[k++] = AliasSlot{1.0, nb[s], nb[s]};
+    // Leftovers take probability 1 (alias_sampling.h:43-53); their loads are independent.
+    for (int32_t s0 = si; s0 < ns + li; s0 += kAliasChunk) {
+        int32_t x[kAliasChunk];
+#pragma unroll
+        for (int c = 0; c < kAliasChunk; ++c) {
+            const int32_t p = s0 + c < ns ? s0 + c : d - 1 - (s0 + c - ns);
+            x[c] = E[max(0, min(p, d - 1))].nbr;
+        }
+#pragma unroll
+        for (int c = 0; c < kAliasChunk; ++c)
+            if (s0 + c < ns + li) t[k + (s0 + c - si)] = AliasSlot{1.0, x[c], x[c]};
     }
-    for (; li < nl; ++li) {
-        const int32_t x = lst[d - 1 - li];
-        t[k++] = AliasSlot{1.0, nb[x], nb[x]};
+    k += ns + li - si;
+    for (int32_t l0 = li; l0 < nl; l0 += kAliasChunk) {
+        int32_t x[kAliasChunk];
+#pragma unroll
+        for (int c = 0; c < kAliasChunk; ++c) x[c] = E[max(d - 1 - (l0 + c), 0)].nbr;
+#pragma unroll
+        for (int c = 0; c < kAliasChunk; ++c)
+            if (l0 + c < nl) t[k + (l0 + c - li)] = AliasSlot{1.0, x[c], x[c]};
     }
 }
 
 // its_initialization (inverse_transformation_sampling.h:19-24).
 __global__ __launch_bounds__(kBuildBlock) void its_build_kernel(DevGraph g, const double *w, int32_t unit,
-                                                                 const int32_t *order, double *out) {
-    const int32_t i = blockIdx.x * kBuildBlock + threadIdx.x;
+                                                                 const int32_t *order, int32_t first, double *out) {
+    const int32_t i = first + blockIdx.x * kBuildBlock + threadIdx.x;
     if (i >= g.nv) return;
     const int32_t v = order ? order[i] : i;
     const int64_t off = g.offset[v];
@@ -102,8 +205,8 @@ __global__ __launch_bounds__(kBuildBlock) void its_build_kernel(DevGraph g, cons
 
 // rs_initialization (rejection_sampling.h:12-14): *std::max_element.
 __global__ __launch_bounds__(kBuildBlock) void rs_build_kernel(DevGraph g, const double *w, int32_t unit,
-                                                                const int32_t *order, double *out) {
-    const int32_t i = blockIdx.x * kBuildBlock + threadIdx.x;
+                                                                const int32_t *order, int32_t first, double *out) {
+    const int32_t i = first + blockIdx.x * kBuildBlock + threadIdx.x;
     if (i >= g.nv) return;
     const int32_t v = order ? order[i] : i;
     const int64_t off = g.offset[v];
@@ -118,6 +221,181 @@ __global__ __launch_bounds__(kBuildBlock) void rs_build_kernel(DevGraph g, const
         if (m < x) m = x;
     }
     out[v] = m;
+}
+
+
+// ---------------------------------------------------------------- hub builds
+//
+// A vertex of degree >= kWaveBuildMin gets a whole wavefront.  The reference's
+// arithmetic stays sequential (a left-to-right FP64 sum, the prefix chain, Vose's
+// pairing), so the tables are the same bytes; what the lanes share is the memory
+// traffic: they load 64 consecutive entries at a time, the sequential chain reads
+// them from registers (readlane), and results are written back 64 at a time.
+
+__device__ __forceinline__ double readlane_f64(double x, int32_t lane) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+__device__ __forceinline__ int32_t readlane_i32(int32_t x, int32_t lane) { return __builtin_amdgcn_readlane(x, lane); }
+
+// std::accumulate(w, w + d, 0.0): the same additions in the same order.
+__device__ __forceinline__ double wave_ordered_sum(const double *w, int32_t unit, int64_t off, int32_t d, int lane) {
+    double sum = 0.0;
+    for (int32_t base = 0; base < d; base += 64) {
+        const int32_t j = base + lane;
+        const double x = j < d ? weight_at(w, unit, off + j) : 0.0;
+        const int32_t n = min(64, d - base);
+        for (int32_t c = 0; c < n; ++c) sum = sum + readlane_f64(x, c);
+    }
+    return sum;
+}
+
+__global__ __launch_bounds__(64) void alias_build_wave_kernel(DevGraph g, const double *w, int32_t unit,
+                                                              const int32_t *order, AliasEntry *scratch,
+                                                              AliasSlot *out, unsigned int *zero_count) {
+    const int lane = static_cast<int>(threadIdx.x);
+    const int32_t v = order[blockIdx.x];
+    const int64_t off = g.offset[v];
+    const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
+    const double avg = wave_ordered_sum(w, unit, off, d, lane) / d;
+    AliasSlot *t = out + off;
+    if (avg == 0.0) {
+        for (int32_t j = lane; j < d; j += 64) t[j] = AliasSlot{0.0, -1, -1};
+        if (lane == 0) atomicAdd(zero_count, 1u);
+        return;
+    }
+
+    // Classification in index order: a ballot ranks each lane's entry among the smalls
+    // (or larges) of its 64-entry group.
+    AliasEntry *E = scratch + off;
+    const int32_t *nb = g.adj + off;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int32_t ns = 0, nl = 0;
+    for (int32_t base = 0; base < d; base += 64) {
+        const int32_t j = base + lane;
+        const bool valid = j < d;
+        const double x = valid ? weight_at(w, unit, off + j) / avg : 0.0;
+        const int32_t nbr = valid ? nb[j] : 0;
+        const bool small = valid && x < 1;
+        const bool large = valid && !(x < 1);
+        const unsigned long long ms = __ballot(small), ml = __ballot(large);
+        if (small) E[ns + __popcll(ms & below)] = AliasEntry{x, nbr, 0};
+        if (large) E[d - 1 - (nl + __popcll(ml & below))] = AliasEntry{x, nbr, 0};
+        ns += __popcll(ms);
+        nl += __popcll(ml);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // entries written by one lane, read by another
+
+    // Pairing, exactly the lane kernel's sequence.  Window sw: small-queue entries
+    // [sw, sw + swn), one per lane; window lw: large entries [lw, lw + 64).
+    int32_t k = 0, si = 0, li = 0;
+    int32_t lw = 0;
+    AliasEntry my_l = E[max(d - 1 - lane, 0)];
+    double wl = nl > 0 ? readlane_f64(my_l.q, 0) : 0.0;
+    int32_t lnb = nl > 0 ? readlane_i32(my_l.nbr, 0) : 0;
+    int32_t lpos = d - 1;
+    int32_t sw = 0, swn = 0;
+    AliasEntry my_s{0.0, 0, 0};
+    AliasSlot my_out{0.0, 0, 0};  // slot kbase + lane of the chunk being filled
+    int32_t kbase = 0;
+    while (si < ns + li && li < nl) {
+        if (si >= sw + swn) {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // residuals written by lane 0
+            sw = si;
+            swn = min(64, ns + li - si);
+            const int32_t p = sw + lane < ns ? sw + lane : d - 1 - (sw + lane - ns);
+            my_s = E[max(0, min(p, d - 1))];
+        }
+        const int32_t c = si - sw;
+        const double ws = readlane_f64(my_s.q, c);
+        const int32_t snb = readlane_i32(my_s.nbr, c);
+        if (lane == (k & 63)) my_out = AliasSlot{ws, snb, lnb};
+        ++k;
+        if ((k & 63) == 0) {
+            t[kbase + lane] = my_out;
+            kbase += 64;
+        }
+        ++si;
+        wl = wl + ws - 1;
+        if (wl < 1) {
+            if (lane == 0) E[lpos].q = wl;
+            ++li;
+            if (li < nl) {
+                if (li - lw == 64) {
+                    lw = li;
+                    my_l = E[max(d - 1 - (lw + lane), 0)];
+                }
+                lpos = d - 1 - li;
+                wl = readlane_f64(my_l.q, li - lw);
+                lnb = readlane_i32(my_l.nbr, li - lw);
+            }
+        }
+    }
+    if (lane < (k & 63)) t[kbase + lane] = my_out;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    // Leftovers take probability 1 (alias_sampling.h:43-53), in queue order.
+    for (int32_t b = si; b < ns + li; b += 64) {
+        const int32_t s2 = b + lane;
+        if (s2 < ns + li) {
+            const int32_t x = E[s2 < ns ? s2 : d - 1 - (s2 - ns)].nbr;
+            t[k + (s2 - si)] = AliasSlot{1.0, x, x};
+        }
+    }
+    k += ns + li - si;
+    for (int32_t b = li; b < nl; b += 64) {
+        const int32_t l2 = b + lane;
+        if (l2 < nl) {
+            const int32_t x = E[d - 1 - l2].nbr;
+            t[k + (l2 - li)] = AliasSlot{1.0, x, x};
+        }
+    }
+}
+
+// its_initialization for hubs: the prefix chain runs on uniform values, each lane keeps
+// the running sum at its own index of the 64-entry group.
+__global__ __launch_bounds__(64) void its_build_wave_kernel(DevGraph g, const double *w, int32_t unit,
+                                                            const int32_t *order, double *out) {
+    const int lane = static_cast<int>(threadIdx.x);
+    const int32_t v = order[blockIdx.x];
+    const int64_t off = g.offset[v];
+    const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
+    double acc = 0.0;
+    for (int32_t base = 0; base < d; base += 64) {
+        const int32_t j = base + lane;
+        const double x = j < d ? weight_at(w, unit, off + j) : 0.0;
+        const int32_t n = min(64, d - base);
+        double mine = 0.0;
+        for (int32_t c = 0; c < n; ++c) {
+            const double y = readlane_f64(x, c);
+            acc = (base + c == 0) ? y : acc + y;  // the first entry is copied, not added to 0.0
+            if (lane == c) mine = acc;
+        }
+        if (j < d) out[off + j] = mine;
+    }
+}
+
+// rs_initialization for hubs: the same comparisons in the same order.
+__global__ __launch_bounds__(64) void rs_build_wave_kernel(DevGraph g, const double *w, int32_t unit,
+                                                           const int32_t *order, double *out) {
+    const int lane = static_cast<int>(threadIdx.x);
+    const int32_t v = order[blockIdx.x];
+    const int64_t off = g.offset[v];
+    const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
+    double m = 0.0;
+    for (int32_t base = 0; base < d; base += 64) {
+        const int32_t j = base + lane;
+        const double x = j < d ? weight_at(w, unit, off + j) : 0.0;
+        const int32_t n = min(64, d - base);
+        for (int32_t c = 0; c < n; ++c) {
+            const double y = readlane_f64(x, c);
+            if (base + c == 0) m = y;
+            else if (m < y) m = y;
+        }
+    }
+    if (lane == 0) out[v] = m;
 }
 
 // Neighbour records: a Move that lands on adj[e] also learns where that
@@ -252,11 +530,15 @@ inline unsigned grid_for(int64_t n) { return static_cast<unsigned>((n + kBuildBl
 
 }  // namespace
 
-hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *qbuf,
-                              int32_t *lists, AliasSlot *out, unsigned int *zero_count, hipStream_t stream) {
+hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
+                              AliasEntry *scratch, AliasSlot *out, unsigned int *zero_count, hipStream_t stream) {
     if (g.nv <= 0) return hipSuccess;
-    hipLaunchKernelGGL(alias_build_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g, w, unit, order,
-                       qbuf, lists, out, zero_count);
+    if (hubs > 0)
+        hipLaunchKernelGGL(alias_build_wave_kernel, dim3(hubs), dim3(64), 0, stream, g, w, unit, order, scratch, out,
+                           zero_count);
+    if (g.nv > hubs)
+        hipLaunchKernelGGL(alias_build_kernel, dim3(grid_for(g.nv - hubs)), dim3(kBuildBlock), 0, stream, g, w, unit,
+                           order, hubs, scratch, out, zero_count);
     return hipGetLastError();
 }
 
@@ -302,17 +584,23 @@ hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *r
     return hipGetLastError();
 }
 
-hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
-                            hipStream_t stream) {
+hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
+                            double *out, hipStream_t stream) {
     if (g.nv <= 0) return hipSuccess;
-    hipLaunchKernelGGL(its_build_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g, w, unit, order, out);
+    if (hubs > 0) hipLaunchKernelGGL(its_build_wave_kernel, dim3(hubs), dim3(64), 0, stream, g, w, unit, order, out);
+    if (g.nv > hubs)
+        hipLaunchKernelGGL(its_build_kernel, dim3(grid_for(g.nv - hubs)), dim3(kBuildBlock), 0, stream, g, w, unit,
+                           order, hubs, out);
     return hipGetLastError();
 }
 
-hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, double *out,
-                           hipStream_t stream) {
+hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
+                           double *out, hipStream_t stream) {
     if (g.nv <= 0) return hipSuccess;
-    hipLaunchKernelGGL(rs_build_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g, w, unit, order, out);
+    if (hubs > 0) hipLaunchKernelGGL(rs_build_wave_kernel, dim3(hubs), dim3(64), 0, stream, g, w, unit, order, out);
+    if (g.nv > hubs)
+        hipLaunchKernelGGL(rs_build_kernel, dim3(grid_for(g.nv - hubs)), dim3(kBuildBlock), 0, stream, g, w, unit,
+                           order, hubs, out);
     return hipGetLastError();
 }
 

@@ -66,7 +66,7 @@ def parse_args():
     ap.add_argument("--qps", type=float, default=10000.0, help="offered load of the PPR serving run")
     ap.add_argument("--queries", type=int, default=10000, help="queries in the PPR serving run")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--only", choices=["its", "mix", "serving"],
+    ap.add_argument("--only", choices=["its", "mix", "serving", "c1", "c3"],
                     help="run one secondary measurement alone and print its figures (for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also launch the copy kernel over this many MiB (byte-counter calibration)")
@@ -112,33 +112,34 @@ def time_launches(ctx, graph, params, io, warmup, steps):
     return ms
 
 
-def reference_cpu_baseline(hg, length):
-    """The reference's DeepWalk (static alias, ring interleaving on) on every host core."""
-    cores = len(os.sched_getaffinity(0))
-    cpu = "unknown"
+def host_cpu():
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    if not os.path.exists(REF_BIN):
+    return "unknown"
+
+
+def reference_run(hg, app, flags):
+    """Runs the reference's own app binary (oracle/_ref/<app>.out, built from its sources) on
+    every host core, on `hg` written as reference CSR files; returns its compute() totals."""
+    binary = os.path.join(ROOT, "oracle", "_ref", app + ".out")
+    if not os.path.exists(binary):
         return None
-    allowed = os.sched_getaffinity(0)
-    if any(c not in allowed for c in range(cores)):
+    cores = len(os.sched_getaffinity(0))
+    if any(c not in os.sched_getaffinity(0) for c in range(cores)):
         log("reference pins threads to CPUs 0..n-1, which are not all available here; skipping it")
         return None
     tmp = tempfile.mkdtemp(prefix="trw_ref_", dir="/tmp")
     try:
-        t0 = time.time()
         hg.store(tmp)
-        log("wrote reference CSR files in %.1fs; running the reference on %d cores" % (time.time() - t0, cores))
-        r = subprocess.run([REF_BIN, "-f", tmp, "-ew", "-n", str(cores), "-l", str(length)], capture_output=True,
-                           text=True, timeout=900)
+        r = subprocess.run([binary, "-f", tmp, "-n", str(cores)] + flags, capture_output=True, text=True,
+                           timeout=900)
         text = r.stdout + r.stderr
         if r.returncode != 0:
-            log("reference exited with %d" % r.returncode)
+            log("reference %s exited with %d" % (app, r.returncode))
             return None
         # Totals are logged by compute() (creeper.h:801-803); per-thread lines carry "Thread".
         totals = "\n".join(ln for ln in text.splitlines() if "func: compute)" in ln)
@@ -149,15 +150,89 @@ def reference_cpu_baseline(hg, length):
         if not thr:
             log("could not find the reference's throughput line")
             return None
-        return {"value": float(thr[-1]), "unit": "steps/s", "cores": cores, "kind": "reference",
-                "sample": "the reference's deepwalk.out (static alias, step interleaving on, -n %d) on the full "
-                          "workload: the same graph as reference CSR files, one walker per non-isolated vertex, "
-                          "length %d; %s steps in %ss of 'Walking time' (graph load and the %ss alias-table "
-                          "build excluded, as the reference times it); CPU: %s"
-                          % (cores, length, steps[-1] if steps else "?", walk_s[-1] if walk_s else "?",
-                             pre[-1] if pre else "?", cpu)}
+        return {"value": float(thr[-1]), "steps": int(steps[-1]) if steps else None,
+                "walk_s": float(walk_s[-1]) if walk_s else None, "pre_s": float(pre[-1]) if pre else None,
+                "cores": cores, "flags": " ".join(flags)}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+def reference_cpu_baseline(hg, length):
+    """The reference's DeepWalk (static alias, ring interleaving on) on every host core."""
+    r = reference_run(hg, "deepwalk", ["-ew", "-l", str(length)])
+    if r is None:
+        return None
+    return {"value": r["value"], "unit": "steps/s", "cores": r["cores"], "kind": "reference",
+            "sample": "the reference's deepwalk.out (static alias, step interleaving on, -n %d) on the full "
+                      "workload: the same graph as reference CSR files, one walker per non-isolated vertex, "
+                      "length %d; %s steps in %ss of 'Walking time' (graph load and the %ss alias-table "
+                      "build excluded, as the reference times it); CPU: %s"
+                      % (r["cores"], length, r["steps"], r["walk_s"], r["pre_s"], host_cpu())}
+
+
+def c1_ppr(ctx, args):
+    """PPR with uniform Moves and termination 0.2 on a 2^18-vertex Chung-Lu graph (exponent 2.1,
+    dmin 2, dmax 4096, ~1.5M undirected edges, unweighted): one walker per non-isolated vertex,
+    and the reference's default of |V| walkers from the max-degree vertex (ppr.cpp:53-56,76)."""
+    hg = T.HostGraph.chung_lu(1 << 18, 1500000, 2.1, 2.0, 4096.0, 1, False, 0)
+    deg, adj, _, _ = hg.arrays()
+    dg = T.DeviceGraph(ctx, deg, adj)
+    out = {"graph": "chung-lu 2^18 V, 1.5M undirected edges, gamma 2.1, dmin 2, dmax 4096, unweighted",
+           "directed_edges": int(len(adj))}
+    prm = T.make_params("ppr", stop_probability=0.2, rng="philox", seed=1)
+    for label, src in (("all_sources", T.sources_all(deg)), ("single_source", T.sources_single(deg, len(deg)))):
+        b = Batch(ctx, src, 1, paths=False)
+        io = b.io()
+        T.walk_async(dg, prm, io)
+        T.synchronize(ctx)
+        b.reset()
+        reps = 20
+        timer = T.DeviceTimer(ctx)
+        timer.begin()
+        for _ in range(reps):
+            T.walk_async(dg, prm, io)
+        ms = timer.end() / reps
+        T.synchronize(ctx)
+        out[label + "_steps_per_s"] = b.steps() / reps / (ms / 1e3)
+        out[label + "_ms"] = ms
+        b.free()
+    dg.close()
+    ref = None if args.no_cpu_baseline else reference_run(hg, "ppr", ["-sp", "0.2"])
+    if ref:
+        out["reference_single_source_steps_per_s"] = ref["value"]
+        out["reference_cores"] = ref["cores"]
+    hg.close()
+    log("C1 PPR: %.3e steps/s (all sources), %.3e (single source), reference %s"
+        % (out["all_sources_steps_per_s"], out["single_source_steps_per_s"],
+           "%.3e" % ref["value"] if ref else "n/a"))
+    return out
+
+
+def c3_node2vec(ctx, args):
+    """Node2Vec p=2, q=0.5 (max-weight rejection) on a 2^22-vertex Chung-Lu graph (exponent 2.0,
+    average degree 32, dmin 2, dmax 65536), length 80, one walker per non-isolated vertex."""
+    t0 = time.time()
+    hg = T.HostGraph.chung_lu(1 << 22, 16 << 22, 2.0, 2.0, 65536.0, 3, True, 0)
+    deg, adj, w, _ = hg.arrays()
+    log("C3 graph: %d directed edges, max degree %d (%.1fs)" % (len(adj), deg.max(), time.time() - t0))
+    dg = T.DeviceGraph(ctx, deg, adj, w)
+    src = T.sources_all(deg)
+    b = Batch(ctx, src, 80)
+    prm = T.make_params("node2vec", length=80, p=2.0, q=0.5, rng="philox", seed=3)
+    ms = time_launches(ctx, dg, prm, b.io(), 1, max(1, min(args.steps, 3)))
+    reps = max(1, min(args.steps, 3))
+    steps = 79 * len(src)
+    out = {"graph": "chung-lu 2^22 V, avg deg 32, gamma 2.0, dmin 2, dmax 65536", "directed_edges": int(len(adj)),
+           "walkers": len(src), "steps_per_launch": steps, "steps_per_s": steps / (ms / 1e3 / reps)}
+    b.free()
+    dg.close()
+    ref = None if args.no_cpu_baseline else reference_run(hg, "node2vec", ["-l", "80", "-p", "2", "-q", "0.5"])
+    if ref:
+        out["reference_steps_per_s"] = ref["value"]
+        out["reference_cores"] = ref["cores"]
+    hg.close()
+    log("C3 Node2Vec: %.3e steps/s, reference %s" % (out["steps_per_s"], "%.3e" % ref["value"] if ref else "n/a"))
+    return out
 
 
 def its_walk(ctx, dg, batch, length, per_launch, args):
@@ -272,9 +347,11 @@ def main():
         print("bench.py measures one GPU; this build has no multi-GPU path (walkers would shard by "
               "contiguous ranges with the graph replicated, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    if args.only == "serving":
+    if args.only in ("serving", "c1", "c3"):
         ctx = T.Context(0)
-        print(json.dumps({"ppr_serving": serving(ctx, args)}), flush=True)
+        fn = {"serving": ("ppr_serving", serving), "c1": ("c1_ppr", c1_ppr), "c3": ("c3_node2vec", c3_node2vec)}
+        key, f = fn[args.only]
+        print(json.dumps({key: f(ctx, args)}), flush=True)
         ctx.close()
         return
     nv = 1 << args.scale
@@ -354,6 +431,9 @@ def main():
     dg.close()
     if not args.quick and not args.no_serving:
         details["ppr_serving"] = serving(ctx, args)
+    if not args.quick:
+        details["c1_ppr"] = c1_ppr(ctx, args)
+        details["c3_node2vec"] = c3_node2vec(ctx, args)
 
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20

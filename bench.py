@@ -419,11 +419,14 @@ def main():
     if not args.quick:
         # Measured ceiling for this access pattern: dependent random 16-byte gathers at full
         # occupancy over a table the size of the alias walk records (and a 32 MiB cache-resident one).
-        for label, nbytes in (("probe_gathers_per_s_4GiB", 4 << 30), ("probe_gathers_per_s_32MiB", 32 << 20)):
+        for label, nbytes, rec in (("probe_gathers_per_s_4GiB", 4 << 30, 16),
+                                   ("probe_gathers_per_s_32MiB", 32 << 20, 16),
+                                   ("probe_32B_records_per_s_2GiB", 2 << 30, 32)):
             v = ctypes.c_double()
-            T._abi.check(T.lib().trw_probe_random_gather(ctx.h, nbytes, 4, 64, ctypes.byref(v)))
+            T._abi.check(T.lib().trw_probe_random_gather(ctx.h, nbytes, rec, 4, 64, ctypes.byref(v)))
             details[label] = v.value
-        log("random-gather ceiling: %.3e gathers/s (4 GiB table)" % details["probe_gathers_per_s_4GiB"])
+        log("random-gather ceiling: %.3e 16-byte gathers/s (4 GiB table), %.3e 32-byte records/s (2 GiB)"
+            % (details["probe_gathers_per_s_4GiB"], details["probe_32B_records_per_s_2GiB"]))
 
     if not args.quick:
         details["mix_ppr_metapath"] = mix_forms(ctx, dg, deg, args)

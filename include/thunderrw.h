@@ -158,11 +158,13 @@ int trw_synchronize(trw_context *ctx, void *stream);
 int trw_timer_begin(trw_context *ctx, void *stream);
 int trw_timer_end(trw_context *ctx, void *stream, double *elapsed_ms);
 
-/* Diagnostic (no reference counterpart): dependent random 16-byte gathers per second
- * over a table of table_bytes, `chains` (1, 2, 4, 8) independent chains per lane at full
- * occupancy, `steps` gathers per chain -- the ceiling for a Move that is one such gather. */
-int trw_probe_random_gather(trw_context *ctx, uint64_t table_bytes, int32_t chains, int32_t steps,
-                            double *gathers_per_s);
+/* Diagnostic (no reference counterpart): dependent random gathers per second of
+ * record_bytes (16: one 16-byte load; 32: two adjacent 16-byte loads, the shape of an
+ * alias walk record) over a table of table_bytes, `chains` (1, 2, 4, 8) independent
+ * chains per lane at full occupancy, `steps` gathers per chain -- the ceiling for a
+ * Move that is one such gather. */
+int trw_probe_random_gather(trw_context *ctx, uint64_t table_bytes, int32_t record_bytes, int32_t chains,
+                            int32_t steps, double *gathers_per_s);
 
 /* ---- graph ----
  * Graph::load_csr (util/graph/graph.cpp:45-101) from host arrays: degree[nv]

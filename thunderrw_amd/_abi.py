@@ -99,7 +99,7 @@ _SIGS = {
     "trw_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "trw_device_zero": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     "trw_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
-    "trw_probe_random_gather": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_int32, C.c_void_p]),
+    "trw_probe_random_gather": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "trw_synchronize": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_timer_begin": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_timer_end": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),

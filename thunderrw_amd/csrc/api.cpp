@@ -194,13 +194,15 @@ int trw_device_copy(trw_context *ctx, void *dst, const void *src, uint64_t bytes
     return TRW_OK;
 }
 
-int trw_probe_random_gather(trw_context *ctx, uint64_t table_bytes, int32_t chains, int32_t steps,
-                            double *gathers_per_s) {
+int trw_probe_random_gather(trw_context *ctx, uint64_t table_bytes, int32_t record_bytes, int32_t chains,
+                            int32_t steps, double *gathers_per_s) {
     if (!ctx || !gathers_per_s || steps <= 0) return fail(TRW_E_INVALID, "bad argument");
+    if (record_bytes != 16 && record_bytes != 32) return fail(TRW_E_INVALID, "record_bytes must be 16 or 32");
     TRW_HIP(hipSetDevice(ctx->device), "hipSetDevice");
     int cus = 0;
     TRW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device), "device attribute");
-    TRW_HIP(trw::run_gather_probe(table_bytes, chains, steps, cus, ctx->stream, gathers_per_s), "gather probe");
+    TRW_HIP(trw::run_gather_probe(table_bytes, record_bytes, chains, steps, cus, ctx->stream, gathers_per_s),
+            "gather probe");
     return TRW_OK;
 }
 

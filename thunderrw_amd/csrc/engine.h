@@ -194,8 +194,8 @@ hipError_t launch_serve_ppr_reduce(const ServeSegment *segs, const int32_t *seg_
                                    size_t temp_bytes, hipStream_t stream);
 hipError_t launch_copy_u128(uint4 *dst, const uint4 *src, int64_t n, hipStream_t stream);
 
-hipError_t run_gather_probe(uint64_t table_bytes, int32_t chains, int32_t steps, int32_t cus, hipStream_t stream,
-                            double *loads_per_s);
+hipError_t run_gather_probe(uint64_t table_bytes, int32_t record_bytes, int32_t chains, int32_t steps, int32_t cus,
+                            hipStream_t stream, double *gathers_per_s);
 
 // Scratch entry of the alias build: an index's normalised weight (or residual) and neighbour id.
 struct AliasEntry {

@@ -30,7 +30,8 @@ __global__ void probe_fill_kernel(uint4 *table, int64_t n) {
     }
 }
 
-template <int CHAINS>
+// RECORD 16-byte loads per gather, adjacent (a RECORD * 16-byte record in one line).
+template <int CHAINS, int RECORD>
 __global__ __launch_bounds__(kProbeBlock) void probe_gather_kernel(const uint4 *__restrict__ table, uint32_t n,
                                                                    int32_t steps, uint32_t *out) {
     const uint32_t lane = blockIdx.x * kProbeBlock + threadIdx.x;
@@ -39,13 +40,18 @@ __global__ __launch_bounds__(kProbeBlock) void probe_gather_kernel(const uint4 *
     for (int c = 0; c < CHAINS; ++c) idx[c] = mix32(lane * CHAINS + c) % n;
     uint32_t acc = 0;
     for (int32_t s = 0; s < steps; ++s) {
-        uint4 r[CHAINS];
+        uint4 r[CHAINS][RECORD];
 #pragma unroll
-        for (int c = 0; c < CHAINS; ++c) r[c] = table[idx[c]];
+        for (int c = 0; c < CHAINS; ++c)
+#pragma unroll
+            for (int q = 0; q < RECORD; ++q) r[c][q] = table[static_cast<uint64_t>(idx[c]) * RECORD + q];
 #pragma unroll
         for (int c = 0; c < CHAINS; ++c) {
-            idx[c] = (r[c].x ^ r[c].z ^ static_cast<uint32_t>(s)) % n;
-            acc += r[c].y;
+            uint32_t h = r[c][0].x ^ r[c][0].z;
+#pragma unroll
+            for (int q = 1; q < RECORD; ++q) h ^= r[c][q].y;
+            idx[c] = (h ^ static_cast<uint32_t>(s)) % n;
+            acc += r[c][0].y;
         }
     }
     out[lane] = acc;
@@ -53,14 +59,26 @@ __global__ __launch_bounds__(kProbeBlock) void probe_gather_kernel(const uint4 *
 
 }  // namespace
 
-hipError_t run_gather_probe(uint64_t table_bytes, int32_t chains, int32_t steps, int32_t cus, hipStream_t stream,
-                            double *loads_per_s) {
-    const int64_t n = static_cast<int64_t>(table_bytes / 16);
+template <int RECORD>
+void launch_probe(int32_t chains, dim3 grid, dim3 block, hipStream_t stream, const uint4 *table, uint32_t n,
+                  int32_t steps, uint32_t *out) {
+    switch (chains) {
+    case 1: hipLaunchKernelGGL((probe_gather_kernel<1, RECORD>), grid, block, 0, stream, table, n, steps, out); break;
+    case 2: hipLaunchKernelGGL((probe_gather_kernel<2, RECORD>), grid, block, 0, stream, table, n, steps, out); break;
+    case 4: hipLaunchKernelGGL((probe_gather_kernel<4, RECORD>), grid, block, 0, stream, table, n, steps, out); break;
+    default: hipLaunchKernelGGL((probe_gather_kernel<8, RECORD>), grid, block, 0, stream, table, n, steps, out); break;
+    }
+}
+
+hipError_t run_gather_probe(uint64_t table_bytes, int32_t record_bytes, int32_t chains, int32_t steps, int32_t cus,
+                            hipStream_t stream, double *loads_per_s) {
+    const int32_t record = record_bytes == 32 ? 2 : 1;
+    const int64_t n = static_cast<int64_t>(table_bytes / (16 * record));  // records
     if (n < 1 || n > 0xffffffffLL) return hipErrorInvalidValue;
     uint4 *table = nullptr;
     uint32_t *out = nullptr;
     const int64_t lanes = static_cast<int64_t>(cus) * 2048;  // 32 wavefronts of 64 lanes per CU
-    hipError_t e = hipMalloc(&table, static_cast<size_t>(n) * 16);
+    hipError_t e = hipMalloc(&table, static_cast<size_t>(n) * 16 * record);
     if (e != hipSuccess) return e;
     e = hipMalloc(&out, static_cast<size_t>(lanes) * 4);
     if (e != hipSuccess) {
@@ -70,16 +88,12 @@ hipError_t run_gather_probe(uint64_t table_bytes, int32_t chains, int32_t steps,
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
-    hipLaunchKernelGGL(probe_fill_kernel, dim3(4096), dim3(256), 0, stream, table, n);
+    hipLaunchKernelGGL(probe_fill_kernel, dim3(4096), dim3(256), 0, stream, table, n * record);
     const dim3 grid(static_cast<unsigned>(lanes / kProbeBlock)), block(kProbeBlock);
     for (int rep = 0; rep < 2; ++rep) {  // first launch warms the caches and the clocks
         (void)hipEventRecord(a, stream);
-        switch (chains) {
-        case 1: hipLaunchKernelGGL((probe_gather_kernel<1>), grid, block, 0, stream, table, static_cast<uint32_t>(n), steps, out); break;
-        case 2: hipLaunchKernelGGL((probe_gather_kernel<2>), grid, block, 0, stream, table, static_cast<uint32_t>(n), steps, out); break;
-        case 4: hipLaunchKernelGGL((probe_gather_kernel<4>), grid, block, 0, stream, table, static_cast<uint32_t>(n), steps, out); break;
-        default: hipLaunchKernelGGL((probe_gather_kernel<8>), grid, block, 0, stream, table, static_cast<uint32_t>(n), steps, out); break;
-        }
+        if (record == 2) launch_probe<2>(chains, grid, block, stream, table, static_cast<uint32_t>(n), steps, out);
+        else launch_probe<1>(chains, grid, block, stream, table, static_cast<uint32_t>(n), steps, out);
         (void)hipEventRecord(b, stream);
     }
     e = hipEventSynchronize(b);

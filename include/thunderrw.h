@@ -189,6 +189,13 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights);
 int trw_copy_alias_table(const trw_graph *g, trw_alias_slot *host_out);
 int trw_copy_prefix_sum(const trw_graph *g, double *host_out);
 int trw_copy_rejection_max(const trw_graph *g, double *host_out);
+/* The reference's table files (graph.cpp:18-30, 184-212; written by alias_preprocess,
+ * its_preprocess and rejection_preprocess): b_edge_weight_alias_table.bin (AliasSlot[E]),
+ * b_edge_weight_prefix_sum.bin (double[E]), b_edge_weight_max.bin (double[V]), raw arrays.
+ * store writes every table that has been built; load reads one into the graph in place of a
+ * build, `weights` naming what the file was built from (TRW_WEIGHTS_*). */
+int trw_graph_store_tables(const trw_graph *g, const char *dir);
+int trw_graph_load_tables(trw_graph *g, const char *dir, int32_t sampling, int32_t weights);
 /* Device time of the last trw_build_tables call in milliseconds. */
 int trw_last_build_ms(const trw_graph *g, double *ms);
 
@@ -267,6 +274,12 @@ int trw_host_graph_chung_lu(int32_t num_vertices, int64_t undirected_edges, doub
  * b_edge_weight.bin / b_edge_label.bin. */
 int trw_host_graph_load(const char *dir, int32_t with_weights, int32_t with_labels, trw_host_graph **out);
 int trw_host_graph_store(const trw_host_graph *h, const char *dir);
+/* Graph::load_edge_list(dir, skip) (graph.cpp:428-519; toolset/edgelist_to_csr.cpp): text
+ * lines "src dst" of <dir>/b_edge_list.bin, lines starting with `skip` ignored, pairs ordered
+ * (min, max), sorted, duplicates and self-loops dropped, symmetrised; vertex count = largest
+ * id + 1.  A line that does not hold two integers, or holds a negative one, is TRW_E_IO (the
+ * reference exits). */
+int trw_host_graph_load_edge_list(const char *dir, int32_t skip, trw_host_graph **out);
 int trw_host_graph_arrays(const trw_host_graph *h, int32_t *num_vertices, int64_t *num_edges,
                           const int32_t **degree, const int32_t **adj, const double **edge_weight,
                           const int32_t **edge_label);

@@ -112,6 +112,8 @@ _SIGS = {
     "trw_copy_prefix_sum": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_copy_rejection_max": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_last_build_ms": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_graph_store_tables": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "trw_graph_load_tables": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]),
     "trw_graph_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "trw_graph_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "trw_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_int32, C.c_void_p]),
@@ -126,6 +128,7 @@ _SIGS = {
     "trw_host_graph_chung_lu": (C.c_int, [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                           C.c_int32, C.c_int32, C.c_void_p]),
     "trw_host_graph_load": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "trw_host_graph_load_edge_list": (C.c_int, [C.c_char_p, C.c_int32, C.c_void_p]),
     "trw_host_graph_store": (C.c_int, [C.c_void_p, C.c_char_p]),
     "trw_host_graph_arrays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]),

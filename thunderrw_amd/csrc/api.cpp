@@ -482,6 +482,106 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
     return TRW_OK;
 }
 
+// ---- table files (graph.cpp:18-30, 184-212) ----
+
+namespace {
+
+const char *table_file(int32_t sampling) {
+    return sampling == TRW_SM_ALIAS ? "b_edge_weight_alias_table.bin"
+           : sampling == TRW_SM_ITS ? "b_edge_weight_prefix_sum.bin"
+                                    : "b_edge_weight_max.bin";
+}
+
+int write_device_array(const std::string &path, const void *dev, size_t bytes) {
+    std::vector<char> host(bytes);
+    if (bytes) TRW_HIP(hipMemcpy(host.data(), dev, bytes, hipMemcpyDeviceToHost), "copy table to host");
+    std::FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return fail(TRW_E_IO, "cannot create %s", path.c_str());
+    const size_t w = bytes ? std::fwrite(host.data(), 1, bytes, f) : 0;
+    const int c = std::fclose(f);
+    if (w != bytes || c != 0) return fail(TRW_E_IO, "write failed: %s", path.c_str());
+    return TRW_OK;
+}
+
+}  // namespace
+
+int trw_graph_store_tables(const trw_graph *g, const char *dir) {
+    if (!g || !dir) return fail(TRW_E_INVALID, "null argument");
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    TRW_HIP(hipStreamSynchronize(g->ctx->stream), "synchronize");
+    const std::string d(dir);
+    const size_t E = static_cast<size_t>(g->ne), V = static_cast<size_t>(g->nv);
+    int written = 0, rc = TRW_OK;
+    if (g->alias && g->alias_weights >= 0 && rc == TRW_OK) {
+        rc = write_device_array(d + "/" + table_file(TRW_SM_ALIAS), g->alias, sizeof(AliasSlot) * E);
+        ++written;
+    }
+    if (g->prefix && g->its_weights >= 0 && rc == TRW_OK) {
+        rc = write_device_array(d + "/" + table_file(TRW_SM_ITS), g->prefix, sizeof(double) * E);
+        ++written;
+    }
+    if (g->rs_max && g->rs_weights >= 0 && rc == TRW_OK) {
+        rc = write_device_array(d + "/" + table_file(TRW_SM_REJECTION), g->rs_max, sizeof(double) * V);
+        ++written;
+    }
+    if (rc != TRW_OK) return rc;
+    if (!written) return fail(TRW_E_STATE, "no table has been built");
+    return TRW_OK;
+}
+
+int trw_graph_load_tables(trw_graph *g, const char *dir, int32_t sampling, int32_t weights) {
+    if (!g || !dir) return fail(TRW_E_INVALID, "null argument");
+    if (sampling != TRW_SM_ALIAS && sampling != TRW_SM_ITS && sampling != TRW_SM_REJECTION)
+        return fail(TRW_E_INVALID, "table files exist for ITS, alias and rejection sampling only");
+    if (weights != TRW_WEIGHTS_EDGE && weights != TRW_WEIGHTS_UNIT) return fail(TRW_E_INVALID, "unknown weights %d", weights);
+    const size_t E = static_cast<size_t>(g->ne), V = static_cast<size_t>(g->nv);
+    const size_t bytes = sampling == TRW_SM_ALIAS ? sizeof(AliasSlot) * E
+                         : sampling == TRW_SM_ITS ? sizeof(double) * E
+                                                  : sizeof(double) * V;
+    const std::string path = std::string(dir) + "/" + table_file(sampling);
+    std::FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return fail(TRW_E_IO, "cannot open %s", path.c_str());
+    std::vector<char> host(bytes + 1);
+    const size_t got = std::fread(host.data(), 1, bytes + 1, f);
+    std::fclose(f);
+    if (got != bytes) return fail(TRW_E_IO, "%s holds %zu bytes, the graph needs %zu", path.c_str(), got, bytes);
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    hipStream_t st = g->ctx->stream;
+    TRW_HIP(hipStreamSynchronize(st), "synchronize");
+    if (sampling == TRW_SM_ALIAS) {
+        if (!g->alias) TRW_HIP(dmalloc(&g->alias, E + 1), "hipMalloc alias table");
+        if (E) TRW_HIP(hipMemcpy(g->alias, host.data(), bytes, hipMemcpyHostToDevice), "copy alias table");
+        g->alias_weights = weights;
+        if (g->vrec) {
+            if (!g->arec) TRW_HIP(dmalloc(&g->arec, 2 * E + 2), "hipMalloc alias walk records");
+            TRW_HIP(trw::launch_alias_records(g->view(), g->arec, st), "alias walk records");
+        }
+    } else if (sampling == TRW_SM_ITS) {
+        if (!g->prefix) TRW_HIP(dmalloc(&g->prefix, E + 2 * trw::kItsFan), "hipMalloc prefix sums");
+        if (E) TRW_HIP(hipMemcpy(g->prefix, host.data(), bytes, hipMemcpyHostToDevice), "copy prefix sums");
+        // Search trees need non-decreasing sums within every vertex (so a file written from
+        // weights with a negative entry gets none, as a device build would).
+        const double *P = reinterpret_cast<const double *>(host.data());
+        std::vector<int32_t> degree(V);
+        if (V) TRW_HIP(hipMemcpy(degree.data(), g->degree, sizeof(int32_t) * V, hipMemcpyDeviceToHost), "copy degree");
+        bool monotone = true;
+        size_t e = 0;
+        for (size_t v = 0; v < V && monotone; ++v) {
+            for (int32_t j = 1; j < degree[v] && monotone; ++j) monotone = P[e + j - 1] <= P[e + j];
+            e += static_cast<size_t>(degree[v]);
+        }
+        g->its_weights = weights;
+        int rc = build_its_trees(g, monotone);
+        if (rc != TRW_OK) return rc;
+    } else {
+        if (!g->rs_max) TRW_HIP(dmalloc(&g->rs_max, V), "hipMalloc rejection maxima");
+        if (V) TRW_HIP(hipMemcpy(g->rs_max, host.data(), bytes, hipMemcpyHostToDevice), "copy rejection maxima");
+        g->rs_weights = weights;
+    }
+    TRW_HIP(hipStreamSynchronize(st), "load tables");
+    return TRW_OK;
+}
+
 int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value) {
     if (!g) return fail(TRW_E_INVALID, "graph is null");
     switch (knob) {
@@ -961,6 +1061,16 @@ int trw_host_graph_load(const char *dir, int32_t with_weights, int32_t with_labe
     std::string err;
     if (!trw::load_reference_csr(dir, with_weights != 0, with_labels != 0, h->g, err))
         return fail(TRW_E_IO, "%s", err.c_str());
+    *out = h.release();
+    return TRW_OK;
+}
+
+int trw_host_graph_load_edge_list(const char *dir, int32_t skip, trw_host_graph **out) {
+    if (!dir || !out) return fail(TRW_E_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<trw_host_graph> h(new trw_host_graph);
+    std::string err;
+    if (!trw::load_edge_list(dir, static_cast<char>(skip), h->g, err)) return fail(TRW_E_IO, "%s", err.c_str());
     *out = h.release();
     return TRW_OK;
 }

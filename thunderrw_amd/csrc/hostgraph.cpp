@@ -2,11 +2,13 @@
 
 #include <algorithm>
 #include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <numeric>
 #include <sys/stat.h>
 
@@ -295,6 +297,99 @@ bool store_reference_csr(const HostGraph &g, const std::string &dir, std::string
     if (!write_array_file(dir + "/b_adj.bin", g.adj.data(), g.ne, err)) return false;
     if (!g.weight.empty() && !write_array_file(dir + "/b_edge_weight.bin", g.weight.data(), g.ne, err)) return false;
     if (!g.label.empty() && !write_array_file(dir + "/b_edge_label.bin", g.label.data(), g.ne, err)) return false;
+    return true;
+}
+
+// Graph::load_edge_list(dir, skip) (util/graph/graph.cpp:428-519): text lines "src dst"
+// of <dir>/b_edge_list.bin, lines starting with `skip` ignored; each pair ordered
+// (min, max), pairs sorted, duplicates and self-loops dropped, both directions added.
+// The vertex count is the largest id seen on any line + 1.  The adjacency of v comes
+// out sorted: its smaller neighbours (edges whose first end is smaller) precede its
+// larger ones, each in ascending order, exactly as the reference's append order gives.
+namespace {
+
+// `ss >> src >> dst` for int32: skips whitespace, optional sign, digits; false on failure.
+bool parse_int32(const char *&p, const char *end, int32_t &out) {
+    while (p < end && (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\v' || *p == '\f')) ++p;
+    bool neg = false;
+    if (p < end && (*p == '+' || *p == '-')) neg = *p++ == '-';
+    if (p >= end || *p < '0' || *p > '9') return false;
+    int64_t v = 0;
+    while (p < end && *p >= '0' && *p <= '9') {
+        v = v * 10 + (*p++ - '0');
+        if (v > int64_t(INT32_MAX) + 1) return false;
+    }
+    if (neg) v = -v;
+    if (v > INT32_MAX || v < INT32_MIN) return false;
+    out = static_cast<int32_t>(v);
+    return true;
+}
+
+}  // namespace
+
+bool load_edge_list(const std::string &dir, char skip, HostGraph &out, std::string &err) {
+    const std::string path = dir + "/b_edge_list.bin";
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        err = "cannot open " + path;
+        return false;
+    }
+    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<uint64_t> keys;
+    int32_t max_id = 0;
+    uint64_t line_no = 0;
+    const char *p = text.data(), *end = text.data() + text.size();
+    while (p < end) {
+        const char *eol = static_cast<const char *>(std::memchr(p, '\n', static_cast<size_t>(end - p)));
+        if (!eol) eol = end;
+        ++line_no;
+        // std::string::operator[] on an empty line gives '\0', which is never the skip
+        // character, so (as in the reference) an empty line must parse as an edge.
+        const char first = p < eol ? *p : '\0';
+        if (first != skip) {
+            const char *q = p;
+            int32_t a, b;
+            if (!parse_int32(q, eol, a) || !parse_int32(q, eol, b)) {
+                err = "cannot convert line " + std::to_string(line_no) + " to edge";
+                return false;
+            }
+            if (a < 0 || b < 0) {
+                err = "negative vertex id on line " + std::to_string(line_no);
+                return false;
+            }
+            if (a > b) std::swap(a, b);
+            keys.push_back((static_cast<uint64_t>(a) << 32) | static_cast<uint32_t>(b));
+            max_id = std::max(max_id, b);
+        }
+        p = eol < end ? eol + 1 : end;
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int32_t nv = max_id + 1;
+    std::vector<int64_t> off(static_cast<size_t>(nv) + 1, 0);
+    std::vector<int32_t> deg(static_cast<size_t>(nv), 0);
+    int64_t ne = 0;
+    for (uint64_t k : keys) {
+        const int32_t a = static_cast<int32_t>(k >> 32), b = static_cast<int32_t>(k & 0xffffffffu);
+        if (a == b) continue;
+        ++deg[a];
+        ++deg[b];
+        ne += 2;
+    }
+    for (int32_t v = 0; v < nv; ++v) off[v + 1] = off[v] + deg[v];
+    std::vector<int32_t> adj(static_cast<size_t>(ne));
+    std::vector<int64_t> fill(off.begin(), off.end() - 1);
+    for (uint64_t k : keys) {
+        const int32_t a = static_cast<int32_t>(k >> 32), b = static_cast<int32_t>(k & 0xffffffffu);
+        if (a == b) continue;
+        adj[fill[a]++] = b;
+        adj[fill[b]++] = a;
+    }
+    out = HostGraph();
+    out.nv = nv;
+    out.ne = ne;
+    out.degree.swap(deg);
+    out.adj.swap(adj);
     return true;
 }
 

@@ -43,6 +43,7 @@ bool chung_lu(int32_t nv, int64_t undirected_edges, double gamma, double dmin, d
               bool weights, int32_t n_labels, HostGraph &out, std::string &err);
 bool load_reference_csr(const std::string &dir, bool weights, bool labels, HostGraph &out, std::string &err);
 bool store_reference_csr(const HostGraph &g, const std::string &dir, std::string &err);
+bool load_edge_list(const std::string &dir, char skip, HostGraph &out, std::string &err);
 
 // Stable per-vertex ordering of segment positions by label (the MetaPath label index).
 void build_label_index(int32_t nv, const int64_t *offset, const int32_t *label, int32_t *sorted_labels,

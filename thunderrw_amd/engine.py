@@ -97,6 +97,13 @@ class HostGraph:
         check(lib().trw_host_graph_load(directory.encode(), int(weights), int(labels), C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def load_edge_list(cls, directory, skip="#"):
+        """<directory>/b_edge_list.bin as text edge list (Graph::load_edge_list semantics)."""
+        h = C.c_void_p()
+        check(lib().trw_host_graph_load_edge_list(directory.encode(), ord(skip), C.byref(h)))
+        return cls(h)
+
     def store(self, directory):
         check(lib().trw_host_graph_store(self.h, directory.encode()))
 
@@ -107,12 +114,12 @@ class HostGraph:
         check(lib().trw_host_graph_arrays(self.h, C.byref(nv), C.byref(ne), C.byref(d), C.byref(a), C.byref(w),
                                           C.byref(l_)))
 
-        def view(p, n, ct, dt):
+        def view(p, n, ct, dt, optional=True):
             if not p.value or n == 0:
-                return None if not p.value else np.zeros(0, dt)
+                return None if (optional and not p.value) else np.zeros(0, dt)
             return np.ctypeslib.as_array(C.cast(p.value, C.POINTER(ct)), shape=(n,))
 
-        return (view(d, nv.value, C.c_int32, np.int32), view(a, ne.value, C.c_int32, np.int32),
+        return (view(d, nv.value, C.c_int32, np.int32, False), view(a, ne.value, C.c_int32, np.int32, False),
                 view(w, ne.value, C.c_double, np.float64), view(l_, ne.value, C.c_int32, np.int32))
 
     def close(self):
@@ -149,6 +156,15 @@ class DeviceGraph:
     def build_tables(self, sampling, weights="edge"):
         w = A.WEIGHTS_EDGE if weights == "edge" else A.WEIGHTS_UNIT
         check(lib().trw_build_tables(self.h, _enum(_SM, sampling), w))
+
+    def store_tables(self, directory):
+        """Writes the built tables as the reference's b_edge_weight_*.bin files."""
+        check(lib().trw_graph_store_tables(self.h, directory.encode()))
+
+    def load_tables(self, directory, sampling, weights="edge"):
+        """Reads one of the reference's table files in place of a build."""
+        w = A.WEIGHTS_EDGE if weights == "edge" else A.WEIGHTS_UNIT
+        check(lib().trw_graph_load_tables(self.h, directory.encode(), _enum(_SM, sampling), w))
 
     def last_build_ms(self):
         ms = C.c_double()

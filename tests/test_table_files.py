@@ -53,7 +53,17 @@ def test_stored_tables_match_oracle_and_reference_tools(gpu_ctx, gname):
                     continue
                 r = subprocess.run([exe, "-f", ref_dir], capture_output=True, text=True, timeout=120)
                 assert r.returncode == 0, r.stderr[-1000:]
-                assert read_bytes(os.path.join(ref_dir, FILES[sm])) == read_bytes(os.path.join(tmp, FILES[sm])), sm
+                ref_b, ours_b = read_bytes(os.path.join(ref_dir, FILES[sm])), read_bytes(os.path.join(tmp, FILES[sm]))
+                if sm == "rejection":
+                    # rejection_preprocess.cpp:26 dereferences max_element of an empty range for an
+                    # isolated vertex (it stores the next vertex's first weight); the build defines
+                    # that maximum as 0.0.  Every other vertex must match bit for bit.
+                    live = g.degree > 0
+                    ref_m, ours_m = np.frombuffer(ref_b, np.uint64), np.frombuffer(ours_b, np.uint64)
+                    assert np.array_equal(ref_m[live], ours_m[live])
+                    assert np.all(np.frombuffer(ours_b, np.float64)[~live] == 0.0)
+                else:
+                    assert ref_b == ours_b, sm
             # Loading the files instead of building gives the same walks.
             dl = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight)
             try:

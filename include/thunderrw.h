@@ -229,7 +229,12 @@ int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_wal
 /* ---- walks ----
  * compute<F>(graph, walkers, f) (creeper.h:689-831) for a built-in app, on a
  * caller stream (hipStream_t as void*; NULL = the context stream).  Asynchronous:
- * returns after the launches are queued. */
+ * returns after the launches are queued (the STEP and GRAPH forms read a live-walker
+ * count back between step chunks, so they return when the walk is done).
+ * A graph's walks use its work-queue head and walker-state scratch: walks on one graph
+ * must not overlap in time, from different host threads or on different streams (the
+ * query server gives each of its slots its own queue head for that reason).  Tables
+ * are built lazily by the first walk that needs them unless trw_build_tables ran. */
 int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream);
 
 /* Two walker kinds in one batch: walkers [0, split) run params_a, [split, n) run
@@ -366,7 +371,9 @@ typedef struct {
 } trw_open_loop_report;
 
 int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server **out);
-/* Stops the dispatcher after the queries in flight finish and frees every result. */
+/* Stops the dispatcher after the queries in flight finish and frees every result.  No other
+ * call on the server may be running; destroy servers before their graph.  The other
+ * trw_server_* calls may be made from any number of host threads at once. */
 int trw_server_destroy(trw_server *s);
 /* Non-blocking; the query's arrival time is the time of the call. */
 int trw_server_submit(trw_server *s, const trw_query *q, uint64_t *query_id);

@@ -155,3 +155,31 @@ def test_open_loop_reports_latency_percentiles(served):
         print("open loop: %.0f qps offered, %.0f achieved, p50 %.3f ms, p99 %.3f ms, %.3g steps/s, "
               "%.1f queries per batch" % (rep["offered_qps"], rep["achieved_qps"], rep["p50_ms"], rep["p99_ms"],
                                           rep["steps_per_s"], rep["mean_batch_queries"]))
+
+
+def test_concurrent_clients_get_their_own_results(served):
+    """Eight host threads submit and wait on one server at once (ctypes releases the GIL
+    during each call); every result equals the one served alone."""
+    import threading
+    g, dg = served
+    qs = make_queries(g, 48, 9)
+    with T.Server(dg, streams=1) as srv:
+        alone = [collect(srv, srv.submit(params_for(k), s, n, b, result=k), k) for k, s, n, b in qs]
+    results = [None] * len(qs)
+    errors = []
+    with T.Server(dg, streams=4, max_batch_walkers=32768) as srv:
+        def client(idx):
+            try:
+                for i in idx:
+                    k, s, n, b = qs[i]
+                    results[i] = collect(srv, srv.submit(params_for(k), s, n, b, result=k), k)
+            except Exception as e:  # surfaced below
+                errors.append(e)
+        threads = [threading.Thread(target=client, args=(range(t, len(qs), 8),)) for t in range(8)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+    assert not errors, errors
+    for i, (a, b) in enumerate(zip(alone, results)):
+        assert same(a, b), i

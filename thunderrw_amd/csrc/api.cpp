@@ -250,7 +250,7 @@ static void graph_release(trw_graph *g) {
     dfree(g->lbl_pos);
     dfree(g->lstart);
     dfree(g->lerec);
-    dfree(g->its_idx);
+    dfree(g->its_lines);
     dfree(g->its_root);
     dfree(g->order);
     dfree(g->vrec);
@@ -388,31 +388,35 @@ int trw_graph_info(const trw_graph *g, int32_t *nv, int64_t *ne, int32_t *max_de
     return TRW_OK;
 }
 
-// Search trees over every vertex's prefix sums (16-key, 128-byte nodes; see its_levels).
-// The tree's upper bound equals the binary search's only for non-decreasing sums, so
-// weights with a negative or NaN entry get none.
+// ITS search structures over every vertex's prefix sums (128-byte lines: a 16-key B+-tree
+// over leaves of 5 prefix sums and their neighbour records; see its_levels).  The tree's
+// upper bound equals the binary search's only for non-decreasing sums, so weights with a
+// negative or NaN entry get none; the leaves carry neighbour records, so graphs without
+// them (2^32 edges or more) get none either.
 static int build_its_trees(trw_graph *g, bool monotone) {
-    dfree(g->its_idx);
-    if (!monotone || g->nv == 0) return TRW_OK;
+    dfree(g->its_lines);
+    if (!monotone || g->nv == 0 || !g->erec) return TRW_OK;
     hipStream_t st = g->ctx->stream;
     const size_t V = static_cast<size_t>(g->nv);
-    if (!g->its_root) TRW_HIP(dmalloc(&g->its_root, V), "hipMalloc ITS tree roots");
-    TRW_HIP(trw::launch_its_tree_sizes(g->view(), g->its_root, st), "ITS tree sizes");
+    if (!g->its_root) TRW_HIP(dmalloc(&g->its_root, V), "hipMalloc ITS structure roots");
+    TRW_HIP(trw::launch_its_tree_sizes(g->view(), g->its_root, st), "ITS structure sizes");
     std::vector<uint32_t> root(V);
-    TRW_HIP(hipMemcpyAsync(root.data(), g->its_root, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, st), "ITS tree sizes");
-    TRW_HIP(hipStreamSynchronize(st), "ITS tree sizes");
+    TRW_HIP(hipMemcpyAsync(root.data(), g->its_root, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, st),
+            "ITS structure sizes");
+    TRW_HIP(hipStreamSynchronize(st), "ITS structure sizes");
     uint64_t total = 0;
     for (size_t v = 0; v < V; ++v) {
         const uint32_t c = root[v];
         root[v] = c ? static_cast<uint32_t>(total) : trw::kNoRange;
         total += c;
     }
-    if (total == 0 || total >= trw::kNoRange) return TRW_OK;  // nothing to index, or roots overflow 32 bits
-    TRW_HIP(hipMemcpyAsync(g->its_root, root.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice, st), "ITS tree roots");
-    TRW_HIP(dmalloc(&g->its_idx, static_cast<size_t>(total) * trw::kItsFan), "hipMalloc ITS trees");
-    TRW_HIP(trw::launch_its_tree_fill(g->view(), g->prefix, g->its_root, g->its_idx, st), "ITS tree fill");
-    TRW_HIP(trw::launch_its_tree_roots_to_records(g->view(), g->its_root, g->erec, st), "ITS tree roots");
-    TRW_HIP(hipStreamSynchronize(st), "ITS trees");  // the host copy of the roots must outlive the upload
+    if (total == 0 || total >= trw::kNoRange) return TRW_OK;  // nothing to index, or line indices overflow 32 bits
+    TRW_HIP(hipMemcpyAsync(g->its_root, root.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice, st),
+            "ITS structure roots");
+    TRW_HIP(dmalloc(&g->its_lines, static_cast<size_t>(total) * 8), "hipMalloc ITS structures");
+    TRW_HIP(trw::launch_its_tree_fill(g->view(), g->prefix, g->its_root, g->its_lines, st), "ITS structure fill");
+    TRW_HIP(trw::launch_its_tree_roots_to_records(g->view(), g->its_root, g->erec, st), "ITS structure roots");
+    TRW_HIP(hipStreamSynchronize(st), "ITS structures");  // the host copy of the roots must outlive the upload
     return TRW_OK;
 }
 

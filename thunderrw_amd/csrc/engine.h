@@ -59,33 +59,31 @@ struct DevGraph {
                                //       with label < k (the start of label k in label order), else null
     const uint4 *lerec;        // ne with lstart: neighbour records in label order,
                                //       lerec[off(v) + j] = erec[off(v) + lbl_pos[off(v) + j]]
-    const double *its_idx;     // ITS search trees (16-key nodes), or null
-    const uint32_t *its_root;  // nv: first node of v's tree in its_idx, kNoRange if v has none
+    const uint4 *its_lines;    // ITS search structures (128-byte lines, 8 uint4 each), or null
+    const uint32_t *its_root;  // nv: first line of v's structure, kNoRange if v has none
 };
 
-// ITS search tree of one vertex.  Its prefix sums P[off, off + d) split into leaf
-// blocks at the 16-element (128-byte) boundaries of the global prefix array; level 1
-// holds the last prefix value of every leaf block, and each higher level the last key
-// of every 16-key node below, up to a single root node.  Nodes are 16 doubles
-// (+inf padded), stored root level first.  The search makes the comparisons
-// `key <= target` of an upper bound, so for non-decreasing sums it returns exactly
-// what the branchless binary search returns.
-constexpr int kItsFan = 16;
+// ITS search structure of one vertex, built from its prefix sums when they are
+// non-decreasing: 128-byte lines, tree first.  Leaf line i holds the prefix sums of the
+// vertex's edges [5i, 5i + 5) (5 doubles, +inf padded, then 8 bytes of padding) and their
+// neighbour records {neighbour, its first edge, its degree, its structure's first line}
+// (5 x 16 bytes).  Above the leaves, for more than one leaf, a B+-tree of 16-key
+// (128-byte) nodes: level 1 holds the last prefix sum of each leaf, each higher level the
+// last key of each node below, up to one root node; levels are stored root first.  A
+// search reads the root, one node per level and one leaf, and that leaf also holds the
+// chosen neighbour's record.  The comparisons are the `key <= target` of an upper bound,
+// so for non-decreasing sums the pick is exactly the branchless binary search's.
+constexpr int kItsFan = 16;  // keys per tree node
+constexpr int kItsLeaf = 5;  // edges per leaf line
 constexpr int kItsMaxLevels = 8;
 
-__host__ __device__ inline int32_t its_leaf_blocks(int64_t off, int32_t d) {
-    if (d <= 0) return 0;
-    const int64_t first_end = ((off >> 4) + 1) << 4;
-    const int64_t end = off + d;
-    if (end <= first_end) return 1;
-    return 1 + static_cast<int32_t>((end - first_end + 15) >> 4);
-}
+__host__ __device__ inline int32_t its_leaves(int32_t d) { return (d + kItsLeaf - 1) / kItsLeaf; }
 
-// Node counts per level (level 1 first); returns the number of levels (0 for one leaf block).
-__host__ __device__ inline int32_t its_levels(int32_t nb, int64_t *nodes) {
-    if (nb <= 1) return 0;
+// Tree node counts per level (level 1 first); returns the number of levels (0 for one leaf).
+__host__ __device__ inline int32_t its_levels(int32_t nl, int64_t *nodes) {
+    if (nl <= 1) return 0;
     int32_t L = 0;
-    int64_t keys = nb;
+    int64_t keys = nl;
     do {
         nodes[L] = (keys + kItsFan - 1) / kItsFan;
         keys = nodes[L];
@@ -211,10 +209,11 @@ hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, 
 hipError_t launch_edge_records(const DevGraph &g, uint4 *erec, hipStream_t stream);
 hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stream);
 hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lerec, hipStream_t stream);
-// ITS search trees: node counts per vertex into root (then the caller turns them into
-// offsets), the trees themselves, and the roots copied into the neighbour records' .w.
+// ITS search structures: line counts per vertex into root (then the caller turns them
+// into offsets), the structures themselves, and the roots copied into the neighbour
+// records' .w.
 hipError_t launch_its_tree_sizes(const DevGraph &g, uint32_t *root, hipStream_t stream);
-hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint32_t *root, double *idx,
+hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint32_t *root, uint4 *lines,
                                 hipStream_t stream);
 hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *root, uint4 *erec, hipStream_t stream);
 hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,

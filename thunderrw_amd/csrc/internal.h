@@ -30,8 +30,8 @@ struct trw_graph {
     uint4 *arec = nullptr;
     uint32_t *lstart = nullptr;
     uint4 *lerec = nullptr;
-    double *its_idx = nullptr;     // ITS search trees (null: not built, or weights not all >= 0)
-    uint32_t *its_root = nullptr;  // nv tree roots
+    uint4 *its_lines = nullptr;    // ITS search structures (null: not built, or sums not non-decreasing)
+    uint32_t *its_root = nullptr;  // nv: first line of each vertex's structure
     bool weights_nonneg = true;    // every edge weight is finite and >= 0
     int32_t build_hubs = 0;        // vertices of degree >= kWaveBuildMin (a prefix of order)
     int64_t tune[3] = {0, 0, 0};   // TRW_TUNE_* knobs
@@ -64,7 +64,7 @@ struct trw_graph {
         g.arec = arec;
         g.lstart = lstart;
         g.lerec = lerec;
-        g.its_idx = its_idx;
+        g.its_lines = its_lines;
         g.its_root = its_root;
         return g;
     }

@@ -446,53 +446,78 @@ __global__ void label_records_kernel(const int64_t *offset, int32_t nv, const in
     }
 }
 
-// ITS search trees.  One thread per vertex; hubs write a few thousand keys once.
+// ITS search structures.  One thread per vertex; a hub writes a few thousand lines once.
 __global__ void its_tree_sizes_kernel(const int64_t *offset, int32_t nv, uint32_t *count) {
     const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v >= nv) return;
-    const int64_t off = offset[v];
+    const int32_t d = static_cast<int32_t>(offset[v + 1] - offset[v]);
+    const int32_t nl = its_leaves(d);
     int64_t nodes[kItsMaxLevels];
-    const int32_t L = its_levels(its_leaf_blocks(off, static_cast<int32_t>(offset[v + 1] - off)), nodes);
-    int64_t total = 0;
+    const int32_t L = its_levels(nl, nodes);
+    int64_t total = nl;
     for (int32_t k = 0; k < L; ++k) total += nodes[k];
     count[v] = static_cast<uint32_t>(total);
 }
 
-__global__ void its_tree_fill_kernel(const int64_t *offset, int32_t nv, const double *P, const uint32_t *root,
-                                     double *idx) {
+__global__ void its_tree_fill_kernel(const int64_t *offset, int32_t nv, const double *P, const uint4 *erec,
+                                     const uint32_t *root, uint4 *lines) {
     const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v >= nv || root[v] == kNoRange) return;
     const int64_t off = offset[v];
     const int32_t d = static_cast<int32_t>(offset[v + 1] - off);
-    const int32_t nb = its_leaf_blocks(off, d);
+    const int32_t nl = its_leaves(d);
     int64_t nodes[kItsMaxLevels];
-    const int32_t L = its_levels(nb, nodes);
-    // Level k (0 = level 1) starts after the levels above it.
+    const int32_t L = its_levels(nl, nodes);
+    // Tree level k (0 = level 1) starts after the levels above it; the leaves follow the tree.
     int64_t base[kItsMaxLevels];
     int64_t acc = 0;
     for (int32_t k = L - 1; k >= 0; --k) {
         base[k] = acc;
         acc += nodes[k];
     }
-    double *tree = idx + static_cast<int64_t>(root[v]) * kItsFan;
-    const int64_t first_end = ((off >> 4) + 1) << 4;
-    // Level 1: the last prefix value of every leaf block.
-    double *l1 = tree + base[0] * kItsFan;
-    for (int64_t i = 0; i < nodes[0] * kItsFan; ++i) {
-        double key = __longlong_as_double(0x7ff0000000000000LL);  // +inf padding
-        if (i < nb) {
-            const int64_t end = (i == 0) ? first_end : first_end + kItsFan * i;
-            key = P[(end < off + d ? end : off + d) - 1];
+    uint4 *V = lines + static_cast<int64_t>(root[v]) * 8;
+    const double inf = __longlong_as_double(0x7ff0000000000000LL);
+    // Leaves: prefix sums, then the neighbour records with each neighbour's first line.
+    uint4 *leaf = V + acc * 8;
+    for (int32_t i = 0; i < nl; ++i) {
+        double pv[6];
+        uint4 rec[kItsLeaf];
+        for (int32_t k = 0; k < kItsLeaf; ++k) {
+            const int32_t j = kItsLeaf * i + k;
+            pv[k] = j < d ? P[off + j] : inf;
+            if (j < d) {
+                const uint4 r = erec[off + j];
+                rec[k] = make_uint4(r.x, r.y, r.z, root[r.x]);
+            } else {
+                rec[k] = make_uint4(0u, 0u, 0u, kNoRange);
+            }
         }
-        l1[i] = key;
+        pv[5] = 0.0;
+        double2 *pl = reinterpret_cast<double2 *>(leaf + 8 * i);
+        pl[0] = make_double2(pv[0], pv[1]);
+        pl[1] = make_double2(pv[2], pv[3]);
+        pl[2] = make_double2(pv[4], pv[5]);
+        for (int32_t k = 0; k < kItsLeaf; ++k) leaf[8 * i + 3 + k] = rec[k];
+    }
+    // Level 1: the last prefix sum of every leaf.
+    if (L > 0) {
+        double *l1 = reinterpret_cast<double *>(V + base[0] * 8);
+        for (int64_t i = 0; i < nodes[0] * kItsFan; ++i) {
+            double key = inf;
+            if (i < nl) {
+                const int64_t last = (i + 1) * kItsLeaf;
+                key = P[off + (last < d ? last : d) - 1];
+            }
+            l1[i] = key;
+        }
     }
     // Higher levels: the last key of every node below.
-    int64_t keys_below = nb;
+    int64_t keys_below = nl;
     for (int32_t k = 1; k < L; ++k) {
-        const double *below = tree + base[k - 1] * kItsFan;
-        double *lk = tree + base[k] * kItsFan;
+        const double *below = reinterpret_cast<const double *>(V + base[k - 1] * 8);
+        double *lk = reinterpret_cast<double *>(V + base[k] * 8);
         for (int64_t i = 0; i < nodes[k] * kItsFan; ++i) {
-            double key = __longlong_as_double(0x7ff0000000000000LL);
+            double key = inf;
             if (i < nodes[k - 1]) {
                 const int64_t last = (i + 1) * kItsFan - 1;
                 key = below[last < keys_below ? last : keys_below - 1];
@@ -570,11 +595,11 @@ hipError_t launch_its_tree_sizes(const DevGraph &g, uint32_t *root, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint32_t *root, double *idx,
+hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint32_t *root, uint4 *lines,
                                 hipStream_t stream) {
     if (g.nv <= 0) return hipSuccess;
     hipLaunchKernelGGL(its_tree_fill_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv, prefix,
-                       root, idx);
+                       g.erec, root, lines);
     return hipGetLastError();
 }
 

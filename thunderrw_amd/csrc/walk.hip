@@ -210,14 +210,14 @@ __device__ __forceinline__ int32_t read_neighbor(const DevGraph &g, int64_t e, u
     return g.adj[e];
 }
 
-// As read_neighbor, also returning the neighbour's ITS tree root (record word .w).
+// As read_neighbor, also returning the neighbour's ITS structure (record word .w).
 __device__ __forceinline__ int32_t read_neighbor_its(const DevGraph &g, int64_t e, uint32_t &noff, uint32_t &ndeg,
                                                      uint32_t &nidx) {
     if (g.erec) {
         const uint4 r = g.erec[e];
         noff = r.y;
         ndeg = r.z;
-        nidx = g.its_idx ? r.w : kNoRange;
+        nidx = g.its_lines ? r.w : kNoRange;
         return static_cast<int32_t>(r.x);
     }
     noff = 0u;
@@ -226,48 +226,71 @@ __device__ __forceinline__ int32_t read_neighbor_its(const DevGraph &g, int64_t 
     return g.adj[e];
 }
 
-// Number of the 16 keys of a 128-byte tree node that are <= target.
-__device__ __forceinline__ int32_t node_count_le(const double *node, double target) {
-    const double2 *n2 = reinterpret_cast<const double2 *>(node);
-    int32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < kItsFan / 2; ++i) {
-        const double2 k = n2[i];
-        c += (k.x <= target) + (k.y <= target);
+// One 128-byte line of an ITS structure, as 8 16-byte loads issued together.  Named
+// members, not an array: a run-time choice among them stays a chain of selects, where
+// an indexed array would be moved to scratch memory.
+struct ItsLine {
+    uint4 a0, a1, a2, a3, a4, a5, a6, a7;
+    __device__ __forceinline__ void load(const uint4 *p) {
+        a0 = p[0];
+        a1 = p[1];
+        a2 = p[2];
+        a3 = p[3];
+        a4 = p[4];
+        a5 = p[5];
+        a6 = p[6];
+        a7 = p[7];
     }
-    return c;
-}
-
-// Upper bound of target in the prefix sums P[off, off + d) through the vertex's search
-// tree (its root node has already been read into `root_node`): the count of entries
-// <= target, d when the target is at least the total.
-__device__ __forceinline__ int32_t its_tree_count(const DevGraph &g, int64_t off, int32_t d, uint32_t root,
-                                                  const double *root_node, int32_t L, const int64_t *nodes,
-                                                  int32_t root_keys, double target) {
-    int64_t node = node_count_le(root_node, target);
-    if (node >= root_keys) return d;
-    int64_t base = 1;  // levels below the root start after its single node
-    for (int32_t k = L - 2; k >= 0; --k) {
-        const double *N = g.its_idx + (static_cast<int64_t>(root) + base + node) * kItsFan;
-        node = node * kItsFan + node_count_le(N, target);
-        base += nodes[k];
+    static __device__ __forceinline__ double lo(const uint4 &v) {
+        return __hiloint2double(static_cast<int>(v.y), static_cast<int>(v.x));
     }
-    // Leaf block `node`: one aligned 16-entry chunk of the prefix array.
-    const int64_t first_end = ((off >> 4) + 1) << 4;
-    const int64_t start = node == 0 ? off : first_end + kItsFan * (node - 1);
-    const int64_t lim = node == 0 ? first_end : start + kItsFan;
-    const int64_t end = lim < off + d ? lim : off + d;
-    const int64_t chunk = (start >> 4) << 4;
-    const double2 *C = reinterpret_cast<const double2 *>(g.prefix + chunk);
-    int32_t cnt = 0;
-#pragma unroll
-    for (int i = 0; i < kItsFan / 2; ++i) {
-        const double2 v = C[i];
-        const int64_t j = chunk + 2 * i;
-        cnt += (j >= start && j < end && v.x <= target) + (j + 1 >= start && j + 1 < end && v.y <= target);
+    static __device__ __forceinline__ double hi(const uint4 &v) {
+        return __hiloint2double(static_cast<int>(v.w), static_cast<int>(v.z));
     }
-    return static_cast<int32_t>(start - off) + cnt;
-}
+    // Tree node: keys 0..15 are the 16 doubles of the line.
+    __device__ __forceinline__ int32_t count_le(double t) const {
+        return (lo(a0) <= t) + (hi(a0) <= t) + (lo(a1) <= t) + (hi(a1) <= t) + (lo(a2) <= t) + (hi(a2) <= t) +
+               (lo(a3) <= t) + (hi(a3) <= t) + (lo(a4) <= t) + (hi(a4) <= t) + (lo(a5) <= t) + (hi(a5) <= t) +
+               (lo(a6) <= t) + (hi(a6) <= t) + (lo(a7) <= t) + (hi(a7) <= t);
+    }
+    __device__ __forceinline__ double key_at(int i) const {  // value selects only
+        double k = lo(a0);
+        k = i == 1 ? hi(a0) : k;
+        k = i == 2 ? lo(a1) : k;
+        k = i == 3 ? hi(a1) : k;
+        k = i == 4 ? lo(a2) : k;
+        k = i == 5 ? hi(a2) : k;
+        k = i == 6 ? lo(a3) : k;
+        k = i == 7 ? hi(a3) : k;
+        k = i == 8 ? lo(a4) : k;
+        k = i == 9 ? hi(a4) : k;
+        k = i == 10 ? lo(a5) : k;
+        k = i == 11 ? hi(a5) : k;
+        k = i == 12 ? lo(a6) : k;
+        k = i == 13 ? hi(a6) : k;
+        k = i == 14 ? lo(a7) : k;
+        k = i == 15 ? hi(a7) : k;
+        return k;
+    }
+    // Leaf: prefix sums are keys 0..4 (a0, a1, a2.lo), records are a3..a7.
+    __device__ __forceinline__ int32_t leaf_count_le(double t, int32_t n) const {
+        return (0 < n && lo(a0) <= t) + (1 < n && hi(a0) <= t) + (2 < n && lo(a1) <= t) + (3 < n && hi(a1) <= t) +
+               (4 < n && lo(a2) <= t);
+    }
+    static __device__ __forceinline__ uint32_t pick5(int32_t j, uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3,
+                                                     uint32_t v4) {
+        uint32_t r = v0;
+        r = j == 1 ? v1 : r;
+        r = j == 2 ? v2 : r;
+        r = j == 3 ? v3 : r;
+        r = j == 4 ? v4 : r;
+        return r;
+    }
+    __device__ __forceinline__ uint4 record(int32_t j) const {
+        return make_uint4(pick5(j, a3.x, a4.x, a5.x, a6.x, a7.x), pick5(j, a3.y, a4.y, a5.y, a6.y, a7.y),
+                          pick5(j, a3.z, a4.z, a5.z, a6.z, a7.z), pick5(j, a3.w, a4.w, a5.w, a6.w, a7.w));
+    }
+};
 
 // Alias slot e (alias_sampling.h:147-172 pick).  Returns false for a vertex
 // whose weights sum to zero.
@@ -349,16 +372,21 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     } else if (MODE == kModeIts) {
         // inverse_transformation_sampling.h:259-286
         uint32_t root = kNoRange;
-        if (g.its_idx && d >= P.its_tree_min) root = s.cidx != kNoRange ? s.cidx : g.its_root[cur];
-        int32_t pos;
+        if (g.its_lines && d >= P.its_tree_min) root = s.cidx != kNoRange ? s.cidx : g.its_root[cur];
         if (root != kNoRange) {
-            // Search tree: the root's last key is the total, P[d - 1].
-            int64_t nodes[kItsMaxLevels];
-            const int32_t nb = its_leaf_blocks(off, d);
-            const int32_t L = its_levels(nb, nodes);
-            const int32_t root_keys = L == 1 ? nb : static_cast<int32_t>(nodes[L - 2]);
-            const double *R = g.its_idx + static_cast<int64_t>(root) * kItsFan;
-            const double total = R[root_keys - 1];
+            // Search structure: the first line is the tree root (its last key is the total,
+            // P[d - 1]) or, for one leaf, the leaf itself.  Level k of the tree has
+            // ceil(nl / 16^k) nodes, computed here rather than stored.
+            const int32_t nl = its_leaves(d);
+            int32_t L = 0;
+            while ((static_cast<int64_t>(nl) + (int64_t(1) << (4 * L)) - 1) >> (4 * L) > 1) ++L;
+            const uint4 *V = g.its_lines + static_cast<int64_t>(root) * 8;
+            ItsLine line;
+            line.load(V);
+            const int32_t root_keys =
+                L == 0 ? d : static_cast<int32_t>((static_cast<int64_t>(nl) + (int64_t(1) << (4 * (L - 1))) - 1) >>
+                                                  (4 * (L - 1)));
+            const double total = line.key_at(root_keys - 1);
             if (total == 0.0) {
                 s.st = kGatherTerminated;
                 return false;
@@ -366,7 +394,35 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             uint32_t u;
             if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
             if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
-            pos = min(its_tree_count(g, off, d, root, R, L, nodes, root_keys, real2(u) * total), d - 1);
+            const double target = real2(u) * total;
+            int64_t leaf = 0;
+            bool beyond = false;  // target >= total: the pick clamps to the last edge
+            if (L > 0) {
+                int64_t node = line.count_le(target);
+                int64_t base = 1;  // lines before the current level (the root's single node)
+                if (node >= root_keys) {
+                    beyond = true;
+                    node = nl - 1;
+                    for (int32_t k = L - 1; k >= 1; --k)
+                        base += (static_cast<int64_t>(nl) + (int64_t(1) << (4 * k)) - 1) >> (4 * k);
+                } else {
+                    for (int32_t k = L - 1; k >= 1; --k) {
+                        line.load(V + (base + node) * 8);
+                        node = node * kItsFan + line.count_le(target);
+                        base += (static_cast<int64_t>(nl) + (int64_t(1) << (4 * k)) - 1) >> (4 * k);
+                    }
+                }
+                leaf = node;
+                line.load(V + (base + leaf) * 8);  // base is now the tree's line count
+            }
+            const int32_t n = min(kItsLeaf, d - kItsLeaf * static_cast<int32_t>(leaf));
+            int32_t j = n - 1;
+            if (!beyond) j = min(line.leaf_count_le(target, n), n - 1);
+            const uint4 r = line.record(j);
+            next = static_cast<int32_t>(r.x);
+            noff = r.y;
+            ndeg = r.z;
+            nidx = r.w;
         } else {
             const double *P_ = g.prefix + off;
             const double total = P_[d - 1];
@@ -379,9 +435,9 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
             // The count is < d whenever the prefix sums are non-decreasing (non-negative
             // weights); the clamp only keeps negative-weight inputs inside the segment.
-            pos = min(count_le(P_, d, real2(u) * total), d - 1);
+            const int32_t pos = min(count_le(P_, d, real2(u) * total), d - 1);
+            next = read_neighbor_its(g, off + pos, noff, ndeg, nidx);
         }
-        next = read_neighbor_its(g, off + pos, noff, ndeg, nidx);
     } else if (MODE == kModeRejection) {
         // rejection_sampling.h:334-366: redraw (u % d, real2 * max) while r > w[pos].
         const double m = g.rs_max[cur];
@@ -889,11 +945,11 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_kernel(DevGraph g, KindP
         WalkerState s = state[w];
         if (s.st == kAlive) {
             const bool kind_b = w >= split;
-            const KindParams &P = kind_b ? B : A;
+            const int32_t plen = kind_b ? B.length : A.length;  // a value, not a reference to either kind
             uint4 blk = first_block<REPLAY>(io, w, s);
             const bool moved =
                 kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
-            if (moved && io.paths && s.len - 1 < P.length)
+            if (moved && io.paths && s.len - 1 < plen)
                 io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
             still = s.st == kAlive;
             state[w] = s;
@@ -956,11 +1012,11 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_compact_kernel(DevGraph 
             w = list_in[i];
             WalkerState s = state[w];
             const bool kind_b = w >= split;
-            const KindParams &P = kind_b ? B : A;
+            const int32_t plen = kind_b ? B.length : A.length;  // a value, not a reference to either kind
             uint4 blk = first_block<REPLAY>(io, w, s);
             const bool moved =
                 kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
-            if (moved && io.paths && s.len - 1 < P.length)
+            if (moved && io.paths && s.len - 1 < plen)
                 io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
             keep = s.st == kAlive;
             state[w] = s;

@@ -431,3 +431,14 @@ def test_full_size_deepwalk_alias_graph(gpu_ctx):
         assert np.all(edge_keys[pos] == q)
     finally:
         dg.close()
+
+
+def test_upload_rejects_vertex_ids_out_of_range(gpu_ctx):
+    g = graphs.chung_lu(3000, 20000, 2.1, 2, 300, 17)
+    for bad in (-1, g.nv):
+        adj = g.adj.copy()
+        adj[len(adj) * 2 // 3] = bad
+        with pytest.raises(T.TrwError) as e:
+            T.DeviceGraph(gpu_ctx, g.degree, adj, g.weight)
+        assert e.value.code == T._abi.E_INVALID
+    T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight).close()

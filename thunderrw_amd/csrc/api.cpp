@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -278,12 +279,13 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
     }
     if (offset[nv] != ne) return fail(TRW_E_INVALID, "degrees sum to %lld, num_edges is %lld",
                                       static_cast<long long>(offset[nv]), static_cast<long long>(ne));
-    bool bad = false;
+    std::atomic<bool> bad{false};
     trw::parallel_for(ne, [&](int64_t b, int64_t e) {
-        for (int64_t i = b; i < e; ++i)
-            if (adj[i] < 0 || adj[i] >= nv) bad = true;
+        bool mine = false;
+        for (int64_t i = b; i < e; ++i) mine = mine || adj[i] < 0 || adj[i] >= nv;
+        if (mine) bad.store(true, std::memory_order_relaxed);
     });
-    if (bad) return fail(TRW_E_INVALID, "adjacency holds a vertex id outside [0, num_vertices)");
+    if (bad.load()) return fail(TRW_E_INVALID, "adjacency holds a vertex id outside [0, num_vertices)");
 
     // Vertices in descending degree order (by power-of-two class): the table
     // builds start hubs' long serial chains first.

@@ -401,24 +401,24 @@ static int build_its_trees(trw_graph *g, bool monotone) {
     hipStream_t st = g->ctx->stream;
     const size_t V = static_cast<size_t>(g->nv);
     if (!g->its_root) TRW_HIP(dmalloc(&g->its_root, V), "hipMalloc ITS structure roots");
-    TRW_HIP(trw::launch_its_tree_sizes(g->view(), g->its_root, st), "ITS structure sizes");
-    std::vector<uint32_t> root(V);
-    TRW_HIP(hipMemcpyAsync(root.data(), g->its_root, sizeof(uint32_t) * V, hipMemcpyDeviceToHost, st),
-            "ITS structure sizes");
-    TRW_HIP(hipStreamSynchronize(st), "ITS structure sizes");
+    // Line starts by a device scan of the per-vertex line counts; only the total comes back.
+    ScratchBuffers sc;
+    uint64_t *start = nullptr;
+    uint8_t *temp = nullptr;
+    size_t temp_bytes = 0;
+    TRW_HIP(trw::launch_its_tree_starts(g->view(), g->its_root, nullptr, nullptr, temp_bytes, st), "ITS structure scan size");
+    TRW_HIP(sc.get(&start, V + 1), "hipMalloc ITS structure starts");
+    TRW_HIP(sc.get(&temp, temp_bytes), "hipMalloc ITS scan storage");
+    TRW_HIP(trw::launch_its_tree_starts(g->view(), g->its_root, start, temp, temp_bytes, st), "ITS structure starts");
     uint64_t total = 0;
-    for (size_t v = 0; v < V; ++v) {
-        const uint32_t c = root[v];
-        root[v] = c ? static_cast<uint32_t>(total) : trw::kNoRange;
-        total += c;
-    }
+    TRW_HIP(hipMemcpyAsync(&total, start + V, sizeof(total), hipMemcpyDeviceToHost, st), "ITS structure size");
+    TRW_HIP(hipStreamSynchronize(st), "ITS structure starts");
     if (total == 0 || total >= trw::kNoRange) return TRW_OK;  // nothing to index, or line indices overflow 32 bits
-    TRW_HIP(hipMemcpyAsync(g->its_root, root.data(), sizeof(uint32_t) * V, hipMemcpyHostToDevice, st),
-            "ITS structure roots");
     TRW_HIP(dmalloc(&g->its_lines, static_cast<size_t>(total) * 8), "hipMalloc ITS structures");
-    TRW_HIP(trw::launch_its_tree_fill(g->view(), g->prefix, g->its_root, g->its_lines, st), "ITS structure fill");
+    TRW_HIP(trw::launch_its_tree_fill(g->view(), g->prefix, start, total, g->its_root, g->its_lines, st),
+            "ITS structure fill");
     TRW_HIP(trw::launch_its_tree_roots_to_records(g->view(), g->its_root, g->erec, st), "ITS structure roots");
-    TRW_HIP(hipStreamSynchronize(st), "ITS structures");  // the host copy of the roots must outlive the upload
+    TRW_HIP(hipStreamSynchronize(st), "ITS structures");  // the scratch is freed on return
     return TRW_OK;
 }
 

@@ -212,9 +212,13 @@ hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lere
 // ITS search structures: line counts per vertex into root (then the caller turns them
 // into offsets), the structures themselves, and the roots copied into the neighbour
 // records' .w.
-hipError_t launch_its_tree_sizes(const DevGraph &g, uint32_t *root, hipStream_t stream);
-hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint32_t *root, uint4 *lines,
-                                hipStream_t stream);
+// ITS structure layout: root[v] = v's first line (kNoRange: none), start[v] likewise
+// with start[nv] = all lines (nv + 1 entries).  With temp null only sizes the scan's
+// temporary storage into temp_bytes.
+hipError_t launch_its_tree_starts(const DevGraph &g, uint32_t *root, uint64_t *start, void *temp, size_t &temp_bytes,
+                                  hipStream_t stream);
+hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint64_t *start, uint64_t lines_total,
+                                const uint32_t *root, uint4 *lines, hipStream_t stream);
 hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *root, uint4 *erec, hipStream_t stream);
 hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
                             double *out, hipStream_t stream);

@@ -6,6 +6,10 @@
 // long serial chains of hub vertices start first.
 #include "engine.h"
 
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+
 namespace trw {
 namespace {
 
@@ -446,9 +450,10 @@ __global__ void label_records_kernel(const int64_t *offset, int32_t nv, const in
     }
 }
 
-// ITS search structures.  One thread per vertex; a hub writes a few thousand lines once.
-__global__ void its_tree_sizes_kernel(const int64_t *offset, int32_t nv, uint32_t *count) {
+// ITS search structures.  Sizes: one thread per vertex writes its line count.
+__global__ void its_tree_sizes_kernel(const int64_t *offset, int32_t nv, uint32_t *count, uint64_t *start) {
     const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v == 0) start[0] = 0;
     if (v >= nv) return;
     const int32_t d = static_cast<int32_t>(offset[v + 1] - offset[v]);
     const int32_t nl = its_leaves(d);
@@ -459,72 +464,78 @@ __global__ void its_tree_sizes_kernel(const int64_t *offset, int32_t nv, uint32_
     count[v] = static_cast<uint32_t>(total);
 }
 
-__global__ void its_tree_fill_kernel(const int64_t *offset, int32_t nv, const double *P, const uint4 *erec,
-                                     const uint32_t *root, uint4 *lines) {
+// start[v] is v's first line (start[nv] = all lines); a vertex with no lines has no root.
+__global__ void its_tree_starts_to_roots_kernel(const uint64_t *start, int32_t nv, uint32_t *root) {
     const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= nv || root[v] == kNoRange) return;
-    const int64_t off = offset[v];
-    const int32_t d = static_cast<int32_t>(offset[v + 1] - off);
-    const int32_t nl = its_leaves(d);
-    int64_t nodes[kItsMaxLevels];
-    const int32_t L = its_levels(nl, nodes);
-    // Tree level k (0 = level 1) starts after the levels above it; the leaves follow the tree.
-    int64_t base[kItsMaxLevels];
-    int64_t acc = 0;
-    for (int32_t k = L - 1; k >= 0; --k) {
-        base[k] = acc;
-        acc += nodes[k];
-    }
-    uint4 *V = lines + static_cast<int64_t>(root[v]) * 8;
+    if (v >= nv) return;
+    root[v] = start[v + 1] > start[v] ? static_cast<uint32_t>(start[v]) : kNoRange;
+}
+
+// One thread per 16-byte piece of every line, so a hub's thousands of lines spread
+// over the whole grid.  v's lines are its tree levels, root level first, then its
+// leaves.  Key j of tree level k (0 = the level over the leaves) is the last prefix
+// sum of leaf min((j + 1) * 16^k, nl) - 1 when j is below the level's real key count
+// (nl at level 0, the node count of level k - 1 above that), +inf otherwise.  A leaf
+// holds 5 prefix sums (+inf past the last edge), 8 bytes of zero, then the 5 edges'
+// neighbour records with each neighbour's root.
+__global__ void its_tree_fill_kernel(const int64_t *offset, const uint64_t *start, int32_t nv, const double *P,
+                                     const uint4 *erec, const uint32_t *root, uint4 *lines, uint64_t pieces) {
     const double inf = __longlong_as_double(0x7ff0000000000000LL);
-    // Leaves: prefix sums, then the neighbour records with each neighbour's first line.
-    uint4 *leaf = V + acc * 8;
-    for (int32_t i = 0; i < nl; ++i) {
-        double pv[6];
-        uint4 rec[kItsLeaf];
-        for (int32_t k = 0; k < kItsLeaf; ++k) {
-            const int32_t j = kItsLeaf * i + k;
-            pv[k] = j < d ? P[off + j] : inf;
+    for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < pieces;
+         t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const uint64_t l = t >> 3;
+        const int32_t q = static_cast<int32_t>(t & 7);
+        // The vertex that owns line l: the last v with start[v] <= l (start[0] = 0 <= l < start[nv]).
+        int32_t lo = 0, hi = nv;
+        while (hi - lo > 1) {
+            const int32_t m = (lo + hi) >> 1;
+            if (start[m] <= l) lo = m;
+            else hi = m;
+        }
+        const int64_t off = offset[lo];
+        const int32_t d = static_cast<int32_t>(offset[lo + 1] - off);
+        const int32_t nl = its_leaves(d);
+        int64_t nodes[kItsMaxLevels];
+        const int32_t L = its_levels(nl, nodes);
+        int64_t tree = 0;
+        for (int32_t k = 0; k < L; ++k) tree += nodes[k];
+        const int64_t r = static_cast<int64_t>(l - start[lo]);
+        uint4 out;
+        if (r < tree) {
+            int32_t k = L - 1;
+            int64_t b = 0;
+            while (r >= b + nodes[k]) b += nodes[k--];
+            const int64_t real = k == 0 ? nl : nodes[k - 1];
+            const int64_t j0 = (r - b) * kItsFan + 2 * q;
+            double key[2];
+            for (int32_t h = 0; h < 2; ++h) {
+                const int64_t j = j0 + h;
+                const int64_t leaf = (j + 1) << (4 * k);
+                const int64_t last = (leaf < nl ? leaf : nl) * kItsLeaf;
+                key[h] = j < real ? P[off + (last < d ? last : d) - 1] : inf;
+            }
+            out = make_uint4(static_cast<uint32_t>(__double_as_longlong(key[0])),
+                             static_cast<uint32_t>(static_cast<uint64_t>(__double_as_longlong(key[0])) >> 32),
+                             static_cast<uint32_t>(__double_as_longlong(key[1])),
+                             static_cast<uint32_t>(static_cast<uint64_t>(__double_as_longlong(key[1])) >> 32));
+        } else if (q < 3) {
+            const int64_t j = (r - tree) * kItsLeaf + 2 * q;
+            const double a = j < d ? P[off + j] : inf;
+            const double c = q == 2 ? 0.0 : j + 1 < d ? P[off + j + 1] : inf;
+            out = make_uint4(static_cast<uint32_t>(__double_as_longlong(a)),
+                             static_cast<uint32_t>(static_cast<uint64_t>(__double_as_longlong(a)) >> 32),
+                             static_cast<uint32_t>(__double_as_longlong(c)),
+                             static_cast<uint32_t>(static_cast<uint64_t>(__double_as_longlong(c)) >> 32));
+        } else {
+            const int64_t j = (r - tree) * kItsLeaf + (q - 3);
             if (j < d) {
-                const uint4 r = erec[off + j];
-                rec[k] = make_uint4(r.x, r.y, r.z, root[r.x]);
+                const uint4 e = erec[off + j];
+                out = make_uint4(e.x, e.y, e.z, root[e.x]);
             } else {
-                rec[k] = make_uint4(0u, 0u, 0u, kNoRange);
+                out = make_uint4(0u, 0u, 0u, kNoRange);
             }
         }
-        pv[5] = 0.0;
-        double2 *pl = reinterpret_cast<double2 *>(leaf + 8 * i);
-        pl[0] = make_double2(pv[0], pv[1]);
-        pl[1] = make_double2(pv[2], pv[3]);
-        pl[2] = make_double2(pv[4], pv[5]);
-        for (int32_t k = 0; k < kItsLeaf; ++k) leaf[8 * i + 3 + k] = rec[k];
-    }
-    // Level 1: the last prefix sum of every leaf.
-    if (L > 0) {
-        double *l1 = reinterpret_cast<double *>(V + base[0] * 8);
-        for (int64_t i = 0; i < nodes[0] * kItsFan; ++i) {
-            double key = inf;
-            if (i < nl) {
-                const int64_t last = (i + 1) * kItsLeaf;
-                key = P[off + (last < d ? last : d) - 1];
-            }
-            l1[i] = key;
-        }
-    }
-    // Higher levels: the last key of every node below.
-    int64_t keys_below = nl;
-    for (int32_t k = 1; k < L; ++k) {
-        const double *below = reinterpret_cast<const double *>(V + base[k - 1] * 8);
-        double *lk = reinterpret_cast<double *>(V + base[k] * 8);
-        for (int64_t i = 0; i < nodes[k] * kItsFan; ++i) {
-            double key = inf;
-            if (i < nodes[k - 1]) {
-                const int64_t last = (i + 1) * kItsFan - 1;
-                key = below[last < keys_below ? last : keys_below - 1];
-            }
-            lk[i] = key;
-        }
-        keys_below = nodes[k - 1];
+        lines[t] = out;
     }
 }
 
@@ -589,17 +600,32 @@ hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lere
     return hipGetLastError();
 }
 
-hipError_t launch_its_tree_sizes(const DevGraph &g, uint32_t *root, hipStream_t stream) {
+hipError_t launch_its_tree_starts(const DevGraph &g, uint32_t *root, uint64_t *start, void *temp, size_t &temp_bytes,
+                                  hipStream_t stream) {
     if (g.nv <= 0) return hipSuccess;
-    hipLaunchKernelGGL(its_tree_sizes_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv, root);
+    const size_t n = static_cast<size_t>(g.nv);
+    if (!temp)
+        return rocprim::inclusive_scan<rocprim::default_config, const uint32_t *, uint64_t *, rocprim::plus<uint64_t>,
+                                       uint64_t>(nullptr, temp_bytes, root, start + 1, n, rocprim::plus<uint64_t>(),
+                                                 stream);
+    hipLaunchKernelGGL(its_tree_sizes_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv, root,
+                       start);
+    hipError_t e = rocprim::inclusive_scan<rocprim::default_config, const uint32_t *, uint64_t *,
+                                           rocprim::plus<uint64_t>, uint64_t>(temp, temp_bytes, root, start + 1, n,
+                                                                              rocprim::plus<uint64_t>(), stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(its_tree_starts_to_roots_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, start, g.nv,
+                       root);
     return hipGetLastError();
 }
 
-hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint32_t *root, uint4 *lines,
-                                hipStream_t stream) {
-    if (g.nv <= 0) return hipSuccess;
-    hipLaunchKernelGGL(its_tree_fill_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv, prefix,
-                       g.erec, root, lines);
+hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint64_t *start, uint64_t lines_total,
+                                const uint32_t *root, uint4 *lines, hipStream_t stream) {
+    if (g.nv <= 0 || lines_total == 0) return hipSuccess;
+    const uint64_t pieces = lines_total * 8;
+    const uint64_t blocks = std::min<uint64_t>((pieces + 255) / 256, uint64_t(1) << 20);
+    hipLaunchKernelGGL(its_tree_fill_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, g.offset, start,
+                       g.nv, prefix, g.erec, root, lines, pieces);
     return hipGetLastError();
 }
 

@@ -19,6 +19,12 @@ constexpr int kAliasLargeChunk = 8;  // alias build: large entries held ahead
 
 __device__ __forceinline__ double weight_at(const double *w, int32_t unit, int64_t e) { return unit ? 1.0 : w[e]; }
 
+// Waits for a conditional refill's loads where they are issued.  Vector memory
+// operations complete in issue order and the counter covers stores too, so a wait
+// placed where the refilled and the untouched paths merge (the compiler's choice)
+// would also wait for every store since, on every pass of the pairing loop.
+__device__ __forceinline__ void refill_wait() { __builtin_amdgcn_s_waitcnt(0); }
+
 // alias_initialization (alias_sampling.h:12-54) without queues.
 //
 // The reference keeps two FIFO queues.  `large` is only ever popped at its
@@ -31,13 +37,8 @@ __device__ __forceinline__ double weight_at(const double *w, int32_t unit, int64
 // start, larges from the segment end backwards, and a large entry's weight is
 // replaced by its residual when it moves to the small side.  Every load address in
 // the pairing loop then depends on the cursors only, never on a loaded value.
-__global__ __launch_bounds__(kBuildBlock) void alias_build_kernel(DevGraph g, const double *w, int32_t unit,
-                                                                   const int32_t *order, int32_t first,
-                                                                   AliasEntry *scratch, AliasSlot *out,
-                                                                   unsigned int *zero_count) {
-    const int32_t i = first + blockIdx.x * kBuildBlock + threadIdx.x;
-    if (i >= g.nv) return;
-    const int32_t v = order ? order[i] : i;
+__device__ __forceinline__ void alias_build_vertex(const DevGraph &g, const double *w, int32_t unit, int32_t v,
+                                                   AliasEntry *scratch, AliasSlot *out, unsigned int *zero_count) {
     const int64_t off = g.offset[v];
     const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
     if (d == 0) return;
@@ -116,6 +117,7 @@ __global__ __launch_bounds__(kBuildBlock) void alias_build_kernel(DevGraph g, co
                         lbase = li;
 #pragma unroll
                         for (int c2 = 0; c2 < kAliasLargeChunk; ++c2) lbuf[c2] = E[max(d - 1 - (lbase + c2), 0)];
+                        refill_wait();
                     }
                     lpos = d - 1 - li;
                     AliasEntry le = lbuf[0];
@@ -155,6 +157,7 @@ __global__ __launch_bounds__(kBuildBlock) void alias_build_kernel(DevGraph g, co
                         lbase = li;
 #pragma unroll
                         for (int c2 = 0; c2 < kAliasLargeChunk; ++c2) lbuf[c2] = E[max(d - 1 - (lbase + c2), 0)];
+                        refill_wait();
                     }
                     lpos = d - 1 - li;
                     AliasEntry le = lbuf[0];
@@ -191,11 +194,8 @@ __global__ __launch_bounds__(kBuildBlock) void alias_build_kernel(DevGraph g, co
 }
 
 // its_initialization (inverse_transformation_sampling.h:19-24).
-__global__ __launch_bounds__(kBuildBlock) void its_build_kernel(DevGraph g, const double *w, int32_t unit,
-                                                                 const int32_t *order, int32_t first, double *out) {
-    const int32_t i = first + blockIdx.x * kBuildBlock + threadIdx.x;
-    if (i >= g.nv) return;
-    const int32_t v = order ? order[i] : i;
+__device__ __forceinline__ void its_build_vertex(const DevGraph &g, const double *w, int32_t unit, int32_t v,
+                                                 double *out) {
     const int64_t off = g.offset[v];
     const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
     if (d == 0) return;
@@ -208,11 +208,8 @@ __global__ __launch_bounds__(kBuildBlock) void its_build_kernel(DevGraph g, cons
 }
 
 // rs_initialization (rejection_sampling.h:12-14): *std::max_element.
-__global__ __launch_bounds__(kBuildBlock) void rs_build_kernel(DevGraph g, const double *w, int32_t unit,
-                                                                const int32_t *order, int32_t first, double *out) {
-    const int32_t i = first + blockIdx.x * kBuildBlock + threadIdx.x;
-    if (i >= g.nv) return;
-    const int32_t v = order ? order[i] : i;
+__device__ __forceinline__ void rs_build_vertex(const DevGraph &g, const double *w, int32_t unit, int32_t v,
+                                                double *out) {
     const int64_t off = g.offset[v];
     const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
     if (d == 0) {
@@ -245,23 +242,32 @@ __device__ __forceinline__ double readlane_f64(double x, int32_t lane) {
 
 __device__ __forceinline__ int32_t readlane_i32(int32_t x, int32_t lane) { return __builtin_amdgcn_readlane(x, lane); }
 
+// Hub loops take their entries 256 at a time: four 64-entry groups are loaded before
+// the sequential chain reads the first, so one memory latency covers 256 entries.
+constexpr int kWaveGroups = 4;
+
 // std::accumulate(w, w + d, 0.0): the same additions in the same order.
 __device__ __forceinline__ double wave_ordered_sum(const double *w, int32_t unit, int64_t off, int32_t d, int lane) {
     double sum = 0.0;
-    for (int32_t base = 0; base < d; base += 64) {
-        const int32_t j = base + lane;
-        const double x = j < d ? weight_at(w, unit, off + j) : 0.0;
-        const int32_t n = min(64, d - base);
-        for (int32_t c = 0; c < n; ++c) sum = sum + readlane_f64(x, c);
+    for (int32_t base = 0; base < d; base += 64 * kWaveGroups) {
+        double x[kWaveGroups];
+#pragma unroll
+        for (int h = 0; h < kWaveGroups; ++h) {
+            const int32_t j = base + 64 * h + lane;
+            x[h] = j < d ? weight_at(w, unit, off + j) : 0.0;
+        }
+#pragma unroll
+        for (int h = 0; h < kWaveGroups; ++h) {
+            const int32_t n = min(64, d - (base + 64 * h));
+            for (int32_t c = 0; c < n; ++c) sum = sum + readlane_f64(x[h], c);
+        }
     }
     return sum;
 }
 
-__global__ __launch_bounds__(64) void alias_build_wave_kernel(DevGraph g, const double *w, int32_t unit,
-                                                              const int32_t *order, AliasEntry *scratch,
-                                                              AliasSlot *out, unsigned int *zero_count) {
+__device__ __forceinline__ void alias_build_hub(const DevGraph &g, const double *w, int32_t unit, int32_t v,
+                                                AliasEntry *scratch, AliasSlot *out, unsigned int *zero_count) {
     const int lane = static_cast<int>(threadIdx.x);
-    const int32_t v = order[blockIdx.x];
     const int64_t off = g.offset[v];
     const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
     const double avg = wave_ordered_sum(w, unit, off, d, lane) / d;
@@ -278,67 +284,124 @@ __global__ __launch_bounds__(64) void alias_build_wave_kernel(DevGraph g, const 
     const int32_t *nb = g.adj + off;
     const unsigned long long below = (1ull << lane) - 1ull;
     int32_t ns = 0, nl = 0;
-    for (int32_t base = 0; base < d; base += 64) {
-        const int32_t j = base + lane;
-        const bool valid = j < d;
-        const double x = valid ? weight_at(w, unit, off + j) / avg : 0.0;
-        const int32_t nbr = valid ? nb[j] : 0;
-        const bool small = valid && x < 1;
-        const bool large = valid && !(x < 1);
-        const unsigned long long ms = __ballot(small), ml = __ballot(large);
-        if (small) E[ns + __popcll(ms & below)] = AliasEntry{x, nbr, 0};
-        if (large) E[d - 1 - (nl + __popcll(ml & below))] = AliasEntry{x, nbr, 0};
-        ns += __popcll(ms);
-        nl += __popcll(ml);
+    for (int32_t base = 0; base < d; base += 64 * kWaveGroups) {
+        double wx[kWaveGroups];
+        int32_t nx[kWaveGroups];
+#pragma unroll
+        for (int h = 0; h < kWaveGroups; ++h) {
+            const int32_t j = base + 64 * h + lane;
+            wx[h] = j < d ? weight_at(w, unit, off + j) : 0.0;
+            nx[h] = j < d ? nb[j] : 0;
+        }
+#pragma unroll
+        for (int h = 0; h < kWaveGroups; ++h) {
+            const bool valid = base + 64 * h + lane < d;
+            const double x = valid ? wx[h] / avg : 0.0;
+            const bool small = valid && x < 1;
+            const bool large = valid && !(x < 1);
+            const unsigned long long ms = __ballot(small), ml = __ballot(large);
+            if (small) E[ns + __popcll(ms & below)] = AliasEntry{x, nx[h], 0};
+            if (large) E[d - 1 - (nl + __popcll(ml & below))] = AliasEntry{x, nx[h], 0};
+            ns += __popcll(ms);
+            nl += __popcll(ml);
+        }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // entries written by one lane, read by another
 
-    // Pairing, exactly the lane kernel's sequence.  Window sw: small-queue entries
-    // [sw, sw + swn), one per lane; window lw: large entries [lw, lw + 64).
-    int32_t k = 0, si = 0, li = 0;
+    // Pairing, exactly the lane kernel's sequence; slot k of the table is filled by the
+    // k-th small consumed, so k == si throughout.  Small-queue position p is entry p of
+    // the segment for p < ns and the residual of large p - ns after that.
+    //   * Larges are held 64 at a time (window lw: my_l); a lane keeps its large's
+    //     residual once it drops (my_res), and the 64 residuals are stored when the
+    //     window moves on.  (Loading the next window ahead would rotate registers
+    //     through every drop, and the rotation waits for the load.)
+    //   * Small-queue entries are consumed from a window of up to 64 final entries
+    //     (initial smalls, and residuals already stored): my_sq / my_snb, each lane also
+    //     keeping the neighbour of the large its entry pairs with (my_lnb), so the
+    //     window's slots are written together.  The next window loads while this one is
+    //     consumed (my_n), as far as its entries are final.  A residual of the current
+    //     large window is read from my_res.
+    //   * The inner loop over a window issues memory operations only when the large
+    //     window moves on.  Small-window switches wait for loads issued a window earlier
+    //     and stores older than that; every small window load follows such a wait, so
+    //     it sees every residual stored before it.
+    const auto small_pos = [&](int32_t p) { return max(0, min(p < ns ? p : d - 1 - (p - ns), d - 1)); };
+    int32_t si = 0, li = 0;
     int32_t lw = 0;
     AliasEntry my_l = E[max(d - 1 - lane, 0)];
+    int32_t sw = 0, swn = 0, nw = 0, nwn = min(64, ns);
+    AliasEntry my_n = E[small_pos(lane)];
+    refill_wait();
     double wl = nl > 0 ? readlane_f64(my_l.q, 0) : 0.0;
     int32_t lnb = nl > 0 ? readlane_i32(my_l.nbr, 0) : 0;
-    int32_t lpos = d - 1;
-    int32_t sw = 0, swn = 0;
-    AliasEntry my_s{0.0, 0, 0};
-    AliasSlot my_out{0.0, 0, 0};  // slot kbase + lane of the chunk being filled
-    int32_t kbase = 0;
-    while (si < ns + li && li < nl) {
-        if (si >= sw + swn) {
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // residuals written by lane 0
+    double my_res = 0.0;
+    double my_sq = 0.0;
+    int32_t my_snb = 0, my_lnb = 0;
+    // The large in use drops below 1: keep its residual, take the next large (slots from
+    // small-window position c_next on pair with it).  True when no large is left.
+    const auto drop = [&](int32_t c_next) {
+        if (lane == li - lw) my_res = wl;
+        ++li;
+        if (li == nl) return true;
+        if (li - lw == 64) {
+            E[d - 1 - (lw + lane)].q = my_res;
+            lw = li;
+            my_l = E[max(d - 1 - (lw + lane), 0)];
+            refill_wait();
+        }
+        wl = readlane_f64(my_l.q, li - lw);
+        lnb = readlane_i32(my_l.nbr, li - lw);
+        if (lane >= c_next) my_lnb = lnb;
+        return false;
+    };
+    bool done = !(ns > 0 && nl > 0);
+    while (!done) {
+        const int32_t jr = si - ns;
+        if (jr >= lw) {  // the next small is a residual of the current large window
+            const double ws = readlane_f64(my_res, jr - lw);
+            const int32_t snb = readlane_i32(my_l.nbr, jr - lw);
+            if (lane == 0) t[si] = AliasSlot{ws, snb, lnb};
+            ++si;
+            wl = wl + ws - 1;
+            if (wl < 1 && drop(64)) break;
+            done = !(si < ns + li);
+            continue;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // residuals stored by other lanes
+        refill_wait();
+        if (lane < swn) t[sw + lane] = AliasSlot{my_sq, my_snb, my_lnb};
+        if (nw == si && nwn > 0) {
+            my_sq = my_n.q;
+            my_snb = my_n.nbr;
+            sw = nw;
+            swn = nwn;
+        } else {
             sw = si;
-            swn = min(64, ns + li - si);
-            const int32_t p = sw + lane < ns ? sw + lane : d - 1 - (sw + lane - ns);
-            my_s = E[max(0, min(p, d - 1))];
+            swn = min(64, ns + lw - si);
+            const AliasEntry e = E[small_pos(sw + lane)];
+            refill_wait();
+            my_sq = e.q;
+            my_snb = e.nbr;
         }
-        const int32_t c = si - sw;
-        const double ws = readlane_f64(my_s.q, c);
-        const int32_t snb = readlane_i32(my_s.nbr, c);
-        if (lane == (k & 63)) my_out = AliasSlot{ws, snb, lnb};
-        ++k;
-        if ((k & 63) == 0) {
-            t[kbase + lane] = my_out;
-            kbase += 64;
-        }
-        ++si;
-        wl = wl + ws - 1;
-        if (wl < 1) {
-            if (lane == 0) E[lpos].q = wl;
-            ++li;
-            if (li < nl) {
-                if (li - lw == 64) {
-                    lw = li;
-                    my_l = E[max(d - 1 - (lw + lane), 0)];
-                }
-                lpos = d - 1 - li;
-                wl = readlane_f64(my_l.q, li - lw);
-                lnb = readlane_i32(my_l.nbr, li - lw);
+        nw = sw + swn;
+        nwn = max(0, min(64, ns + lw - nw));
+        if (nwn > 0) my_n = E[small_pos(nw + lane)];
+        my_lnb = lnb;
+        int32_t c = 0;
+        do {
+            const double ws = readlane_f64(my_sq, c);
+            ++c;
+            wl = wl + ws - 1;
+            if (wl < 1 && drop(c)) {
+                done = true;
+                break;
             }
-        }
+        } while (c < swn);
+        si = sw + c;
+        if (!done) done = !(si < ns + li);
     }
-    if (lane < (k & 63)) t[kbase + lane] = my_out;
+    if (lane < min(si - sw, swn)) t[sw + lane] = AliasSlot{my_sq, my_snb, my_lnb};
+    int32_t k = si;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     // Leftovers take probability 1 (alias_sampling.h:43-53), in queue order.
     for (int32_t b = si; b < ns + li; b += 64) {
@@ -360,46 +423,99 @@ __global__ __launch_bounds__(64) void alias_build_wave_kernel(DevGraph g, const 
 
 // its_initialization for hubs: the prefix chain runs on uniform values, each lane keeps
 // the running sum at its own index of the 64-entry group.
-__global__ __launch_bounds__(64) void its_build_wave_kernel(DevGraph g, const double *w, int32_t unit,
-                                                            const int32_t *order, double *out) {
+__device__ __forceinline__ void its_build_hub(const DevGraph &g, const double *w, int32_t unit, int32_t v,
+                                              double *out) {
     const int lane = static_cast<int>(threadIdx.x);
-    const int32_t v = order[blockIdx.x];
     const int64_t off = g.offset[v];
     const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
     double acc = 0.0;
-    for (int32_t base = 0; base < d; base += 64) {
-        const int32_t j = base + lane;
-        const double x = j < d ? weight_at(w, unit, off + j) : 0.0;
-        const int32_t n = min(64, d - base);
-        double mine = 0.0;
-        for (int32_t c = 0; c < n; ++c) {
-            const double y = readlane_f64(x, c);
-            acc = (base + c == 0) ? y : acc + y;  // the first entry is copied, not added to 0.0
-            if (lane == c) mine = acc;
+    for (int32_t base = 0; base < d; base += 64 * kWaveGroups) {
+        double x[kWaveGroups];
+#pragma unroll
+        for (int h = 0; h < kWaveGroups; ++h) {
+            const int32_t j = base + 64 * h + lane;
+            x[h] = j < d ? weight_at(w, unit, off + j) : 0.0;
         }
-        if (j < d) out[off + j] = mine;
+#pragma unroll
+        for (int h = 0; h < kWaveGroups; ++h) {
+            const int32_t g0 = base + 64 * h;
+            const int32_t n = min(64, d - g0);
+            double mine = 0.0;
+            for (int32_t c = 0; c < n; ++c) {
+                const double y = readlane_f64(x[h], c);
+                acc = (g0 + c == 0) ? y : acc + y;  // the first entry is copied, not added to 0.0
+                if (lane == c) mine = acc;
+            }
+            if (g0 + lane < d) out[off + g0 + lane] = mine;
+        }
     }
 }
 
 // rs_initialization for hubs: the same comparisons in the same order.
-__global__ __launch_bounds__(64) void rs_build_wave_kernel(DevGraph g, const double *w, int32_t unit,
-                                                           const int32_t *order, double *out) {
+__device__ __forceinline__ void rs_build_hub(const DevGraph &g, const double *w, int32_t unit, int32_t v,
+                                             double *out) {
     const int lane = static_cast<int>(threadIdx.x);
-    const int32_t v = order[blockIdx.x];
     const int64_t off = g.offset[v];
     const int32_t d = static_cast<int32_t>(g.offset[v + 1] - off);
     double m = 0.0;
-    for (int32_t base = 0; base < d; base += 64) {
-        const int32_t j = base + lane;
-        const double x = j < d ? weight_at(w, unit, off + j) : 0.0;
-        const int32_t n = min(64, d - base);
-        for (int32_t c = 0; c < n; ++c) {
-            const double y = readlane_f64(x, c);
-            if (base + c == 0) m = y;
-            else if (m < y) m = y;
+    for (int32_t base = 0; base < d; base += 64 * kWaveGroups) {
+        double x[kWaveGroups];
+#pragma unroll
+        for (int h = 0; h < kWaveGroups; ++h) {
+            const int32_t j = base + 64 * h + lane;
+            x[h] = j < d ? weight_at(w, unit, off + j) : 0.0;
+        }
+#pragma unroll
+        for (int h = 0; h < kWaveGroups; ++h) {
+            const int32_t g0 = base + 64 * h;
+            const int32_t n = min(64, d - g0);
+            for (int32_t c = 0; c < n; ++c) {
+                const double y = readlane_f64(x[h], c);
+                if (g0 + c == 0) m = y;
+                else if (m < y) m = y;
+            }
         }
     }
     if (lane == 0) out[v] = m;
+}
+
+// One launch builds every vertex: blocks [0, hubs) take one hub each with the whole
+// wavefront (hubs lead the degree-class order, largest first); later blocks take 64
+// vertices, one per lane.  The hubs' long sequential chains run beside the lane builds,
+// at raised wave priority so the SIMD issues their next instruction first.
+__device__ __forceinline__ int32_t lane_vertex(const int32_t *order, int32_t hubs) {
+    const int32_t i = hubs + (static_cast<int32_t>(blockIdx.x) - hubs) * 64 + static_cast<int32_t>(threadIdx.x);
+    return order ? order[i] : i;
+}
+
+__device__ __forceinline__ bool lane_in_range(int32_t hubs, int32_t nv) {
+    return hubs + (static_cast<int32_t>(blockIdx.x) - hubs) * 64 + static_cast<int32_t>(threadIdx.x) < nv;
+}
+
+__global__ __launch_bounds__(64) void alias_build_kernel(DevGraph g, const double *w, int32_t unit,
+                                                         const int32_t *order, int32_t hubs, AliasEntry *scratch,
+                                                         AliasSlot *out, unsigned int *zero_count) {
+    if (static_cast<int32_t>(blockIdx.x) < hubs) {
+        __builtin_amdgcn_s_setprio(3);
+        alias_build_hub(g, w, unit, order[blockIdx.x], scratch, out, zero_count);
+    } else if (lane_in_range(hubs, g.nv))
+        alias_build_vertex(g, w, unit, lane_vertex(order, hubs), scratch, out, zero_count);
+}
+
+__global__ __launch_bounds__(64) void its_build_kernel(DevGraph g, const double *w, int32_t unit,
+                                                       const int32_t *order, int32_t hubs, double *out) {
+    if (static_cast<int32_t>(blockIdx.x) < hubs) {
+        __builtin_amdgcn_s_setprio(3);
+        its_build_hub(g, w, unit, order[blockIdx.x], out);
+    } else if (lane_in_range(hubs, g.nv)) its_build_vertex(g, w, unit, lane_vertex(order, hubs), out);
+}
+
+__global__ __launch_bounds__(64) void rs_build_kernel(DevGraph g, const double *w, int32_t unit,
+                                                      const int32_t *order, int32_t hubs, double *out) {
+    if (static_cast<int32_t>(blockIdx.x) < hubs) {
+        __builtin_amdgcn_s_setprio(3);
+        rs_build_hub(g, w, unit, order[blockIdx.x], out);
+    } else if (lane_in_range(hubs, g.nv)) rs_build_vertex(g, w, unit, lane_vertex(order, hubs), out);
 }
 
 // Neighbour records: a Move that lands on adj[e] also learns where that
@@ -566,15 +682,15 @@ inline unsigned grid_for(int64_t n) { return static_cast<unsigned>((n + kBuildBl
 
 }  // namespace
 
+inline unsigned build_grid(int32_t nv, int32_t hubs) {
+    return static_cast<unsigned>(hubs + (static_cast<int64_t>(nv) - hubs + 63) / 64);
+}
+
 hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
                               AliasEntry *scratch, AliasSlot *out, unsigned int *zero_count, hipStream_t stream) {
     if (g.nv <= 0) return hipSuccess;
-    if (hubs > 0)
-        hipLaunchKernelGGL(alias_build_wave_kernel, dim3(hubs), dim3(64), 0, stream, g, w, unit, order, scratch, out,
-                           zero_count);
-    if (g.nv > hubs)
-        hipLaunchKernelGGL(alias_build_kernel, dim3(grid_for(g.nv - hubs)), dim3(kBuildBlock), 0, stream, g, w, unit,
-                           order, hubs, scratch, out, zero_count);
+    hipLaunchKernelGGL(alias_build_kernel, dim3(build_grid(g.nv, hubs)), dim3(64), 0, stream, g, w, unit, order, hubs,
+                       scratch, out, zero_count);
     return hipGetLastError();
 }
 
@@ -638,20 +754,16 @@ hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *r
 hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
                             double *out, hipStream_t stream) {
     if (g.nv <= 0) return hipSuccess;
-    if (hubs > 0) hipLaunchKernelGGL(its_build_wave_kernel, dim3(hubs), dim3(64), 0, stream, g, w, unit, order, out);
-    if (g.nv > hubs)
-        hipLaunchKernelGGL(its_build_kernel, dim3(grid_for(g.nv - hubs)), dim3(kBuildBlock), 0, stream, g, w, unit,
-                           order, hubs, out);
+    hipLaunchKernelGGL(its_build_kernel, dim3(build_grid(g.nv, hubs)), dim3(64), 0, stream, g, w, unit, order, hubs,
+                       out);
     return hipGetLastError();
 }
 
 hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
                            double *out, hipStream_t stream) {
     if (g.nv <= 0) return hipSuccess;
-    if (hubs > 0) hipLaunchKernelGGL(rs_build_wave_kernel, dim3(hubs), dim3(64), 0, stream, g, w, unit, order, out);
-    if (g.nv > hubs)
-        hipLaunchKernelGGL(rs_build_kernel, dim3(grid_for(g.nv - hubs)), dim3(kBuildBlock), 0, stream, g, w, unit,
-                           order, hubs, out);
+    hipLaunchKernelGGL(rs_build_kernel, dim3(build_grid(g.nv, hubs)), dim3(64), 0, stream, g, w, unit, order, hubs,
+                       out);
     return hipGetLastError();
 }
 

@@ -125,6 +125,58 @@ def test_random_graphs_match_oracle(gpu_ctx, seed):
             dg.close()
 
 
+
+def directed_graph(n, seed):
+    """A directed CSR (the upload takes any CSR) in which about 5% of the vertices have no
+    out-edges, so walks reach dead ends mid-walk; vertex 0 is isolated, vertex 2 a hub."""
+    rng = np.random.default_rng(seed)
+    deg = rng.integers(1, 40, n)
+    deg[rng.random(n) < 0.05] = 0
+    deg[:3] = [0, 1, 700]
+    rows = []
+    for v in range(n):
+        cand = rng.choice(n - 1, int(deg[v]), replace=False)
+        rows.append(np.sort(cand + (cand >= v)))  # no self-loops, sorted as the reference keeps lists
+    adj = np.concatenate(rows).astype(np.int32)
+    return graphs.CSR(deg, adj, rng.uniform(1.0, 5.0, len(adj)), rng.integers(0, 5, len(adj)))
+
+
+def test_dead_ends_and_empty_batches_match_oracle(gpu_ctx):
+    """A walker at a vertex without out-edges stops with status DEAD_END (the reference's
+    u % 0 is undefined there), at its source or mid-walk, in every app, sampler and form;
+    an empty batch is a no-op in every form."""
+    g = directed_graph(3000, 41)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        rng = np.random.default_rng(41)
+        src = np.concatenate([np.arange(g.nv), rng.integers(0, g.nv, 500)]).astype(np.int32)
+        for app, em, sm, L, extra in RANDOM_CASES:
+            case = {"app": app, "execution": em, "sampling": sm, "length": L, "extra": extra}
+            K, KT = 4 * L + 256, 160
+            ref = oracle_walk(g, case, src, K, KT, 43, 44)
+            if app not in ("ppr", "metapath"):
+                assert np.count_nonzero(ref["status"] == T._abi.WALKER_DEAD_END) > len(src) // 10, app
+            draws = T.replay_fill(43, len(src), K)
+            tdraws = T.replay_fill(44, len(src), KT)
+            for form in forms_for(app):
+                r = T.walk(dg, case_params(case, form), src, draws, tdraws)
+                tag = (app, em, sm, form)
+                for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                    assert np.array_equal(r[key], ref[key]), (tag, key)
+                assert r["steps"] == ref["steps"], tag
+                assert r["dead_ends"] == np.count_nonzero(ref["status"] == T._abi.WALKER_DEAD_END), tag
+        empty = np.zeros(0, np.int32)
+        for app, em, sm, L, extra in RANDOM_CASES[:1] + RANDOM_CASES[6:7]:
+            case = {"app": app, "execution": em, "sampling": sm, "length": L, "extra": extra}
+            for form in forms_for(app):
+                for rngk in ("replay", "philox"):
+                    draws = T.replay_fill(1, 0, 4 * L) if rngk == "replay" else None
+                    tdraws = T.replay_fill(2, 0, 8) if rngk == "replay" else None
+                    r = T.walk(dg, case_params(case, form, rng=rngk), empty, draws, tdraws)
+                    assert r["steps"] == 0 and len(r["lengths"]) == 0 and r["paths"].shape == (0, max(L, 1))
+    finally:
+        dg.close()
+
 @pytest.mark.parametrize("n_labels,schema", [(8, "7,0,3,9,7,-1,5"), (12, "11,0,3,7,9")])
 def test_metapath_label_alphabets_match_oracle(gpu_ctx, n_labels, schema):
     """Labels in [0, 8) take the label-start records (schema labels outside the alphabet

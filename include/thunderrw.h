@@ -70,6 +70,8 @@ extern "C" {
 #define TRW_WALKER_REPLAY_EXHAUSTED 3  /* replay draws ran out: an error, never a wrap */
 #define TRW_WALKER_TRIAL_CAP 4         /* rejection trial cap hit */
 #define TRW_WALKER_BAD_SOURCE 5        /* source vertex outside [0, num_vertices) */
+#define TRW_WALKER_STEP_CAP 6          /* drop-in functor engine: the app's update() had not ended the
+                                          walk within its move cap (the reference would run on) */
 
 /* Weights F::weight gives static_initialization (creeper.h:24-99): DeepWalk and
  * GenericWalk return edge_weight_[eid] (deepwalk.cpp:19-21), PPR returns 1.0 (ppr.cpp:23-25). */
@@ -199,6 +201,43 @@ int trw_graph_load_tables(trw_graph *g, const char *dir, int32_t sampling, int32
 /* Device time of the last trw_build_tables call in milliseconds. */
 int trw_last_build_ms(const trw_graph *g, double *ms);
 
+/* static_initialization (creeper.h:102-175) for an arbitrary functor: static_gather
+ * (creeper.h:24-99) evaluates F::weight for every edge, here done by the caller's kernel into
+ * device_weights (DEVICE array of num_edges doubles, weight of edge e at [e]); the alias, ITS or
+ * rejection table is then built from those weights exactly as trw_build_tables builds it from
+ * edge weights.  The table is tagged TRW_WEIGHTS_CUSTOM (a later walk of a built-in app that
+ * needs edge or unit weights rebuilds it).  Rejection keeps a copy of the weights (its Move
+ * reads them, rejection_sampling.h:334-366). */
+#define TRW_WEIGHTS_CUSTOM 2
+int trw_build_tables_from(trw_graph *g, int32_t sampling, const double *device_weights);
+
+/* The graph's arrays in HBM (DEVICE pointers, null where absent), for kernels compiled outside
+ * the library that walk it: the fields of class Graph (util/graph/graph.h:51-83) the reference's
+ * Moves and functors read -- offset_ (exclusive prefix of degree_, num_vertices + 1 entries),
+ * degree_, adj_, edge_weight_, edge_label_ -- and the static tables (creeper.h:108-170):
+ * edge_weight_alias_table_, edge_weight_prefix_sum_, edge_weight_rejection_max_ and the
+ * weights the rejection maxima were built from (edge_weight_rejection_; null for unit
+ * weights).  A table pointer is null until that table is built; *_weights give the
+ * TRW_WEIGHTS_* it was built from (-1: not built).  Valid until the graph is freed or the
+ * table rebuilt. */
+typedef struct {
+    int32_t num_vertices;
+    int32_t max_degree;
+    int64_t num_edges;
+    const int64_t *offset;
+    const int32_t *degree;
+    const int32_t *adj;
+    const double *edge_weight;
+    const int32_t *edge_label;
+    const trw_alias_slot *alias_table;
+    const double *prefix_sum;
+    const double *rejection_max;
+    const double *rejection_weight;
+    int32_t alias_weights, its_weights, rejection_weights;
+    int32_t reserved;
+} trw_device_view;
+int trw_graph_device_view(const trw_graph *g, trw_device_view *out);
+
 /* ---- per-graph tuning ----
  * Launch choices that change speed, never results (no reference counterpart; the
  * reference fixes its ring sizes at compile time, types.h:17-31).  trw_autotune picks
@@ -236,6 +275,14 @@ int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_wal
  * query server gives each of its slots its own queue head for that reason).  Tables
  * are built lazily by the first walk that needs them unless trw_build_tables ran. */
 int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream);
+
+/* Whether the library has a Move kernel for params' (app, execution, sampling): TRW_OK, or
+ * TRW_E_UNSUPPORTED (TRW_E_INVALID for values outside the reference's enums).  The pairs
+ * the reference's apps accept (-em/-sm of apps/<app>.cpp) that the library has no kernel for --
+ * walker-dependent weights outside Node2Vec's max-weight rejection and MetaPath's dynamic ITS,
+ * or max-weight rejection for DeepWalk/GenericWalk -- run in the drop-in functor engine
+ * (thunderrw_amd/dropin/functor_walk.h).  No graph or device is touched. */
+int trw_walk_supported(const trw_walk_params *params);
 
 /* Two walker kinds in one batch: walkers [0, split) run params_a, [split, n) run
  * params_b (the uneven PPR + MetaPath mix).  form: QUEUE, STEP or GRAPH. */

@@ -4,7 +4,8 @@
 // the reference's sources where they lie (util/graph/graph.cpp,
 // util/log/log.cpp, random_walk/walker_generator.cpp, SFMT.c) and with one
 // reference app translation unit (random_walk/apps/<app>.cpp, whose main() is
-// renamed away), into oracle/_ref/harness_<app>.  Nothing in the product uses
+// renamed away; or tests/dropin_apps/customwalk.cpp, an app written against the
+// reference's interface), into oracle/_ref/harness_<app>.  Nothing in the product uses
 // it; tests/golden/make_golden.py runs it in the build container to write the
 // fixtures under tests/golden/.
 //
@@ -31,6 +32,8 @@
 #include "apps/metapath.cpp"
 #elif defined(REF_APP_genericwalk)
 #include "apps/genericwalk.cpp"
+#elif defined(REF_APP_customwalk)
+#include "customwalk.cpp"  // tests/dropin_apps: a functor without a library Move
 #else
 #error "define REF_APP_<app>"
 #endif
@@ -101,6 +104,9 @@ int main(int argc, char **argv) {
     DeepWalk f(L, &graph);
 #elif defined(REF_APP_genericwalk)
     GenericWalk f(L, &graph);
+#elif defined(REF_APP_customwalk)
+    BiasedWalk f(L, static_cast<int>(opt_l(cmd, "-kind", 0)), 1.0 / opt_d(cmd, "-p", 2.0), 1.0 / opt_d(cmd, "-q", 0.5),
+                 &graph);
 #elif defined(REF_APP_ppr)
     PPR f(opt_d(cmd, "-sp", 0.2));
 #elif defined(REF_APP_node2vec)

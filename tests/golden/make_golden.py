@@ -8,8 +8,11 @@ Two sources, both reference-held or reference-computed:
   * graph_<name>.npz, tables_<name>_<app>.npz, walk_<case>.npz: outputs of the
     reference's own functions, run by oracle/_ref/harness_<app> (built from the
     reference sources by `make -C oracle ref`) on small seeded graphs.
+  * functor_<case>.npz: the same harness driving functors the library has no Move
+    kernel for (reference apps in other mode/sampler pairs, and
+    tests/dropin_apps/customwalk.cpp), walkers in generate_all_source order.
 
-Usage:  python tests/golden/make_golden.py [--ref /root/reference]
+Usage:  python tests/golden/make_golden.py [--ref /root/reference] [--only functor]
 """
 import argparse
 import json
@@ -61,6 +64,50 @@ CASES = [
     ("metapath_alias_small", "small", "metapath", 2, 2, 20, {"-s": "1,3"}, 256, 0),
 ]
 
+# Walks of functors the library has no Move kernel for (the drop-in functor engine): the
+# reference apps in (mode, sampler) pairs outside the library's kernels, and the test app
+# tests/dropin_apps/customwalk.cpp.  Sources are generate_all_source's order, as the apps
+# themselves walk them.  name: (graph, app, em, sm, length, extra-args dict, K)
+FUNCTOR_CASES = [
+    ("node2vec_dyn_its_small", "small", "node2vec", 2, 1, 20, {"-p": "2", "-q": "0.5"}, 64),
+    ("node2vec_dyn_its_hub", "hub", "node2vec", 2, 1, 20, {"-p": "2", "-q": "0.5"}, 64),
+    ("node2vec_dyn_alias_small", "small", "node2vec", 2, 2, 20, {"-p": "2", "-q": "0.5"}, 64),
+    ("node2vec_dyn_alias_hub", "hub", "node2vec", 2, 2, 20, {"-p": "2", "-q": "0.5"}, 64),
+    ("node2vec_dyn_rs_small", "small", "node2vec", 2, 3, 20, {"-p": "0.5", "-q": "2"}, 1024),
+    ("node2vec_dyn_rs_hub", "hub", "node2vec", 2, 3, 20, {"-p": "2", "-q": "0.5"}, 1024),
+    ("metapath_dyn_alias_small", "small", "metapath", 2, 2, 20, {"-s": "0,1,2,3,4"}, 64),
+    ("metapath_dyn_alias_hub", "hub", "metapath", 2, 2, 20, {"-s": "0,1,2,3,4"}, 64),
+    ("metapath_dyn_rs_small", "small", "metapath", 2, 3, 20, {"-s": "1,3"}, 1024),
+    ("metapath_dyn_rs_hub", "hub", "metapath", 2, 3, 20, {"-s": "0,1,2,3,4"}, 1024),
+    ("deepwalk_dyn_maxrs_small", "small", "deepwalk", 2, 4, 20, {}, 1024),
+    ("deepwalk_dyn_maxrs_hub", "hub", "deepwalk", 2, 4, 20, {}, 1024),
+    ("custom0_static_its_small", "small", "customwalk", 1, 1, 20, {"-kind": "0"}, 64),
+    ("custom0_static_its_hub", "hub", "customwalk", 1, 1, 20, {"-kind": "0"}, 64),
+    ("custom0_static_alias_small", "small", "customwalk", 1, 2, 20, {"-kind": "0"}, 64),
+    ("custom0_static_alias_hub", "hub", "customwalk", 1, 2, 20, {"-kind": "0"}, 64),
+    ("custom0_static_rs_small", "small", "customwalk", 1, 3, 20, {"-kind": "0"}, 4096),
+    ("custom0_dyn_its_small", "small", "customwalk", 2, 1, 20, {"-kind": "0"}, 64),
+    ("custom0_dyn_alias_small", "small", "customwalk", 2, 2, 20, {"-kind": "0"}, 64),
+    ("custom0_dyn_rs_small", "small", "customwalk", 2, 3, 20, {"-kind": "0"}, 4096),
+    ("custom1_dyn_its_small", "small", "customwalk", 2, 1, 20, {"-kind": "1"}, 64),
+    ("custom1_dyn_its_hub", "hub", "customwalk", 2, 1, 20, {"-kind": "1"}, 64),
+    ("custom1_dyn_alias_small", "small", "customwalk", 2, 2, 20, {"-kind": "1"}, 64),
+    ("custom1_dyn_alias_hub", "hub", "customwalk", 2, 2, 20, {"-kind": "1"}, 64),
+    ("custom1_dyn_rs_small", "small", "customwalk", 2, 3, 20, {"-kind": "1"}, 2048),
+    ("custom1_dyn_rs_hub", "hub", "customwalk", 2, 3, 20, {"-kind": "1"}, 2048),
+    ("custom1_dyn_maxrs_small", "small", "customwalk", 2, 4, 20, {"-kind": "1"}, 2048),
+    ("custom1_dyn_maxrs_hub", "hub", "customwalk", 2, 4, 20, {"-kind": "1"}, 2048),
+    ("custom2_dyn_its_small", "small", "customwalk", 2, 1, 20, {"-kind": "2"}, 64),
+    ("custom2_dyn_its_hub", "hub", "customwalk", 2, 1, 20, {"-kind": "2"}, 64),
+    ("custom2_dyn_alias_small", "small", "customwalk", 2, 2, 20, {"-kind": "2"}, 64),
+    ("custom2_dyn_alias_hub", "hub", "customwalk", 2, 2, 20, {"-kind": "2"}, 64),
+    ("custom2_dyn_rs_small", "small", "customwalk", 2, 3, 20, {"-kind": "2"}, 2048),
+    ("custom2_dyn_rs_hub", "hub", "customwalk", 2, 3, 20, {"-kind": "2"}, 2048),
+    ("custom2_uniform_small", "small", "customwalk", 0, 0, 20, {"-kind": "2"}, 64),
+    ("custom2_uniform_hub", "hub", "customwalk", 0, 0, 20, {"-kind": "2"}, 64),
+    ("custom4_dyn_maxrs_small", "small", "customwalk", 2, 4, 20, {"-kind": "4", "-p": "2", "-q": "0.5"}, 1024),
+]
+
 
 def build_graph(spec):
     if spec["kind"] == "chung_lu":
@@ -95,10 +142,65 @@ def load(path, dtype):
     return np.fromfile(path, dtype=dtype)
 
 
+def run_case(tmp, case, gdir, src, app, em, sm, length, extra, K, KT):
+    out = os.path.join(tmp, case)
+    os.makedirs(out)
+    src_path = os.path.join(out, "src.bin")
+    src.tofile(src_path)
+    cmd = [os.path.join(REF_BIN, f"harness_{app}"), "-f", gdir, "-ew", "-el", "-o", out,
+           "-mode", "walk", "-em", str(em), "-sm", str(sm), "-l", str(length), "-src", src_path,
+           "-K", str(K), "-KT", str(max(KT, 1)), "-seed", str(SEED), "-tseed", str(TSEED)]
+    for k, v in extra.items():
+        cmd += [k, v]
+    run(cmd)
+    return dict(paths=load(os.path.join(out, "paths.bin"), np.int32).reshape(len(src), -1),
+                lengths=load(os.path.join(out, "lengths.bin"), np.int32),
+                ends=load(os.path.join(out, "ends.bin"), np.int32),
+                status=load(os.path.join(out, "status.bin"), np.int32),
+                draws_used=load(os.path.join(out, "draws_used.bin"), np.int32),
+                steps=load(os.path.join(out, "steps.bin"), np.uint64))
+
+
+def functor_fixtures(index):
+    """functor_<case>.npz from the committed graphs (graph_<name>.npz), sources in
+    generate_all_source order."""
+    sys.path.insert(0, ROOT)
+    import thunderrw_amd as T
+
+    index["functor_walks"] = []
+    with tempfile.TemporaryDirectory() as tmp:
+        gdirs, gobjs = {}, {}
+        for name in GRAPHS:
+            z = np.load(os.path.join(HERE, f"graph_{name}.npz"))
+            g = graphs.CSR(z["degree"], z["adj"], z["weight"], z["label"])
+            d = os.path.join(tmp, "graph_" + name)
+            graphs.write_reference_csr(g, d)
+            gdirs[name], gobjs[name] = d, g
+        for case, gname, app, em, sm, length, extra, K in FUNCTOR_CASES:
+            src = T.sources_all(gobjs[gname].degree)
+            r = run_case(tmp, case, gdirs[gname], src, app, em, sm, length, extra, K, 0)
+            # Every walker's draws must fit its chunk (a walker that needs more reads its
+            # neighbour's chunk in the harness, which is no replay contract).
+            assert (r["draws_used"] >= 0).all(), case
+            np.savez_compressed(os.path.join(HERE, f"functor_{case}.npz"), **r)
+            index["functor_walks"].append({"case": case, "graph": gname, "app": app, "execution": em, "sampling": sm,
+                                           "length": length, "extra": extra, "K": K})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--only", choices=["functor"], help="write only these fixtures (index entries kept)")
     args = ap.parse_args()
+
+    if args.only == "functor":
+        with open(os.path.join(HERE, "index.json")) as f:
+            index = json.load(f)
+        functor_fixtures(index)
+        with open(os.path.join(HERE, "index.json"), "w") as f:
+            json.dump(index, f, indent=1, sort_keys=True)
+        print("wrote", len(index["functor_walks"]), "functor-engine walk fixtures")
+        return
 
     gen, arr = parse_kat(os.path.join(args.ref, "dependency/SFMT-src-1.5.1/SFMT.19937.out.txt"))
     np.savez_compressed(os.path.join(HERE, "sfmt_kat.npz"), init_gen_rand_1234=gen,
@@ -131,29 +233,16 @@ def main():
         for case, gname, app, em, sm, length, extra, K, KT in CASES:
             g = gobjs[gname]
             src = sources_for(g, gname)
-            out = os.path.join(tmp, case)
-            os.makedirs(out)
-            src_path = os.path.join(out, "src.bin")
-            src.tofile(src_path)
-            cmd = [os.path.join(REF_BIN, f"harness_{app}"), "-f", gdirs[gname], "-ew", "-el", "-o", out,
-                   "-mode", "walk", "-em", str(em), "-sm", str(sm), "-l", str(length), "-src", src_path,
-                   "-K", str(K), "-KT", str(max(KT, 1)), "-seed", str(SEED), "-tseed", str(TSEED)]
-            for k, v in extra.items():
-                cmd += [k, v]
-            run(cmd)
-            np.savez_compressed(os.path.join(HERE, f"walk_{case}.npz"),
-                                paths=load(os.path.join(out, "paths.bin"), np.int32).reshape(len(src), -1),
-                                lengths=load(os.path.join(out, "lengths.bin"), np.int32),
-                                ends=load(os.path.join(out, "ends.bin"), np.int32),
-                                status=load(os.path.join(out, "status.bin"), np.int32),
-                                draws_used=load(os.path.join(out, "draws_used.bin"), np.int32),
-                                steps=load(os.path.join(out, "steps.bin"), np.uint64))
+            r = run_case(tmp, case, gdirs[gname], src, app, em, sm, length, extra, K, KT)
+            np.savez_compressed(os.path.join(HERE, f"walk_{case}.npz"), **r)
             index["walks"].append({"case": case, "graph": gname, "app": app, "execution": em, "sampling": sm,
                                    "length": length, "extra": extra, "K": K, "KT": KT})
 
+    functor_fixtures(index)
     with open(os.path.join(HERE, "index.json"), "w") as f:
         json.dump(index, f, indent=1, sort_keys=True)
-    print("wrote", len(index["walks"]), "walk fixtures and", len(index["tables"]), "table fixtures")
+    print("wrote", len(index["walks"]), "walk fixtures,", len(index["functor_walks"]), "functor-engine walk fixtures and",
+          len(index["tables"]), "table fixtures")
 
 
 if __name__ == "__main__":

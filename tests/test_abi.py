@@ -35,7 +35,7 @@ def test_library_exports_every_declared_function():
 STRUCTS = {"trw_walk_params": A.WalkParams, "trw_walk_io": A.WalkIO, "trw_alias_slot": A.AliasSlot,
            "trw_server_config": A.ServerConfig, "trw_query": A.Query, "trw_query_info": A.QueryInfo,
            "trw_server_stats": A.ServerStats, "trw_open_loop_report": A.OpenLoopReport,
-           "trw_tune_report": A.TuneReport}
+           "trw_tune_report": A.TuneReport, "trw_device_view": A.DeviceView}
 
 
 def test_struct_layouts_match_the_header():
@@ -158,3 +158,30 @@ def test_context_without_device_reports_hip_error():
     with pytest.raises(T.TrwError) as e:
         T.Context(0)
     assert e.value.code == A.E_HIP
+
+
+def test_walk_supported_names_the_library_moves():
+    """trw_walk_supported: the (app, mode, sampler) pairs the library has Move kernels for;
+    the rest of what the reference's apps accept is the drop-in functor engine's."""
+    def rc(app, em, sm):
+        return T.lib().trw_walk_supported(ctypes.byref(T.make_params(app, execution=em, sampling=sm)))
+
+    import ctypes
+    for app in ("deepwalk", "genericwalk", "ppr", "node2vec", "metapath"):
+        assert rc(app, 0, 0) == A.OK  # uniform_move for every app
+    for app in ("deepwalk", "genericwalk", "ppr"):
+        for em in (1, 2):
+            for sm in (1, 2, 3):
+                assert rc(app, em, sm) == A.OK
+            assert rc(app, em, 4) == A.E_UNSUPPORTED
+            assert rc(app, em, 0) == A.E_UNSUPPORTED
+    assert rc("node2vec", 2, 4) == A.OK
+    for sm in (1, 2, 3):
+        assert rc("node2vec", 2, sm) == A.E_UNSUPPORTED
+        assert rc("node2vec", 1, sm) == A.E_UNSUPPORTED
+    assert rc("metapath", 2, 1) == A.OK
+    for sm in (2, 3, 4):
+        assert rc("metapath", 2, sm) == A.E_UNSUPPORTED
+    bad = T.make_params("deepwalk")
+    bad.execution = 7
+    assert T.lib().trw_walk_supported(ctypes.byref(bad)) == A.E_INVALID

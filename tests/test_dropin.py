@@ -1,7 +1,12 @@
 """The reference's own app sources, compiled unchanged against thunderrw_amd/dropin/creeper_main.h
 (thunderrw_amd/dropin/bin/<app>, built from the reference checkout), run their walks on the GPU;
-in replay mode their dumped paths equal the oracle's."""
+in replay mode their dumped paths equal the oracle's.  Functors the library has no Move kernel
+for (the reference apps in other mode/sampler pairs, tests/dropin_apps/customwalk.cpp) run in
+the functor engine and walk as the reference's own engine does (harness fixtures), and as the
+library's kernels do where the functor equals a built-in app."""
+import json
 import os
+import re
 import subprocess
 import tempfile
 
@@ -14,9 +19,9 @@ import thunderrw_amd as T
 from conftest import ROOT, load_golden_graph
 
 BIN = os.path.join(ROOT, "thunderrw_amd", "dropin", "bin")
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(not os.path.exists(os.path.join(BIN, "deepwalk")),
-                                 reason="drop-in apps not built (needs the reference checkout at build time)")]
+pytestmark = [pytest.mark.gpu]
+needs_reference_apps = pytest.mark.skipif(not os.path.exists(os.path.join(BIN, "deepwalk")),
+                                          reason="drop-in apps not built (needs the reference checkout at build time)")
 
 # app, args, oracle (app, execution, sampling, length, extra), sources kind, K, KT
 CASES = [
@@ -33,6 +38,7 @@ CASES = [
 ]
 
 
+@needs_reference_apps
 @pytest.mark.parametrize("gname", ["small", "hub"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: c[0] + "_" + "_".join(c[1][1:4]))
 def test_reference_app_runs_unchanged_on_the_gpu(gname, case):
@@ -67,3 +73,82 @@ def test_reference_app_runs_unchanged_on_the_gpu(gname, case):
     ref = ob.walk(og, oapp, em, sm, L, src, d, K, td, KT1, **extra)
     assert np.array_equal(paths.reshape(n, -1), ref["paths"])
     assert np.array_equal(lengths, ref["lengths"])
+
+
+# ---- the functor engine: functors the library has no Move kernel for ----
+
+def _functor_cases():
+    with open(os.path.join(ROOT, "tests", "golden", "index.json")) as f:
+        return json.load(f)["functor_walks"]
+
+
+def _run_app(app, g, args, env_extra):
+    """Runs dropin/bin/<app> on g (written as reference CSR files) and returns its dumped walk,
+    the steps it logged and its configuration line."""
+    with tempfile.TemporaryDirectory() as tmp:
+        gdir = os.path.join(tmp, "g")
+        graphs.write_reference_csr(g, gdir)
+        out = os.path.join(tmp, "out")
+        os.makedirs(out)
+        env = dict(os.environ, TRW_DUMP_DIR=out, **env_extra)
+        r = subprocess.run([os.path.join(BIN, app), "-f", gdir, "-ew", "-el"] + args, env=env, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-2000:]
+        res = {k: np.fromfile(os.path.join(out, k + ".bin"), np.int32) for k in ("paths", "lengths", "status")}
+    res["steps"] = int(re.search(r"Num of steps: (\d+)", r.stderr).group(1))
+    res["config"] = re.search(r"Configuration: (.*)", r.stderr).group(1)
+    return res
+
+
+def _app_args(case):
+    args = ["-l", str(case["length"]), "-em", str(case["execution"]), "-sm", str(case["sampling"])]
+    for k, v in case["extra"].items():
+        args += [k, v]
+    return args
+
+
+@pytest.mark.parametrize("case", _functor_cases(), ids=lambda c: c["case"])
+def test_functor_engine_matches_the_reference_harness(case):
+    """Replay mode: the reference's own Moves, gather and update driving the same functor
+    (oracle/_ref/harness_<app>, fixtures tests/golden/functor_*.npz) and the functor engine
+    running it on the GPU give the same walks, bit for bit."""
+    if not os.path.exists(os.path.join(BIN, case["app"])):
+        pytest.skip("drop-in app %s not built" % case["app"])
+    g, _ = load_golden_graph(case["graph"])
+    ref = np.load(os.path.join(ROOT, "tests", "golden", "functor_%s.npz" % case["case"]))
+    r = _run_app(case["app"], g, _app_args(case),
+                 {"TRW_RNG": "replay", "TRW_SEED": "1234", "TRW_REPLAY_K": str(case["K"])})
+    assert r["config"].endswith("functor engine")
+    n = len(ref["lengths"])
+    assert np.array_equal(r["paths"].reshape(n, -1), ref["paths"])
+    assert np.array_equal(r["lengths"], ref["lengths"])
+    assert np.array_equal(r["status"], ref["status"])  # 0 done, 1 gather-terminated in both
+    assert r["steps"] == int(ref["steps"][0])
+
+
+# customwalk kinds 3 and 4 are DeepWalk and Node2Vec under other names: in free-running
+# mode the functor engine must walk exactly as the library's kernels for those apps.
+TWINS = [(3, 1, 2, "deepwalk"), (3, 1, 1, "deepwalk"), (3, 1, 3, "deepwalk"), (3, 2, 1, "deepwalk"),
+         (3, 2, 2, "deepwalk"), (3, 2, 3, "deepwalk"), (3, 0, 0, "deepwalk"), (4, 2, 4, "node2vec")]
+
+
+@pytest.mark.parametrize("gname", ["small", "hub"])
+@pytest.mark.parametrize("twin", TWINS, ids=lambda t: "kind%d_em%d_sm%d" % t[:3])
+def test_functor_engine_walks_as_the_library_in_free_running_mode(gpu_ctx, gname, twin):
+    kind, em, sm, app = twin
+    if not os.path.exists(os.path.join(BIN, "customwalk")):
+        pytest.skip("test app not built")
+    g, _ = load_golden_graph(gname)
+    r = _run_app("customwalk", g, ["-l", "20", "-em", str(em), "-sm", str(sm), "-kind", str(kind)],
+                 {"TRW_RNG": "philox", "TRW_SEED": "77"})
+    assert r["config"].endswith("functor engine")
+    src = T.sources_all(g.degree)
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight, g.label)
+    try:
+        lib = T.walk(dg, T.make_params(app, execution=em, sampling=sm, length=20, rng="philox", seed=77), src)
+    finally:
+        dg.close()
+    n = len(src)
+    assert np.array_equal(r["paths"].reshape(n, -1), lib["paths"])
+    assert np.array_equal(r["lengths"], lib["lengths"])
+    assert r["steps"] == lib["steps"]

@@ -13,13 +13,13 @@ SM_UNIFORM, SM_ITS, SM_ALIAS, SM_REJECTION, SM_MAX_WEIGHT_REJECTION = 0, 1, 2, 3
 APP_DEEPWALK, APP_PPR, APP_NODE2VEC, APP_METAPATH, APP_GENERIC = 0, 1, 2, 3, 4
 RNG_REPLAY, RNG_PHILOX = 0, 1
 FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP, FORM_GRAPH = 0, 1, 2, 3, 4
-WEIGHTS_EDGE, WEIGHTS_UNIT = 0, 1
+WEIGHTS_EDGE, WEIGHTS_UNIT, WEIGHTS_CUSTOM = 0, 1, 2
 WALK_WEIGHTS_APP, WALK_WEIGHTS_EDGE, WALK_WEIGHTS_UNIT = 0, 1, 2
 TUNE_WALKERS_PER_THREAD, TUNE_ITS_TREE_MIN_DEGREE, TUNE_QUEUE_CHUNK = 0, 1, 2
 RESULT_PPR, RESULT_PATHS = 0, 1
 QUERY_PENDING, QUERY_RUNNING, QUERY_DONE, QUERY_FAILED = 0, 1, 2, 3
 WALKER_DONE, WALKER_GATHER_TERMINATED, WALKER_DEAD_END, WALKER_REPLAY_EXHAUSTED, WALKER_TRIAL_CAP, \
-    WALKER_BAD_SOURCE = 0, 1, 2, 3, 4, 5
+    WALKER_BAD_SOURCE, WALKER_STEP_CAP = 0, 1, 2, 3, 4, 5, 6
 
 APPS = {"deepwalk": APP_DEEPWALK, "ppr": APP_PPR, "node2vec": APP_NODE2VEC, "metapath": APP_METAPATH,
         "genericwalk": APP_GENERIC}
@@ -33,6 +33,14 @@ class TrwError(RuntimeError):
 
 class AliasSlot(C.Structure):
     _fields_ = [("alias_value", C.c_double), ("first", C.c_int32), ("second", C.c_int32)]
+
+
+class DeviceView(C.Structure):
+    _fields_ = [("num_vertices", C.c_int32), ("max_degree", C.c_int32), ("num_edges", C.c_int64),
+                ("offset", C.c_void_p), ("degree", C.c_void_p), ("adj", C.c_void_p), ("edge_weight", C.c_void_p),
+                ("edge_label", C.c_void_p), ("alias_table", C.c_void_p), ("prefix_sum", C.c_void_p),
+                ("rejection_max", C.c_void_p), ("rejection_weight", C.c_void_p), ("alias_weights", C.c_int32),
+                ("its_weights", C.c_int32), ("rejection_weights", C.c_int32), ("reserved", C.c_int32)]
 
 
 class WalkParams(C.Structure):
@@ -112,12 +120,15 @@ _SIGS = {
     "trw_copy_prefix_sum": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_copy_rejection_max": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_last_build_ms": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_build_tables_from": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "trw_graph_device_view": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_graph_store_tables": (C.c_int, [C.c_void_p, C.c_char_p]),
     "trw_graph_load_tables": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]),
     "trw_graph_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "trw_graph_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "trw_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_int32, C.c_void_p]),
     "trw_walk_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "trw_walk_supported": (C.c_int, [C.c_void_p]),
     "trw_walk_mix_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "trw_walk_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p,
                                 C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

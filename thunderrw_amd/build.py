@@ -62,6 +62,21 @@ def build(verbose=False):
 
 DROPIN = os.path.join(HERE, "dropin")
 DROPIN_APPS = ["deepwalk", "ppr", "node2vec", "metapath", "genericwalk"]
+# Apps of this repository's tests whose functors have no built-in Move (functor engine).
+TEST_APPS_DIR = os.path.join(ROOT, "tests", "dropin_apps")
+
+
+def compile_app(src, exe):
+    """One app translation unit against dropin/creeper_main.h and this library, with hipcc
+    (the functor engine instantiates kernels for the app's functor)."""
+    cmd = [hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", "-x", "hip", "--offload-arch=" + ARCH, "-Xclang",
+           "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument", "-I", DROPIN,
+           "-I", os.path.join(ROOT, "include"), src, "-o", exe, "-L", HERE, "-lthunderrw",
+           "-Wl,-rpath,$ORIGIN/../.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in build failed: %s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
+    return exe
 
 
 def build_dropin(reference_dir, verbose=False):
@@ -73,27 +88,32 @@ def build_dropin(reference_dir, verbose=False):
         return []
     out_dir = os.path.join(DROPIN, "bin")
     os.makedirs(out_dir, exist_ok=True)
-    cxx = shutil.which("g++") or "g++"
-
-    def one(app):
-        exe = os.path.join(out_dir, app)
-        cmd = [cxx, "-std=c++17", "-O2", "-I", DROPIN, "-I", os.path.join(ROOT, "include"),
-               os.path.join(apps_dir, app + ".cpp"), "-o", exe, "-L", HERE, "-lthunderrw",
-               "-Wl,-rpath,$ORIGIN/../.."]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError("drop-in build failed: %s\n%s" % (" ".join(cmd), r.stderr))
-        return exe
-
-    with cf.ThreadPoolExecutor(max_workers=len(DROPIN_APPS)) as ex:
-        exes = list(ex.map(one, DROPIN_APPS))
+    jobs = [(os.path.join(apps_dir, app + ".cpp"), os.path.join(out_dir, app)) for app in DROPIN_APPS]
+    with cf.ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        exes = list(ex.map(lambda j: compile_app(*j), jobs))
     if verbose:
         print("built drop-in apps:", " ".join(DROPIN_APPS))
     return exes
 
 
+def build_test_apps(verbose=False):
+    """tests/dropin_apps/<name>.cpp -> dropin/bin/<name> (no reference checkout needed)."""
+    if not os.path.isdir(TEST_APPS_DIR):
+        return []
+    out_dir = os.path.join(DROPIN, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    srcs = sorted(f for f in os.listdir(TEST_APPS_DIR) if f.endswith(".cpp"))
+    jobs = [(os.path.join(TEST_APPS_DIR, f), os.path.join(out_dir, f[:-4])) for f in srcs]
+    with cf.ThreadPoolExecutor(max_workers=max(1, len(jobs))) as ex:
+        exes = list(ex.map(lambda j: compile_app(*j), jobs))
+    if verbose and exes:
+        print("built test apps:", " ".join(os.path.basename(e) for e in exes))
+    return exes
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_test_apps(verbose=True)
     if len(sys.argv) > 1:
         build_dropin(sys.argv[1], verbose=True)
     sys.exit(0)

@@ -260,6 +260,7 @@ static void graph_release(trw_graph *g) {
     dfree(g->alias);
     dfree(g->prefix);
     dfree(g->rs_max);
+    dfree(g->rs_custom);
     dfree(g->counters);
     dfree(g->state);
     dfree(g->active);
@@ -422,17 +423,17 @@ static int build_its_trees(trw_graph *g, bool monotone) {
     return TRW_OK;
 }
 
-int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
-    if (!g) return fail(TRW_E_INVALID, "graph is null");
-    if (weights != TRW_WEIGHTS_EDGE && weights != TRW_WEIGHTS_UNIT) return fail(TRW_E_INVALID, "unknown weights %d", weights);
-    if (weights == TRW_WEIGHTS_EDGE && !g->weight) return fail(TRW_E_STATE, "graph was uploaded without edge weights");
-    if (sampling != TRW_SM_ALIAS && sampling != TRW_SM_ITS && sampling != TRW_SM_REJECTION)
-        return fail(TRW_E_INVALID, "static tables exist for ITS, alias and rejection sampling only");
+// static_initialization's table build (creeper.h:102-175) from per-edge weights w (device,
+// ignored when unit): `weights` is the TRW_WEIGHTS_* tag recorded for the table, `monotone`
+// says the weights are all finite and >= 0 (ITS search structures are built only then).
+static int build_tables(trw_graph *g, int32_t sampling, const double *w, int32_t unit, int32_t weights,
+                        bool monotone) {
     TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
     hipStream_t st = g->ctx->stream;
     const DevGraph dv = g->view();
-    const int32_t unit = weights == TRW_WEIGHTS_UNIT;
     const size_t E = static_cast<size_t>(g->ne), V = static_cast<size_t>(g->nv);
+    // The table is undefined until this build succeeds.
+    (sampling == TRW_SM_ALIAS ? g->alias_weights : sampling == TRW_SM_ITS ? g->its_weights : g->rs_weights) = -1;
     hipEvent_t t0, t1;
     TRW_HIP(hipEventCreate(&t0), "hipEventCreate");
     TRW_HIP(hipEventCreate(&t1), "hipEventCreate");
@@ -449,7 +450,7 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
         TRW_HIP(dmalloc(&scratch, E + 1), "hipMalloc alias scratch");
         trw::launch_fill_u32(g->counters + 2, 0u, 1, st);
         (void)hipEventRecord(t0, st);
-        hipError_t e = trw::launch_alias_build(dv, g->weight, unit, g->order, g->build_hubs, scratch, g->alias,
+        hipError_t e = trw::launch_alias_build(dv, w, unit, g->order, g->build_hubs, scratch, g->alias,
                                               g->counters + 2, st);
         (void)hipEventRecord(t1, st);
         hipError_t e2 = hipStreamSynchronize(st);
@@ -467,9 +468,9 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
         // Padded so that a search may read the whole aligned 16-entry chunk holding any entry.
         if (!g->prefix) TRW_HIP(dmalloc(&g->prefix, E + 2 * trw::kItsFan), "hipMalloc prefix sums");
         (void)hipEventRecord(t0, st);
-        hipError_t e = trw::launch_its_build(dv, g->weight, unit, g->order, g->build_hubs, g->prefix, st);
+        hipError_t e = trw::launch_its_build(dv, w, unit, g->order, g->build_hubs, g->prefix, st);
         if (e != hipSuccess) return hip_fail(e, "ITS build launch");
-        int rc = build_its_trees(g, unit || g->weights_nonneg);
+        int rc = build_its_trees(g, unit || monotone);
         (void)hipEventRecord(t1, st);
         if (rc != TRW_OK) return rc;
         TRW_HIP(hipStreamSynchronize(st), "ITS build");
@@ -477,7 +478,7 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
     } else {
         if (!g->rs_max) TRW_HIP(dmalloc(&g->rs_max, V), "hipMalloc rejection maxima");
         (void)hipEventRecord(t0, st);
-        hipError_t e = trw::launch_rs_build(dv, g->weight, unit, g->order, g->build_hubs, g->rs_max, st);
+        hipError_t e = trw::launch_rs_build(dv, w, unit, g->order, g->build_hubs, g->rs_max, st);
         (void)hipEventRecord(t1, st);
         if (e != hipSuccess) return hip_fail(e, "rejection build launch");
         TRW_HIP(hipStreamSynchronize(st), "rejection build");
@@ -485,6 +486,58 @@ int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
     }
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess) g->last_build_ms = ms;
+    return TRW_OK;
+}
+
+int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
+    if (!g) return fail(TRW_E_INVALID, "graph is null");
+    if (weights != TRW_WEIGHTS_EDGE && weights != TRW_WEIGHTS_UNIT) return fail(TRW_E_INVALID, "unknown weights %d", weights);
+    if (weights == TRW_WEIGHTS_EDGE && !g->weight) return fail(TRW_E_STATE, "graph was uploaded without edge weights");
+    if (sampling != TRW_SM_ALIAS && sampling != TRW_SM_ITS && sampling != TRW_SM_REJECTION)
+        return fail(TRW_E_INVALID, "static tables exist for ITS, alias and rejection sampling only");
+    return build_tables(g, sampling, g->weight, weights == TRW_WEIGHTS_UNIT, weights, g->weights_nonneg);
+}
+
+int trw_build_tables_from(trw_graph *g, int32_t sampling, const double *device_weights) {
+    if (!g) return fail(TRW_E_INVALID, "graph is null");
+    if (sampling != TRW_SM_ALIAS && sampling != TRW_SM_ITS && sampling != TRW_SM_REJECTION)
+        return fail(TRW_E_INVALID, "static tables exist for ITS, alias and rejection sampling only");
+    if (!device_weights && g->ne > 0) return fail(TRW_E_INVALID, "device_weights is null");
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    const double *w = device_weights;
+    if (sampling == TRW_SM_REJECTION) {
+        // static_rs_move reads the weights beside the maxima (rejection_sampling.h:334-366): keep a copy.
+        if (!g->rs_custom) TRW_HIP(dmalloc(&g->rs_custom, static_cast<size_t>(g->ne) + 1), "hipMalloc rejection weights");
+        if (g->ne > 0)
+            TRW_HIP(hipMemcpyAsync(g->rs_custom, device_weights, sizeof(double) * static_cast<size_t>(g->ne),
+                                   hipMemcpyDeviceToDevice, g->ctx->stream),
+                    "copy rejection weights");
+        w = g->rs_custom;
+    }
+    // Custom weights get no ITS search structures (their walks binary-search the prefix sums).
+    return build_tables(g, sampling, w, 0, TRW_WEIGHTS_CUSTOM, false);
+}
+
+int trw_graph_device_view(const trw_graph *g, trw_device_view *out) {
+    if (!g || !out) return fail(TRW_E_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->num_vertices = g->nv;
+    out->max_degree = g->max_degree;
+    out->num_edges = g->ne;
+    out->offset = g->offset;
+    out->degree = g->degree;
+    out->adj = g->adj;
+    out->edge_weight = g->weight;
+    out->edge_label = g->label;
+    out->alias_table = g->alias_weights >= 0 ? reinterpret_cast<const trw_alias_slot *>(g->alias) : nullptr;
+    out->prefix_sum = g->its_weights >= 0 ? g->prefix : nullptr;
+    out->rejection_max = g->rs_weights >= 0 ? g->rs_max : nullptr;
+    out->rejection_weight = g->rs_weights == TRW_WEIGHTS_EDGE     ? g->weight
+                            : g->rs_weights == TRW_WEIGHTS_CUSTOM ? g->rs_custom
+                                                                  : nullptr;
+    out->alias_weights = g->alias_weights;
+    out->its_weights = g->its_weights;
+    out->rejection_weights = g->rs_weights;
     return TRW_OK;
 }
 
@@ -648,6 +701,30 @@ int trw_copy_rejection_max(const trw_graph *g, double *host_out) {
 
 // ---------------------------------------------------------------- walks
 
+int trw_walk_supported(const trw_walk_params *p) {
+    if (!p) return fail(TRW_E_INVALID, "params is null");
+    if (p->app < TRW_APP_DEEPWALK || p->app > TRW_APP_GENERIC) return fail(TRW_E_INVALID, "unknown app %d", p->app);
+    if (p->execution < TRW_EXEC_UNIFORM || p->execution > TRW_EXEC_DYNAMIC)
+        return fail(TRW_E_INVALID, "unknown execution mode %d", p->execution);
+    if (p->sampling < TRW_SM_UNIFORM || p->sampling > TRW_SM_MAX_WEIGHT_REJECTION)
+        return fail(TRW_E_INVALID, "unknown sampling method %d", p->sampling);
+    if (p->execution == TRW_EXEC_UNIFORM) return TRW_OK;  // uniform_move for every app
+    if (p->app == TRW_APP_NODE2VEC) {
+        if (p->execution != TRW_EXEC_DYNAMIC || p->sampling != TRW_SM_MAX_WEIGHT_REJECTION)
+            return fail(TRW_E_UNSUPPORTED, "Node2Vec runs in Dynamic mode with max-weight rejection sampling");
+        return TRW_OK;
+    }
+    if (p->app == TRW_APP_METAPATH) {
+        if (p->execution != TRW_EXEC_DYNAMIC || p->sampling != TRW_SM_ITS)
+            return fail(TRW_E_UNSUPPORTED, "MetaPath runs in Dynamic mode with inverse-transform sampling");
+        return TRW_OK;
+    }
+    if (p->sampling == TRW_SM_UNIFORM || p->sampling == TRW_SM_MAX_WEIGHT_REJECTION)
+        return fail(TRW_E_UNSUPPORTED, "sampling method %d is not defined for this app in mode %d", p->sampling,
+                    p->execution);
+    return TRW_OK;
+}
+
 // Maps an app + g_para to a device mode, building the static tables the mode
 // reads when they are missing.
 static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, bool allow_build = true) {
@@ -679,13 +756,15 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
     k.max_trials = p->max_trials;
     k.stop_prob = p->stop_probability;
 
+    {
+        const int rc = trw_walk_supported(p);
+        if (rc != TRW_OK) return rc;
+    }
     if (p->execution == TRW_EXEC_UNIFORM) {
         k.mode = trw::kModeUniform;
         return TRW_OK;
     }
     if (p->app == TRW_APP_NODE2VEC) {
-        if (p->execution != TRW_EXEC_DYNAMIC || p->sampling != TRW_SM_MAX_WEIGHT_REJECTION)
-            return fail(TRW_E_UNSUPPORTED, "Node2Vec runs in Dynamic mode with max-weight rejection sampling");
         if (!(p->p > 0.0) || !(p->q > 0.0)) return fail(TRW_E_INVALID, "Node2Vec p and q must be > 0");
         k.mode = trw::kModeNode2Vec;
         k.n2v_p = p->p;  // Node2Vec::p_ (node2vec.cpp:18, after the inversion at :69)
@@ -694,8 +773,6 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
         return TRW_OK;
     }
     if (p->app == TRW_APP_METAPATH) {
-        if (p->execution != TRW_EXEC_DYNAMIC || p->sampling != TRW_SM_ITS)
-            return fail(TRW_E_UNSUPPORTED, "MetaPath runs in Dynamic mode with inverse-transform sampling");
         if (!g->label) return fail(TRW_E_STATE, "MetaPath needs edge labels");
         if (p->schema_length <= 0 || p->schema_length > 16) return fail(TRW_E_INVALID, "schema length must be 1..16");
         k.mode = trw::kModeMetaPath;
@@ -705,9 +782,6 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
     }
     // DeepWalk, GenericWalk (edge weights) and PPR (unit weights): walker-independent
     // weights, so the Dynamic gather rebuilds exactly the Static tables every step.
-    if (p->sampling == TRW_SM_UNIFORM || p->sampling == TRW_SM_MAX_WEIGHT_REJECTION)
-        return fail(TRW_E_UNSUPPORTED, "sampling method %d is not defined for this app in mode %d", p->sampling,
-                    p->execution);
     const int32_t wsrc = p->weights == TRW_WALK_WEIGHTS_EDGE   ? TRW_WEIGHTS_EDGE
                          : p->weights == TRW_WALK_WEIGHTS_UNIT ? TRW_WEIGHTS_UNIT
                          : ppr                                  ? TRW_WEIGHTS_UNIT

@@ -38,6 +38,7 @@ struct trw_graph {
     trw::AliasSlot *alias = nullptr;
     double *prefix = nullptr;
     double *rs_max = nullptr;
+    double *rs_custom = nullptr;   // rejection weights of a TRW_WEIGHTS_CUSTOM build (E)
     int32_t alias_weights = -1, its_weights = -1, rs_weights = -1;  // TRW_WEIGHTS_* the table was built from
     unsigned int *counters = nullptr;                                 // [0] queue head [1] alive [2] zero-sum count
     trw::WalkerState *state = nullptr;

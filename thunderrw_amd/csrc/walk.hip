@@ -17,6 +17,7 @@
 // order with contraction off (-ffp-contract=off); u % d is 32-bit unsigned, which
 // equals the reference's uint32 % int64 for 1 <= d < 2^31.
 #include "engine.h"
+#include "thunderrw_device.h"
 
 #include <algorithm>
 
@@ -31,31 +32,9 @@ constexpr int kTileMulti = 32;  // ... by walk_fused_multi_kernel (128-byte row 
 constexpr int kQueueBlock = 256;
 constexpr int kStepBlock = 256;
 
-constexpr double kToReal1 = 1.0 / 4294967295.0;  // SFMT.h:193-207
-constexpr double kToReal2 = 1.0 / 4294967296.0;  // SFMT.h:214-228
-
-__device__ __forceinline__ double real1(uint32_t v) { return static_cast<double>(v) * kToReal1; }
-__device__ __forceinline__ double real2(uint32_t v) { return static_cast<double>(v) * kToReal2; }
-
-// Philox-4x32-10 (Salmon et al., SC'11).
-__device__ __forceinline__ uint4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
-                                               uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        if (r) {
-            k0 += 0x9E3779B9u;
-            k1 += 0xBB67AE85u;
-        }
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-        c0 = n0;
-        c1 = lo1;
-        c2 = n2;
-        c3 = lo0;
-    }
-    return make_uint4(c0, c1, c2, c3);
-}
+using trw_dev::philox4x32_10;
+using trw_dev::real1;
+using trw_dev::real2;
 
 // Free-running stream: one Philox block per (global walker id, step, trial).
 // A step's move draws are words .x/.y of its trial's block; the PPR stop draw is
@@ -118,13 +97,7 @@ __device__ __forceinline__ bool draw_stop(const BatchIO &io, int32_t w, WalkerSt
 // Utility::upper_bound_branchless (util/utility.h:18-27): the number of
 // entries <= target in a non-decreasing array.
 __device__ __forceinline__ int32_t count_le(const double *src, int32_t n, double target) {
-    const double *base = src;
-    while (n > 1) {
-        const int32_t half = n >> 1;
-        base = (base[half] <= target) ? base + half : base;
-        n -= half;
-    }
-    return static_cast<int32_t>((*base <= target) + (base - src));
+    return trw_dev::upper_bound_branchless(src, n, target);
 }
 
 // Graph::is_neighbor (util/graph/graph.h:180-199) with both endpoints' ranges

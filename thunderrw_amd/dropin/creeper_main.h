@@ -6,13 +6,19 @@
 // InputParser, Utility::split, the walker generators, the SFMT API their functors
 // seed, and compute<F>(Graph&, std::vector<WalkerMeta>&, F).  This header provides
 // all of them with the same names, signatures and meanings, so those app sources
-// compile unchanged against it (g++ -std=c++17 -I thunderrw_amd/dropin -I include
-// app.cpp -L thunderrw_amd -lthunderrw) and run their walks on the GPU.
+// compile unchanged against it (hipcc -x hip --offload-arch=gfx950 -std=c++17
+// -I thunderrw_amd/dropin -I include app.cpp -L thunderrw_amd -lthunderrw) and run their
+// walks on the GPU.
 //
-// compute<F> recognises the reference's built-in functors by their members
+// compute<F> recognises the reference's built-in functors by type name and members
 // (PPR: tp_; MetaPath: l_, schema_length_, schema_; Node2Vec: l_, p_, q_; DeepWalk and
-// GenericWalk: l_) and runs the matching device Move; a functor with other members is
-// rejected at compile time, since its weight()/update() would have to run on the device.
+// GenericWalk: l_) and runs the library's Move kernel for them where the library has one for
+// the (execution mode, sampling method) pair (trw_walk_supported).  Every other functor, and
+// those pairs the library lacks (e.g. Node2Vec with dynamic ITS, MetaPath with dynamic alias
+// or rejection, DeepWalk with max-weight rejection), runs in the functor engine
+// (functor_walk.h): kernels instantiated for F in the app's translation unit call its
+// weight(), update() and max_weight() on the device.  That needs hipcc; built with g++ the
+// header serves the library's Moves only.
 //
 // Environment (no counterpart in the reference):
 //   TRW_DEVICE      HIP device (default 0)
@@ -22,6 +28,9 @@
 //   TRW_SEED, TRW_TSEED, TRW_REPLAY_K, TRW_REPLAY_KT
 //   TRW_DUMP_DIR    write paths.bin (walkers x length int32, LOG_SEQUENCE layout),
 //                   lengths.bin, ends.bin, status.bin there after the walk
+//   TRW_MAX_MOVES   functor engine: moves per walker before it ends as TRW_WALKER_STEP_CAP
+//                   (default max(65536, 4 x length); the reference has no cap)
+//   TRW_MAX_TRIALS  functor engine: rejection trials per move (default 65536)
 #ifndef THUNDERRW_DROPIN_CREEPER_MAIN_H
 #define THUNDERRW_DROPIN_CREEPER_MAIN_H
 
@@ -48,6 +57,13 @@
 
 #include "SFMT.h"
 #include "thunderrw.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define TRW_DROPIN_HD __host__ __device__
+#else
+#define TRW_DROPIN_HD
+#endif
 
 typedef int intT;
 typedef unsigned int uintT;
@@ -174,6 +190,7 @@ public:
     Graph(const Graph &) = delete;
     Graph &operator=(const Graph &) = delete;
     ~Graph() {
+        if (image_) trw_device_free(trw_dropin::context(), image_);
         if (device_) trw_graph_free(device_);
         if (host_) trw_host_graph_free(host_);
     }
@@ -210,20 +227,26 @@ public:
                              max_degree_);
     }
 
-    intT num_vertices() const { return num_vertices_; }
-    int64_t num_edges() const { return num_edges_; }
-    double avg_degree() const { return num_edges_ / static_cast<double>(num_vertices_); }
-    intT max_degree() const { return max_degree_; }
-    intT node_with_max_degree() const { return node_with_max_degree_; }
-    intT degree(intT u) const { return degree_[u]; }
-    std::pair<intT *, intT> neighbors(intT u) const { return {adj_ + offset_[u], degree(u)}; }
-    std::pair<int64_t, int64_t> edges(intT u) const { return {offset_[u], offset_[u + 1]}; }
-    double edge_weight(int64_t eid) const { return edge_weight_[eid]; }
-    int edge_label(int64_t eid) const { return edge_label_[eid]; }
+    // The accessors functors call; under hipcc they also run on the device, where the
+    // functor engine hands functors a copy of this object whose arrays are the graph's in HBM.
+    TRW_DROPIN_HD intT num_vertices() const { return num_vertices_; }
+    TRW_DROPIN_HD int64_t num_edges() const { return num_edges_; }
+    TRW_DROPIN_HD double avg_degree() const { return num_edges_ / static_cast<double>(num_vertices_); }
+    TRW_DROPIN_HD intT max_degree() const { return max_degree_; }
+    TRW_DROPIN_HD intT node_with_max_degree() const { return node_with_max_degree_; }
+    TRW_DROPIN_HD intT degree(intT u) const { return degree_[u]; }
+    TRW_DROPIN_HD std::pair<intT *, intT> neighbors(intT u) const { return {adj_ + offset_[u], degree(u)}; }
+    TRW_DROPIN_HD std::pair<int64_t, int64_t> edges(intT u) const { return {offset_[u], offset_[u + 1]}; }
+    TRW_DROPIN_HD double edge_weight(int64_t eid) const { return edge_weight_[eid]; }
+    TRW_DROPIN_HD int edge_label(int64_t eid) const { return edge_label_[eid]; }
 
     // graph.h:180-199: the larger-degree endpoint searched in the smaller one's list.
-    bool is_neighbor(intT u, intT v) const {
-        if (degree(u) < degree(v)) std::swap(u, v);
+    TRW_DROPIN_HD bool is_neighbor(intT u, intT v) const {
+        if (degree(u) < degree(v)) {
+            const intT t = u;
+            u = v;
+            v = t;
+        }
         auto nbrs = neighbors(v);
         int b = 0, e = nbrs.second - 1;
         while (b <= e) {
@@ -242,9 +265,35 @@ public:
         return device_;
     }
 
+    // A device copy of this object whose array fields (offset_, adj_, degree_, edge_weight_,
+    // edge_label_) are the graph's arrays in HBM; made once, freed with the graph.
+    const void *device_image() {
+        if (image_) return image_;
+        trw_device_view v;
+        if (trw_graph_device_view(device(), &v) != TRW_OK) trw_dropin::die("trw_graph_device_view");
+        std::vector<unsigned char> bytes(sizeof(Graph));
+        std::memcpy(bytes.data(), static_cast<const void *>(this), sizeof(Graph));
+        auto put = [&](const void *field, const void *value) {
+            const size_t at = static_cast<size_t>(static_cast<const char *>(field) - reinterpret_cast<const char *>(this));
+            std::memcpy(bytes.data() + at, &value, sizeof(value));
+        };
+        put(&offset_, v.offset);
+        put(&adj_, v.adj);
+        put(&degree_, v.degree);
+        put(&edge_weight_, v.edge_weight);
+        put(&edge_label_, v.edge_label);
+        put(&device_, nullptr);
+        put(&host_, nullptr);
+        if (trw_device_alloc(trw_dropin::context(), sizeof(Graph), &image_) != TRW_OK ||
+            trw_copy_to_device(trw_dropin::context(), image_, bytes.data(), sizeof(Graph)) != TRW_OK)
+            trw_dropin::die("graph device image");
+        return image_;
+    }
+
 private:
     trw_host_graph *host_ = nullptr;
     std::vector<int64_t> offsets_;
+    void *image_ = nullptr;
 };
 
 // walker_generator.cpp:9-25
@@ -297,15 +346,36 @@ TRW_DROPIN_MEMBER_TRAIT(p_)
 TRW_DROPIN_MEMBER_TRAIT(q_)
 TRW_DROPIN_MEMBER_TRAIT(schema_)
 TRW_DROPIN_MEMBER_TRAIT(schema_length_)
+TRW_DROPIN_MEMBER_TRAIT(sfmt_)
 #undef TRW_DROPIN_MEMBER_TRAIT
 
-// The reference functor F as a built-in app and its constructor arguments.
+// Whether F's type is named `want` (the compiler's function signature spells "F = <type>").
 template <class F>
-void describe(const F &f, trw_walk_params &p) {
+bool functor_named(const char *want) {
+    const char *p = __PRETTY_FUNCTION__;
+    for (; *p; ++p) {
+        if (p[0] == 'F' && p[1] == ' ' && p[2] == '=' && p[3] == ' ') {
+            const char *q = p + 4;
+            size_t i = 0;
+            for (; want[i]; ++i)
+                if (q[i] != want[i]) return false;
+            return q[i] == ']' || q[i] == ';';
+        }
+    }
+    return false;
+}
+
+// The reference's built-in functors, recognised by name and members, as library apps with
+// their constructor arguments.  Returns false for any other functor.
+template <class F>
+bool describe(const F &f, trw_walk_params &p) {
     if constexpr (has_tp_<F>::value) {
+        if (!functor_named<F>("PPR")) return false;
         p.app = TRW_APP_PPR;  // apps/ppr.cpp:10-34
         p.stop_probability = f.tp_;
+        return true;
     } else if constexpr (has_schema_<F>::value && has_schema_length_<F>::value && has_l_<F>::value) {
+        if (!functor_named<F>("MetaPath")) return false;
         p.app = TRW_APP_METAPATH;  // apps/metapath.cpp:9-39
         if (f.schema_length_ < 1 || f.schema_length_ > 16) {
             std::fprintf(stderr, "[thunderrw] MetaPath schema length must be 1..16\n");
@@ -313,14 +383,26 @@ void describe(const F &f, trw_walk_params &p) {
         }
         p.schema_length = f.schema_length_;
         for (int i = 0; i < f.schema_length_; ++i) p.schema[i] = f.schema_[i];
+        return true;
     } else if constexpr (has_p_<F>::value && has_q_<F>::value && has_l_<F>::value) {
+        if (!functor_named<F>("Node2Vec")) return false;
         p.app = TRW_APP_NODE2VEC;  // apps/node2vec.cpp:9-47
         p.p = f.p_;
         p.q = f.q_;
+        return true;
     } else if constexpr (has_l_<F>::value) {
+        if (!functor_named<F>("DeepWalk") && !functor_named<F>("GenericWalk")) return false;
         p.app = TRW_APP_DEEPWALK;  // apps/deepwalk.cpp:9-30, apps/genericwalk.cpp:9-30
+        return true;
     } else {
-        static_assert(has_l_<F>::value, "compute<F>: only the reference's built-in functors run on the device");
+#if !defined(__HIPCC__)
+        static_assert(has_l_<F>::value,
+                      "compute<F>: a functor other than the reference's runs its weight()/update() on the device; "
+                      "compile the app with hipcc");
+#endif
+        (void)f;
+        (void)p;
+        return false;
     }
 }
 
@@ -340,16 +422,147 @@ void dump(const std::string &dir, const char *name, const std::vector<T> &v) {
     f.write(reinterpret_cast<const char *>(v.data()), static_cast<std::streamsize>(sizeof(T) * v.size()));
 }
 
+// Device buffers of one compute() call, freed together.
+struct Buffers {
+    trw_context *ctx;
+    std::vector<void *> held;
+    explicit Buffers(trw_context *c) : ctx(c) {}
+    ~Buffers() {
+        for (void *q : held) trw_device_free(ctx, q);
+    }
+    void *get(uint64_t bytes) {
+        void *q = nullptr;
+        if (trw_device_alloc(ctx, bytes ? bytes : 4, &q) != TRW_OK) die("trw_device_alloc");
+        held.push_back(q);
+        return q;
+    }
+};
+
 }  // namespace trw_dropin
 
-// compute<F> (random_walk/creeper.h:689-831) on the GPU.
+#if defined(__HIPCC__)
+#include "functor_walk.h"
+
+namespace trw_dropin {
+
+// The functor engine (functor_walk.h): F's methods run on the device.  io holds the walker
+// batch (paths non-null, with readable guard rows before and after).
+template <class F>
+void run_functor(Graph &graph, F &f, const trw_walk_params &p, const trw_walk_io &io, Buffers &bufs) {
+    trw_context *ctx = context();
+    trw_graph *g = graph.device();
+    void *sv = nullptr;
+    trw_context_stream(ctx, &sv);
+    const hipStream_t st = static_cast<hipStream_t>(sv);
+    trw_device_view v;
+    if (trw_graph_device_view(g, &v) != TRW_OK) die("trw_graph_device_view");
+    const size_t nv = static_cast<size_t>(graph.num_vertices_), ne = static_cast<size_t>(graph.num_edges_);
+
+    // The functor's bytes for the device: its Graph pointer (and pointers into the graph's
+    // arrays) now address the device copies.
+    std::vector<trw_functor::Relocation> rel = {
+        {&graph, sizeof(Graph), graph.device_image()},
+        {graph.offset_, 8 * (nv + 1), v.offset},
+        {graph.degree_, 4 * nv, v.degree},
+        {graph.adj_, 4 * ne, v.adj},
+        {graph.edge_weight_, 8 * ne, v.edge_weight},
+        {graph.edge_label_, 4 * ne, v.edge_label},
+    };
+    if constexpr (has_schema_<F>::value && has_schema_length_<F>::value) {
+        // MetaPath's schema (apps/metapath.cpp:13-14) lives in host memory.
+        const size_t bytes = 4 * static_cast<size_t>(f.schema_length_ > 0 ? f.schema_length_ : 0);
+        void *d = bufs.get(bytes);
+        if (bytes) trw_copy_to_device(ctx, d, f.schema_, bytes);
+        rel.push_back({f.schema_, bytes, d});
+    }
+    std::vector<unsigned char> fb(sizeof(F));
+    std::memcpy(fb.data(), static_cast<const void *>(&f), sizeof(F));
+    const int64_t bad = trw_functor::relocate(fb.data(), sizeof(F), rel.data(), static_cast<int>(rel.size()));
+    if (bad >= 0) {
+        // Possibly a host pointer member (a device read through it would fault), possibly an
+        // integer and padding that happen to look like one: warn, the walk still runs.
+        log_line("warning: the functor's word at byte %lld holds a host address; the device can reach graph "
+                 "data only through the functor's Graph pointer",
+                 static_cast<long long>(bad));
+    }
+    F *df = static_cast<F *>(bufs.get(sizeof(F)));
+    trw_copy_to_device(ctx, df, fb.data(), sizeof(F));
+
+    if (p.execution == TRW_EXEC_STATIC) {
+        // static_initialization (creeper.h:102-175): F::weight per edge, then the tables.
+        double *w = static_cast<double *>(bufs.get(8 * ne));
+        hipError_t e = trw_functor::launch_weights<F, WalkerMeta>(df, v, w, st);
+        if (e != hipSuccess) {
+            std::fprintf(stderr, "[thunderrw] static_gather: %s\n", hipGetErrorString(e));
+            std::exit(-1);
+        }
+        if (trw_build_tables_from(g, p.sampling, w) != TRW_OK) die("static_initialization");
+        double ms = 0;
+        trw_last_build_ms(g, &ms);
+        log_line("Preprocessing time: %.6lf seconds", ms / 1e3);
+        if (trw_graph_device_view(g, &v) != TRW_OK) die("trw_graph_device_view");
+    }
+
+    trw_functor::Args a;
+    std::memset(&a, 0, sizeof(a));
+    a.offset = v.offset;
+    a.adj = v.adj;
+    a.alias = v.alias_table;
+    a.prefix = v.prefix_sum;
+    a.rs_max = v.rejection_max;
+    a.rs_weight = v.rejection_weight;
+    a.nv = v.num_vertices;
+    a.execution = p.execution;
+    a.sampling = p.sampling;
+    a.length = p.length;
+    a.max_trials = p.max_trials > 0 ? p.max_trials : static_cast<int32_t>(env_long("TRW_MAX_TRIALS", 1 << 16));
+    a.max_moves = static_cast<int32_t>(env_long("TRW_MAX_MOVES", std::max<long>(1 << 16, 4L * p.length)));
+    a.sources = io.sources;
+    a.n = io.n_walkers;
+    a.path_stride = io.path_stride;
+    a.paths = io.paths;
+    a.lengths = io.lengths;
+    a.ends = io.ends;
+    a.status = io.status;
+    a.draws_used = io.draws_used;
+    a.counters = reinterpret_cast<unsigned long long *>(io.counters);
+    a.draws = p.rng == TRW_RNG_REPLAY ? io.draws : nullptr;
+    a.K = p.rng == TRW_RNG_REPLAY ? io.draws_per_walker : 0;
+    a.seed = p.seed;
+    a.id_base = static_cast<uint64_t>(io.walker_id_base);
+    if (p.rng == TRW_RNG_REPLAY && !a.draws) {
+        // No draws: a zero-length replay stream (every walker that needs one is exhausted).
+        a.draws = static_cast<const uint32_t *>(bufs.get(4));
+    }
+    if (p.execution == TRW_EXEC_DYNAMIC && p.sampling != TRW_SM_MAX_WEIGHT_REJECTION) {
+        const int32_t maxd = std::max<int32_t>(v.max_degree, 1);
+        const int64_t waves = trw_functor::gather_waves(a.n, maxd);
+        const uint64_t rows = static_cast<uint64_t>(waves) * static_cast<uint64_t>(maxd);
+        a.scratch_stride = maxd;
+        a.scratch_w = static_cast<double *>(bufs.get(8 * rows));
+        a.scratch_s = static_cast<trw_functor::Entry *>(bufs.get(sizeof(trw_functor::Entry) * rows));
+        a.scratch_g = static_cast<trw_functor::Entry *>(bufs.get(sizeof(trw_functor::Entry) * rows));
+    }
+    const hipError_t e = trw_functor::launch_walk<F, WalkerMeta>(df, a, st);
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "[thunderrw] functor walk launch: %s\n", hipGetErrorString(e));
+        std::exit(-1);
+    }
+}
+
+}  // namespace trw_dropin
+#endif  // __HIPCC__
+
+// compute<F> (random_walk/creeper.h:689-831) on the GPU: the library's Move kernels for the
+// reference's five functors where the library has that (mode, sampler) pair, the functor
+// engine for everything else (built with hipcc).
 template <class F>
 void compute(Graph &graph, std::vector<WalkerMeta> &walkers, F f) {
     static const char *kExec[] = {"Uniform", "Static", "Dynamic"};
     static const char *kSample[] = {"Uniform", "ITS", "Alias", "RejectionSampling", "MaxRejectionSampling"};
     trw_walk_params p;
     std::memset(&p, 0, sizeof(p));
-    trw_dropin::describe(f, p);
+    const bool builtin = trw_dropin::describe(f, p);
     p.execution = g_para.execution_;
     p.sampling = g_para.sample_;
     p.length = g_para.length_;
@@ -358,11 +571,21 @@ void compute(Graph &graph, std::vector<WalkerMeta> &walkers, F f) {
     p.rng = replay ? TRW_RNG_REPLAY : TRW_RNG_PHILOX;
     p.seed = static_cast<uint64_t>(trw_dropin::env_long("TRW_SEED", replay ? 1234 : 0));
     p.form = TRW_FORM_AUTO;
-    trw_dropin::log_line("Configuration: %s, %s, GPU %ld, length %d", kExec[g_para.execution_ % 3],
-                         kSample[g_para.sample_ % 5], trw_dropin::env_long("TRW_DEVICE", 0), g_para.length_);
+    const bool library_move = builtin && trw_walk_supported(&p) == TRW_OK;
+    trw_dropin::log_line("Configuration: %s, %s, GPU %ld, length %d, %s", kExec[g_para.execution_ % 3],
+                         kSample[g_para.sample_ % 5], trw_dropin::env_long("TRW_DEVICE", 0), g_para.length_,
+                         library_move ? "library Move" : "functor engine");
+#if !defined(__HIPCC__)
+    if (!library_move) {
+        std::fprintf(stderr, "[thunderrw] this functor / mode / sampler runs in the functor engine: build the app "
+                             "with hipcc\n");
+        std::exit(-1);
+    }
+#endif
+    if (!builtin) p.app = TRW_APP_GENERIC;
 
     trw_graph *g = graph.device();
-    if (g_para.execution_ == Static) {  // static_initialization (creeper.h:102-175)
+    if (library_move && g_para.execution_ == Static) {  // static_initialization (creeper.h:102-175)
         const int32_t w = p.app == TRW_APP_PPR ? TRW_WEIGHTS_UNIT : TRW_WEIGHTS_EDGE;
         if (trw_build_tables(g, g_para.sample_, w) != TRW_OK) trw_dropin::die("static_initialization");
         double ms = 0;
@@ -379,37 +602,44 @@ void compute(Graph &graph, std::vector<WalkerMeta> &walkers, F f) {
     std::vector<int32_t> paths(dump_dir ? static_cast<size_t>(n) * L : 0);
 
     trw_context *ctx = trw_dropin::context();
-    std::vector<void *> bufs;
-    auto dalloc = [&](uint64_t bytes) {
-        void *q = nullptr;
-        if (trw_device_alloc(ctx, bytes ? bytes : 4, &q) != TRW_OK) trw_dropin::die("trw_device_alloc");
-        bufs.push_back(q);
-        return q;
-    };
+    trw_dropin::Buffers bufs(ctx);
     trw_walk_io io;
     std::memset(&io, 0, sizeof(io));
     io.n_walkers = n;
     io.path_stride = L;
-    io.sources = static_cast<const int32_t *>(dalloc(4ull * n));
+    io.sources = static_cast<const int32_t *>(bufs.get(4ull * n));
     trw_copy_to_device(ctx, const_cast<int32_t *>(io.sources), src.data(), 4ull * n);
-    io.lengths = static_cast<int32_t *>(dalloc(4ull * n));
-    io.ends = static_cast<int32_t *>(dalloc(4ull * n));
-    io.status = static_cast<int32_t *>(dalloc(4ull * n));
-    io.counters = static_cast<uint64_t *>(dalloc(32));
+    io.lengths = static_cast<int32_t *>(bufs.get(4ull * n));
+    io.ends = static_cast<int32_t *>(bufs.get(4ull * n));
+    io.status = static_cast<int32_t *>(bufs.get(4ull * n));
+    io.counters = static_cast<uint64_t *>(bufs.get(32));
     trw_device_zero(ctx, io.counters, 32, nullptr);
-    if (dump_dir) io.paths = static_cast<int32_t *>(dalloc(4ull * n * L));
+    if (!library_move) {
+        // The functor engine always keeps paths: functors read WalkerMeta::seq_ (node2vec.cpp:29).
+        // Guard rows on both sides keep a functor's seq_[-1] or seq_[length] readable.
+        const uint64_t guard = 64;
+        int32_t *base = static_cast<int32_t *>(bufs.get(4ull * (static_cast<uint64_t>(n) * L + 2 * guard)));
+        void *sv = nullptr;
+        trw_context_stream(ctx, &sv);
+#if defined(__HIPCC__)
+        (void)hipMemsetAsync(base, 0xff, 4ull * (static_cast<uint64_t>(n) * L + 2 * guard), static_cast<hipStream_t>(sv));
+#endif
+        io.paths = base + guard;
+    } else if (dump_dir) {
+        io.paths = static_cast<int32_t *>(bufs.get(4ull * n * L));
+    }
     if (replay) {
         const int64_t K = trw_dropin::env_long("TRW_REPLAY_K", trw_dropin::replay_budget(p));
         std::vector<uint32_t> d(static_cast<size_t>(K) * n);
         trw_replay_fill(static_cast<uint32_t>(p.seed), 0, n, K, d.data());
-        io.draws = static_cast<const uint32_t *>(dalloc(4ull * d.size()));
+        io.draws = static_cast<const uint32_t *>(bufs.get(4ull * d.size()));
         trw_copy_to_device(ctx, const_cast<uint32_t *>(io.draws), d.data(), 4ull * d.size());
         io.draws_per_walker = K;
         if (p.app == TRW_APP_PPR) {
             const int64_t KT = trw_dropin::env_long("TRW_REPLAY_KT", 4096);
             std::vector<uint32_t> t(static_cast<size_t>(KT) * n);
             trw_replay_fill(static_cast<uint32_t>(trw_dropin::env_long("TRW_TSEED", 4321)), 0, n, KT, t.data());
-            io.term_draws = static_cast<const uint32_t *>(dalloc(4ull * t.size()));
+            io.term_draws = static_cast<const uint32_t *>(bufs.get(4ull * t.size()));
             trw_copy_to_device(ctx, const_cast<uint32_t *>(io.term_draws), t.data(), 4ull * t.size());
             io.term_draws_per_walker = KT;
         }
@@ -417,7 +647,20 @@ void compute(Graph &graph, std::vector<WalkerMeta> &walkers, F f) {
 
     trw_dropin::log_line("Compute...");
     trw_timer_begin(ctx, nullptr);
-    if (trw_walk_async(g, &p, &io, nullptr) != TRW_OK) trw_dropin::die("compute");
+    if (library_move) {
+        if (trw_walk_async(g, &p, &io, nullptr) != TRW_OK) trw_dropin::die("compute");
+    } else {
+#if defined(__HIPCC__)
+        if constexpr (trw_dropin::has_sfmt_<F>::value) {
+            // PPR's update draws from the functor's own host SFMT stream (apps/ppr.cpp:27-29).
+            std::fprintf(stderr, "[thunderrw] a functor with a host SFMT stream runs only in the library's Moves "
+                                 "(sampling methods 0-3)\n");
+            std::exit(-1);
+        } else {
+            trw_dropin::run_functor(graph, f, p, io, bufs);
+        }
+#endif
+    }
     double ms = 0;
     trw_timer_end(ctx, nullptr, &ms);
     uint64_t counters[4] = {0, 0, 0, 0};
@@ -426,7 +669,6 @@ void compute(Graph &graph, std::vector<WalkerMeta> &walkers, F f) {
     trw_copy_to_host(ctx, end.data(), io.ends, 4ull * n);
     trw_copy_to_host(ctx, st.data(), io.status, 4ull * n);
     if (dump_dir) trw_copy_to_host(ctx, paths.data(), io.paths, 4ull * n * L);
-    for (void *q : bufs) trw_device_free(ctx, q);
 
     for (int32_t i = 0; i < n; ++i) {
         walkers[i].length_ = len[i];
@@ -446,7 +688,13 @@ void compute(Graph &graph, std::vector<WalkerMeta> &walkers, F f) {
     }
 }
 
+#if defined(__HIPCC__) && !defined(TRW_DROPIN_NO_FORCE_HD)
+#pragma clang force_cuda_host_device begin
+#endif
 void execute(Graph &graph, InputParser &cmd_parser);
+#if defined(__HIPCC__) && !defined(TRW_DROPIN_NO_FORCE_HD)
+#pragma clang force_cuda_host_device end
+#endif
 
 int g_num_threads = 1;
 
@@ -470,5 +718,18 @@ int main(int argc, char *argv[]) {
     trw_dropin::log_line("Done.");
     return 0;
 }
+
+// Under hipcc, everything the app defines after this header -- its functor above all -- is
+// compiled for the host and the device, so that the functor engine can call weight(),
+// update() and max_weight() in its kernels.  In the device pass execute() is renamed to an
+// unused internal function, so its host-only body (argument parsing, I/O) is never emitted
+// for the GPU while the kernels compute<F> instantiates are.  Define TRW_DROPIN_NO_FORCE_HD
+// to mark the functor __host__ __device__ yourself instead.
+#if defined(__HIPCC__) && !defined(TRW_DROPIN_NO_FORCE_HD)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define execute(...) __attribute__((unused)) static trw_dropin_execute_host_only(__VA_ARGS__)
+#endif
+#pragma clang force_cuda_host_device begin
+#endif
 
 #endif  // THUNDERRW_DROPIN_CREEPER_MAIN_H

@@ -24,11 +24,27 @@
 namespace trw {
 namespace {
 
-constexpr int kFusedBlock = 256;
-constexpr int kTile = 16;       // path positions staged per flush by walk_fused_kernel (64-byte row chunks)
-constexpr int kTileMulti = 32;  // ... by walk_fused_multi_kernel (128-byte row chunks: whole L2 lines); its
-                                // Moves are one gather, so the occupancy the larger tile costs matters less
-                                // than writing whole lines (measured: +2.5% alias, -21% Node2Vec if shared)
+// Fused-form launch shapes: threads per block and the LDS bytes per block that stage path
+// positions (see fused_tile).  Each form stages whole rows when they fit: the rows of a
+// wavefront's 64 consecutive walkers are then one contiguous run of memory, written out with
+// coalesced stores at the end of the walk; longer walks stage chunks of a multiple of 32
+// positions (whole 128-byte lines per row).  Measured on the DeepWalk-alias headline (2^22
+// vertices, length 80, one walker per lane): 32-position chunks 4.02e10 steps/s, 64-position
+// chunks 4.60e10, whole rows 4.79e10, no path output at all 4.93e10 -- the one-gather Move
+// tolerates the occupancy the tile costs, while the HBM write stream wants long runs.  Moves of
+// several dependent reads need wavefronts in flight instead: Node2Vec loses 22% with
+// 32-position chunks at 256 threads and 60% with whole rows, so it keeps 16-position chunks.
+#ifndef TRW_MULTI_BLOCK1
+#define TRW_MULTI_BLOCK1 128
+#endif
+#ifndef TRW_MULTI_BLOCK2
+#define TRW_MULTI_BLOCK2 64
+#endif
+#ifndef TRW_MULTI_BLOCK4
+#define TRW_MULTI_BLOCK4 64
+#endif
+constexpr int kMaxFusedBlock = 256;  // launch bound of both fused kernels
+constexpr int kTileBudget = 65536;   // walk_fused_multi_kernel (uniform / alias walk records)
 constexpr int kQueueBlock = 256;
 constexpr int kStepBlock = 256;
 
@@ -596,10 +612,11 @@ __device__ __forceinline__ void init_walker(const DevGraph &g, const BatchIO &io
 // makes fast wavefronts wait for slow ones.
 typedef int32_t v4i32 __attribute__((ext_vector_type(4)));
 
-template <int WPT, int T>
+template <int WPT>
 struct WaveTile {
     int32_t *region;  // WPT * 64 rows of T + 1 ints
     int32_t lane;
+    int32_t T;        // positions per row
 
     __device__ __forceinline__ void put(int j, int32_t p, int32_t v) { region[(j * 64 + lane) * (T + 1) + p] = v; }
 
@@ -612,11 +629,13 @@ struct WaveTile {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // Streaming stores: path rows are written once and not read back by the walk, so
         // they should not displace the records the walk gathers from the L2.
-        constexpr int32_t Q = T / 4;  // 16-byte pieces of a full tile row
-        if (Q >= 1 && cnt == T && (io.path_stride & 3) == 0 && (t0 & 3) == 0 &&
+        const int32_t Q = T >> 2;  // 16-byte pieces of a full tile row
+        if ((T & 3) == 0 && Q >= 1 && cnt == T && (io.path_stride & 3) == 0 && (t0 & 3) == 0 &&
             (reinterpret_cast<uintptr_t>(io.paths) & 15) == 0) {
+            // Piece i is piece q of row lw: consecutive lanes store consecutive pieces of a row,
+            // and consecutive rows too when the rows are whole (path_stride == T).
             for (int32_t i = lane; i < WPT * 64 * Q; i += 64) {
-                const int32_t lw = i / Q;  // Q is a power of two
+                const int32_t lw = i / Q;
                 const int32_t q = i - lw * Q;
                 const int32_t gw = first_walker + (lw >> 6) * stride_j + (lw & 63);
                 const int32_t *r = region + lw * (T + 1) + 4 * q;
@@ -646,13 +665,14 @@ struct WaveTile {
 };
 
 // Thread per walker, the whole fixed-length walk in one launch (any Move).
+// T path positions are staged per row (dynamic LDS: blockDim x (T + 1) ints).
 template <int MODE, bool REPLAY>
-__global__ __launch_bounds__(kFusedBlock) void walk_fused_kernel(DevGraph g, KindParams P, BatchIO io) {
-    __shared__ int32_t tile[kFusedBlock * (kTile + 1)];
+__global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_kernel(DevGraph g, KindParams P, BatchIO io, int32_t T) {
+    extern __shared__ int32_t tile[];
     const int32_t tid = threadIdx.x;
-    const int32_t w = blockIdx.x * kFusedBlock + tid;
+    const int32_t w = blockIdx.x * blockDim.x + tid;
     const int32_t wave_first = w - (tid & 63);
-    WaveTile<1, kTile> out{tile + (tid >> 6) * 64 * (kTile + 1), tid & 63};
+    WaveTile<1> out{tile + (tid >> 6) * 64 * (T + 1), tid & 63, T};
     const bool valid = w < io.n;
     WalkerState s = empty_walker();
     if (valid) init_walker(g, io, w, s);
@@ -670,7 +690,7 @@ __global__ __launch_bounds__(kFusedBlock) void walk_fused_kernel(DevGraph g, Kin
         }
         if (paths) {
             out.put(0, pos - t0, v);
-            if (pos - t0 == kTile - 1 || pos == L - 1) {
+            if (pos - t0 == T - 1 || pos == L - 1) {
                 out.flush(io, wave_first, 0, t0, pos - t0 + 1);
                 t0 = pos + 1;
             }
@@ -693,12 +713,13 @@ __global__ __launch_bounds__(kFusedBlock) void walk_fused_kernel(DevGraph g, Kin
 // of the eight wavefronts per SIMD that occupancy provides.  Walker (t, j) of a
 // block is block_base + j * blockDim + t, so every per-j access is coalesced.
 template <int MODE, bool REPLAY, int WPT>
-__global__ __launch_bounds__(kFusedBlock) void walk_fused_multi_kernel(DevGraph g, KindParams P, BatchIO io) {
-    constexpr int T = kTileMulti / WPT;  // staged positions per flush: ~33 KiB of LDS per block for any WPT
-    __shared__ int32_t tile[WPT * kFusedBlock * (T + 1)];
+__global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGraph g, KindParams P, BatchIO io,
+                                                                          int32_t T) {
+    extern __shared__ int32_t tile[];  // WPT x blockDim rows of T + 1 ints
     const int32_t tid = threadIdx.x;
-    const int32_t wbase = blockIdx.x * (kFusedBlock * WPT);
-    WaveTile<WPT, T> out{tile + (tid >> 6) * (WPT * 64 * (T + 1)), tid & 63};
+    const int32_t block = static_cast<int32_t>(blockDim.x);
+    const int32_t wbase = blockIdx.x * (block * WPT);
+    WaveTile<WPT> out{tile + (tid >> 6) * (WPT * 64 * (T + 1)), tid & 63, T};
     const int32_t wave_first = wbase + (tid & ~63);
     const bool paths = io.paths != nullptr;
     const int32_t L = P.length;
@@ -706,7 +727,7 @@ __global__ __launch_bounds__(kFusedBlock) void walk_fused_multi_kernel(DevGraph 
     uint4 blk[WPT];
 #pragma unroll
     for (int j = 0; j < WPT; ++j) {
-        const int32_t w = wbase + j * kFusedBlock + tid;
+        const int32_t w = wbase + j * block + tid;
         s[j] = empty_walker();
         if (w < io.n) init_walker(g, io, w, s[j]);
         blk[j] = first_block<REPLAY>(io, w, s[j]);
@@ -728,7 +749,7 @@ __global__ __launch_bounds__(kFusedBlock) void walk_fused_multi_kernel(DevGraph 
         }
 #pragma unroll
         for (int j = 0; j < WPT; ++j) {
-            const int32_t w = wbase + j * kFusedBlock + tid;
+            const int32_t w = wbase + j * block + tid;
             const uint32_t step = static_cast<uint32_t>(s[j].len - 1);
             u0[j] = u1[j] = 0;
             if (!go[j]) continue;
@@ -758,7 +779,7 @@ __global__ __launch_bounds__(kFusedBlock) void walk_fused_multi_kernel(DevGraph 
         // the gathers just issued, so the stores overlap them instead of delaying
         // the next step's wait.
         if (paths && pos - t0 == T) {
-            out.flush(io, wave_first, kFusedBlock, t0, T);
+            out.flush(io, wave_first, block, t0, T);
             t0 = pos;
         }
         if (!REPLAY) {
@@ -797,11 +818,11 @@ __global__ __launch_bounds__(kFusedBlock) void walk_fused_multi_kernel(DevGraph 
             for (int j = 0; j < WPT; ++j) out.put(j, pos - t0, next[j]);
         }
     }
-    if (paths) out.flush(io, wave_first, kFusedBlock, t0, L - t0);
+    if (paths) out.flush(io, wave_first, block, t0, L - t0);
     unsigned long long gterm = 0, dead = 0, err = 0;
 #pragma unroll
     for (int j = 0; j < WPT; ++j) {
-        const int32_t w = wbase + j * kFusedBlock + tid;
+        const int32_t w = wbase + j * block + tid;
         if (w < io.n) {
             if (s[j].st == kAlive) s[j].st = kDone;
             write_outputs(io, w, s[j]);
@@ -1071,41 +1092,74 @@ __global__ void copy_u128_kernel(uint4 *__restrict__ dst, const uint4 *__restric
 
 inline unsigned grid_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
 
+// Positions per row a fused kernel with `rows` walker rows per block stages in LDS: the whole
+// walk when it fits `budget` bytes, else the largest multiple of 32 that does (of 4 below 32).
+int32_t fused_tile(int32_t L, int32_t rows, int32_t budget) {
+    const int32_t tmax = budget / (4 * rows) - 1;
+    if (L <= tmax) return L < 1 ? 1 : L;
+    if (tmax >= 32) return tmax & ~31;
+    return std::max(4, tmax & ~3);
+}
+
 template <int MODE, bool REPLAY>
 void fused_multi_launch(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream) {
-    const dim3 block(kFusedBlock);
-    switch (p.walkers_per_thread == 0 ? 1 : p.walkers_per_thread) {
+    const int32_t wpt = p.walkers_per_thread == 4 ? 4 : p.walkers_per_thread == 2 ? 2 : 1;
+    const int32_t block = wpt == 4 ? TRW_MULTI_BLOCK4 : wpt == 2 ? TRW_MULTI_BLOCK2 : TRW_MULTI_BLOCK1;
+    const int32_t T = fused_tile(p.length, wpt * block, kTileBudget);
+    const size_t lds = static_cast<size_t>(wpt) * block * (T + 1) * sizeof(int32_t);
+    const dim3 grid(grid_for(io.n, wpt * block));
+    switch (wpt) {
     case 4:
-        hipLaunchKernelGGL((walk_fused_multi_kernel<MODE, REPLAY, 4>), dim3(grid_for(io.n, 4 * kFusedBlock)), block, 0,
-                           stream, g, p, io);
+        hipLaunchKernelGGL((walk_fused_multi_kernel<MODE, REPLAY, 4>), grid, dim3(block), lds, stream, g, p, io, T);
         break;
     case 2:
-        hipLaunchKernelGGL((walk_fused_multi_kernel<MODE, REPLAY, 2>), dim3(grid_for(io.n, 2 * kFusedBlock)), block, 0,
-                           stream, g, p, io);
+        hipLaunchKernelGGL((walk_fused_multi_kernel<MODE, REPLAY, 2>), grid, dim3(block), lds, stream, g, p, io, T);
         break;
     default:
-        hipLaunchKernelGGL((walk_fused_multi_kernel<MODE, REPLAY, 1>), dim3(grid_for(io.n, kFusedBlock)), block, 0,
-                           stream, g, p, io);
+        hipLaunchKernelGGL((walk_fused_multi_kernel<MODE, REPLAY, 1>), grid, dim3(block), lds, stream, g, p, io, T);
         break;
     }
 }
 
+// walk_fused_kernel's launch shape per Move: threads per block and LDS bytes for path tiles.
+struct FusedShape {
+    int32_t block;
+    int32_t budget;
+};
+#ifndef TRW_FUSED_SHAPE_ITS
+#define TRW_FUSED_SHAPE_ITS {64, 32768}     // whole rows (+4.6% over 16-position chunks)
+#endif
+#ifndef TRW_FUSED_SHAPE_RS
+#define TRW_FUSED_SHAPE_RS {256, 33792}     // 32-position chunks (+6%)
+#endif
+#ifndef TRW_FUSED_SHAPE_OTHER
+#define TRW_FUSED_SHAPE_OTHER {256, 17408}  // 16-position chunks
+#endif
+FusedShape fused_shape(int32_t mode) {
+    if (mode == kModeIts) return TRW_FUSED_SHAPE_ITS;
+    if (mode == kModeRejection) return TRW_FUSED_SHAPE_RS;
+    return TRW_FUSED_SHAPE_OTHER;
+}
+
 template <bool REPLAY>
 hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream) {
-    const dim3 grid(grid_for(io.n, kFusedBlock)), block(kFusedBlock);
     const bool records = (p.mode == kModeUniform) ? g.erec != nullptr : g.arec != nullptr;
     if (records && !p.ppr && (p.mode == kModeUniform || p.mode == kModeAlias)) {
         if (p.mode == kModeUniform) fused_multi_launch<kModeUniform, REPLAY>(g, p, io, stream);
         else fused_multi_launch<kModeAlias, REPLAY>(g, p, io, stream);
         return hipGetLastError();
     }
+    const FusedShape sh = fused_shape(p.mode);
+    const int32_t T = fused_tile(p.length, sh.block, sh.budget);
+    const size_t lds = static_cast<size_t>(sh.block) * (T + 1) * sizeof(int32_t);
+    const dim3 grid(grid_for(io.n, sh.block)), block(sh.block);
     switch (p.mode) {
-    case kModeUniform: hipLaunchKernelGGL((walk_fused_kernel<kModeUniform, REPLAY>), grid, block, 0, stream, g, p, io); break;
-    case kModeAlias: hipLaunchKernelGGL((walk_fused_kernel<kModeAlias, REPLAY>), grid, block, 0, stream, g, p, io); break;
-    case kModeIts: hipLaunchKernelGGL((walk_fused_kernel<kModeIts, REPLAY>), grid, block, 0, stream, g, p, io); break;
-    case kModeRejection: hipLaunchKernelGGL((walk_fused_kernel<kModeRejection, REPLAY>), grid, block, 0, stream, g, p, io); break;
-    case kModeNode2Vec: hipLaunchKernelGGL((walk_fused_kernel<kModeNode2Vec, REPLAY>), grid, block, 0, stream, g, p, io); break;
-    default: hipLaunchKernelGGL((walk_fused_kernel<kModeMetaPath, REPLAY>), grid, block, 0, stream, g, p, io); break;
+    case kModeUniform: hipLaunchKernelGGL((walk_fused_kernel<kModeUniform, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
+    case kModeAlias: hipLaunchKernelGGL((walk_fused_kernel<kModeAlias, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
+    case kModeIts: hipLaunchKernelGGL((walk_fused_kernel<kModeIts, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
+    case kModeRejection: hipLaunchKernelGGL((walk_fused_kernel<kModeRejection, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
+    case kModeNode2Vec: hipLaunchKernelGGL((walk_fused_kernel<kModeNode2Vec, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
+    default: hipLaunchKernelGGL((walk_fused_kernel<kModeMetaPath, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     }
     return hipGetLastError();
 }

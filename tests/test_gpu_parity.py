@@ -494,3 +494,51 @@ def test_upload_rejects_vertex_ids_out_of_range(gpu_ctx):
             T.DeviceGraph(gpu_ctx, g.degree, adj, g.weight)
         assert e.value.code == T._abi.E_INVALID
     T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight).close()
+
+
+def test_fused_path_tiles_for_every_walk_length(gpu_ctx):
+    """The fused kernels stage whole path rows when they fit their LDS budget and chunks of a
+    multiple of 32 (or 4) positions otherwise, per Move and walkers per lane: every shape --
+    rows shorter than, equal to and longer than a tile, a partial last chunk, a batch that is
+    not a whole number of blocks, a row stride wider than the walk -- writes the oracle's paths."""
+    g = graphs.chung_lu(3000, 15000, 2.1, 2, 900, 81)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        src = np.nonzero(g.degree > 0)[0].astype(np.int32)[:2777]
+        n = len(src)
+        cases = [("deepwalk", 1, 2, {}, [1, 2, 4]), ("deepwalk", 0, 0, {}, [1, 4]), ("deepwalk", 1, 1, {}, [0]),
+                 ("deepwalk", 1, 3, {}, [0]), ("node2vec", 2, 4, {"-p": "2", "-q": "0.5"}, [0])]
+        for L in (2, 3, 31, 33, 64, 127, 128, 129, 300):
+            for app, em, sm, extra, wpts in cases:
+                case = {"app": app, "execution": em, "sampling": sm, "length": L, "extra": extra}
+                K = 32 * L if sm in (3, 4) else 2 * L
+                ref = oracle_walk(g, case, src, K, 1, 82, 83)
+                draws = T.replay_fill(82, n, K)
+                for wpt in wpts:
+                    a = golden_case_args(case)
+                    prm = T.make_params(app, execution=em, sampling=sm, length=L, p=a["p"], q=a["q"], rng="replay",
+                                        form="fused", walkers_per_thread=wpt)
+                    r = T.walk(dg, prm, src, draws)
+                    tag = (app, em, sm, L, wpt)
+                    assert np.array_equal(r["paths"], ref["paths"]), tag
+                    assert np.array_equal(r["lengths"], ref["lengths"]), tag
+                    assert r["steps"] == ref["steps"], tag
+        # A row stride wider than the walk: rows are no longer contiguous; the gap stays untouched.
+        L, stride = 80, 83
+        case = {"app": "deepwalk", "execution": 1, "sampling": 2, "length": L, "extra": {}}
+        ref = oracle_walk(g, case, src, 2 * L, 1, 84, 85)
+        sb = T.DeviceBuffer.from_array(gpu_ctx, src)
+        db = T.DeviceBuffer.from_array(gpu_ctx, T.replay_fill(84, n, 2 * L))
+        pb = T.DeviceBuffer.filled(gpu_ctx, n * stride, np.int32, -7)
+        try:
+            io = T.make_io(sb, n, stride, 0, db, 2 * L, None, 0, pb)
+            T.walk_async(dg, T.make_params("deepwalk", length=L, rng="replay", form="fused"), io)
+            T.synchronize(gpu_ctx)
+            rows = pb.to_numpy(np.int32, (n, stride))
+        finally:
+            for b in (sb, db, pb):
+                b.free()
+        assert np.array_equal(rows[:, :L], ref["paths"])
+        assert np.all(rows[:, L:] == -7)
+    finally:
+        dg.close()

@@ -542,3 +542,37 @@ def test_fused_path_tiles_for_every_walk_length(gpu_ctx):
         assert np.all(rows[:, L:] == -7)
     finally:
         dg.close()
+
+
+def test_node2vec_membership_structures_match_oracle(gpu_ctx):
+    """Node2Vec's is_neighbor tests run through 32-key membership trees for adjacencies longer
+    than 32 (one, two and three tree levels here: hubs of degree 100, 3000 and 40000) and by
+    binary search below; on an undirected and on a directed graph (where which endpoint's list
+    is searched decides the answer) the walks equal the oracle's."""
+    rng = np.random.default_rng(91)
+    n = 60000
+    parts = [rng.integers(0, n, (150000, 2))]
+    for hub, deg in [(0, 40000)] + [(h, 3000) for h in range(1, 6)] + [(h, 100) for h in range(6, 40)]:
+        nb = rng.choice(np.arange(40, n), size=deg, replace=False)
+        parts.append(np.stack([np.full(deg, hub), nb], 1))
+    parts.append(np.array([(a, b) for a in range(40) for b in range(a + 1, 40) if rng.random() < 0.5]))
+    g = graphs.csr_from_undirected(n, np.concatenate(parts), rng)
+    d = directed_graph(3000, 93)
+    for gg in (g, d):
+        dg = dgraph(gpu_ctx, gg)
+        try:
+            live = np.nonzero(gg.degree > 0)[0].astype(np.int32)
+            top = np.argsort(gg.degree)[::-1][:8].astype(np.int32)
+            src = np.concatenate([rng.choice(live, 6000), np.repeat(top, 64)]).astype(np.int32)
+            for p, q in ((2.0, 0.5), (0.5, 2.0), (1.0, 4.0)):
+                case = {"app": "node2vec", "execution": 2, "sampling": 4, "length": 24,
+                        "extra": {"-p": str(p), "-q": str(q)}}
+                K = 64 * 24
+                ref = oracle_walk(gg, case, src, K, 1, 94, 95)
+                draws = T.replay_fill(94, len(src), K)
+                for form in ("fused", "queue"):
+                    r = T.walk(dg, case_params(case, form), src, draws)
+                    for key in ("paths", "lengths", "status", "draws_used"):
+                        assert np.array_equal(r[key], ref[key]), (p, q, form, key)
+        finally:
+            dg.close()

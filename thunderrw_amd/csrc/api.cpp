@@ -253,6 +253,9 @@ static void graph_release(trw_graph *g) {
     dfree(g->lerec);
     dfree(g->its_lines);
     dfree(g->its_root);
+    dfree(g->mlines);
+    dfree(g->nrec);
+    dfree(g->mroot);
     dfree(g->order);
     dfree(g->vrec);
     dfree(g->erec);
@@ -486,6 +489,51 @@ static int build_tables(trw_graph *g, int32_t sampling, const double *w, int32_t
     }
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, t0, t1) == hipSuccess) g->last_build_ms = ms;
+    return TRW_OK;
+}
+
+// Membership structures over every adjacency of degree > kMemberMin, and neighbour records
+// carrying each neighbour's root, for Node2Vec's is_neighbor tests (built by the first Node2Vec
+// walk; a graph without neighbour records, 2^32 edges or more, keeps the binary search).
+static int build_member_trees(trw_graph *g) {
+    if (g->nrec || g->nv == 0 || !g->erec) return TRW_OK;
+    hipStream_t st = g->ctx->stream;
+    const size_t V = static_cast<size_t>(g->nv);
+    ScratchBuffers sc;
+    uint64_t *start = nullptr;
+    uint8_t *temp = nullptr;
+    size_t temp_bytes = 0;
+    uint32_t *root = nullptr;
+    uint4 *lines = nullptr, *nrec = nullptr;
+    struct Owned {  // freed unless handed to the graph
+        uint32_t *&r;
+        uint4 *&l, *&n;
+        ~Owned() {
+            dfree(r);
+            dfree(l);
+            dfree(n);
+        }
+    } owned{root, lines, nrec};
+    TRW_HIP(dmalloc(&root, V), "hipMalloc membership roots");
+    TRW_HIP(trw::launch_member_tree_starts(g->view(), root, nullptr, nullptr, temp_bytes, st), "membership scan size");
+    TRW_HIP(sc.get(&start, V + 1), "hipMalloc membership starts");
+    TRW_HIP(sc.get(&temp, temp_bytes), "hipMalloc membership scan storage");
+    TRW_HIP(trw::launch_member_tree_starts(g->view(), root, start, temp, temp_bytes, st), "membership starts");
+    uint64_t total = 0;
+    TRW_HIP(hipMemcpyAsync(&total, start + V, sizeof(total), hipMemcpyDeviceToHost, st), "membership size");
+    TRW_HIP(hipStreamSynchronize(st), "membership starts");
+    if (total >= trw::kNoRange) return TRW_OK;  // line indices would overflow 32 bits: binary search
+    if (total) TRW_HIP(dmalloc(&lines, static_cast<size_t>(total) * 8), "hipMalloc membership structures");
+    TRW_HIP(dmalloc(&nrec, static_cast<size_t>(g->ne) + 1), "hipMalloc Node2Vec neighbour records");
+    TRW_HIP(trw::launch_member_tree_fill(g->view(), start, total, lines, st), "membership structure fill");
+    TRW_HIP(trw::launch_member_records(g->view(), root, nrec, st), "Node2Vec neighbour records");
+    TRW_HIP(hipStreamSynchronize(st), "membership structures");
+    g->mroot = root;
+    g->mlines = lines;
+    g->nrec = nrec;
+    root = nullptr;
+    lines = nullptr;
+    nrec = nullptr;
     return TRW_OK;
 }
 
@@ -767,6 +815,10 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
     if (p->app == TRW_APP_NODE2VEC) {
         if (!(p->p > 0.0) || !(p->q > 0.0)) return fail(TRW_E_INVALID, "Node2Vec p and q must be > 0");
         k.mode = trw::kModeNode2Vec;
+        if (allow_build) {
+            const int rc = build_member_trees(g);
+            if (rc != TRW_OK) return rc;
+        }
         k.n2v_p = p->p;  // Node2Vec::p_ (node2vec.cpp:18, after the inversion at :69)
         k.n2v_q = p->q;  // Node2Vec::q_ (:76)
         k.n2v_max = std::max(1.0, std::max(k.n2v_q, k.n2v_p));  // node2vec.cpp:19

@@ -61,6 +61,9 @@ struct DevGraph {
                                //       lerec[off(v) + j] = erec[off(v) + lbl_pos[off(v) + j]]
     const uint4 *its_lines;    // ITS search structures (128-byte lines, 8 uint4 each), or null
     const uint32_t *its_root;  // nv: first line of v's structure, kNoRange if v has none
+    const uint4 *nrec;         // ne: neighbour records with .w = the neighbour's membership structure root
+    const uint4 *mlines;       // membership structures (128-byte lines), or null
+    const uint32_t *mroot;     // nv: first line of v's membership structure, kNoRange if v has none
 };
 
 // ITS search structure of one vertex, built from its prefix sums when they are
@@ -89,6 +92,31 @@ __host__ __device__ inline int32_t its_levels(int32_t nl, int64_t *nodes) {
         keys = nodes[L];
         ++L;
     } while (nodes[L - 1] > 1 && L < kItsMaxLevels);
+    return L;
+}
+
+// Membership structure of one vertex's sorted adjacency (Graph::is_neighbor, graph.h:180-199),
+// for vertices of degree > kMemberMin: 128-byte lines of 32 int32 keys.  Leaf i holds the
+// neighbours [32i, 32i + 32) (INT32_MAX past the last); above more than one leaf, a B+-tree of
+// 32-key nodes (level 1: the last neighbour of each leaf; higher levels: the last key of each
+// node below; INT32_MAX past the real keys), stored root first, then the leaves.  A search
+// reads one node per level and one leaf.
+constexpr int kMemberFan = 32;
+constexpr int kMemberMin = 32;
+constexpr int kMemberMaxLevels = 8;
+
+__host__ __device__ inline int32_t member_leaves(int32_t d) { return (d + kMemberFan - 1) / kMemberFan; }
+
+// Tree node counts per level (level 1 first); returns the number of levels (0 for one leaf).
+__host__ __device__ inline int32_t member_levels(int32_t nl, int64_t *nodes) {
+    if (nl <= 1) return 0;
+    int32_t L = 0;
+    int64_t keys = nl;
+    do {
+        nodes[L] = (keys + kMemberFan - 1) / kMemberFan;
+        keys = nodes[L];
+        ++L;
+    } while (nodes[L - 1] > 1 && L < kMemberMaxLevels);
     return L;
 }
 
@@ -143,7 +171,9 @@ struct WalkerState {
     uint32_t pdeg;
     uint32_t gid_lo;  // global walker id: the Philox counter's walker words
     uint32_t gid_hi;
-    uint32_t cidx;    // current vertex: root of its ITS search tree (kNoRange: not known / none)
+    uint32_t cidx;    // current vertex: root of its ITS search tree (ITS) or membership structure
+                      // (Node2Vec); kNoRange: not known / none
+    uint32_t pidx;    // previous vertex: root of its membership structure (Node2Vec)
 };
 constexpr uint32_t kNoRange = 0xffffffffu;
 
@@ -220,6 +250,13 @@ hipError_t launch_its_tree_starts(const DevGraph &g, uint32_t *root, uint64_t *s
 hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint64_t *start, uint64_t lines_total,
                                 const uint32_t *root, uint4 *lines, hipStream_t stream);
 hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *root, uint4 *erec, hipStream_t stream);
+// Membership structures: as the ITS ones (line counts into root, then offsets by a scan, then
+// one thread per 16-byte piece), plus the neighbour records carrying each neighbour's root.
+hipError_t launch_member_tree_starts(const DevGraph &g, uint32_t *root, uint64_t *start, void *temp,
+                                     size_t &temp_bytes, hipStream_t stream);
+hipError_t launch_member_tree_fill(const DevGraph &g, const uint64_t *start, uint64_t lines_total, uint4 *lines,
+                                   hipStream_t stream);
+hipError_t launch_member_records(const DevGraph &g, const uint32_t *root, uint4 *nrec, hipStream_t stream);
 hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
                             double *out, hipStream_t stream);
 hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,

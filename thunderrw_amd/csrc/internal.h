@@ -32,6 +32,9 @@ struct trw_graph {
     uint4 *lerec = nullptr;
     uint4 *its_lines = nullptr;    // ITS search structures (null: not built, or sums not non-decreasing)
     uint32_t *its_root = nullptr;  // nv: first line of each vertex's structure
+    uint4 *nrec = nullptr;         // neighbour records with membership roots (built by the first Node2Vec walk)
+    uint4 *mlines = nullptr;       // membership structures
+    uint32_t *mroot = nullptr;     // nv: first line of each vertex's membership structure
     bool weights_nonneg = true;    // every edge weight is finite and >= 0
     int32_t build_hubs = 0;        // vertices of degree >= kWaveBuildMin (a prefix of order)
     int64_t tune[3] = {0, 0, 0};   // TRW_TUNE_* knobs
@@ -67,6 +70,9 @@ struct trw_graph {
         g.lerec = lerec;
         g.its_lines = its_lines;
         g.its_root = its_root;
+        g.nrec = nrec;
+        g.mlines = mlines;
+        g.mroot = mroot;
         return g;
     }
 };

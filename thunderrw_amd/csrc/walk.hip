@@ -141,6 +141,85 @@ __device__ __forceinline__ bool is_neighbor(const DevGraph &g, int32_t u, int64_
     return false;
 }
 
+// Half of a 128-byte membership line: 16 int32 keys (4 16-byte loads issued together).  A
+// search loads the second half of a line only when the first cannot decide: a whole line in
+// flight would take 32 registers and cost the Node2Vec kernel almost half its wavefronts.
+struct MemberHalf {
+    uint4 a0, a1, a2, a3;
+    __device__ __forceinline__ void load(const uint4 *p) {
+        a0 = p[0];
+        a1 = p[1];
+        a2 = p[2];
+        a3 = p[3];
+    }
+    static __device__ __forceinline__ int32_t lt4(const uint4 &v, int32_t x) {
+        return (static_cast<int32_t>(v.x) < x) + (static_cast<int32_t>(v.y) < x) + (static_cast<int32_t>(v.z) < x) +
+               (static_cast<int32_t>(v.w) < x);
+    }
+    static __device__ __forceinline__ bool eq4(const uint4 &v, uint32_t x) {
+        return (v.x == x) | (v.y == x) | (v.z == x) | (v.w == x);
+    }
+    __device__ __forceinline__ int32_t count_lt(int32_t x) const { return lt4(a0, x) + lt4(a1, x) + lt4(a2, x) + lt4(a3, x); }
+    __device__ __forceinline__ bool contains(int32_t x) const {
+        const uint32_t y = static_cast<uint32_t>(x);
+        return eq4(a0, y) | eq4(a1, y) | eq4(a2, y) | eq4(a3, y);
+    }
+    __device__ __forceinline__ int32_t last() const { return static_cast<int32_t>(a3.w); }
+};
+
+// Whether x is among the d sorted neighbours whose membership structure starts at line root:
+// one node per tree level, then one leaf.  Level k (0 = over the leaves) has ceil(nl / 32^(k+1))
+// nodes and nl / 32^k real keys in all; a key count past a node's real keys means x exceeds
+// every neighbour under it.
+__device__ __forceinline__ bool member_search(const uint4 *lines, uint32_t root, int32_t d, int32_t x) {
+    // 32-bit arithmetic throughout (line indices fit: a graph's structures have < 2^32 lines).
+    const uint32_t nl = static_cast<uint32_t>(member_leaves(d));
+    const uint32_t bits = 32u - static_cast<uint32_t>(__clz(static_cast<int>(nl - 1u)));  // nl >= 2
+    const uint32_t L = (bits + 4u) / 5u;  // tree levels: 32^L >= nl
+    MemberHalf h;
+    uint32_t line = root, node = 0;
+    for (uint32_t k = L; k-- > 0;) {
+        const uint4 *p = lines + static_cast<uint64_t>(line + node) * 8;
+        const uint32_t real = (nl + (1u << (5 * k)) - 1u) >> (5 * k);  // keys at level k, all nodes
+        const uint32_t here = min(real - node * kMemberFan, static_cast<uint32_t>(kMemberFan));
+        h.load(p);
+        uint32_t c = static_cast<uint32_t>(h.count_lt(x));
+        if (c == 16u && here > 16u) {
+            h.load(p + 4);
+            c += static_cast<uint32_t>(h.count_lt(x));
+        }
+        if (c >= here) return false;
+        line += (real + kMemberFan - 1u) / kMemberFan;  // nodes at level k
+        node = node * kMemberFan + c;
+    }
+    const uint4 *p = lines + static_cast<uint64_t>(line + node) * 8;
+    h.load(p);
+    if (h.last() < x) h.load(p + 4);  // x lies past the first 16 (padding keys are INT32_MAX)
+    return h.contains(x);
+}
+
+// Graph::is_neighbor (util/graph/graph.h:180-199) with both endpoints' ranges known: the
+// larger-degree endpoint searched in the smaller-degree endpoint's sorted adjacency -- through
+// that endpoint's membership structure when it has one (root carried by the walker or the
+// neighbour record, else looked up), by binary search otherwise.  The answer is the same.
+__device__ __forceinline__ bool is_neighbor_tree(const DevGraph &g, int32_t u, int64_t uoff, int32_t udeg,
+                                                 uint32_t uroot, int32_t v, int64_t voff, int32_t vdeg,
+                                                 uint32_t vroot) {
+    if (udeg < vdeg) {
+        const int32_t t = u;
+        u = v;
+        v = t;
+        voff = uoff;
+        vdeg = udeg;
+        vroot = uroot;
+    }
+    if (g.mlines && vdeg > kMemberMin) {
+        const uint32_t root = vroot != kNoRange ? vroot : g.mroot[v];
+        if (root != kNoRange) return member_search(g.mlines, root, vdeg, u);
+    }
+    return is_neighbor(g, u, voff, vdeg, v, voff, vdeg);  // equal degrees: searches u in v's list
+}
+
 __device__ __forceinline__ int32_t first_ge(const int32_t *a, int32_t n, int32_t x) {
     int32_t lo = 0, hi = n;
     while (lo < hi) {
@@ -213,6 +292,20 @@ __device__ __forceinline__ int32_t read_neighbor_its(const DevGraph &g, int64_t 
     ndeg = kNoRange;
     nidx = kNoRange;
     return g.adj[e];
+}
+
+// As read_neighbor, also returning the neighbour's membership structure root (Node2Vec).
+__device__ __forceinline__ int32_t read_neighbor_member(const DevGraph &g, int64_t e, uint32_t &noff, uint32_t &ndeg,
+                                                        uint32_t &nroot) {
+    if (g.nrec) {
+        const uint4 r = g.nrec[e];
+        noff = r.y;
+        ndeg = r.z;
+        nroot = r.w;
+        return static_cast<int32_t>(r.x);
+    }
+    nroot = kNoRange;
+    return read_neighbor(g, e, noff, ndeg);
 }
 
 // One 128-byte line of an ITS structure, as 8 16-byte loads issued together.  Named
@@ -311,6 +404,7 @@ __device__ __forceinline__ void finish_move(const DevGraph &g, const KindParams 
     s.prev = cur;
     s.poff = static_cast<uint32_t>(off);
     s.pdeg = g.vrec ? static_cast<uint32_t>(d) : kNoRange;
+    s.pidx = s.cidx;
     s.cur = next;
     s.coff = noff;
     s.cdeg = ndeg;
@@ -455,6 +549,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         // runs only when the draw falls between the two possible weights.
         const double lo = fmin(1.0, P.n2v_q), hi = fmax(1.0, P.n2v_q);
         int32_t c = -1;
+        uint32_t croot = kNoRange;
         for (uint32_t trial = 0;; ++trial) {
             if (P.max_trials > 0 && trial >= static_cast<uint32_t>(P.max_trials)) {
                 s.st = kTrialCap;
@@ -462,7 +557,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             }
             uint32_t ua, ub;
             if (!draw_pair<REPLAY>(io, w, s, step, trial, blk, ua, ub)) return false;
-            c = read_neighbor(g, off + (ua % ud), noff, ndeg);
+            c = read_neighbor_member(g, off + (ua % ud), noff, ndeg, croot);
             const double y = real2(ub) * P.n2v_max;
             bool accept;
             if (s.len == 1) accept = (y <= P.n2v_max);
@@ -474,11 +569,12 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
                 int32_t pd, cd;
                 known_range(g, s.prev, s.poff, s.pdeg, poff, pd);
                 known_range(g, c, noff, ndeg, coff, cd);
-                accept = (y <= (is_neighbor(g, s.prev, poff, pd, c, coff, cd) ? 1.0 : P.n2v_q));
+                accept = (y <= (is_neighbor_tree(g, s.prev, poff, pd, s.pidx, c, coff, cd, croot) ? 1.0 : P.n2v_q));
             }
             if (accept) break;
         }
         next = c;
+        nidx = croot;
         if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
     } else {  // kModeMetaPath
         // dynamic_gather (creeper.h:178-296) with MetaPath::weight (metapath.cpp:22-30)
@@ -591,7 +687,7 @@ __device__ __forceinline__ void write_outputs(const BatchIO &io, int32_t w, cons
 }
 
 __device__ __forceinline__ WalkerState empty_walker() {
-    return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange, 0u, 0u, kNoRange};
+    return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange, 0u, 0u, kNoRange, kNoRange};
 }
 
 __device__ __forceinline__ void init_walker(const DevGraph &g, const BatchIO &io, int32_t w, WalkerState &s) {
@@ -665,9 +761,11 @@ struct WaveTile {
 };
 
 // Thread per walker, the whole fixed-length walk in one launch (any Move).
-// T path positions are staged per row (dynamic LDS: blockDim x (T + 1) ints).
+// T path positions are staged per row (dynamic LDS: blockDim x (T + 1) ints).  At least five
+// wavefronts per SIMD: the Node2Vec Move (membership searches) would otherwise take 99
+// registers and four wavefronts; five, with 12 bytes per lane spilled, runs it 14% faster.
 template <int MODE, bool REPLAY>
-__global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_kernel(DevGraph g, KindParams P, BatchIO io, int32_t T) {
+__global__ __launch_bounds__(kMaxFusedBlock) __attribute__((amdgpu_waves_per_eu(5))) void walk_fused_kernel(DevGraph g, KindParams P, BatchIO io, int32_t T) {
     extern __shared__ int32_t tile[];
     const int32_t tid = threadIdx.x;
     const int32_t w = blockIdx.x * blockDim.x + tid;

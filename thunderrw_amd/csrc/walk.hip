@@ -110,6 +110,34 @@ __device__ __forceinline__ bool draw_stop(const BatchIO &io, int32_t w, WalkerSt
     return true;
 }
 
+// Replay: the draws a walker's next move takes -- words s.k and s.k + 1 of its stream (0
+// past the end) -- loaded a step ahead, while the current step's gather is in flight, so a
+// replay move is one dependent read as a free-running one is.
+template <int NDRAW>
+__device__ __forceinline__ uint4 replay_prefetch(const BatchIO &io, int32_t w, const WalkerState &s) {
+    uint4 b = make_uint4(0u, 0u, 0u, 0u);
+    const int64_t k = s.k;
+    if (k < io.K) b.x = io.draws[k * io.n + w];
+    if (NDRAW > 1 && k + 1 < io.K) b.y = io.draws[(k + 1) * io.n + w];
+    return b;
+}
+
+// Takes the prefetched draws (replay_take's accounting: the cursor advances per word taken).
+template <int NDRAW>
+__device__ __forceinline__ bool replay_take_prefetched(const BatchIO &io, WalkerState &s, const uint4 &b, uint32_t &u0,
+                                                       uint32_t &u1) {
+    const int64_t k = s.k;
+    if (k >= io.K || (NDRAW > 1 && k + 1 >= io.K)) {
+        if (k < io.K) s.k += 1;
+        s.st = kReplayExhausted;
+        return false;
+    }
+    u0 = b.x;
+    u1 = b.y;
+    s.k += NDRAW;
+    return true;
+}
+
 // Utility::upper_bound_branchless (util/utility.h:18-27): the number of
 // entries <= target in a non-decreasing array.
 __device__ __forceinline__ int32_t count_le(const double *src, int32_t n, double target) {
@@ -823,12 +851,14 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
     const int32_t L = P.length;
     WalkerState s[WPT];
     uint4 blk[WPT];
+    constexpr int kDraws = MODE == kModeUniform ? 1 : 2;  // move draws per step
 #pragma unroll
     for (int j = 0; j < WPT; ++j) {
         const int32_t w = wbase + j * block + tid;
         s[j] = empty_walker();
         if (w < io.n) init_walker(g, io, w, s[j]);
-        blk[j] = first_block<REPLAY>(io, w, s[j]);
+        if (REPLAY) blk[j] = s[j].st == kAlive ? replay_prefetch<kDraws>(io, w, s[j]) : make_uint4(0u, 0u, 0u, 0u);
+        else blk[j] = first_block<REPLAY>(io, w, s[j]);
         if (paths) out.put(j, 0, s[j].cur);
     }
     unsigned long long steps = 0;
@@ -856,7 +886,8 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
                 go[j] = false;
                 continue;
             }
-            if (MODE == kModeUniform) go[j] = draw_one<REPLAY>(io, w, s[j], blk[j], u0[j]);
+            if (REPLAY) go[j] = replay_take_prefetched<kDraws>(io, s[j], blk[j], u0[j], u1[j]);
+            else if (MODE == kModeUniform) go[j] = draw_one<REPLAY>(io, w, s[j], blk[j], u0[j]);
             else go[j] = draw_pair<REPLAY>(io, w, s[j], step, 0u, blk[j], u0[j], u1[j]);
         }
         // Issue every walker's gather, then compute next steps' Philox blocks in
@@ -884,6 +915,10 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
 #pragma unroll
             for (int j = 0; j < WPT; ++j)
                 if (go[j]) blk[j] = philox_block(io, s[j], static_cast<uint32_t>(s[j].len), 0u);
+        } else {
+#pragma unroll
+            for (int j = 0; j < WPT; ++j)
+                if (go[j]) blk[j] = replay_prefetch<kDraws>(io, wbase + j * block + tid, s[j]);
         }
         int32_t next[WPT];
 #pragma unroll

@@ -17,6 +17,9 @@
 // kind 3: DeepWalk's weight and update under another name (apps/deepwalk.cpp:19-25).
 // kind 4: Node2Vec's weight and update under another name, p_ = 1/p, q_ = 1/q
 //         (apps/node2vec.cpp:24-42, 64-76).
+// kind 5: DeepWalk's weight with an update() that never ends a walk (the reference would run
+//         forever; the functor engine stops it at its move cap).
+// kind 6: every weight 0 (max-weight rejection never accepts; the engine's trial cap ends it).
 #include "creeper_main.h"
 
 HyperParameter g_para = {ExecutionMode::Dynamic, SamplingMethod::InverseTransformationSampling, 0, {nullptr, 0}};
@@ -47,7 +50,8 @@ struct BiasedWalk {
             return g_->is_neighbor(prev, end) ? ew : 2.0 * ew;
         }
         if (kind_ == 2) return ((end + w.length_) % 4 == 0) ? ew : 0.0;
-        if (kind_ == 3) return ew;
+        if (kind_ == 3 || kind_ == 5) return ew;
+        if (kind_ == 6) return 0.0;
         if (w.length_ == 1) return max_;
         const int prev = w.seq_[w.length_ - 2];
         if (end == prev) return p_;
@@ -55,6 +59,7 @@ struct BiasedWalk {
     }
 
     inline bool update(WalkerMeta& w, intT begin, intT end, int64_t eid) {
+        if (kind_ == 5) return false;
         return w.length_ == l_ || (kind_ == 2 && g_->degree(end) == 1);
     }
 

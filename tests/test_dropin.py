@@ -17,6 +17,7 @@ import graphs
 import oracle_bind as ob
 import thunderrw_amd as T
 from conftest import ROOT, load_golden_graph
+from thunderrw_amd import _abi as A
 
 BIN = os.path.join(ROOT, "thunderrw_amd", "dropin", "bin")
 pytestmark = [pytest.mark.gpu]
@@ -152,3 +153,31 @@ def test_functor_engine_walks_as_the_library_in_free_running_mode(gpu_ctx, gname
     assert np.array_equal(r["paths"].reshape(n, -1), lib["paths"])
     assert np.array_equal(r["lengths"], lib["lengths"])
     assert r["steps"] == lib["steps"]
+
+
+def test_functor_engine_caps_walks_the_reference_would_never_end(gpu_ctx):
+    """A functor whose update() never ends a walk stops at the engine's move cap
+    (TRW_WALKER_STEP_CAP) with its first moves those of the walk it is; one whose weights are
+    all zero stops at the rejection trial cap (TRW_WALKER_TRIAL_CAP).  Every kernel ends."""
+    if not os.path.exists(os.path.join(BIN, "customwalk")):
+        pytest.skip("test app not built")
+    g, _ = load_golden_graph("small")
+    src = T.sources_all(g.degree)
+    n, L = len(src), 20
+    r = _run_app("customwalk", g, ["-l", str(L), "-em", "1", "-sm", "2", "-kind", "5"],
+                 {"TRW_RNG": "philox", "TRW_SEED": "5", "TRW_MAX_MOVES": "50"})
+    assert np.all(r["status"] == A.WALKER_STEP_CAP)
+    assert np.all(r["lengths"] == 51)
+    assert r["steps"] == 50 * n
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight, g.label)
+    try:
+        lib = T.walk(dg, T.make_params("deepwalk", execution="static", sampling="alias", length=L, rng="philox",
+                                       seed=5), src)
+    finally:
+        dg.close()
+    assert np.array_equal(r["paths"].reshape(n, -1), lib["paths"])  # the rows hold the first L positions
+    r = _run_app("customwalk", g, ["-l", str(L), "-em", "2", "-sm", "4", "-kind", "6"],
+                 {"TRW_RNG": "philox", "TRW_SEED": "5", "TRW_MAX_TRIALS": "100"})
+    assert np.all(r["status"] == A.WALKER_TRIAL_CAP)
+    assert np.all(r["lengths"] == 1)
+    assert r["steps"] == 0

@@ -576,3 +576,44 @@ def test_node2vec_membership_structures_match_oracle(gpu_ctx):
                         assert np.array_equal(r[key], ref[key]), (p, q, form, key)
         finally:
             dg.close()
+
+
+def test_tables_from_caller_weights_equal_the_edge_weight_tables(gpu_ctx):
+    """trw_build_tables_from builds the static tables from a device array of per-edge weights
+    (the functor engine's static_gather output): from the edge weights themselves it gives the
+    tables trw_build_tables(EDGE) gives, byte for byte, tagged CUSTOM in the device view."""
+    import ctypes
+    from thunderrw_amd import _abi as A
+
+    g, _ = load_golden_graph("hub")
+    dg = dgraph(gpu_ctx, g)
+    wbuf = T.DeviceBuffer.from_array(gpu_ctx, np.ascontiguousarray(g.weight, dtype=np.float64))
+    try:
+        ref = {}
+        for sm in ("alias", "its", "rejection"):
+            dg.build_tables(sm, "edge")
+        ref = (dg.alias_table().copy(), dg.prefix_sum().copy(), dg.rejection_max().copy())
+        for code in (A.SM_ALIAS, A.SM_ITS, A.SM_REJECTION):
+            A.check(T.lib().trw_build_tables_from(dg.h, code, ctypes.c_void_p(wbuf.data_ptr())))
+        assert np.array_equal(dg.alias_table().view(np.uint8), ref[0].view(np.uint8))
+        assert np.array_equal(dg.prefix_sum().view(np.uint64), ref[1].view(np.uint64))
+        assert np.array_equal(dg.rejection_max(), ref[2])
+        view = A.DeviceView()
+        A.check(T.lib().trw_graph_device_view(dg.h, ctypes.byref(view)))
+        assert (view.num_vertices, view.num_edges) == (g.nv, g.ne)
+        assert view.max_degree == int(g.degree.max())
+        assert (view.alias_weights, view.its_weights, view.rejection_weights) == (A.WEIGHTS_CUSTOM,) * 3
+        for f in ("offset", "degree", "adj", "edge_weight", "edge_label", "alias_table", "prefix_sum",
+                  "rejection_max", "rejection_weight"):
+            assert getattr(view, f), f
+        # The rejection weights are the library's own copy, not the caller's buffer.
+        assert view.rejection_weight != wbuf.data_ptr()
+        # A built-in walk that needs edge-weight tables rebuilds them: the walk equals the oracle's.
+        case = {"app": "deepwalk", "execution": 1, "sampling": 2, "length": 12, "extra": {}}
+        src = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        refw = oracle_walk(g, case, src, 24, 1, 96, 97)
+        r = T.walk(dg, case_params(case, "fused"), src, T.replay_fill(96, len(src), 24))
+        assert np.array_equal(r["paths"], refw["paths"])
+    finally:
+        wbuf.free()
+        dg.close()

@@ -66,7 +66,7 @@ def parse_args():
     ap.add_argument("--qps", type=float, default=10000.0, help="offered load of the PPR serving run")
     ap.add_argument("--queries", type=int, default=10000, help="queries in the PPR serving run")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--only", choices=["its", "mix", "serving", "c1", "c3"],
+    ap.add_argument("--only", choices=["its", "mix", "serving", "c1", "c3", "functor"],
                     help="run one secondary measurement alone and print its figures (for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also launch the copy kernel over this many MiB (byte-counter calibration)")
@@ -110,6 +110,58 @@ def time_launches(ctx, graph, params, io, warmup, steps):
     ms = timer.end()
     T.synchronize(ctx)
     return ms
+
+
+FUNCTOR_APP = os.path.join(ROOT, "thunderrw_amd", "dropin", "bin", "customwalk")
+
+
+def functor_engine(ctx, dg, hg, deg, args):
+    """The drop-in functor engine -- app functors the library has no kernel for, their weight()
+    and update() run on the device -- on the same graph, through tests/dropin_apps/customwalk.cpp
+    built against thunderrw_amd/dropin (its walk time is HIP events around the walk alone):
+      * a Node2Vec functor under another name (max-weight rejection, a lane per walker, p = 2,
+        q = 0.5), one walker per non-isolated vertex, beside the library's own Node2Vec kernel on
+        the same walkers and Philox stream (the two walk identically);
+      * DeepWalk's weights by dynamic ITS, and a second-order walker-dependent functor by dynamic
+        alias: a wavefront per walker and a gather of every neighbour's weight per move, as the
+        reference's Dynamic mode does, on 2^14 walkers from random sources."""
+    if not os.path.exists(FUNCTOR_APP):
+        log("functor engine: %s not built; skipped" % FUNCTOR_APP)
+        return None
+    tmp = tempfile.mkdtemp(prefix="trw_functor_", dir="/tmp")
+    try:
+        hg.store(tmp)
+        env = dict(os.environ, TRW_RNG="philox", TRW_SEED="11")
+
+        def run(extra):
+            r = subprocess.run([FUNCTOR_APP, "-f", tmp, "-ew", "-l", "80"] + extra, env=env, capture_output=True,
+                               text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError("functor app %s failed: %s" % (extra, r.stderr[-500:]))
+            steps = int(re.search(r"Num of steps: (\d+)", r.stderr).group(1))
+            secs = float(re.search(r"Walking time \(seconds\): ([0-9.]+)", r.stderr).group(1))
+            return steps, secs
+
+        out = {}
+        steps, secs = run(["-kind", "4", "-em", "2", "-sm", "4"])
+        out["node2vec_functor_steps_per_s"] = steps / secs
+        src = T.sources_all(deg)
+        b = Batch(ctx, src, 80)
+        prm = T.make_params("node2vec", length=80, rng="philox", seed=11)
+        ms = time_launches(ctx, dg, prm, b.io(), 1, 3)
+        b.free()
+        out["node2vec_library_steps_per_s"] = steps / (ms / 1e3 / 3)
+        steps, secs = run(["-kind", "3", "-em", "2", "-sm", "1", "-wn", str(1 << 14)])
+        out["dynamic_its_sample_steps_per_s"] = steps / secs
+        steps, secs = run(["-kind", "1", "-em", "2", "-sm", "2", "-wn", str(1 << 14)])
+        out["second_order_dynamic_alias_sample_steps_per_s"] = steps / secs
+        log("functor engine: Node2Vec functor %.3e steps/s (library kernel %.3e); dynamic ITS %.3e, "
+            "second-order dynamic alias %.3e steps/s (2^14-walker samples)"
+            % (out["node2vec_functor_steps_per_s"], out["node2vec_library_steps_per_s"],
+               out["dynamic_its_sample_steps_per_s"], out["second_order_dynamic_alias_sample_steps_per_s"]))
+        return out
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 def host_cpu():
@@ -368,8 +420,11 @@ def main():
     alias_build_ms = dg.last_build_ms()
     log("alias tables built on the device in %.1f ms" % alias_build_ms)
 
-    if args.only == "mix":
-        print(json.dumps({"mix_ppr_metapath": mix_forms(ctx, dg, deg, args)}), flush=True)
+    if args.only in ("mix", "functor"):
+        if args.only == "mix":
+            print(json.dumps({"mix_ppr_metapath": mix_forms(ctx, dg, deg, args)}), flush=True)
+        else:
+            print(json.dumps({"functor_engine": functor_engine(ctx, dg, hg, deg, args)}), flush=True)
         dg.close()
         ctx.close()
         return
@@ -430,6 +485,7 @@ def main():
 
     if not args.quick:
         details["mix_ppr_metapath"] = mix_forms(ctx, dg, deg, args)
+        details["functor_engine"] = functor_engine(ctx, dg, hg, deg, args)
     batch.free()
     dg.close()
     if not args.quick and not args.no_serving:

@@ -7,7 +7,10 @@
 //   * against thunderrw_amd/dropin (thunderrw_amd/dropin/bin/customwalk), where compute<F>
 //     runs BiasedWalk's weight()/update()/max_weight() on the GPU in the functor engine.
 //
-//   customwalk -f <graph dir> -ew [-l 20] [-kind k] [-em e] [-sm s] [-p 2] [-q 0.5]
+//   customwalk -f <graph dir> -ew [-l 20] [-kind k] [-em e] [-sm s] [-p 2] [-q 0.5] [-wn n]
+//
+// -wn n walks n walkers from random sources (generate_random_source, seed 0) instead of one per
+// non-isolated vertex.
 //
 // kind 0: walker-independent, edge weight over (1 + neighbour degree) (Static and Dynamic).
 // kind 1: second order: the edge weight, x0.25 back to the previous vertex, x1 to a common
@@ -83,6 +86,9 @@ void execute(Graph &graph, InputParser &cmd_parser) {
     g_para.length_ = l;
 
     std::vector<WalkerMeta> walkers;
-    generate_all_source(graph, walkers);
+    if (!cmd_parser.get_cmd_option("-wn").empty())
+        generate_random_source(graph, walkers, std::stoi(cmd_parser.get_cmd_option("-wn")), 0);
+    else
+        generate_all_source(graph, walkers);
     compute(graph, walkers, BiasedWalk(l, kind, 1.0 / p, 1.0 / q, &graph));
 }

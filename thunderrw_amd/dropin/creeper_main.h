@@ -445,10 +445,20 @@ struct Buffers {
 
 namespace trw_dropin {
 
-// The functor engine (functor_walk.h): F's methods run on the device.  io holds the walker
-// batch (paths non-null, with readable guard rows before and after).
+// A functor walk ready to launch: the device copy of the functor and the kernel arguments.
 template <class F>
-void run_functor(Graph &graph, F &f, const trw_walk_params &p, const trw_walk_io &io, Buffers &bufs) {
+struct FunctorWalk {
+    F *functor = nullptr;
+    trw_functor::Args args;
+    hipStream_t stream = nullptr;
+};
+
+// The functor engine (functor_walk.h): F's methods run on the device.  io holds the walker
+// batch (paths non-null, with readable guard rows before and after).  Everything before the
+// walk itself -- the functor's device copy and, in Static mode, static_gather and the tables --
+// happens here, so the walk's timing covers the walk alone (as creeper.h:700-803 times it).
+template <class F>
+FunctorWalk<F> prepare_functor(Graph &graph, F &f, const trw_walk_params &p, const trw_walk_io &io, Buffers &bufs) {
     trw_context *ctx = context();
     trw_graph *g = graph.device();
     void *sv = nullptr;
@@ -536,14 +546,26 @@ void run_functor(Graph &graph, F &f, const trw_walk_params &p, const trw_walk_io
     }
     if (p.execution == TRW_EXEC_DYNAMIC && p.sampling != TRW_SM_MAX_WEIGHT_REJECTION) {
         const int32_t maxd = std::max<int32_t>(v.max_degree, 1);
-        const int64_t waves = trw_functor::gather_waves(a.n, maxd);
+        const int64_t waves = trw_functor::gather_waves(a.n, maxd, p.sampling);
         const uint64_t rows = static_cast<uint64_t>(waves) * static_cast<uint64_t>(maxd);
         a.scratch_stride = maxd;
         a.scratch_w = static_cast<double *>(bufs.get(8 * rows));
-        a.scratch_s = static_cast<trw_functor::Entry *>(bufs.get(sizeof(trw_functor::Entry) * rows));
-        a.scratch_g = static_cast<trw_functor::Entry *>(bufs.get(sizeof(trw_functor::Entry) * rows));
+        if (p.sampling == TRW_SM_ALIAS) {
+            a.scratch_s = static_cast<trw_functor::Entry *>(bufs.get(sizeof(trw_functor::Entry) * rows));
+            a.scratch_g = static_cast<trw_functor::Entry *>(bufs.get(sizeof(trw_functor::Entry) * rows));
+        }
     }
-    const hipError_t e = trw_functor::launch_walk<F, WalkerMeta>(df, a, st);
+    if (trw_synchronize(ctx, nullptr) != TRW_OK) die("functor walk set-up");
+    FunctorWalk<F> fw;
+    fw.functor = df;
+    fw.args = a;
+    fw.stream = st;
+    return fw;
+}
+
+template <class F>
+void launch_functor(const FunctorWalk<F> &fw) {
+    const hipError_t e = trw_functor::launch_walk<F, WalkerMeta>(fw.functor, fw.args, fw.stream);
     if (e != hipSuccess) {
         std::fprintf(stderr, "[thunderrw] functor walk launch: %s\n", hipGetErrorString(e));
         std::exit(-1);
@@ -646,8 +668,8 @@ void compute(Graph &graph, std::vector<WalkerMeta> &walkers, F f) {
     }
 
     trw_dropin::log_line("Compute...");
-    trw_timer_begin(ctx, nullptr);
     if (library_move) {
+        trw_timer_begin(ctx, nullptr);
         if (trw_walk_async(g, &p, &io, nullptr) != TRW_OK) trw_dropin::die("compute");
     } else {
 #if defined(__HIPCC__)
@@ -657,7 +679,9 @@ void compute(Graph &graph, std::vector<WalkerMeta> &walkers, F f) {
                                  "(sampling methods 0-3)\n");
             std::exit(-1);
         } else {
-            trw_dropin::run_functor(graph, f, p, io, bufs);
+            const auto fw = trw_dropin::prepare_functor(graph, f, p, io, bufs);
+            trw_timer_begin(ctx, nullptr);
+            trw_dropin::launch_functor(fw);
         }
 #endif
     }

@@ -269,12 +269,81 @@ __global__ void __launch_bounds__(256) lane_kernel(const F *fsrc, Args a) {
     flush_counters(a, steps, gterm, dead, errs);
 }
 
+constexpr int kGatherBatch = 4;
+
+// dynamic_gather's weights for one move, and the serial FP64 chain its sampler folds them with
+// (ITS: the in-place prefix sums w[x] += w[x - 1], creeper.h:263-265, stored to W; alias:
+// std::accumulate from 0.0, alias_sampling.h:16, weights stored to W; rejection:
+// std::max_element, creeper.h:284, weights stored to W).  acc ends as the total / sum / maximum.
+//
+// The chain runs in every lane on the same values: its operands come from the wavefront's LDS
+// row as broadcast reads and its prefix sums go back there, so each element costs one vector
+// ALU instruction (the add, or the compare-select of the maximum); reading them across lanes
+// with readlane cost about fifteen.  lds_in / lds_out are this wavefront's 64-entry rows.
+template <int SM>
+__device__ __forceinline__ void chain_step(double &acc, double x, double *__restrict__ lds_out, int32_t j) {
+    if (SM == TRW_SM_ITS) {
+        acc = acc + x;
+        lds_out[j] = acc;
+    } else if (SM == TRW_SM_ALIAS) {
+        acc = acc + x;
+    } else {
+        acc = acc < x ? x : acc;
+    }
+}
+
+template <int SM, class F, class WM>
+__device__ __forceinline__ void gather_chain(F &f, const Args &a, WM &wm, int32_t cur, int64_t off, int32_t d,
+                                             int lane, double *W, double *__restrict__ lds_in,
+                                             double *__restrict__ lds_out, double &acc) {
+    for (int32_t base = 0; base < d; base += 64 * kGatherBatch) {
+        double wv[kGatherBatch];
+#pragma unroll
+        for (int q = 0; q < kGatherBatch; ++q) {
+            const int32_t i = base + 64 * q + lane;
+            wv[q] = 0.0;
+            if (i < d) wv[q] = f.weight(wm, cur, a.adj[off + i], off + i);
+        }
+#pragma unroll
+        for (int q = 0; q < kGatherBatch; ++q) {
+            const int32_t cb = base + 64 * q;
+            if (cb >= d) break;
+            const int32_t cnt = min(64, d - cb);
+            lds_in[lane] = wv[q];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            int32_t j = 0;
+            if (cb == 0) {  // the chain starts at the first weight
+                acc = lds_in[0];
+                if (SM == TRW_SM_ITS) lds_out[0] = acc;
+                if (SM == TRW_SM_ALIAS) acc = 0.0 + acc;
+                j = 1;
+            }
+            // Eight operands read ahead of their adds, so the reads' latency overlaps.
+            for (; j + 8 <= cnt; j += 8) {
+                double x[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) x[t] = lds_in[j + t];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) chain_step<SM>(acc, x[t], lds_out, j + t);
+            }
+            for (; j < cnt; ++j) chain_step<SM>(acc, lds_in[j], lds_out, j);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (cb + lane < d) W[cb + lane] = SM == TRW_SM_ITS ? lds_out[lane] : wv[q];
+            __builtin_amdgcn_wave_barrier();  // the next chunk's writes of lds_in follow every read
+        }
+    }
+}
+
 // Dynamic ITS / alias / rejection: one walker per wavefront.  Everything but the weight
 // evaluation is wave-uniform: every lane runs the same serial chains on the same values, so
 // no result needs broadcasting.  W, S and G are this wavefront's scratch rows.
 template <class F, class WM, bool REPLAY>
 __device__ int32_t walk_wave(F &f, const Args &a, int32_t w, WM &wm, Draws<REPLAY> &dr, unsigned long long &steps,
-                             double *W, Entry *S, Entry *G) {
+                             double *W, Entry *S, Entry *G, double *lds_in, double *lds_out) {
     const int lane = static_cast<int>(threadIdx.x & 63);
     const int32_t src = a.sources[w];
     wm.id_ = w;
@@ -293,28 +362,16 @@ __device__ int32_t walk_wave(F &f, const Args &a, int32_t w, WM &wm, Draws<REPLA
         const uint32_t step = static_cast<uint32_t>(wm.length_ - 1);
         const uint32_t ud = static_cast<uint32_t>(d);
 
-        // Gather (creeper.h:178-247): weight i by lane i % 64, chunk by chunk; the serial chain
-        // of each sampler reads the chunk's weights across lanes in index order.
+        // Gather (creeper.h:178-247): weight i by lane i % 64, kGatherBatch chunks of 64 at a time
+        // (their weight reads overlap); the serial chain of each sampler then reads each chunk's
+        // weights across lanes in index order.
         double acc = 0.0;  // ITS: prefix sum; alias: accumulate from 0.0; RS: max_element
-        for (int32_t base = 0; base < d; base += 64) {
-            const int32_t i = base + lane;
-            double wi = 0.0;
-            if (i < d) wi = f.weight(wm, cur, a.adj[off + i], off + i);
-            const int32_t cnt = min(64, d - base);
-            double mine = 0.0;
-            for (int32_t j = 0; j < cnt; ++j) {
-                const double x = readlane_f64(wi, j);
-                if (a.sampling == TRW_SM_ITS) {
-                    acc = (base == 0 && j == 0) ? x : acc + x;  // w[x] += w[x - 1] (creeper.h:263-265)
-                    if (lane == j) mine = acc;
-                } else if (a.sampling == TRW_SM_ALIAS) {
-                    acc = acc + x;                               // std::accumulate (alias_sampling.h:16)
-                } else {
-                    acc = (base == 0 && j == 0) ? x : (acc < x ? x : acc);  // std::max_element (creeper.h:284)
-                }
-            }
-            if (i < d) W[i] = a.sampling == TRW_SM_ITS ? mine : wi;
-        }
+        if (a.sampling == TRW_SM_ITS)
+            gather_chain<TRW_SM_ITS>(f, a, wm, cur, off, d, lane, W, lds_in, lds_out, acc);
+        else if (a.sampling == TRW_SM_ALIAS)
+            gather_chain<TRW_SM_ALIAS>(f, a, wm, cur, off, d, lane, W, lds_in, lds_out, acc);
+        else
+            gather_chain<TRW_SM_REJECTION>(f, a, wm, cur, off, d, lane, W, lds_in, lds_out, acc);
         wave_publish();
 
         int32_t next;
@@ -422,15 +479,18 @@ __global__ void __launch_bounds__(256) gather_kernel(const F *fsrc, Args a) {
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = (static_cast<int64_t>(gridDim.x) * blockDim.x) >> 6;
     double *W = a.scratch_w + wave * a.scratch_stride;
-    Entry *S = a.scratch_s + wave * a.scratch_stride;
-    Entry *G = a.scratch_g + wave * a.scratch_stride;
+    Entry *S = a.scratch_s ? a.scratch_s + wave * a.scratch_stride : nullptr;  // alias only
+    Entry *G = a.scratch_g ? a.scratch_g + wave * a.scratch_stride : nullptr;
+    __shared__ double chain_in[256], chain_out[256];  // per wavefront: 64 chain operands, 64 prefix sums
+    double *lds_in = chain_in + (threadIdx.x & ~63u);
+    double *lds_out = chain_out + (threadIdx.x & ~63u);
     unsigned long long steps = 0, gterm = 0, dead = 0, errs = 0;
     const bool lead = (threadIdx.x & 63) == 0;
     for (int64_t w = wave; w < a.n; w += nwaves) {
         WM wm;
         Draws<REPLAY> dr(a, static_cast<int32_t>(w));
         unsigned long long s = 0;
-        const int32_t st = walk_wave<F, WM, REPLAY>(f, a, static_cast<int32_t>(w), wm, dr, s, W, S, G);
+        const int32_t st = walk_wave<F, WM, REPLAY>(f, a, static_cast<int32_t>(w), wm, dr, s, W, S, G, lds_in, lds_out);
         if (lead) {
             steps += s;
             tally(st, gterm, dead, errs);
@@ -523,10 +583,18 @@ hipError_t launch_weights(const F *dev_f, const trw_device_view &v, double *out,
     return hipGetLastError();
 }
 
-// Wavefronts gather_kernel runs at most (one walker each at a time), and its scratch bytes.
-inline int64_t gather_waves(int32_t n, int32_t max_degree) {
-    const int64_t per_wave = static_cast<int64_t>(max_degree > 0 ? max_degree : 1) * (8 + 2 * sizeof(Entry));
-    int64_t waves = 4096;  // 256 CUs x 16 wavefronts
+// Scratch bytes per entry of a wavefront's row: the weights (prefix sums), and for alias the
+// two queues as well.
+inline int64_t gather_entry_bytes(int32_t sampling) {
+    return sampling == TRW_SM_ALIAS ? 8 + 2 * static_cast<int64_t>(sizeof(Entry)) : 8;
+}
+
+// Wavefronts gather_kernel runs at most (one walker each at a time): up to 32 per CU, fewer when
+// their scratch rows (max degree entries each) would pass 4 GiB.  A Dynamic move is a serial
+// FP64 chain over the vertex's neighbours, so throughput grows with the walkers in flight.
+inline int64_t gather_waves(int32_t n, int32_t max_degree, int32_t sampling) {
+    const int64_t per_wave = static_cast<int64_t>(max_degree > 0 ? max_degree : 1) * gather_entry_bytes(sampling);
+    int64_t waves = 8192;  // 256 CUs x 32 wavefronts
     while (waves > 64 && waves * per_wave > (int64_t(4) << 30)) waves >>= 1;
     if (waves > n) waves = n > 0 ? n : 1;
     return (waves + 3) & ~int64_t(3);  // whole 256-thread blocks
@@ -537,7 +605,7 @@ hipError_t launch_walk(const F *dev_f, const Args &a, hipStream_t st) {
     if (a.n == 0) return hipSuccess;
     const bool replay = a.draws != nullptr;
     if (a.execution == TRW_EXEC_DYNAMIC && a.sampling != TRW_SM_MAX_WEIGHT_REJECTION) {
-        const int64_t waves = gather_waves(a.n, static_cast<int32_t>(a.scratch_stride));
+        const int64_t waves = gather_waves(a.n, static_cast<int32_t>(a.scratch_stride), a.sampling);
         const dim3 grid(static_cast<unsigned>(waves / 4)), block(256);
         if (replay) hipLaunchKernelGGL((gather_kernel<F, WM, true>), grid, block, 0, st, dev_f, a);
         else hipLaunchKernelGGL((gather_kernel<F, WM, false>), grid, block, 0, st, dev_f, a);

@@ -124,13 +124,6 @@ struct Draws {
     }
 };
 
-__device__ __forceinline__ double readlane_f64(double v, int j) {
-    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
-    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), j));
-    const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), j));
-    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-
 // Stores of one wavefront made visible to its own later loads of other lanes' addresses.
 __device__ __forceinline__ void wave_publish() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");

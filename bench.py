@@ -471,6 +471,21 @@ def main():
 
         details.update(its_walk(ctx, dg, batch, length, per_launch, args))
 
+        # The same walk through the host-memory entry point: sources in and 1.3 GB of paths out over
+        # PCIe (pageable host buffers), device buffers allocated by the call.  Reported beside the
+        # headline, never as it.
+        paths_h = np.empty((n, length), dtype=np.int32)
+        steps_h = ctypes.c_uint64()
+        t2 = time.perf_counter()
+        T._abi.check(T.lib().trw_walk_host(dg.h, ctypes.byref(prm), src.ctypes.data, n, None, 0, None, 0,
+                                           paths_h.ctypes.data, None, None, None, ctypes.byref(steps_h)))
+        secs_h = time.perf_counter() - t2
+        del paths_h
+        details["host_buffers_steps_per_s"] = steps_h.value / secs_h
+        details["host_buffers_ms"] = secs_h * 1e3
+        log("same walk through trw_walk_host (paths to pageable host memory): %.1f ms, %.3e steps/s"
+            % (secs_h * 1e3, steps_h.value / secs_h))
+
     if not args.quick:
         # Measured ceiling for this access pattern: dependent random 16-byte gathers at full
         # occupancy over a table the size of the alias walk records (and a 32 MiB cache-resident one).

@@ -789,12 +789,10 @@ struct WaveTile {
 };
 
 // Thread per walker, the whole fixed-length walk in one launch (any Move).
-// T path positions are staged per row (dynamic LDS: blockDim x (T + 1) ints).  At least five
-// wavefronts per SIMD: the Node2Vec Move (membership searches) would otherwise take 99
-// registers and four wavefronts; five, with 12 bytes per lane spilled, runs it 14% faster.
+// T path positions are staged per row in `tile` (dynamic LDS: blockDim x (T + 1) ints).
 template <int MODE, bool REPLAY>
-__global__ __launch_bounds__(kMaxFusedBlock) __attribute__((amdgpu_waves_per_eu(5))) void walk_fused_kernel(DevGraph g, KindParams P, BatchIO io, int32_t T) {
-    extern __shared__ int32_t tile[];
+__device__ __forceinline__ void walk_fused_body(const DevGraph &g, const KindParams &P, const BatchIO &io, int32_t T,
+                                                int32_t *tile) {
     const int32_t tid = threadIdx.x;
     const int32_t w = blockIdx.x * blockDim.x + tid;
     const int32_t wave_first = w - (tid & 63);
@@ -829,6 +827,22 @@ __global__ __launch_bounds__(kMaxFusedBlock) __attribute__((amdgpu_waves_per_eu(
         classify(s.st, gterm, dead, err);
     }
     flush_counters(io, steps, gterm, dead, err);
+}
+
+template <int MODE, bool REPLAY>
+__global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_kernel(DevGraph g, KindParams P, BatchIO io, int32_t T) {
+    extern __shared__ int32_t tile[];
+    walk_fused_body<MODE, REPLAY>(g, P, io, T, tile);
+}
+
+// The Node2Vec Move (membership searches) takes 99 registers, so four wavefronts per SIMD;
+// asking for five spills 12 bytes per lane and runs 14% faster.  The same request costs the
+// ITS Move a quarter of its rate, hence a kernel of its own.
+template <bool REPLAY>
+__global__ __launch_bounds__(kMaxFusedBlock) __attribute__((amdgpu_waves_per_eu(5))) void walk_fused_node2vec_kernel(
+    DevGraph g, KindParams P, BatchIO io, int32_t T) {
+    extern __shared__ int32_t tile[];
+    walk_fused_body<kModeNode2Vec, REPLAY>(g, P, io, T, tile);
 }
 
 // Fused form, uniform and alias Moves, WPT walkers interleaved per lane.
@@ -1291,7 +1305,7 @@ hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO 
     case kModeAlias: hipLaunchKernelGGL((walk_fused_kernel<kModeAlias, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     case kModeIts: hipLaunchKernelGGL((walk_fused_kernel<kModeIts, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     case kModeRejection: hipLaunchKernelGGL((walk_fused_kernel<kModeRejection, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
-    case kModeNode2Vec: hipLaunchKernelGGL((walk_fused_kernel<kModeNode2Vec, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
+    case kModeNode2Vec: hipLaunchKernelGGL((walk_fused_node2vec_kernel<REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     default: hipLaunchKernelGGL((walk_fused_kernel<kModeMetaPath, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     }
     return hipGetLastError();

@@ -100,7 +100,7 @@ typedef struct {
     uint64_t seed;          /* Philox key (PHILOX) */
     int32_t form;           /* TRW_FORM_* */
     int32_t max_trials;     /* rejection trials per step before TRW_WALKER_TRIAL_CAP; 0 = no cap */
-    int32_t walkers_per_thread; /* FUSED uniform/alias: walkers interleaved per lane (1, 2, 4); 0 = tuned default */
+    int32_t walkers_per_thread; /* FUSED uniform/alias/static ITS: walkers interleaved per lane (1, 2, 4); 0 = tuned default */
     int32_t weights;        /* TRW_WALK_WEIGHTS_*: the weights Static/Dynamic sampling uses */
 } trw_walk_params;
 
@@ -242,7 +242,7 @@ int trw_graph_device_view(const trw_graph *g, trw_device_view *out);
  * Launch choices that change speed, never results (no reference counterpart; the
  * reference fixes its ring sizes at compile time, types.h:17-31).  trw_autotune picks
  * them by measurement; 0 means the built-in default. */
-#define TRW_TUNE_WALKERS_PER_THREAD 0  /* FUSED uniform/alias walkers per lane when params say 0: 1, 2, 4 */
+#define TRW_TUNE_WALKERS_PER_THREAD 0  /* FUSED uniform/alias/static ITS walkers per lane when params say 0: 1, 2, 4 */
 #define TRW_TUNE_ITS_TREE_MIN_DEGREE 1 /* ITS: vertices of at least this degree search their 128-byte-node
                                           tree, the rest binary-search their prefix sums */
 #define TRW_TUNE_QUEUE_CHUNK 2         /* QUEUE: walker indices a wavefront takes per atomic (64..4096) */

@@ -617,3 +617,116 @@ def test_tables_from_caller_weights_equal_the_edge_weight_tables(gpu_ctx):
     finally:
         wbuf.free()
         dg.close()
+
+
+def _its_change_points(P, total):
+    """For each i in [1, d): the least u in [0, 2^32] with P[i - 1] <= real2(u) * total, the draw
+    at which the ITS pick (upper bound of real2(u) * total) moves past edge i - 1; bisection on
+    the Move's own expression (2^32: never)."""
+    lo = np.zeros(len(P) - 1, np.int64)  # predicate false below lo unless it holds at 0
+    hi = np.full(len(P) - 1, 1 << 32, np.int64)
+    key = P[:-1]
+    t0 = (np.float64(0.0) * (1.0 / 4294967296.0)) * total
+    at0 = key <= t0
+    while True:
+        open_ = (hi - lo > 1) & ~at0
+        if not open_.any():
+            break
+        mid = (lo + hi) // 2
+        ok = key <= (mid.astype(np.float64) * (1.0 / 4294967296.0)) * total
+        hi = np.where(open_ & ok, mid, hi)
+        lo = np.where(open_ & ~ok, mid, lo)
+    return np.where(at0, 0, hi)
+
+
+def _bucket_edges(m):
+    """First and last draw of each of m buckets floor(u * m / 2^32)."""
+    k = np.arange(m + 1, dtype=object)
+    first = np.array([((int(x) << 32) + m - 1) // m for x in k], dtype=np.int64)
+    return np.concatenate([first[:-1], first[1:] - 1])
+
+
+def test_its_guide_records_decide_every_pick_boundary(gpu_ctx):
+    """Static ITS in the fused form walks one gather per step through guide records (buckets
+    of the draw space whose picks span at most two edges, with the draw threshold between
+    them).  Replay draws placed at every pick change point (u - 1, u, u + 1) and at the edges
+    of the buckets at one to four buckets per edge, on vertices whose weights span six decades
+    (so some buckets span many edges and take the prefix-sum search), unit-range weights, a
+    hub, a degree-one vertex and a vertex whose weights are all zero: every pick equals the
+    oracle's upper bound, and the queue form's tree search agrees."""
+    rng = np.random.default_rng(77)
+    g = graphs.hub_graph(3000, 2500, 6000, 78)
+    w = rng.uniform(1.0, 5.0, g.ne)
+    skew = np.repeat(rng.random(g.nv) < 0.5, g.degree)
+    w[skew] = 10.0 ** rng.uniform(-4.0, 2.0, int(skew.sum()))
+    live = np.nonzero(g.degree > 0)[0]
+    zero_v = int(live[np.argmin(g.degree[live] + (g.degree[live] < 3) * 10**6)])  # a small vertex of degree >= 3
+    w[g.offset[zero_v]:g.offset[zero_v + 1]] = 0.0
+    gg = graphs.CSR(g.degree, g.adj, w, g.label)
+    dg = dgraph(gpu_ctx, gg)
+    try:
+        dg.build_tables("its", "edge")
+        P_all = dg.prefix_sum()
+        hub = int(np.argmax(g.degree))
+        one = live[g.degree[live] == 1]
+        pick = list(rng.choice(live, 120, replace=False)) + [hub, zero_v] + list(one[:3])
+        src, draws = [], []
+        for v in dict.fromkeys(int(x) for x in pick):
+            d = int(g.degree[v])
+            P = P_all[g.offset[v]:g.offset[v + 1]]
+            total = P[-1]
+            u = [0, 1, (1 << 32) - 1, 1 << 31]
+            if total > 0 and d > 1:
+                c = _its_change_points(P, total)
+                c = c[c < (1 << 32)]
+                u += list(c - 1) + list(c) + list(c + 1)
+            for c in (1, 2, 3, 4):
+                u += list(_bucket_edges(c * d))
+            u = np.unique(np.clip(np.array(u, np.int64), 0, (1 << 32) - 1)).astype(np.uint32)
+            src.append(np.full(len(u), v, np.int32))
+            draws.append(u)
+        src = np.concatenate(src)
+        draws = np.concatenate(draws)
+        og = ob.Graph(gg, None, ob.its_build(gg, w), w, None)
+        ref = ob.walk(og, "deepwalk", 1, 1, 2, src, draws.reshape(-1, 1), 1)
+        assert (ref["status"] == 1).sum() > 0  # the zero-weight vertex ends its walkers without a draw
+        for form in ("fused", "queue"):
+            r = T.walk(dg, T.make_params("deepwalk", execution="static", sampling="its", length=2, rng="replay",
+                                         form=form), src, draws.reshape(1, -1))
+            for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                assert np.array_equal(r[key], ref[key]), (form, key)
+            assert r["steps"] == ref["steps"], form
+    finally:
+        dg.close()
+
+
+@pytest.mark.parametrize("seed", [41, 42])
+def test_its_guide_walks_match_oracle_on_skewed_weights(gpu_ctx, seed):
+    """Whole ITS walks (length 40) where a third of the vertices have weights spanning six
+    decades: replay mode against the oracle in every form, and the Philox stream identical in
+    the fused (guide) and queue (search) forms."""
+    rng = np.random.default_rng(seed)
+    g = graphs.chung_lu(5000, 25000, 2.0, 2, 2000, seed)
+    w = g.weight.copy()
+    skew = np.repeat(rng.random(g.nv) < 0.33, g.degree)
+    w[skew] = 10.0 ** rng.uniform(-4.0, 2.0, int(skew.sum()))
+    gg = graphs.CSR(g.degree, g.adj, w, g.label)
+    dg = dgraph(gpu_ctx, gg)
+    try:
+        live = np.nonzero(gg.degree > 0)[0].astype(np.int32)
+        src = np.concatenate([live, rng.choice(live, 2000)]).astype(np.int32)
+        case = {"app": "deepwalk", "execution": 1, "sampling": 1, "length": 40, "extra": {}}
+        K = 60
+        ref = oracle_walk(gg, case, src, K, 1, seed, seed + 7)
+        draws = T.replay_fill(seed, len(src), K)
+        for form in forms_for("deepwalk"):
+            r = T.walk(dg, case_params(case, form), src, draws, T.replay_fill(seed + 7, len(src), 1))
+            for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                assert np.array_equal(r[key], ref[key]), (form, key)
+            assert r["steps"] == ref["steps"], form
+        outs = [T.walk(dg, T.make_params("deepwalk", execution="static", sampling="its", length=40, rng="philox",
+                                         seed=seed, form=form), src) for form in ("fused", "queue")]
+        for key in ("paths", "lengths", "status"):
+            assert np.array_equal(outs[0][key], outs[1][key]), key
+    finally:
+        dg.close()

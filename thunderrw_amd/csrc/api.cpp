@@ -253,6 +253,8 @@ static void graph_release(trw_graph *g) {
     dfree(g->lerec);
     dfree(g->its_lines);
     dfree(g->its_root);
+    dfree(g->grec);
+    dfree(g->gvrec);
     dfree(g->mlines);
     dfree(g->nrec);
     dfree(g->mroot);
@@ -426,6 +428,40 @@ static int build_its_trees(trw_graph *g, bool monotone) {
     return TRW_OK;
 }
 
+// ITS guide records over every vertex's prefix sums (one gather per static ITS step; the
+// record layout is described with DevGraph).  Like the search structures they reproduce the
+// upper bound only for non-decreasing sums, and they carry neighbours' bucket ranges in 32
+// bits (a walker holds a bucket count as a positive int32), so graphs whose bucket total
+// reaches 2^32 - 1 or whose degree reaches 2^29 get none.
+static int build_its_guide(trw_graph *g, bool monotone) {
+    dfree(g->grec);
+    if (!monotone || g->nv == 0 || !g->vrec || g->max_degree >= (1 << 29)) return TRW_OK;
+    hipStream_t st = g->ctx->stream;
+    const size_t V = static_cast<size_t>(g->nv);
+    ScratchBuffers sc;
+    uint32_t *flag = nullptr, *count = nullptr;
+    uint64_t *start = nullptr;
+    uint8_t *temp = nullptr;
+    size_t temp_bytes = 0;
+    TRW_HIP(trw::launch_its_guide_starts(g->view(), g->prefix, nullptr, nullptr, nullptr, nullptr, temp_bytes, st),
+            "ITS guide scan size");
+    TRW_HIP(sc.get(&flag, V), "hipMalloc ITS guide flags");
+    TRW_HIP(sc.get(&count, V), "hipMalloc ITS guide counts");
+    TRW_HIP(sc.get(&start, V + 1), "hipMalloc ITS guide starts");
+    TRW_HIP(sc.get(&temp, temp_bytes), "hipMalloc ITS guide scan storage");
+    TRW_HIP(trw::launch_its_guide_starts(g->view(), g->prefix, flag, count, start, temp, temp_bytes, st),
+            "ITS guide starts");
+    uint64_t total = 0;
+    TRW_HIP(hipMemcpyAsync(&total, start + V, sizeof(total), hipMemcpyDeviceToHost, st), "ITS guide size");
+    TRW_HIP(hipStreamSynchronize(st), "ITS guide starts");
+    if (total >= trw::kNoRange) return TRW_OK;
+    if (!g->gvrec) TRW_HIP(dmalloc(&g->gvrec, V), "hipMalloc ITS guide ranges");
+    TRW_HIP(dmalloc(&g->grec, 2 * static_cast<size_t>(total) + 2), "hipMalloc ITS guide records");
+    TRW_HIP(trw::launch_its_guide_fill(g->view(), g->prefix, start, total, g->gvrec, g->grec, st), "ITS guide fill");
+    TRW_HIP(hipStreamSynchronize(st), "ITS guide records");  // the scratch is freed on return
+    return TRW_OK;
+}
+
 // static_initialization's table build (creeper.h:102-175) from per-edge weights w (device,
 // ignored when unit): `weights` is the TRW_WEIGHTS_* tag recorded for the table, `monotone`
 // says the weights are all finite and >= 0 (ITS search structures are built only then).
@@ -474,6 +510,7 @@ static int build_tables(trw_graph *g, int32_t sampling, const double *w, int32_t
         hipError_t e = trw::launch_its_build(dv, w, unit, g->order, g->build_hubs, g->prefix, st);
         if (e != hipSuccess) return hip_fail(e, "ITS build launch");
         int rc = build_its_trees(g, unit || monotone);
+        if (rc == TRW_OK) rc = build_its_guide(g, unit || monotone);
         (void)hipEventRecord(t1, st);
         if (rc != TRW_OK) return rc;
         TRW_HIP(hipStreamSynchronize(st), "ITS build");
@@ -671,14 +708,18 @@ int trw_graph_load_tables(trw_graph *g, const char *dir, int32_t sampling, int32
         const double *P = reinterpret_cast<const double *>(host.data());
         std::vector<int32_t> degree(V);
         if (V) TRW_HIP(hipMemcpy(degree.data(), g->degree, sizeof(int32_t) * V, hipMemcpyDeviceToHost), "copy degree");
-        bool monotone = true;
+        bool monotone = true, nonneg = true;
         size_t e = 0;
         for (size_t v = 0; v < V && monotone; ++v) {
+            if (degree[v] > 0) nonneg = nonneg && P[e] >= 0.0;
             for (int32_t j = 1; j < degree[v] && monotone; ++j) monotone = P[e + j - 1] <= P[e + j];
             e += static_cast<size_t>(degree[v]);
         }
         g->its_weights = weights;
         int rc = build_its_trees(g, monotone);
+        // Guide records also need sums >= 0: they end a walk at a vertex whose sums total zero
+        // without a draw, which for sums running below zero would not be the Move's pick.
+        if (rc == TRW_OK) rc = build_its_guide(g, monotone && nonneg);
         if (rc != TRW_OK) return rc;
     } else {
         if (!g->rs_max) TRW_HIP(dmalloc(&g->rs_max, V), "hipMalloc rejection maxima");

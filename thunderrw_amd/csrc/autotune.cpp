@@ -127,7 +127,8 @@ int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_wal
     if (form == TRW_FORM_AUTO) form = (ppr || metapath) ? TRW_FORM_QUEUE : TRW_FORM_FUSED;
     const bool fused_records = form == TRW_FORM_FUSED && params->walkers_per_thread == 0 &&
                                (params->execution == TRW_EXEC_UNIFORM ||
-                                (params->execution != TRW_EXEC_DYNAMIC && params->sampling == TRW_SM_ALIAS));
+                                (params->execution != TRW_EXEC_DYNAMIC &&
+                                 (params->sampling == TRW_SM_ALIAS || params->sampling == TRW_SM_ITS)));
     const bool its = params->execution != TRW_EXEC_UNIFORM && params->sampling == TRW_SM_ITS && !metapath;
     struct Knob {
         int32_t id;

@@ -61,6 +61,8 @@ struct DevGraph {
                                //       lerec[off(v) + j] = erec[off(v) + lbl_pos[off(v) + j]]
     const uint4 *its_lines;    // ITS search structures (128-byte lines, 8 uint4 each), or null
     const uint32_t *its_root;  // nv: first line of v's structure, kNoRange if v has none
+    const uint4 *grec;         // ITS guide records (2 uint4 per bucket), or null; see its_guide below
+    const uint2 *gvrec;        // nv: {first bucket, bucket count} per vertex ({kNoRange, 0}: weights sum to zero)
     const uint4 *nrec;         // ne: neighbour records with .w = the neighbour's membership structure root
     const uint4 *mlines;       // membership structures (128-byte lines), or null
     const uint32_t *mroot;     // nv: first line of v's membership structure, kNoRange if v has none
@@ -94,6 +96,19 @@ __host__ __device__ inline int32_t its_levels(int32_t nl, int64_t *nodes) {
     } while (nodes[L - 1] > 1 && L < kItsMaxLevels);
     return L;
 }
+
+// ITS guide records: static ITS as one gather per step.  A vertex of degree d gets m = c * d
+// buckets, c the least of 1..kGuideMaxC whose buckets all span at most two edges; bucket k
+// holds the draws u with floor(u * m / 2^32) == k, so a draw's bucket is one multiply.  The
+// walk's pick (the upper bound of real2(u) * total in the prefix sums) is non-decreasing in
+// u, so over a bucket it runs from pick(first u) to pick(last u); when those differ by at
+// most one, the bucket's 32-byte record decides every draw in it:
+//   {u threshold, lower pick's neighbour, upper pick's neighbour, 0}
+//   {lower's first bucket, lower's bucket count, upper's first bucket, upper's bucket count}
+// with the upper pick taken iff u >= threshold (the least u of the bucket whose pick is the
+// upper one, found with the Move's own expression).  A bucket spanning more edges has .w = 1
+// and the walk searches the prefix sums for that step.  Built only for non-decreasing sums.
+constexpr uint32_t kGuideMaxC = 4;
 
 // Membership structure of one vertex's sorted adjacency (Graph::is_neighbor, graph.h:180-199),
 // for vertices of degree > kMemberMin: 128-byte lines of 32 int32 keys.  Leaf i holds the
@@ -256,6 +271,12 @@ hipError_t launch_member_tree_starts(const DevGraph &g, uint32_t *root, uint64_t
                                      size_t &temp_bytes, hipStream_t stream);
 hipError_t launch_member_tree_fill(const DevGraph &g, const uint64_t *start, uint64_t lines_total, uint4 *lines,
                                    hipStream_t stream);
+// ITS guide records (tables.hip): flag and size pass plus scan (temp == null: scan storage
+// size only), then ranges and records.  start has nv + 1 entries.
+hipError_t launch_its_guide_starts(const DevGraph &g, const double *prefix, uint32_t *flag, uint32_t *count,
+                                   uint64_t *start, void *temp, size_t &temp_bytes, hipStream_t stream);
+hipError_t launch_its_guide_fill(const DevGraph &g, const double *prefix, const uint64_t *start, uint64_t buckets,
+                                 uint2 *gvrec, uint4 *grec, hipStream_t stream);
 hipError_t launch_member_records(const DevGraph &g, const uint32_t *root, uint4 *nrec, hipStream_t stream);
 hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
                             double *out, hipStream_t stream);

@@ -32,6 +32,8 @@ struct trw_graph {
     uint4 *lerec = nullptr;
     uint4 *its_lines = nullptr;    // ITS search structures (null: not built, or sums not non-decreasing)
     uint32_t *its_root = nullptr;  // nv: first line of each vertex's structure
+    uint4 *grec = nullptr;         // ITS guide records (2 per bucket), with the ITS table when its sums are monotone
+    uint2 *gvrec = nullptr;        // nv: {first bucket, bucket count}
     uint4 *nrec = nullptr;         // neighbour records with membership roots (built by the first Node2Vec walk)
     uint4 *mlines = nullptr;       // membership structures
     uint32_t *mroot = nullptr;     // nv: first line of each vertex's membership structure
@@ -70,6 +72,8 @@ struct trw_graph {
         g.lerec = lerec;
         g.its_lines = its_lines;
         g.its_root = its_root;
+        g.grec = grec;
+        g.gvrec = gvrec;
         g.nrec = nrec;
         g.mlines = mlines;
         g.mroot = mroot;

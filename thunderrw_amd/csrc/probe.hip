@@ -45,11 +45,13 @@ __global__ __launch_bounds__(kProbeBlock) void probe_gather_kernel(const uint4 *
         for (int c = 0; c < CHAINS; ++c)
 #pragma unroll
             for (int q = 0; q < RECORD; ++q) r[c][q] = table[static_cast<uint64_t>(idx[c]) * RECORD + q];
+        // Every word of the record feeds the next index, so the compiler keeps each load a
+        // whole 16-byte load (with words unused it narrows them, which is not a walk's read).
 #pragma unroll
         for (int c = 0; c < CHAINS; ++c) {
-            uint32_t h = r[c][0].x ^ r[c][0].z;
+            uint32_t h = 0;
 #pragma unroll
-            for (int q = 1; q < RECORD; ++q) h ^= r[c][q].y;
+            for (int q = 0; q < RECORD; ++q) h ^= r[c][q].x ^ mix32(r[c][q].y) ^ (r[c][q].z * 3u) ^ (r[c][q].w * 5u);
             idx[c] = (h ^ static_cast<uint32_t>(s)) % n;
             acc += r[c][0].y;
         }

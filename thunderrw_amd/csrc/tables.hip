@@ -5,6 +5,7 @@
 // serial prefix sums.  Vertices are visited in descending degree order so the
 // long serial chains of hub vertices start first.
 #include "engine.h"
+#include "thunderrw_device.h"
 
 #include <rocprim/device/device_scan.hpp>
 
@@ -874,6 +875,162 @@ hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, con
     if (g.nv <= 0) return hipSuccess;
     hipLaunchKernelGGL(rs_build_kernel, dim3(build_grid(g.nv, hubs)), dim3(64), 0, stream, g, w, unit, order, hubs,
                        out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- ITS guide records
+namespace {
+
+// The walk's ITS pick for draw u (inverse_transformation_sampling.h:259-286): the number
+// of prefix sums <= real2(u) * total, clamped to the last edge.  Same expression as the
+// Move's, so the records below reproduce its picks exactly.
+__device__ __forceinline__ int32_t guide_pick(const double *P, int32_t d, double total, uint32_t u) {
+    const int32_t c = trw_dev::upper_bound_branchless(P, d, trw_dev::real2(u) * total);
+    return c < d - 1 ? c : d - 1;
+}
+
+// First draw of bucket k of m: the least u with floor(u * m / 2^32) >= k (2^32 for k = m).
+__device__ __forceinline__ uint64_t guide_first_u(uint64_t k, uint64_t m) { return ((k << 32) + m - 1) / m; }
+
+// Pass c (1..kGuideMaxC), one thread per edge: at c buckets per edge, edge e's share is the
+// buckets [c * k, c * k + c) of its vertex (k = e's position); flag bit c - 1 of the vertex is
+// set when one of them reaches more than two edges.  Pass c > 1 runs only for vertices whose
+// pass c - 1 set its bit.
+__global__ void its_guide_flag_kernel(const int64_t *offset, int32_t nv, int64_t ne, const double *P, int32_t c,
+                                      uint32_t *flag) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int32_t lo = 0, hi = nv;  // the vertex v with offset[v] <= e < offset[v + 1]
+        while (hi - lo > 1) {
+            const int32_t m = (lo + hi) >> 1;
+            if (offset[m] <= e) lo = m;
+            else hi = m;
+        }
+        if (c > 1 && !(flag[lo] & (1u << (c - 2)))) continue;
+        const int64_t off = offset[lo];
+        const int32_t d = static_cast<int32_t>(offset[lo + 1] - off);
+        const double total = P[off + d - 1];
+        if (!(total > 0.0)) continue;
+        const uint64_t m = static_cast<uint64_t>(c) * static_cast<uint64_t>(d);
+        for (uint64_t k = static_cast<uint64_t>(c) * static_cast<uint64_t>(e - off), kend = k + c; k < kend; ++k) {
+            const int32_t a = guide_pick(P + off, d, total, static_cast<uint32_t>(guide_first_u(k, m)));
+            const int32_t b = guide_pick(P + off, d, total, static_cast<uint32_t>(guide_first_u(k + 1, m) - 1));
+            if (b - a > 1) {
+                atomicOr(flag + lo, 1u << (c - 1));
+                break;
+            }
+        }
+    }
+}
+
+// Bucket counts: c * d for the least c whose buckets all reach at most two edges (kGuideMaxC
+// when none does), none for a vertex of degree 0 or whose weights sum to zero.
+__global__ void its_guide_sizes_kernel(const int64_t *offset, int32_t nv, const double *P, const uint32_t *flag,
+                                       uint32_t *count, uint64_t *start) {
+    const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v == 0) start[0] = 0;
+    if (v >= nv) return;
+    const int64_t off = offset[v];
+    const int32_t d = static_cast<int32_t>(offset[v + 1] - off);
+    uint32_t c = 1;
+    while (c < kGuideMaxC && (flag[v] & (1u << (c - 1)))) ++c;
+    count[v] = (d > 0 && P[off + d - 1] > 0.0) ? static_cast<uint32_t>(d) * c : 0u;
+}
+
+// {first bucket, bucket count} per vertex; a vertex of positive degree whose weights sum to
+// zero is {kNoRange, 0} (the walk ends there without a draw, as the Move does).
+// Written without a short-circuit: hipcc (ROCm 7.2, -O3) compiled the `m == 0 && degree > 0`
+// form of this selection so that a degree-0 vertex also got kNoRange (the -1 was moved into
+// the m == 0 branch over the register holding start[v]).  Both loads here are unconditional
+// and the selection is one v_cndmask.
+__global__ void its_guide_ranges_kernel(const int64_t *offset, int32_t nv, const uint64_t *start, uint2 *gvrec) {
+    const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const uint64_t s0 = start[v], s1 = start[v + 1];
+    const int64_t d = offset[v + 1] - offset[v];
+    const uint32_t m = static_cast<uint32_t>(s1 - s0);
+    const uint32_t zero_total = static_cast<uint32_t>(m == 0u) & static_cast<uint32_t>(d > 0);
+    gvrec[v] = make_uint2(zero_total ? kNoRange : static_cast<uint32_t>(s0), m);
+}
+
+// One thread per bucket.  Bucket k of vertex v (m buckets) holds the draws u in
+// [guide_first_u(k), guide_first_u(k + 1)); the picks over it run from a = pick(first) to
+// b = pick(last).  When b <= a + 1 the record decides the step: the least u of the bucket
+// whose pick is b (found by bisection on P[a] <= real2(u) * total, which is pick(u) > a),
+// both candidates and their bucket ranges.  Otherwise the record says so (.w = 1) and the
+// walk searches the prefix sums.
+__global__ void its_guide_fill_kernel(const int64_t *offset, int32_t nv, const int32_t *adj, const double *P,
+                                      const uint64_t *start, const uint2 *gvrec, uint4 *grec, uint64_t buckets) {
+    for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < buckets;
+         t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        int32_t lo = 0, hi = nv;  // the last v with start[v] <= t: the vertex whose buckets hold t
+        while (hi - lo > 1) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (start[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const int32_t v = lo;
+        const int64_t off = offset[v];
+        const int32_t d = static_cast<int32_t>(offset[v + 1] - off);
+        const uint64_t m = start[v + 1] - start[v], k = t - start[v];
+        const double *Pv = P + off;
+        const double total = Pv[d - 1];
+        const uint32_t u_lo = static_cast<uint32_t>(guide_first_u(k, m));
+        const uint32_t u_hi = static_cast<uint32_t>(guide_first_u(k + 1, m) - 1);
+        const int32_t a = guide_pick(Pv, d, total, u_lo);
+        const int32_t b = guide_pick(Pv, d, total, u_hi);
+        if (b - a > 1) {
+            grec[2 * t] = make_uint4(0u, 0u, 0u, 1u);
+            grec[2 * t + 1] = make_uint4(0u, 0u, 0u, 0u);
+            continue;
+        }
+        uint32_t thr = 0u;
+        if (b > a) {
+            const double pa = Pv[a];
+            uint32_t l = u_lo, h = u_hi;  // pick(l) = a, pick(h) = b
+            while (h - l > 1) {
+                const uint32_t mid = l + ((h - l) >> 1);
+                if (pa <= trw_dev::real2(mid) * total) h = mid;
+                else l = mid;
+            }
+            thr = h;
+        }
+        const int32_t na = adj[off + a], nb = adj[off + b];
+        const uint2 ra = gvrec[na], rb = gvrec[nb];
+        grec[2 * t] = make_uint4(thr, static_cast<uint32_t>(na), static_cast<uint32_t>(nb), 0u);
+        grec[2 * t + 1] = make_uint4(ra.x, ra.y, rb.x, rb.y);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_its_guide_starts(const DevGraph &g, const double *prefix, uint32_t *flag, uint32_t *count,
+                                   uint64_t *start, void *temp, size_t &temp_bytes, hipStream_t stream) {
+    if (g.nv <= 0) return hipSuccess;
+    const size_t n = static_cast<size_t>(g.nv);
+    if (!temp)
+        return rocprim::inclusive_scan<rocprim::default_config, const uint32_t *, uint64_t *, rocprim::plus<uint64_t>,
+                                       uint64_t>(nullptr, temp_bytes, count, start + 1, n, rocprim::plus<uint64_t>(),
+                                                 stream);
+    launch_fill_u32(flag, 0u, g.nv, stream);
+    for (int32_t c = 1; c < static_cast<int32_t>(kGuideMaxC) && g.ne > 0; ++c)
+        hipLaunchKernelGGL(its_guide_flag_kernel, dim3(8192), dim3(256), 0, stream, g.offset, g.nv, g.ne, prefix, c,
+                           flag);
+    hipLaunchKernelGGL(its_guide_sizes_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv,
+                       prefix, flag, count, start);
+    return rocprim::inclusive_scan<rocprim::default_config, const uint32_t *, uint64_t *, rocprim::plus<uint64_t>,
+                                   uint64_t>(temp, temp_bytes, count, start + 1, n, rocprim::plus<uint64_t>(), stream);
+}
+
+hipError_t launch_its_guide_fill(const DevGraph &g, const double *prefix, const uint64_t *start, uint64_t buckets,
+                                 uint2 *gvrec, uint4 *grec, hipStream_t stream) {
+    if (g.nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(its_guide_ranges_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv,
+                       start, gvrec);
+    if (buckets == 0) return hipGetLastError();
+    const uint64_t blocks = std::min<uint64_t>((buckets + 255) / 256, uint64_t(1) << 20);
+    hipLaunchKernelGGL(its_guide_fill_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, g.offset,
+                       g.nv, g.adj, prefix, start, gvrec, grec, buckets);
     return hipGetLastError();
 }
 

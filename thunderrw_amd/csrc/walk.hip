@@ -336,6 +336,36 @@ __device__ __forceinline__ int32_t read_neighbor_member(const DevGraph &g, int64
     return read_neighbor(g, e, noff, ndeg);
 }
 
+// ITS guide range of the walker's current vertex: {first bucket, bucket count}, read once
+// per walk (the records carry the next vertex's).  A count of 0 is degree 0, or weights that
+// sum to zero when the first bucket is kNoRange.
+__device__ __forceinline__ void guide_range(const DevGraph &g, int32_t v, uint32_t roff, uint32_t rdeg, int64_t &off,
+                                            int32_t &m) {
+    if (rdeg == kNoRange) {
+        const uint2 r = g.gvrec[v];
+        roff = r.x;
+        rdeg = r.y;
+    }
+    off = static_cast<int64_t>(roff);
+    m = static_cast<int32_t>(rdeg);
+}
+
+// A guide bucket whose draws reach more than two edges: the Move's own search of the prefix
+// sums (inverse_transformation_sampling.h:259-286), then the chosen neighbour's bucket range.
+__device__ __forceinline__ int32_t guide_search(const DevGraph &g, int32_t cur, uint32_t u, uint32_t &noff,
+                                                uint32_t &ndeg) {
+    int64_t off;
+    int32_t d;
+    vertex_range(g, cur, off, d);
+    const double *P_ = g.prefix + off;
+    const int32_t pos = min(count_le(P_, d, real2(u) * P_[d - 1]), d - 1);
+    const int32_t next = g.adj[off + pos];
+    const uint2 r = g.gvrec[next];
+    noff = r.x;
+    ndeg = r.y;
+    return next;
+}
+
 // One 128-byte line of an ITS structure, as 8 16-byte loads issued together.  Named
 // members, not an array: a run-time choice among them stays a chain of selects, where
 // an indexed array would be moved to scratch memory.
@@ -845,10 +875,10 @@ __global__ __launch_bounds__(kMaxFusedBlock) __attribute__((amdgpu_waves_per_eu(
     walk_fused_body<kModeNode2Vec, REPLAY>(g, P, io, T, tile);
 }
 
-// Fused form, uniform and alias Moves, WPT walkers interleaved per lane.
+// Fused form, uniform, alias and static ITS Moves, WPT walkers interleaved per lane.
 //
-// With neighbour / alias walk records a step is one dependent gather (the chosen
-// entry, which also carries the next vertex's range).  Issuing the WPT walkers'
+// With neighbour / alias walk records / ITS guide records a step is one dependent
+// gather (the chosen entry, which also carries the next vertex's range).  Issuing the WPT walkers'
 // gathers back to back keeps WPT independent requests in flight per lane, on top
 // of the eight wavefronts per SIMD that occupancy provides.  Walker (t, j) of a
 // block is block_base + j * blockDim + t, so every per-j access is coalesced.
@@ -865,7 +895,7 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
     const int32_t L = P.length;
     WalkerState s[WPT];
     uint4 blk[WPT];
-    constexpr int kDraws = MODE == kModeUniform ? 1 : 2;  // move draws per step
+    constexpr int kDraws = MODE == kModeAlias ? 2 : 1;  // move draws per step
 #pragma unroll
     for (int j = 0; j < WPT; ++j) {
         const int32_t w = wbase + j * block + tid;
@@ -887,7 +917,9 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
             go[j] = s[j].st == kAlive;
             off[j] = 0;
             d[j] = 1;
-            if (go[j]) known_range(g, s[j].cur, s[j].coff, s[j].cdeg, off[j], d[j]);
+            if (!go[j]) continue;
+            if (MODE == kModeIts) guide_range(g, s[j].cur, s[j].coff, s[j].cdeg, off[j], d[j]);  // d: buckets
+            else known_range(g, s[j].cur, s[j].coff, s[j].cdeg, off[j], d[j]);
         }
 #pragma unroll
         for (int j = 0; j < WPT; ++j) {
@@ -896,12 +928,14 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
             u0[j] = u1[j] = 0;
             if (!go[j]) continue;
             if (d[j] <= 0) {
-                s[j].st = kDeadEnd;
+                // ITS: no buckets at a vertex of positive degree means its weights sum to zero,
+                // which ends the walk before a draw, as the Move's total check does.
+                s[j].st = (MODE == kModeIts && off[j] == static_cast<int64_t>(kNoRange)) ? kGatherTerminated : kDeadEnd;
                 go[j] = false;
                 continue;
             }
             if (REPLAY) go[j] = replay_take_prefetched<kDraws>(io, s[j], blk[j], u0[j], u1[j]);
-            else if (MODE == kModeUniform) go[j] = draw_one<REPLAY>(io, w, s[j], blk[j], u0[j]);
+            else if (MODE != kModeAlias) go[j] = draw_one<REPLAY>(io, w, s[j], blk[j], u0[j]);
             else go[j] = draw_pair<REPLAY>(io, w, s[j], step, 0u, blk[j], u0[j], u1[j]);
         }
         // Issue every walker's gather, then compute next steps' Philox blocks in
@@ -910,6 +944,14 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
 #pragma unroll
         for (int j = 0; j < WPT; ++j) {
             if (!go[j]) continue;
+            if (MODE == kModeIts) {
+                // The draw's bucket: floor(u * m / 2^32).
+                const int64_t e =
+                    off[j] + static_cast<int64_t>((static_cast<uint64_t>(u0[j]) * static_cast<uint32_t>(d[j])) >> 32);
+                ra[j] = g.grec[2 * e];
+                rb[j] = g.grec[2 * e + 1];
+                continue;
+            }
             const int64_t e = off[j] + (u0[j] % static_cast<uint32_t>(d[j]));
             if (MODE == kModeUniform) {
                 ra[j] = g.erec[e];
@@ -944,6 +986,15 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
                 next[j] = static_cast<int32_t>(ra[j].x);
                 noff = ra[j].y;
                 ndeg = ra[j].z;
+            } else if (MODE == kModeIts) {
+                if (ra[j].w != 0u) {
+                    next[j] = guide_search(g, s[j].cur, u0[j], noff, ndeg);
+                } else {
+                    const bool upper = u0[j] >= ra[j].x;
+                    next[j] = static_cast<int32_t>(upper ? ra[j].z : ra[j].y);
+                    noff = upper ? rb[j].z : rb[j].x;
+                    ndeg = upper ? rb[j].w : rb[j].y;
+                }
             } else {
                 if (static_cast<int32_t>(ra[j].z) < 0) {  // the vertex's weights sum to zero
                     if (REPLAY) s[j].k -= 2;
@@ -1290,10 +1341,14 @@ FusedShape fused_shape(int32_t mode) {
 
 template <bool REPLAY>
 hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream) {
-    const bool records = (p.mode == kModeUniform) ? g.erec != nullptr : g.arec != nullptr;
-    if (records && !p.ppr && (p.mode == kModeUniform || p.mode == kModeAlias)) {
+    const bool records = p.mode == kModeUniform ? g.erec != nullptr
+                         : p.mode == kModeAlias ? g.arec != nullptr
+                         : p.mode == kModeIts   ? g.grec != nullptr
+                                                : false;
+    if (records && !p.ppr) {
         if (p.mode == kModeUniform) fused_multi_launch<kModeUniform, REPLAY>(g, p, io, stream);
-        else fused_multi_launch<kModeAlias, REPLAY>(g, p, io, stream);
+        else if (p.mode == kModeAlias) fused_multi_launch<kModeAlias, REPLAY>(g, p, io, stream);
+        else fused_multi_launch<kModeIts, REPLAY>(g, p, io, stream);
         return hipGetLastError();
     }
     const FusedShape sh = fused_shape(p.mode);

@@ -730,3 +730,55 @@ def test_its_guide_walks_match_oracle_on_skewed_weights(gpu_ctx, seed):
             assert np.array_equal(outs[0][key], outs[1][key]), key
     finally:
         dg.close()
+
+
+def test_rejection_records_walk_like_the_oracle(gpu_ctx, tmp_path):
+    """Static rejection reads one 32-byte record per trial (the edge's weight beside the
+    neighbour's range and maximum weight) and carries the maximum into the next step.  Against
+    the oracle in replay mode, in every form: weights spanning four decades (many rejected
+    trials), vertices whose weights are all zero (the walk ends without a draw), unit weights,
+    a trial cap, and a table loaded from the reference's file instead of built."""
+    rng = np.random.default_rng(91)
+    g = graphs.chung_lu(4000, 20000, 2.0, 2, 1500, 92)
+    w = 10.0 ** rng.uniform(-2.0, 2.0, g.ne)
+    live = np.nonzero(g.degree > 0)[0]
+    for v in rng.choice(live, 40, replace=False):
+        w[g.offset[v]:g.offset[v + 1]] = 0.0
+    gg = graphs.CSR(g.degree, g.adj, w, g.label)
+    src = np.concatenate([live, rng.choice(live, 1500)]).astype(np.int32)
+    L, K = 25, 2500
+    draws_wm = ob.sfmt_stream(93, len(src) * K).reshape(len(src), K)
+    draws = np.ascontiguousarray(draws_wm.T)
+    dg = dgraph(gpu_ctx, gg)
+    try:
+        for weights, cap in (("edge", 0), ("unit", 0), ("edge", 3)):
+            ww = w if weights == "edge" else np.ones(gg.ne)
+            og = ob.Graph(gg, None, None, ww, ob.rs_build(gg, ww))
+            ref = ob.walk(og, "deepwalk", 1, 3, L, src, draws_wm, K, max_trials=cap)
+            if weights == "edge":
+                assert (ref["status"] == T._abi.WALKER_GATHER_TERMINATED).sum() > 0
+            if cap:
+                assert (ref["status"] == T._abi.WALKER_TRIAL_CAP).sum() > 0
+            for form in forms_for("deepwalk"):
+                prm = T.make_params("deepwalk", execution="static", sampling="rejection", length=L, rng="replay",
+                                    form=form, max_trials=cap, weights=weights)
+                r = T.walk(dg, prm, src, draws)
+                tag = (weights, cap, form)
+                for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                    assert np.array_equal(r[key], ref[key]), (tag, key)
+                assert r["steps"] == ref["steps"], tag
+        dg.build_tables("rejection", "edge")
+        dg.store_tables(str(tmp_path))
+        og = ob.Graph(gg, None, None, w, ob.rs_build(gg, w))
+        ref = ob.walk(og, "deepwalk", 1, 3, L, src, draws_wm, K)
+    finally:
+        dg.close()
+    dg2 = dgraph(gpu_ctx, gg)
+    try:
+        dg2.load_tables(str(tmp_path), "rejection", "edge")
+        r = T.walk(dg2, T.make_params("deepwalk", execution="static", sampling="rejection", length=L, rng="replay",
+                                      form="fused"), src, draws)
+        for key in ("paths", "lengths", "ends", "status", "draws_used"):
+            assert np.array_equal(r[key], ref[key]), ("loaded", key)
+    finally:
+        dg2.close()

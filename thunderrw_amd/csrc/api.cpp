@@ -266,6 +266,7 @@ static void graph_release(trw_graph *g) {
     dfree(g->prefix);
     dfree(g->rs_max);
     dfree(g->rs_custom);
+    dfree(g->rrec);
     dfree(g->counters);
     dfree(g->state);
     dfree(g->active);
@@ -522,6 +523,12 @@ static int build_tables(trw_graph *g, int32_t sampling, const double *w, int32_t
         (void)hipEventRecord(t1, st);
         if (e != hipSuccess) return hip_fail(e, "rejection build launch");
         TRW_HIP(hipStreamSynchronize(st), "rejection build");
+        if (g->vrec) {
+            // Rejection walk records: weight, neighbour range and neighbour maximum (32 bytes per edge).
+            if (!g->rrec) TRW_HIP(dmalloc(&g->rrec, 2 * E + 2), "hipMalloc rejection walk records");
+            TRW_HIP(trw::launch_rs_records(g->view(), w, unit, g->rrec, st), "rejection walk records");
+            TRW_HIP(hipStreamSynchronize(st), "rejection walk records");
+        }
         g->rs_weights = weights;
     }
     float ms = 0.f;
@@ -724,6 +731,13 @@ int trw_graph_load_tables(trw_graph *g, const char *dir, int32_t sampling, int32
     } else {
         if (!g->rs_max) TRW_HIP(dmalloc(&g->rs_max, V), "hipMalloc rejection maxima");
         if (V) TRW_HIP(hipMemcpy(g->rs_max, host.data(), bytes, hipMemcpyHostToDevice), "copy rejection maxima");
+        if (g->vrec && (weights == TRW_WEIGHTS_UNIT || g->weight)) {
+            if (!g->rrec) TRW_HIP(dmalloc(&g->rrec, 2 * E + 2), "hipMalloc rejection walk records");
+            TRW_HIP(trw::launch_rs_records(g->view(), g->weight, weights == TRW_WEIGHTS_UNIT, g->rrec, st),
+                    "rejection walk records");
+        } else {
+            dfree(g->rrec);
+        }
         g->rs_weights = weights;
     }
     TRW_HIP(hipStreamSynchronize(st), "load tables");

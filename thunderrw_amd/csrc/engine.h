@@ -61,6 +61,8 @@ struct DevGraph {
                                //       lerec[off(v) + j] = erec[off(v) + lbl_pos[off(v) + j]]
     const uint4 *its_lines;    // ITS search structures (128-byte lines, 8 uint4 each), or null
     const uint32_t *its_root;  // nv: first line of v's structure, kNoRange if v has none
+    const uint4 *rrec;         // 2*ne: rejection walk records {weight lo, weight hi, adj[e], first edge of adj[e]}
+                               //       {degree of adj[e], its maximum weight lo, hi, 0}, or null
     const uint4 *grec;         // ITS guide records (2 uint4 per bucket), or null; see its_guide below
     const uint2 *gvrec;        // nv: {first bucket, bucket count} per vertex ({kNoRange, 0}: weights sum to zero)
     const uint4 *nrec;         // ne: neighbour records with .w = the neighbour's membership structure root
@@ -189,6 +191,7 @@ struct WalkerState {
     uint32_t cidx;    // current vertex: root of its ITS search tree (ITS) or membership structure
                       // (Node2Vec); kNoRange: not known / none
     uint32_t pidx;    // previous vertex: root of its membership structure (Node2Vec)
+    double cmax;      // rejection: the current vertex's maximum weight (from a rejection record), < 0: not known
 };
 constexpr uint32_t kNoRange = 0xffffffffu;
 
@@ -253,6 +256,8 @@ hipError_t launch_alias_build(const DevGraph &g, const double *w, int32_t unit, 
                               AliasEntry *scratch, AliasSlot *out, unsigned int *zero_count, hipStream_t stream);
 hipError_t launch_edge_records(const DevGraph &g, uint4 *erec, hipStream_t stream);
 hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stream);
+// Rejection walk records from the weights the maxima were built from (w, or 1.0 when unit).
+hipError_t launch_rs_records(const DevGraph &g, const double *w, int32_t unit, uint4 *rrec, hipStream_t stream);
 hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lerec, hipStream_t stream);
 // ITS search structures: line counts per vertex into root (then the caller turns them
 // into offsets), the structures themselves, and the roots copied into the neighbour

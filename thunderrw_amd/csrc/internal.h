@@ -32,6 +32,7 @@ struct trw_graph {
     uint4 *lerec = nullptr;
     uint4 *its_lines = nullptr;    // ITS search structures (null: not built, or sums not non-decreasing)
     uint32_t *its_root = nullptr;  // nv: first line of each vertex's structure
+    uint4 *rrec = nullptr;         // rejection walk records (2 per edge), with the rejection table
     uint4 *grec = nullptr;         // ITS guide records (2 per bucket), with the ITS table when its sums are monotone
     uint2 *gvrec = nullptr;        // nv: {first bucket, bucket count}
     uint4 *nrec = nullptr;         // neighbour records with membership roots (built by the first Node2Vec walk)
@@ -72,6 +73,7 @@ struct trw_graph {
         g.lerec = lerec;
         g.its_lines = its_lines;
         g.its_root = its_root;
+        g.rrec = rrec;
         g.grec = grec;
         g.gvrec = gvrec;
         g.nrec = nrec;

@@ -754,6 +754,22 @@ __global__ void alias_records_kernel(const AliasSlot *alias, const uint2 *vrec, 
     }
 }
 
+// Rejection walk records: an edge's weight beside its neighbour's range and maximum weight, so
+// a rejection trial is one read and the accepted one also carries the next step's range.
+__global__ void rs_records_kernel(const int32_t *adj, const uint2 *vrec, const double *rs_max, const double *w,
+                                  int32_t unit, int64_t ne, uint4 *rrec) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t nb = adj[e];
+        const unsigned long long wb = static_cast<unsigned long long>(__double_as_longlong(unit ? 1.0 : w[e]));
+        const unsigned long long mb = static_cast<unsigned long long>(__double_as_longlong(rs_max[nb]));
+        const uint2 r = vrec[nb];
+        rrec[2 * e] = make_uint4(static_cast<uint32_t>(wb), static_cast<uint32_t>(wb >> 32), static_cast<uint32_t>(nb),
+                                 r.x);
+        rrec[2 * e + 1] = make_uint4(r.y, static_cast<uint32_t>(mb), static_cast<uint32_t>(mb >> 32), 0u);
+    }
+}
+
 inline unsigned grid_for(int64_t n) { return static_cast<unsigned>((n + kBuildBlock - 1) / kBuildBlock); }
 
 }  // namespace
@@ -779,6 +795,13 @@ hipError_t launch_edge_records(const DevGraph &g, uint4 *erec, hipStream_t strea
 hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stream) {
     if (g.ne <= 0) return hipSuccess;
     hipLaunchKernelGGL(alias_records_kernel, dim3(8192), dim3(256), 0, stream, g.alias, g.vrec, g.ne, arec);
+    return hipGetLastError();
+}
+
+hipError_t launch_rs_records(const DevGraph &g, const double *w, int32_t unit, uint4 *rrec, hipStream_t stream) {
+    if (g.ne <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rs_records_kernel, dim3(8192), dim3(256), 0, stream, g.adj, g.vrec, g.rs_max, w, unit, g.ne,
+                       rrec);
     return hipGetLastError();
 }
 

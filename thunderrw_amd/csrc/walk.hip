@@ -491,6 +491,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     const uint32_t ud = static_cast<uint32_t>(d);
     int32_t next;
     uint32_t noff = 0u, ndeg = kNoRange, nidx = kNoRange;
+    double nmax = -1.0;  // rejection: the next vertex's maximum weight, when a record gave it
     uint4 nblk = blk;
 
     if (MODE == kModeUniform) {
@@ -580,13 +581,16 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             next = read_neighbor_its(g, off + pos, noff, ndeg, nidx);
         }
     } else if (MODE == kModeRejection) {
-        // rejection_sampling.h:334-366: redraw (u % d, real2 * max) while r > w[pos].
-        const double m = g.rs_max[cur];
+        // rejection_sampling.h:334-366: redraw (u % d, real2 * max) while r > w[pos].  With
+        // rejection records a trial is one read (the edge's weight), the accepted one also
+        // yields the neighbour's range and maximum, and the walker carries that maximum.
+        const double m = s.cmax >= 0.0 ? s.cmax : g.rs_max[cur];
         if (m == 0.0) {
             s.st = kGatherTerminated;
             return false;
         }
         uint32_t pos = 0;
+        uint4 ra = make_uint4(0u, 0u, 0u, 0u), rb = ra;
         for (uint32_t trial = 0;; ++trial) {
             if (P.max_trials > 0 && trial >= static_cast<uint32_t>(P.max_trials)) {
                 s.st = kTrialCap;
@@ -596,10 +600,24 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             if (!draw_pair<REPLAY>(io, w, s, step, trial, blk, ua, ub)) return false;
             pos = ua % ud;
             const double r = real2(ub) * m;
-            const double wt = P.unit_weights ? 1.0 : g.weight[off + pos];
+            double wt;
+            if (g.rrec) {
+                ra = g.rrec[2 * (off + pos)];
+                rb = g.rrec[2 * (off + pos) + 1];
+                wt = __hiloint2double(static_cast<int>(ra.y), static_cast<int>(ra.x));
+            } else {
+                wt = P.unit_weights ? 1.0 : g.weight[off + pos];
+            }
             if (!(r > wt)) break;
         }
-        next = read_neighbor(g, off + pos, noff, ndeg);
+        if (g.rrec) {
+            next = static_cast<int32_t>(ra.z);
+            noff = ra.w;
+            ndeg = rb.x;
+            nmax = __hiloint2double(static_cast<int>(rb.z), static_cast<int>(rb.y));
+        } else {
+            next = read_neighbor(g, off + pos, noff, ndeg);
+        }
         if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
     } else if (MODE == kModeNode2Vec) {
         // rejection_sampling.h:368-402 with Node2Vec::weight (node2vec.cpp:24-38):
@@ -674,6 +692,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     }
 
     finish_move(g, P, s, cur, off, d, next, noff, ndeg, nidx);
+    if (MODE == kModeRejection) s.cmax = nmax;
     const uint4 this_blk = blk;
     blk = nblk;
     if (P.ppr) {
@@ -745,7 +764,7 @@ __device__ __forceinline__ void write_outputs(const BatchIO &io, int32_t w, cons
 }
 
 __device__ __forceinline__ WalkerState empty_walker() {
-    return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange, 0u, 0u, kNoRange, kNoRange};
+    return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange, 0u, 0u, kNoRange, kNoRange, -1.0};
 }
 
 __device__ __forceinline__ void init_walker(const DevGraph &g, const BatchIO &io, int32_t w, WalkerState &s) {

@@ -288,19 +288,22 @@ def c3_node2vec(ctx, args):
 
 
 def its_walk(ctx, dg, batch, length, per_launch, args):
-    """DeepWalk with static ITS sampling on the same graph and walkers (prefix sums and
-    search trees built on the device, tree threshold autotuned)."""
+    """DeepWalk with static ITS and with static rejection sampling on the same graph and walkers
+    (paths written).  ITS: prefix sums, search trees and guide records built on the device (a
+    fused step is one guide-record read); rejection: maxima and rejection records (a trial is
+    one record read).  Launch knobs autotuned."""
     out = {}
-    dg.build_tables("its", "edge")
-    out["its_table_build_ms"] = round(dg.last_build_ms(), 3)
-    prm_i = T.make_params("deepwalk", execution="static", sampling="its", length=length, rng="philox", seed=7,
-                          form="fused")
-    tune = dg.autotune(prm_i, sample_walkers=1 << 21, seed=7)
-    out["its_autotune"] = tune["chosen"]
-    batch.reset()
-    ms_i = time_launches(ctx, dg, prm_i, batch.io(), 1, args.steps)
-    out["its_steps_per_s"] = per_launch / (ms_i / 1e3 / args.steps)
-    log("fused ITS (philox, tuned %s): %.3e steps/s" % (tune["chosen"], out["its_steps_per_s"]))
+    for sm, key in (("its", "its"), ("rejection", "rejection")):
+        dg.build_tables(sm, "edge")
+        out[key + "_table_build_ms"] = round(dg.last_build_ms(), 3)
+        prm = T.make_params("deepwalk", execution="static", sampling=sm, length=length, rng="philox", seed=7,
+                            form="fused")
+        tune = dg.autotune(prm, sample_walkers=1 << 21, seed=7)
+        out[key + "_autotune"] = tune["chosen"]
+        batch.reset()
+        ms = time_launches(ctx, dg, prm, batch.io(), 1, args.steps)
+        out[key + "_steps_per_s"] = per_launch / (ms / 1e3 / args.steps)
+        log("fused %s (philox, tuned %s): %.3e steps/s" % (sm, tune["chosen"], out[key + "_steps_per_s"]))
     return out
 
 

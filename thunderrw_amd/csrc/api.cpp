@@ -364,13 +364,13 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
             TRW_HIP(hipMemcpy(g->lbl_sorted, ls.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy label index");
             TRW_HIP(hipMemcpy(g->lbl_pos, lp.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice), "copy label index");
         }
-        // Small label alphabets (all labels in [0, 8)): per-vertex label starts and
-        // label-ordered neighbour records make a MetaPath move two gathers.
+        // Small label alphabets (all labels in [0, 8)): per-vertex label starts and label-ordered
+        // neighbour records carrying the neighbour's label starts make a MetaPath move one gather.
         bool small = g->erec != nullptr;
         for (size_t e = 0; e < E && small; ++e) small = edge_label[e] >= 0 && edge_label[e] < 8;
         if (small) {
             TRW_HIP(dmalloc(&g->lstart, 8 * V + 8), "hipMalloc label starts");
-            TRW_HIP(dmalloc(&g->lerec, E + 1), "hipMalloc label records");
+            TRW_HIP(dmalloc(&g->lerec, 4 * E + 4), "hipMalloc label records");
             TRW_HIP(trw::launch_label_records(g->view(), g->lstart, g->lerec, ctx->stream), "label records");
             TRW_HIP(hipStreamSynchronize(ctx->stream), "label records");
         }

@@ -57,8 +57,9 @@ struct DevGraph {
                                //       {first's edge, first's degree, second's edge, second's degree}
     const uint32_t *lstart;    // 8*nv when every label is in [0, 8): lstart[8v + k] = neighbours of v
                                //       with label < k (the start of label k in label order), else null
-    const uint4 *lerec;        // ne with lstart: neighbour records in label order,
-                               //       lerec[off(v) + j] = erec[off(v) + lbl_pos[off(v) + j]]
+    const uint4 *lerec;        // 4*ne with lstart: 64-byte neighbour records in label order, entry j of v:
+                               //       lerec[4 (off(v) + j)] = erec[off(v) + lbl_pos[off(v) + j]], then the
+                               //       neighbour's 8 label starts (2 uint4), then 16 bytes of zero
     const uint4 *its_lines;    // ITS search structures (128-byte lines, 8 uint4 each), or null
     const uint32_t *its_root;  // nv: first line of v's structure, kNoRange if v has none
     const uint4 *rrec;         // 2*ne: rejection walk records {weight lo, weight hi, adj[e], first edge of adj[e]}
@@ -184,8 +185,9 @@ struct WalkerState {
     uint32_t kt;    // replay termination-draw cursor
     uint32_t coff;  // current vertex: first edge and degree (cdeg == kNoRange: not known yet)
     uint32_t cdeg;
-    uint32_t poff;  // previous vertex: first edge and degree (Node2Vec membership test)
-    uint32_t pdeg;
+    uint32_t poff;  // previous vertex: first edge and degree (Node2Vec membership test); MetaPath: the
+    uint32_t pdeg;  //   current vertex's label-order range of this step's label, from the record that
+                    //   brought the walker here (pdeg == kNoRange: not known)
     uint32_t gid_lo;  // global walker id: the Philox counter's walker words
     uint32_t gid_hi;
     uint32_t cidx;    // current vertex: root of its ITS search tree (ITS) or membership structure

@@ -550,10 +550,11 @@ __global__ void label_start_kernel(const int64_t *offset, const int32_t *lbl_sor
     }
 }
 
-// Neighbour records in label order: entry j of v's label-sorted segment is the
-// neighbour record of segment position lbl_pos[off + j].
+// Neighbour records in label order, 64 bytes each: entry j of v's label-sorted segment is the
+// neighbour record of segment position lbl_pos[off + j], followed by that neighbour's label
+// starts, so the move that picks it also has the next step's label range.
 __global__ void label_records_kernel(const int64_t *offset, int32_t nv, const int32_t *lbl_pos, const uint4 *erec,
-                                     int64_t ne, uint4 *lerec) {
+                                     const uint32_t *lstart, int64_t ne, uint4 *lerec) {
     for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
          e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         // The vertex whose segment holds e: the last v with offset[v] <= e.
@@ -563,7 +564,12 @@ __global__ void label_records_kernel(const int64_t *offset, int32_t nv, const in
             if (offset[m] <= e) lo = m;
             else hi = m;
         }
-        lerec[e] = erec[offset[lo] + lbl_pos[e]];
+        const uint4 r = erec[offset[lo] + lbl_pos[e]];
+        const uint4 *ls = reinterpret_cast<const uint4 *>(lstart + 8 * static_cast<int64_t>(r.x));
+        lerec[4 * e] = r;
+        lerec[4 * e + 1] = ls[0];
+        lerec[4 * e + 2] = ls[1];
+        lerec[4 * e + 3] = make_uint4(0u, 0u, 0u, 0u);
     }
 }
 
@@ -811,7 +817,7 @@ hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lere
                        g.nv, lstart);
     if (g.ne > 0)
         hipLaunchKernelGGL(label_records_kernel, dim3(8192), dim3(256), 0, stream, g.offset, g.nv, g.lbl_pos, g.erec,
-                           g.ne, lerec);
+                           lstart, g.ne, lerec);
     return hipGetLastError();
 }
 

@@ -492,6 +492,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     int32_t next;
     uint32_t noff = 0u, ndeg = kNoRange, nidx = kNoRange;
     double nmax = -1.0;  // rejection: the next vertex's maximum weight, when a record gave it
+    uint32_t nlo = 0u, nhi = kNoRange;  // MetaPath: the next step's label range at the next vertex
     uint4 nblk = blk;
 
     if (MODE == kModeUniform) {
@@ -659,9 +660,13 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         const int32_t target = P.schema[(s.len - 1) % P.schema_length];
         int32_t a, K;
         if (g.lerec) {
-            // Matches of `target` are [lstart[target], lstart[target + 1]) in label order.
+            // Matches of `target` are [lstart[target], lstart[target + 1]) in label order; after
+            // the first move the record that brought the walker here carried them.
             uint32_t lo = 0u, hi = 0u;
-            if (target >= 0 && target < 8) {
+            if (s.pdeg != kNoRange) {
+                lo = s.poff;
+                hi = s.pdeg;
+            } else if (target >= 0 && target < 8) {
                 const uint32_t *L = g.lstart + 8 * static_cast<int64_t>(cur);
                 lo = L[target];
                 hi = target < 7 ? L[target + 1] : ud;
@@ -681,10 +686,19 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
         const int32_t t = static_cast<int32_t>(real2(u) * static_cast<double>(K));
         if (g.lerec) {
-            const uint4 r = g.lerec[off + a + t];
+            const uint4 *R = g.lerec + 4 * (off + a + t);
+            const uint4 r = R[0];
             next = static_cast<int32_t>(r.x);
             noff = r.y;
             ndeg = r.z;
+            // The next step's label range at `next`, from its label starts in the same record.
+            const int32_t nt = P.schema[s.len % P.schema_length];
+            nlo = nhi = 0u;
+            if (nt >= 0 && nt < 8) {
+                const uint32_t *NL = reinterpret_cast<const uint32_t *>(R + 1);
+                nlo = NL[nt];
+                nhi = nt < 7 ? NL[nt + 1] : ndeg;
+            }
         } else {
             next = read_neighbor(g, off + g.lbl_pos[off + a + t], noff, ndeg);
         }
@@ -693,6 +707,10 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
 
     finish_move(g, P, s, cur, off, d, next, noff, ndeg, nidx);
     if (MODE == kModeRejection) s.cmax = nmax;
+    if (MODE == kModeMetaPath) {  // the previous vertex's range is Node2Vec's alone
+        s.poff = nlo;
+        s.pdeg = nhi;
+    }
     const uint4 this_blk = blk;
     blk = nblk;
     if (P.ppr) {

@@ -545,10 +545,10 @@ def test_fused_path_tiles_for_every_walk_length(gpu_ctx):
 
 
 def test_node2vec_membership_structures_match_oracle(gpu_ctx):
-    """Node2Vec's is_neighbor tests run through 32-key membership trees for adjacencies longer
-    than 32 (one, two and three tree levels here: hubs of degree 100, 3000 and 40000) and by
-    binary search below; on an undirected and on a directed graph (where which endpoint's list
-    is searched decides the answer) the walks equal the oracle's."""
+    """Node2Vec's is_neighbor tests run through hashed membership tables (128-byte buckets) for
+    adjacencies longer than 32 (8, 256 and 4096 buckets here: hubs of degree 100, 3000 and 40000)
+    and by binary search below; on an undirected and on a directed graph (where which endpoint's
+    list is searched decides the answer) the walks equal the oracle's."""
     rng = np.random.default_rng(91)
     n = 60000
     parts = [rng.integers(0, n, (150000, 2))]
@@ -782,3 +782,44 @@ def test_rejection_records_walk_like_the_oracle(gpu_ctx, tmp_path):
             assert np.array_equal(r[key], ref[key]), ("loaded", key)
     finally:
         dg2.close()
+
+
+def _member_bucket(u, bits):
+    return ((u * 0x9E3779B1) & 0xFFFFFFFF) >> (32 - bits)
+
+
+def test_node2vec_membership_bucket_overflow_matches_oracle(gpu_ctx):
+    """A vertex of degree 60 (four buckets of 32 slots) whose neighbours all hash to its first
+    bucket: 32 of them fill it and 28 overflow into the next.  Walks from it and around a clique
+    of hubs test hubs that are its neighbours (in either bucket) and hubs that are not against its
+    table (it is the smaller endpoint), in replay mode against the oracle."""
+    rng = np.random.default_rng(101)
+    n = 20000
+    same = [u for u in range(1, n) if _member_bucket(u, 2) == 0]
+    other = [u for u in range(1, n) if _member_bucket(u, 2) != 0]
+    nb0 = np.array(same[:60])
+    hubs = np.concatenate([nb0[:6], np.array(other[:6])])
+    parts = [np.stack([np.zeros(60, np.int64), nb0], 1)]
+    for h in hubs:
+        far = rng.choice(np.arange(1, n), 300, replace=False)
+        parts.append(np.stack([np.full(300, h), far], 1))
+    parts.append(np.array([(a, b) for i, a in enumerate(hubs) for b in hubs[i + 1:]]))
+    bg = rng.integers(1, n, (40000, 2))
+    parts.append(bg)
+    g = graphs.csr_from_undirected(n, np.concatenate(parts), rng)
+    assert g.degree[0] == 60
+    dg = dgraph(gpu_ctx, g)
+    try:
+        src = np.concatenate([np.zeros(20000, np.int32), np.repeat(hubs.astype(np.int32), 500)])
+        for p, q in ((2.0, 0.5), (1.0, 4.0)):
+            case = {"app": "node2vec", "execution": 2, "sampling": 4, "length": 12,
+                    "extra": {"-p": str(p), "-q": str(q)}}
+            K = 64 * 12
+            ref = oracle_walk(g, case, src, K, 1, 102, 103)
+            draws = T.replay_fill(102, len(src), K)
+            for form in ("fused", "queue"):
+                r = T.walk(dg, case_params(case, form), src, draws)
+                for key in ("paths", "lengths", "status", "draws_used"):
+                    assert np.array_equal(r[key], ref[key]), (p, q, form, key)
+    finally:
+        dg.close()

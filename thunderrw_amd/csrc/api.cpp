@@ -536,9 +536,10 @@ static int build_tables(trw_graph *g, int32_t sampling, const double *w, int32_t
     return TRW_OK;
 }
 
-// Membership structures over every adjacency of degree > kMemberMin, and neighbour records
-// carrying each neighbour's root, for Node2Vec's is_neighbor tests (built by the first Node2Vec
-// walk; a graph without neighbour records, 2^32 edges or more, keeps the binary search).
+// Membership tables (hash sets in 128-byte buckets) over every adjacency of degree > kMemberMin,
+// and neighbour records carrying each neighbour's root, for Node2Vec's is_neighbor tests (built
+// by the first Node2Vec walk; a graph without neighbour records, 2^32 edges or more, keeps the
+// binary search).
 static int build_member_trees(trw_graph *g) {
     if (g->nrec || g->nv == 0 || !g->erec) return TRW_OK;
     hipStream_t st = g->ctx->stream;
@@ -569,7 +570,9 @@ static int build_member_trees(trw_graph *g) {
     if (total >= trw::kNoRange) return TRW_OK;  // line indices would overflow 32 bits: binary search
     if (total) TRW_HIP(dmalloc(&lines, static_cast<size_t>(total) * 8), "hipMalloc membership structures");
     TRW_HIP(dmalloc(&nrec, static_cast<size_t>(g->ne) + 1), "hipMalloc Node2Vec neighbour records");
-    TRW_HIP(trw::launch_member_tree_fill(g->view(), start, total, lines, st), "membership structure fill");
+    uint32_t *fill_count = nullptr;
+    TRW_HIP(sc.get(&fill_count, static_cast<size_t>(total)), "hipMalloc membership bucket counters");
+    TRW_HIP(trw::launch_member_tree_fill(g->view(), start, total, lines, fill_count, st), "membership table fill");
     TRW_HIP(trw::launch_member_records(g->view(), root, nrec, st), "Node2Vec neighbour records");
     TRW_HIP(hipStreamSynchronize(st), "membership structures");
     g->mroot = root;

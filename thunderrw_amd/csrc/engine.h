@@ -113,29 +113,24 @@ __host__ __device__ inline int32_t its_levels(int32_t nl, int64_t *nodes) {
 // and the walk searches the prefix sums for that step.  Built only for non-decreasing sums.
 constexpr uint32_t kGuideMaxC = 4;
 
-// Membership structure of one vertex's sorted adjacency (Graph::is_neighbor, graph.h:180-199),
-// for vertices of degree > kMemberMin: 128-byte lines of 32 int32 keys.  Leaf i holds the
-// neighbours [32i, 32i + 32) (INT32_MAX past the last); above more than one leaf, a B+-tree of
-// 32-key nodes (level 1: the last neighbour of each leaf; higher levels: the last key of each
-// node below; INT32_MAX past the real keys), stored root first, then the leaves.  A search
-// reads one node per level and one leaf.
-constexpr int kMemberFan = 32;
+// Membership table of one vertex's adjacency (for Graph::is_neighbor, graph.h:180-199), for
+// vertices of degree d > kMemberMin: a hash set of its neighbours in nb = 2^member_bucket_bits(d)
+// buckets of 128 bytes (32 int32 slots), about 8 to 16 keys per bucket.  Neighbour u lives in
+// bucket member_hash(u) or, when that one is full, the next bucket (cyclically) with room; a
+// bucket's keys fill its slots from the first, the other slots hold kMemberEmpty.  A test reads
+// u's bucket (its second half only when the first is full of other keys) and moves to the next
+// bucket only when this one is full without u, so it is almost always one 128-byte line.
+constexpr int kMemberSlots = 32;
 constexpr int kMemberMin = 32;
-constexpr int kMemberMaxLevels = 8;
+constexpr uint32_t kMemberEmpty = 0xffffffffu;
 
-__host__ __device__ inline int32_t member_leaves(int32_t d) { return (d + kMemberFan - 1) / kMemberFan; }
+__host__ __device__ inline uint32_t member_bucket_bits(int32_t d) {
+    const uint32_t need = static_cast<uint32_t>((d + 15) / 16);  // >= 3 for d > kMemberMin
+    return need <= 1u ? 0u : 32u - static_cast<uint32_t>(__builtin_clz(need - 1u));
+}
 
-// Tree node counts per level (level 1 first); returns the number of levels (0 for one leaf).
-__host__ __device__ inline int32_t member_levels(int32_t nl, int64_t *nodes) {
-    if (nl <= 1) return 0;
-    int32_t L = 0;
-    int64_t keys = nl;
-    do {
-        nodes[L] = (keys + kMemberFan - 1) / kMemberFan;
-        keys = nodes[L];
-        ++L;
-    } while (nodes[L - 1] > 1 && L < kMemberMaxLevels);
-    return L;
+__host__ __device__ inline uint32_t member_hash(int32_t u, uint32_t bits) {
+    return (static_cast<uint32_t>(u) * 0x9E3779B1u) >> (32u - bits);  // bits >= 2
 }
 
 // Per walker kind launch parameters (app functor + g_para).
@@ -272,12 +267,13 @@ hipError_t launch_its_tree_starts(const DevGraph &g, uint32_t *root, uint64_t *s
 hipError_t launch_its_tree_fill(const DevGraph &g, const double *prefix, const uint64_t *start, uint64_t lines_total,
                                 const uint32_t *root, uint4 *lines, hipStream_t stream);
 hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *root, uint4 *erec, hipStream_t stream);
-// Membership structures: as the ITS ones (line counts into root, then offsets by a scan, then
-// one thread per 16-byte piece), plus the neighbour records carrying each neighbour's root.
+// Membership tables: bucket counts into root, then offsets by a scan (as the ITS structures),
+// then the insertion of every neighbour (fill_count: one zeroed counter per bucket, scratch),
+// plus the neighbour records carrying each neighbour's root.
 hipError_t launch_member_tree_starts(const DevGraph &g, uint32_t *root, uint64_t *start, void *temp,
                                      size_t &temp_bytes, hipStream_t stream);
 hipError_t launch_member_tree_fill(const DevGraph &g, const uint64_t *start, uint64_t lines_total, uint4 *lines,
-                                   hipStream_t stream);
+                                   uint32_t *fill_count, hipStream_t stream);
 // ITS guide records (tables.hip): flag and size pass plus scan (temp == null: scan storage
 // size only), then ranges and records.  start has nv + 1 entries.
 hipError_t launch_its_guide_starts(const DevGraph &g, const double *prefix, uint32_t *flag, uint32_t *count,

@@ -668,69 +668,44 @@ __global__ void its_tree_roots_kernel(const int32_t *adj, int64_t ne, const uint
         erec[e].w = root[adj[e]];
 }
 
-// Membership structures.  Sizes: one thread per vertex writes its line count (0 for degree
-// <= kMemberMin: such a list spans at most two lines and is binary-searched in place).
+// Membership tables.  Sizes: one thread per vertex writes its bucket count (none for degree <=
+// kMemberMin, whose tests binary-search the adjacency itself).
 __global__ void member_tree_sizes_kernel(const int64_t *offset, int32_t nv, uint32_t *count, uint64_t *start) {
     const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v == 0) start[0] = 0;
     if (v >= nv) return;
     const int32_t d = static_cast<int32_t>(offset[v + 1] - offset[v]);
-    if (d <= kMemberMin) {
-        count[v] = 0;
-        return;
-    }
-    const int32_t nl = member_leaves(d);
-    int64_t nodes[kMemberMaxLevels];
-    const int32_t L = member_levels(nl, nodes);
-    int64_t total = nl;
-    for (int32_t k = 0; k < L; ++k) total += nodes[k];
-    count[v] = static_cast<uint32_t>(total);
+    count[v] = d > kMemberMin ? 1u << member_bucket_bits(d) : 0u;
 }
 
-// One thread per 16-byte piece (4 keys) of every line.  v's lines are its tree levels, root
-// level first, then its leaves.  Key j of tree level k (0 = the level over the leaves) is the
-// last neighbour under child j -- adjacency entry min((j + 1) * 32^(k + 1), d) - 1 -- when j
-// is below the level's real key count (nl at level 0, the node count of level k - 1 above
-// that), INT32_MAX otherwise.  Leaf i holds entries [32i, 32i + 32), INT32_MAX past d.
+// One thread per edge e of a vertex with a table: its neighbour takes the next free slot of its
+// bucket, or of the next bucket with room (fill_count[b] counts the claims on bucket b, so slots
+// [0, min(count, 32)) end up holding keys; the slots were set to kMemberEmpty beforehand).  The
+// order of the keys inside a bucket depends on the order of the claims; which keys a bucket holds
+// does not matter to a test, which looks at all of them.
 __global__ void member_tree_fill_kernel(const int64_t *offset, const uint64_t *start, int32_t nv, const int32_t *adj,
-                                        uint4 *lines, uint64_t pieces) {
-    constexpr uint32_t kPad = 0x7fffffffu;
-    for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < pieces;
-         t += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-        const uint64_t l = t >> 3;
-        const int32_t q = static_cast<int32_t>(t & 7);
-        int32_t lo = 0, hi = nv;  // the owner of line l: the last v with start[v] <= l
+                                        int64_t ne, uint32_t *slots, uint32_t *fill_count) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        int32_t lo = 0, hi = nv;  // the vertex v with offset[v] <= e < offset[v + 1]
         while (hi - lo > 1) {
             const int32_t m = (lo + hi) >> 1;
-            if (start[m] <= l) lo = m;
+            if (offset[m] <= e) lo = m;
             else hi = m;
         }
-        const int64_t off = offset[lo];
-        const int32_t d = static_cast<int32_t>(offset[lo + 1] - off);
-        const int32_t nl = member_leaves(d);
-        int64_t nodes[kMemberMaxLevels];
-        const int32_t L = member_levels(nl, nodes);
-        int64_t tree = 0;
-        for (int32_t k = 0; k < L; ++k) tree += nodes[k];
-        const int64_t r = static_cast<int64_t>(l - start[lo]);
-        uint32_t key[4];
-        if (r < tree) {
-            int32_t k = L - 1;
-            int64_t b = 0;
-            while (r >= b + nodes[k]) b += nodes[k--];
-            const int64_t real = k == 0 ? nl : nodes[k - 1];
-            for (int32_t h = 0; h < 4; ++h) {
-                const int64_t j = (r - b) * kMemberFan + 4 * q + h;
-                const int64_t last = (j + 1) << (5 * (k + 1));
-                key[h] = j < real ? static_cast<uint32_t>(adj[off + (last < d ? last : d) - 1]) : kPad;
-            }
-        } else {
-            for (int32_t h = 0; h < 4; ++h) {
-                const int64_t j = (r - tree) * kMemberFan + 4 * q + h;
-                key[h] = j < d ? static_cast<uint32_t>(adj[off + j]) : kPad;
+        const int32_t d = static_cast<int32_t>(offset[lo + 1] - offset[lo]);
+        if (d <= kMemberMin) continue;
+        const uint32_t bits = member_bucket_bits(d), mask = (1u << bits) - 1u;
+        const int32_t u = adj[e];
+        const uint64_t base = start[lo];
+        // At most d <= 16 nb keys in 32 nb slots: a free slot exists, the loop ends.
+        for (uint32_t b = member_hash(u, bits);; b = (b + 1u) & mask) {
+            const uint32_t slot = atomicAdd(fill_count + base + b, 1u);
+            if (slot < static_cast<uint32_t>(kMemberSlots)) {
+                slots[(base + b) * kMemberSlots + slot] = static_cast<uint32_t>(u);
+                break;
             }
         }
-        lines[t] = make_uint4(key[0], key[1], key[2], key[3]);
     }
 }
 
@@ -876,12 +851,15 @@ hipError_t launch_member_tree_starts(const DevGraph &g, uint32_t *root, uint64_t
 }
 
 hipError_t launch_member_tree_fill(const DevGraph &g, const uint64_t *start, uint64_t lines_total, uint4 *lines,
-                                   hipStream_t stream) {
+                                   uint32_t *fill_count, hipStream_t stream) {
     if (g.nv <= 0 || lines_total == 0) return hipSuccess;
-    const uint64_t pieces = lines_total * 8;
-    const uint64_t blocks = std::min<uint64_t>((pieces + 255) / 256, uint64_t(1) << 20);
-    hipLaunchKernelGGL(member_tree_fill_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, g.offset,
-                       start, g.nv, g.adj, lines, pieces);
+    uint32_t *slots = reinterpret_cast<uint32_t *>(lines);
+    hipError_t e = launch_fill_u32(slots, kMemberEmpty, static_cast<int64_t>(lines_total) * kMemberSlots, stream);
+    if (e != hipSuccess) return e;
+    e = launch_fill_u32(fill_count, 0u, static_cast<int64_t>(lines_total), stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(member_tree_fill_kernel, dim3(8192), dim3(256), 0, stream, g.offset, start, g.nv, g.adj, g.ne,
+                       slots, fill_count);
     return hipGetLastError();
 }
 

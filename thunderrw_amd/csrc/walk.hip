@@ -169,9 +169,9 @@ __device__ __forceinline__ bool is_neighbor(const DevGraph &g, int32_t u, int64_
     return false;
 }
 
-// Half of a 128-byte membership line: 16 int32 keys (4 16-byte loads issued together).  A
-// search loads the second half of a line only when the first cannot decide: a whole line in
-// flight would take 32 registers and cost the Node2Vec kernel almost half its wavefronts.
+// Half of a 128-byte membership bucket: 16 int32 keys (4 16-byte loads issued together).  A
+// search loads the second half only when the first cannot decide: a whole line in flight would
+// take 32 registers and cost the Node2Vec kernel almost half its wavefronts.
 struct MemberHalf {
     uint4 a0, a1, a2, a3;
     __device__ __forceinline__ void load(const uint4 *p) {
@@ -180,14 +180,9 @@ struct MemberHalf {
         a2 = p[2];
         a3 = p[3];
     }
-    static __device__ __forceinline__ int32_t lt4(const uint4 &v, int32_t x) {
-        return (static_cast<int32_t>(v.x) < x) + (static_cast<int32_t>(v.y) < x) + (static_cast<int32_t>(v.z) < x) +
-               (static_cast<int32_t>(v.w) < x);
-    }
     static __device__ __forceinline__ bool eq4(const uint4 &v, uint32_t x) {
         return (v.x == x) | (v.y == x) | (v.z == x) | (v.w == x);
     }
-    __device__ __forceinline__ int32_t count_lt(int32_t x) const { return lt4(a0, x) + lt4(a1, x) + lt4(a2, x) + lt4(a3, x); }
     __device__ __forceinline__ bool contains(int32_t x) const {
         const uint32_t y = static_cast<uint32_t>(x);
         return eq4(a0, y) | eq4(a1, y) | eq4(a2, y) | eq4(a3, y);
@@ -195,35 +190,24 @@ struct MemberHalf {
     __device__ __forceinline__ int32_t last() const { return static_cast<int32_t>(a3.w); }
 };
 
-// Whether x is among the d sorted neighbours whose membership structure starts at line root:
-// one node per tree level, then one leaf.  Level k (0 = over the leaves) has ceil(nl / 32^(k+1))
-// nodes and nl / 32^k real keys in all; a key count past a node's real keys means x exceeds
-// every neighbour under it.
+// Whether x is among the d neighbours whose membership table starts at line root: x's bucket,
+// first half, then second half when the first is full of other keys, then the next bucket only
+// when this one is full without x (a bucket's keys fill its slots from the first).
 __device__ __forceinline__ bool member_search(const uint4 *lines, uint32_t root, int32_t d, int32_t x) {
-    // 32-bit arithmetic throughout (line indices fit: a graph's structures have < 2^32 lines).
-    const uint32_t nl = static_cast<uint32_t>(member_leaves(d));
-    const uint32_t bits = 32u - static_cast<uint32_t>(__clz(static_cast<int>(nl - 1u)));  // nl >= 2
-    const uint32_t L = (bits + 4u) / 5u;  // tree levels: 32^L >= nl
+    const uint32_t bits = member_bucket_bits(d), nb = 1u << bits;
     MemberHalf h;
-    uint32_t line = root, node = 0;
-    for (uint32_t k = L; k-- > 0;) {
-        const uint4 *p = lines + static_cast<uint64_t>(line + node) * 8;
-        const uint32_t real = (nl + (1u << (5 * k)) - 1u) >> (5 * k);  // keys at level k, all nodes
-        const uint32_t here = min(real - node * kMemberFan, static_cast<uint32_t>(kMemberFan));
+    uint32_t b = member_hash(x, bits);
+    for (uint32_t i = 0; i < nb; ++i) {
+        const uint4 *p = lines + static_cast<uint64_t>(root + b) * 8;
         h.load(p);
-        uint32_t c = static_cast<uint32_t>(h.count_lt(x));
-        if (c == 16u && here > 16u) {
-            h.load(p + 4);
-            c += static_cast<uint32_t>(h.count_lt(x));
-        }
-        if (c >= here) return false;
-        line += (real + kMemberFan - 1u) / kMemberFan;  // nodes at level k
-        node = node * kMemberFan + c;
+        if (h.contains(x)) return true;
+        if (static_cast<uint32_t>(h.last()) == kMemberEmpty) return false;
+        h.load(p + 4);
+        if (h.contains(x)) return true;
+        if (static_cast<uint32_t>(h.last()) == kMemberEmpty) return false;
+        b = (b + 1u) & (nb - 1u);
     }
-    const uint4 *p = lines + static_cast<uint64_t>(line + node) * 8;
-    h.load(p);
-    if (h.last() < x) h.load(p + 4);  // x lies past the first 16 (padding keys are INT32_MAX)
-    return h.contains(x);
+    return false;
 }
 
 // Graph::is_neighbor (util/graph/graph.h:180-199) with both endpoints' ranges known: the

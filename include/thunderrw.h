@@ -246,6 +246,9 @@ int trw_graph_device_view(const trw_graph *g, trw_device_view *out);
 #define TRW_TUNE_ITS_TREE_MIN_DEGREE 1 /* ITS: vertices of at least this degree search their 128-byte-node
                                           tree, the rest binary-search their prefix sums */
 #define TRW_TUNE_QUEUE_CHUNK 2         /* QUEUE: walker indices a wavefront takes per atomic (64..4096) */
+#define TRW_TUNE_ITS_GUIDE_OFF 3       /* ITS: 0 = steps read the guide records where the table has them
+                                          (one gather), 1 = they search the prefix-sum structures (the
+                                          picks are identical; not autotuned) */
 int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value);
 int trw_graph_get_tuning(const trw_graph *g, int32_t knob, int64_t *value);
 

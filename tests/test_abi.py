@@ -185,3 +185,13 @@ def test_walk_supported_names_the_library_moves():
     bad = T.make_params("deepwalk")
     bad.execution = 7
     assert T.lib().trw_walk_supported(ctypes.byref(bad)) == A.E_INVALID
+
+
+def test_tuning_knob_ids_match_the_header():
+    """Every TRW_TUNE_* knob the header defines has the same id in the Python mirror, and the
+    mirror's knob names cover exactly those ids."""
+    defs = dict(re.findall(r"^#define TRW_TUNE_(\w+)\s+(\d+)", open(HEADER).read(), re.M))
+    assert defs, "no TRW_TUNE_* defines found"
+    for name, val in defs.items():
+        assert getattr(A, "TUNE_" + name) == int(val), name
+    assert set(T.DeviceGraph.TUNING.values()) == {int(v) for v in defs.values()}

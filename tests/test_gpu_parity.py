@@ -203,9 +203,10 @@ def test_metapath_label_alphabets_match_oracle(gpu_ctx, n_labels, schema):
 
 
 def test_tuning_changes_no_walk(gpu_ctx):
-    """ITS with the search tree on every vertex, on hubs only, and off; the queue form with
-    several chunk sizes; the fused form with 1, 2 and 4 walkers per lane: identical replay
-    walks, equal to the oracle's.  Weights with a negative entry build no tree."""
+    """ITS through guide records, and with them switched off through the search tree on every
+    vertex, on hubs only, and off; the queue form with several chunk sizes; the fused form with
+    1, 2 and 4 walkers per lane: identical replay walks, equal to the oracle's.  Weights with a
+    negative entry build no tree and no guide."""
     g = graphs.hub_graph(5000, 4500, 20000, 71)
     dg = dgraph(gpu_ctx, g)
     try:
@@ -214,7 +215,8 @@ def test_tuning_changes_no_walk(gpu_ctx):
         K = 4 * 40 + 64
         draws = T.replay_fill(72, len(src), K)
         cases = [({"app": "deepwalk", "execution": 1, "sampling": 1, "length": 40, "extra": {}},
-                  [dict(its_tree_min_degree=v) for v in (0, 64, 1 << 30)], ["fused", "queue"]),
+                  [dict(its_guide_off=0)] + [dict(its_guide_off=1, its_tree_min_degree=v) for v in (0, 64, 1 << 30)],
+                  ["fused", "queue", "step"]),
                  ({"app": "deepwalk", "execution": 1, "sampling": 2, "length": 40, "extra": {}},
                   [dict(walkers_per_thread=v) for v in (1, 2, 4)], ["fused"]),
                  ({"app": "metapath", "execution": 2, "sampling": 1, "length": 40, "extra": {"-s": "0,1,2,3,4"}},
@@ -227,7 +229,7 @@ def test_tuning_changes_no_walk(gpu_ctx):
                     r = T.walk(dg, case_params(case, form), src, draws, T.replay_fill(73, len(src), 1))
                     for key in ("paths", "lengths", "ends", "status", "draws_used"):
                         assert np.array_equal(r[key], ref[key]), (case["app"], kn, form, key)
-            dg.set_tuning(walkers_per_thread=0, its_tree_min_degree=0, queue_chunk=0)
+            dg.set_tuning(walkers_per_thread=0, its_tree_min_degree=0, queue_chunk=0, its_guide_off=0)
     finally:
         dg.close()
     # Negative weights: no tree; the walk still equals the oracle's (binary search, clamped pick).
@@ -653,7 +655,8 @@ def test_its_guide_records_decide_every_pick_boundary(gpu_ctx):
     of the buckets at one to four buckets per edge, on vertices whose weights span six decades
     (so some buckets span many edges and take the prefix-sum search), unit-range weights, a
     hub, a degree-one vertex and a vertex whose weights are all zero: every pick equals the
-    oracle's upper bound, and the queue form's tree search agrees."""
+    oracle's upper bound, in the fused and queue forms, and with the guide switched off (the
+    search structures) too."""
     rng = np.random.default_rng(77)
     g = graphs.hub_graph(3000, 2500, 6000, 78)
     w = rng.uniform(1.0, 5.0, g.ne)
@@ -690,12 +693,14 @@ def test_its_guide_records_decide_every_pick_boundary(gpu_ctx):
         og = ob.Graph(gg, None, ob.its_build(gg, w), w, None)
         ref = ob.walk(og, "deepwalk", 1, 1, 2, src, draws.reshape(-1, 1), 1)
         assert (ref["status"] == 1).sum() > 0  # the zero-weight vertex ends its walkers without a draw
-        for form in ("fused", "queue"):
-            r = T.walk(dg, T.make_params("deepwalk", execution="static", sampling="its", length=2, rng="replay",
-                                         form=form), src, draws.reshape(1, -1))
-            for key in ("paths", "lengths", "ends", "status", "draws_used"):
-                assert np.array_equal(r[key], ref[key]), (form, key)
-            assert r["steps"] == ref["steps"], form
+        for off in (0, 1):
+            dg.set_tuning(its_guide_off=off)
+            for form in ("fused", "queue"):
+                r = T.walk(dg, T.make_params("deepwalk", execution="static", sampling="its", length=2,
+                                             rng="replay", form=form), src, draws.reshape(1, -1))
+                for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                    assert np.array_equal(r[key], ref[key]), (off, form, key)
+                assert r["steps"] == ref["steps"], (off, form)
     finally:
         dg.close()
 
@@ -703,8 +708,8 @@ def test_its_guide_records_decide_every_pick_boundary(gpu_ctx):
 @pytest.mark.parametrize("seed", [41, 42])
 def test_its_guide_walks_match_oracle_on_skewed_weights(gpu_ctx, seed):
     """Whole ITS walks (length 40) where a third of the vertices have weights spanning six
-    decades: replay mode against the oracle in every form, and the Philox stream identical in
-    the fused (guide) and queue (search) forms."""
+    decades: replay mode against the oracle in every form, and the Philox stream identical through
+    the guide records and through the search structures."""
     rng = np.random.default_rng(seed)
     g = graphs.chung_lu(5000, 25000, 2.0, 2, 2000, seed)
     w = g.weight.copy()
@@ -724,8 +729,11 @@ def test_its_guide_walks_match_oracle_on_skewed_weights(gpu_ctx, seed):
             for key in ("paths", "lengths", "ends", "status", "draws_used"):
                 assert np.array_equal(r[key], ref[key]), (form, key)
             assert r["steps"] == ref["steps"], form
-        outs = [T.walk(dg, T.make_params("deepwalk", execution="static", sampling="its", length=40, rng="philox",
-                                         seed=seed, form=form), src) for form in ("fused", "queue")]
+        outs = []
+        for off, form in ((0, "fused"), (1, "queue")):
+            dg.set_tuning(its_guide_off=off)
+            outs.append(T.walk(dg, T.make_params("deepwalk", execution="static", sampling="its", length=40,
+                                                 rng="philox", seed=seed, form=form), src))
         for key in ("paths", "lengths", "status"):
             assert np.array_equal(outs[0][key], outs[1][key]), key
     finally:
@@ -823,3 +831,29 @@ def test_node2vec_membership_bucket_overflow_matches_oracle(gpu_ctx):
                     assert np.array_equal(r[key], ref[key]), (p, q, form, key)
     finally:
         dg.close()
+
+
+def test_its_guide_from_a_loaded_prefix_sum_file(gpu_ctx, tmp_path):
+    """The ITS table read from the reference's b_edge_weight_prefix_sum.bin (instead of built)
+    gets guide records too, and the fused walk over them equals the oracle's in replay mode."""
+    g = graphs.chung_lu(3000, 15000, 2.0, 2, 800, 111)
+    src = np.nonzero(g.degree > 0)[0].astype(np.int32)
+    L, K = 30, 40
+    draws_wm = ob.sfmt_stream(112, len(src) * K).reshape(len(src), K)
+    og = ob.Graph(g, None, ob.its_build(g, g.weight), g.weight, None)
+    ref = ob.walk(og, "deepwalk", 1, 1, L, src, draws_wm, K)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        dg.build_tables("its", "edge")
+        dg.store_tables(str(tmp_path))
+    finally:
+        dg.close()
+    dg2 = dgraph(gpu_ctx, g)
+    try:
+        dg2.load_tables(str(tmp_path), "its", "edge")
+        r = T.walk(dg2, T.make_params("deepwalk", execution="static", sampling="its", length=L, rng="replay",
+                                      form="fused"), src, np.ascontiguousarray(draws_wm.T))
+        for key in ("paths", "lengths", "ends", "status", "draws_used"):
+            assert np.array_equal(r[key], ref[key]), key
+    finally:
+        dg2.close()

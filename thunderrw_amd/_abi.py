@@ -15,7 +15,7 @@ RNG_REPLAY, RNG_PHILOX = 0, 1
 FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP, FORM_GRAPH = 0, 1, 2, 3, 4
 WEIGHTS_EDGE, WEIGHTS_UNIT, WEIGHTS_CUSTOM = 0, 1, 2
 WALK_WEIGHTS_APP, WALK_WEIGHTS_EDGE, WALK_WEIGHTS_UNIT = 0, 1, 2
-TUNE_WALKERS_PER_THREAD, TUNE_ITS_TREE_MIN_DEGREE, TUNE_QUEUE_CHUNK = 0, 1, 2
+TUNE_WALKERS_PER_THREAD, TUNE_ITS_TREE_MIN_DEGREE, TUNE_QUEUE_CHUNK, TUNE_ITS_GUIDE_OFF = 0, 1, 2, 3
 RESULT_PPR, RESULT_PATHS = 0, 1
 QUERY_PENDING, QUERY_RUNNING, QUERY_DONE, QUERY_FAILED = 0, 1, 2, 3
 WALKER_DONE, WALKER_GATHER_TERMINATED, WALKER_DEAD_END, WALKER_REPLAY_EXHAUSTED, WALKER_TRIAL_CAP, \

@@ -457,7 +457,15 @@ static int build_its_guide(trw_graph *g, bool monotone) {
     TRW_HIP(hipStreamSynchronize(st), "ITS guide starts");
     if (total >= trw::kNoRange) return TRW_OK;
     if (!g->gvrec) TRW_HIP(dmalloc(&g->gvrec, V), "hipMalloc ITS guide ranges");
-    TRW_HIP(dmalloc(&g->grec, 2 * static_cast<size_t>(total) + 2), "hipMalloc ITS guide records");
+    // The records only speed the walk up (about 80 bytes per edge at U[1,5) weights): without
+    // room for them the table stands and the fused walk searches it, as the other forms do.
+    const hipError_t ae = dmalloc(&g->grec, 2 * static_cast<size_t>(total) + 2);
+    if (ae == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        g->grec = nullptr;
+        return TRW_OK;
+    }
+    TRW_HIP(ae, "hipMalloc ITS guide records");
     TRW_HIP(trw::launch_its_guide_fill(g->view(), g->prefix, start, total, g->gvrec, g->grec, st), "ITS guide fill");
     TRW_HIP(hipStreamSynchronize(st), "ITS guide records");  // the scratch is freed on return
     return TRW_OK;
@@ -759,6 +767,9 @@ int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value) {
     case TRW_TUNE_QUEUE_CHUNK:
         if (value < 0 || value > 4096) return fail(TRW_E_INVALID, "queue chunk outside [0, 4096]");
         break;
+    case TRW_TUNE_ITS_GUIDE_OFF:
+        if (value != 0 && value != 1) return fail(TRW_E_INVALID, "ITS guide switch: 0 or 1");
+        break;
     default:
         return fail(TRW_E_INVALID, "unknown tuning knob %d", knob);
     }
@@ -768,7 +779,7 @@ int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value) {
 
 int trw_graph_get_tuning(const trw_graph *g, int32_t knob, int64_t *value) {
     if (!g || !value) return fail(TRW_E_INVALID, "null argument");
-    if (knob < 0 || knob >= 3) return fail(TRW_E_INVALID, "unknown tuning knob %d", knob);
+    if (knob < 0 || knob > TRW_TUNE_ITS_GUIDE_OFF) return fail(TRW_E_INVALID, "unknown tuning knob %d", knob);
     *value = g->tune[knob];
     return TRW_OK;
 }
@@ -857,6 +868,7 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
     k.walkers_per_thread = p->walkers_per_thread ? p->walkers_per_thread : static_cast<int32_t>(g->tune[TRW_TUNE_WALKERS_PER_THREAD]);
     k.its_tree_min = static_cast<int32_t>(g->tune[TRW_TUNE_ITS_TREE_MIN_DEGREE]);
     k.queue_chunk = static_cast<int32_t>(g->tune[TRW_TUNE_QUEUE_CHUNK]);
+    k.its_guide = g->tune[TRW_TUNE_ITS_GUIDE_OFF] == 0;
     k.ppr = ppr;
     k.length = p->length;
     k.max_trials = p->max_trials;

@@ -147,6 +147,7 @@ struct KindParams {
     int32_t schema[16];
     int32_t its_tree_min;   // ITS: vertices of at least this degree search their tree (when built)
     int32_t queue_chunk;    // QUEUE: walker indices a wavefront takes per atomic (0 = from the batch size)
+    int32_t its_guide;      // ITS: read the guide records when the graph has them
 };
 
 // Walker batch (device pointers).

@@ -466,9 +466,14 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     const int32_t cur = s.cur;
     int64_t off;
     int32_t d;
-    known_range(g, cur, s.coff, s.cdeg, off, d);
+    // ITS with guide records: the walker's range is its vertex's buckets, {first bucket, count};
+    // no buckets at a vertex of positive degree (first bucket kNoRange) means weights summing to
+    // zero, which ends the walk before a draw as the total check below does.
+    const bool guide = MODE == kModeIts && g.grec != nullptr && P.its_guide;
+    if (guide) guide_range(g, cur, s.coff, s.cdeg, off, d);
+    else known_range(g, cur, s.coff, s.cdeg, off, d);
     if (d <= 0) {
-        s.st = kDeadEnd;
+        s.st = (guide && off == static_cast<int64_t>(kNoRange)) ? kGatherTerminated : kDeadEnd;
         return false;
     }
     const uint32_t step = static_cast<uint32_t>(s.len - 1);
@@ -495,6 +500,21 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             if (REPLAY) s.k -= 2;  // the vertex's weights sum to zero: terminate without consuming draws
             s.st = kGatherTerminated;
             return false;
+        }
+    } else if (MODE == kModeIts && guide) {
+        // inverse_transformation_sampling.h:259-286 through the guide record of the draw's bucket.
+        uint32_t u;
+        if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
+        const int64_t e = off + static_cast<int64_t>((static_cast<uint64_t>(u) * ud) >> 32);
+        const uint4 ra = g.grec[2 * e], rb = g.grec[2 * e + 1];
+        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+        if (ra.w != 0u) {
+            next = guide_search(g, cur, u, noff, ndeg);
+        } else {
+            const bool upper = u >= ra.x;
+            next = static_cast<int32_t>(upper ? ra.z : ra.y);
+            noff = upper ? rb.z : rb.x;
+            ndeg = upper ? rb.w : rb.y;
         }
     } else if (MODE == kModeIts) {
         // inverse_transformation_sampling.h:259-286
@@ -1364,7 +1384,7 @@ template <bool REPLAY>
 hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream) {
     const bool records = p.mode == kModeUniform ? g.erec != nullptr
                          : p.mode == kModeAlias ? g.arec != nullptr
-                         : p.mode == kModeIts   ? g.grec != nullptr
+                         : p.mode == kModeIts   ? g.grec != nullptr && p.its_guide
                                                 : false;
     if (records && !p.ppr) {
         if (p.mode == kModeUniform) fused_multi_launch<kModeUniform, REPLAY>(g, p, io, stream);

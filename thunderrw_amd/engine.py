@@ -172,6 +172,7 @@ class DeviceGraph:
         return ms.value
 
     TUNING = {"walkers_per_thread": A.TUNE_WALKERS_PER_THREAD, "its_tree_min_degree": A.TUNE_ITS_TREE_MIN_DEGREE,
+              "its_guide_off": A.TUNE_ITS_GUIDE_OFF,
               "queue_chunk": A.TUNE_QUEUE_CHUNK}
 
     def set_tuning(self, **knobs):

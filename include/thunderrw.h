@@ -218,8 +218,10 @@ int trw_build_tables_from(trw_graph *g, int32_t sampling, const double *device_w
  * edge_weight_alias_table_, edge_weight_prefix_sum_, edge_weight_rejection_max_ and the
  * weights the rejection maxima were built from (edge_weight_rejection_; null for unit
  * weights).  A table pointer is null until that table is built; *_weights give the
- * TRW_WEIGHTS_* it was built from (-1: not built).  Valid until the graph is freed or the
- * table rebuilt. */
+ * TRW_WEIGHTS_* it was built from (-1: not built).  membership_lines / membership_root: the
+ * tables Graph::is_neighbor (graph.h:180-199) can be answered from on the device
+ * (trw_dev::member_contains in thunderrw_device.h; null until trw_build_membership).  Valid
+ * until the graph is freed or the table rebuilt. */
 typedef struct {
     int32_t num_vertices;
     int32_t max_degree;
@@ -235,8 +237,17 @@ typedef struct {
     const double *rejection_weight;
     int32_t alias_weights, its_weights, rejection_weights;
     int32_t reserved;
+    const void *membership_lines;
+    const uint32_t *membership_root;
 } trw_device_view;
 int trw_graph_device_view(const trw_graph *g, trw_device_view *out);
+
+/* Membership tables for is_neighbor on the device (no reference counterpart: graph.h:180-199
+ * binary-searches the smaller endpoint's sorted list; these answer the same question in one
+ * 128-byte read): a hash set of the neighbours of every vertex of degree > 32, in 128-byte
+ * buckets, layout in thunderrw_device.h.  Built once per graph (the first Node2Vec walk builds
+ * them too); a graph with 2^32 edges or more gets none and its is_neighbor keeps the search. */
+int trw_build_membership(trw_graph *g);
 
 /* ---- per-graph tuning ----
  * Launch choices that change speed, never results (no reference counterpart; the

@@ -7,7 +7,9 @@
  *   - Philox-4x32-10 (Salmon et al., SC'11) as the library keys it: key = the walk seed's
  *     two 32-bit halves, counter = (global walker id low, high, step, trial); a move's draws
  *     are words .x/.y of its trial's block, a PPR stop draw word .w of the step's trial-0 block,
- *   - Utility::upper_bound_branchless (util/utility.h:18-27).
+ *   - Utility::upper_bound_branchless (util/utility.h:18-27),
+ *   - the membership tables Graph::is_neighbor (graph.h:180-199) is answered from on the device
+ *     (trw_build_membership builds them; trw_graph_device_view gives their address).
  * HIP only (include after <hip/hip_runtime.h>).
  */
 #ifndef THUNDERRW_DEVICE_H
@@ -60,6 +62,62 @@ __device__ __forceinline__ int32_t upper_bound_branchless(const double *src, int
         n -= half;
     }
     return static_cast<int32_t>((*base <= target) + (base - src));
+}
+
+// Membership table of one vertex's adjacency, for vertices of degree d > kMemberMin: a hash
+// set of its neighbours in nb = 2^member_bucket_bits(d) buckets of 128 bytes (32 int32 slots,
+// about 8 to 16 keys per bucket).  Neighbour u lives in bucket member_hash(u) or, when that one
+// is full, the next bucket (cyclically) with room; a bucket's keys fill its slots from the first,
+// the other slots hold kMemberEmpty.  A vertex's table starts at line root[v] (kNoRoot: none).
+constexpr int kMemberSlots = 32;
+constexpr int kMemberMin = 32;
+constexpr uint32_t kMemberEmpty = 0xffffffffu;
+constexpr uint32_t kNoRoot = 0xffffffffu;
+
+__host__ __device__ inline uint32_t member_bucket_bits(int32_t d) {
+    const uint32_t need = static_cast<uint32_t>((d + 15) / 16);  // >= 3 for d > kMemberMin
+    return need <= 1u ? 0u : 32u - static_cast<uint32_t>(__builtin_clz(need - 1u));
+}
+
+__host__ __device__ inline uint32_t member_hash(int32_t u, uint32_t bits) {
+    return (static_cast<uint32_t>(u) * 0x9E3779B1u) >> (32u - bits);  // bits >= 2
+}
+
+// Half of a bucket: 16 keys in 4 16-byte loads issued together.  A search loads the second
+// half only when the first cannot decide: a whole line in flight would take 32 registers.
+struct MemberHalf {
+    uint4 a0, a1, a2, a3;
+    __host__ __device__ inline void load(const uint4 *p) {
+        a0 = p[0];
+        a1 = p[1];
+        a2 = p[2];
+        a3 = p[3];
+    }
+    static __host__ __device__ inline bool eq4(const uint4 &v, uint32_t x) {
+        return (v.x == x) | (v.y == x) | (v.z == x) | (v.w == x);
+    }
+    __host__ __device__ inline bool contains(uint32_t x) const { return eq4(a0, x) | eq4(a1, x) | eq4(a2, x) | eq4(a3, x); }
+    __host__ __device__ inline bool full() const { return a3.w != kMemberEmpty; }
+};
+
+// Whether x is among the d neighbours whose table starts at line root (lines: 8 uint4 each):
+// x's bucket, its second half only when the first is full of other keys, the next bucket only
+// when this one is full without x.  Almost always one 128-byte line.
+__host__ __device__ inline bool member_contains(const uint4 *lines, uint32_t root, int32_t d, int32_t x) {
+    const uint32_t bits = member_bucket_bits(d), nb = 1u << bits, y = static_cast<uint32_t>(x);
+    MemberHalf h;
+    uint32_t b = member_hash(x, bits);
+    for (uint32_t i = 0; i < nb; ++i) {
+        const uint4 *p = lines + static_cast<uint64_t>(root + b) * 8;
+        h.load(p);
+        if (h.contains(y)) return true;
+        if (!h.full()) return false;
+        h.load(p + 4);
+        if (h.contains(y)) return true;
+        if (!h.full()) return false;
+        b = (b + 1u) & (nb - 1u);
+    }
+    return false;
 }
 
 }  // namespace trw_dev

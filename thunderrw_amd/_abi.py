@@ -40,7 +40,8 @@ class DeviceView(C.Structure):
                 ("offset", C.c_void_p), ("degree", C.c_void_p), ("adj", C.c_void_p), ("edge_weight", C.c_void_p),
                 ("edge_label", C.c_void_p), ("alias_table", C.c_void_p), ("prefix_sum", C.c_void_p),
                 ("rejection_max", C.c_void_p), ("rejection_weight", C.c_void_p), ("alias_weights", C.c_int32),
-                ("its_weights", C.c_int32), ("rejection_weights", C.c_int32), ("reserved", C.c_int32)]
+                ("its_weights", C.c_int32), ("rejection_weights", C.c_int32), ("reserved", C.c_int32),
+                ("membership_lines", C.c_void_p), ("membership_root", C.c_void_p)]
 
 
 class WalkParams(C.Structure):
@@ -122,6 +123,7 @@ _SIGS = {
     "trw_last_build_ms": (C.c_int, [C.c_void_p, C.c_void_p]),
     "trw_build_tables_from": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "trw_graph_device_view": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "trw_build_membership": (C.c_int, [C.c_void_p]),
     "trw_graph_store_tables": (C.c_int, [C.c_void_p, C.c_char_p]),
     "trw_graph_load_tables": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]),
     "trw_graph_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),

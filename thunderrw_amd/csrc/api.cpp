@@ -592,6 +592,12 @@ static int build_member_trees(trw_graph *g) {
     return TRW_OK;
 }
 
+int trw_build_membership(trw_graph *g) {
+    if (!g) return fail(TRW_E_INVALID, "graph is null");
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    return build_member_trees(g);
+}
+
 int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
     if (!g) return fail(TRW_E_INVALID, "graph is null");
     if (weights != TRW_WEIGHTS_EDGE && weights != TRW_WEIGHTS_UNIT) return fail(TRW_E_INVALID, "unknown weights %d", weights);
@@ -641,6 +647,8 @@ int trw_graph_device_view(const trw_graph *g, trw_device_view *out) {
     out->alias_weights = g->alias_weights;
     out->its_weights = g->its_weights;
     out->rejection_weights = g->rs_weights;
+    out->membership_lines = g->mlines;
+    out->membership_root = g->mroot;
     return TRW_OK;
 }
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "thunderrw_device.h"
+
 #include <cstdint>
 
 namespace trw {
@@ -113,25 +115,12 @@ __host__ __device__ inline int32_t its_levels(int32_t nl, int64_t *nodes) {
 // and the walk searches the prefix sums for that step.  Built only for non-decreasing sums.
 constexpr uint32_t kGuideMaxC = 4;
 
-// Membership table of one vertex's adjacency (for Graph::is_neighbor, graph.h:180-199), for
-// vertices of degree d > kMemberMin: a hash set of its neighbours in nb = 2^member_bucket_bits(d)
-// buckets of 128 bytes (32 int32 slots), about 8 to 16 keys per bucket.  Neighbour u lives in
-// bucket member_hash(u) or, when that one is full, the next bucket (cyclically) with room; a
-// bucket's keys fill its slots from the first, the other slots hold kMemberEmpty.  A test reads
-// u's bucket (its second half only when the first is full of other keys) and moves to the next
-// bucket only when this one is full without u, so it is almost always one 128-byte line.
-constexpr int kMemberSlots = 32;
-constexpr int kMemberMin = 32;
-constexpr uint32_t kMemberEmpty = 0xffffffffu;
-
-__host__ __device__ inline uint32_t member_bucket_bits(int32_t d) {
-    const uint32_t need = static_cast<uint32_t>((d + 15) / 16);  // >= 3 for d > kMemberMin
-    return need <= 1u ? 0u : 32u - static_cast<uint32_t>(__builtin_clz(need - 1u));
-}
-
-__host__ __device__ inline uint32_t member_hash(int32_t u, uint32_t bits) {
-    return (static_cast<uint32_t>(u) * 0x9E3779B1u) >> (32u - bits);  // bits >= 2
-}
+// Membership tables (Graph::is_neighbor's search on the device): thunderrw_device.h.
+using trw_dev::kMemberEmpty;
+using trw_dev::kMemberMin;
+using trw_dev::kMemberSlots;
+using trw_dev::member_bucket_bits;
+using trw_dev::member_hash;
 
 // Per walker kind launch parameters (app functor + g_para).
 struct KindParams {

@@ -169,47 +169,6 @@ __device__ __forceinline__ bool is_neighbor(const DevGraph &g, int32_t u, int64_
     return false;
 }
 
-// Half of a 128-byte membership bucket: 16 int32 keys (4 16-byte loads issued together).  A
-// search loads the second half only when the first cannot decide: a whole line in flight would
-// take 32 registers and cost the Node2Vec kernel almost half its wavefronts.
-struct MemberHalf {
-    uint4 a0, a1, a2, a3;
-    __device__ __forceinline__ void load(const uint4 *p) {
-        a0 = p[0];
-        a1 = p[1];
-        a2 = p[2];
-        a3 = p[3];
-    }
-    static __device__ __forceinline__ bool eq4(const uint4 &v, uint32_t x) {
-        return (v.x == x) | (v.y == x) | (v.z == x) | (v.w == x);
-    }
-    __device__ __forceinline__ bool contains(int32_t x) const {
-        const uint32_t y = static_cast<uint32_t>(x);
-        return eq4(a0, y) | eq4(a1, y) | eq4(a2, y) | eq4(a3, y);
-    }
-    __device__ __forceinline__ int32_t last() const { return static_cast<int32_t>(a3.w); }
-};
-
-// Whether x is among the d neighbours whose membership table starts at line root: x's bucket,
-// first half, then second half when the first is full of other keys, then the next bucket only
-// when this one is full without x (a bucket's keys fill its slots from the first).
-__device__ __forceinline__ bool member_search(const uint4 *lines, uint32_t root, int32_t d, int32_t x) {
-    const uint32_t bits = member_bucket_bits(d), nb = 1u << bits;
-    MemberHalf h;
-    uint32_t b = member_hash(x, bits);
-    for (uint32_t i = 0; i < nb; ++i) {
-        const uint4 *p = lines + static_cast<uint64_t>(root + b) * 8;
-        h.load(p);
-        if (h.contains(x)) return true;
-        if (static_cast<uint32_t>(h.last()) == kMemberEmpty) return false;
-        h.load(p + 4);
-        if (h.contains(x)) return true;
-        if (static_cast<uint32_t>(h.last()) == kMemberEmpty) return false;
-        b = (b + 1u) & (nb - 1u);
-    }
-    return false;
-}
-
 // Graph::is_neighbor (util/graph/graph.h:180-199) with both endpoints' ranges known: the
 // larger-degree endpoint searched in the smaller-degree endpoint's sorted adjacency -- through
 // that endpoint's membership structure when it has one (root carried by the walker or the
@@ -227,7 +186,7 @@ __device__ __forceinline__ bool is_neighbor_tree(const DevGraph &g, int32_t u, i
     }
     if (g.mlines && vdeg > kMemberMin) {
         const uint32_t root = vroot != kNoRange ? vroot : g.mroot[v];
-        if (root != kNoRange) return member_search(g.mlines, root, vdeg, u);
+        if (root != kNoRange) return trw_dev::member_contains(g.mlines, root, vdeg, u);
     }
     return is_neighbor(g, u, voff, vdeg, v, voff, vdeg);  // equal degrees: searches u in v's list
 }

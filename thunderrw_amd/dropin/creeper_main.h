@@ -60,6 +60,8 @@
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+
+#include "thunderrw_device.h"
 #define TRW_DROPIN_HD __host__ __device__
 #else
 #define TRW_DROPIN_HD
@@ -186,6 +188,10 @@ public:
 
     trw_graph *device_ = nullptr;
 
+    // Set in the device image only: the hashed neighbour sets is_neighbor reads there.
+    const void *member_lines_ = nullptr;
+    const uint32_t *member_root_ = nullptr;
+
     Graph() = default;
     Graph(const Graph &) = delete;
     Graph &operator=(const Graph &) = delete;
@@ -241,12 +247,20 @@ public:
     TRW_DROPIN_HD int edge_label(int64_t eid) const { return edge_label_[eid]; }
 
     // graph.h:180-199: the larger-degree endpoint searched in the smaller one's list.
+    // On the device, a vertex with a membership table answers from it (one 128-byte read for
+    // the same answer as the search).
     TRW_DROPIN_HD bool is_neighbor(intT u, intT v) const {
         if (degree(u) < degree(v)) {
             const intT t = u;
             u = v;
             v = t;
         }
+#if defined(__HIPCC__)
+        if (member_root_) {
+            const uint32_t r = member_root_[v];
+            if (r != trw_dev::kNoRoot) return trw_dev::member_contains(static_cast<const uint4 *>(member_lines_), r, degree(v), u);
+        }
+#endif
         auto nbrs = neighbors(v);
         int b = 0, e = nbrs.second - 1;
         while (b <= e) {
@@ -266,9 +280,11 @@ public:
     }
 
     // A device copy of this object whose array fields (offset_, adj_, degree_, edge_weight_,
-    // edge_label_) are the graph's arrays in HBM; made once, freed with the graph.
+    // edge_label_) are the graph's arrays in HBM, with the membership tables is_neighbor reads
+    // there (built here); made once, freed with the graph.
     const void *device_image() {
         if (image_) return image_;
+        if (trw_build_membership(device()) != TRW_OK) trw_dropin::die("trw_build_membership");
         trw_device_view v;
         if (trw_graph_device_view(device(), &v) != TRW_OK) trw_dropin::die("trw_graph_device_view");
         std::vector<unsigned char> bytes(sizeof(Graph));
@@ -282,6 +298,8 @@ public:
         put(&degree_, v.degree);
         put(&edge_weight_, v.edge_weight);
         put(&edge_label_, v.edge_label);
+        put(&member_lines_, v.membership_lines);
+        put(&member_root_, v.membership_root);
         put(&device_, nullptr);
         put(&host_, nullptr);
         if (trw_device_alloc(trw_dropin::context(), sizeof(Graph), &image_) != TRW_OK ||

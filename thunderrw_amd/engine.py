@@ -157,6 +157,10 @@ class DeviceGraph:
         w = A.WEIGHTS_EDGE if weights == "edge" else A.WEIGHTS_UNIT
         check(lib().trw_build_tables(self.h, _enum(_SM, sampling), w))
 
+    def build_membership(self):
+        """trw_build_membership: the hashed neighbour sets is_neighbor is answered from on the device."""
+        check(lib().trw_build_membership(self.h))
+
     def store_tables(self, directory):
         """Writes the built tables as the reference's b_edge_weight_*.bin files."""
         check(lib().trw_graph_store_tables(self.h, directory.encode()))

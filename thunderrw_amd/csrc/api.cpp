@@ -548,7 +548,7 @@ static int build_tables(trw_graph *g, int32_t sampling, const double *w, int32_t
 // and neighbour records carrying each neighbour's root, for Node2Vec's is_neighbor tests (built
 // by the first Node2Vec walk; a graph without neighbour records, 2^32 edges or more, keeps the
 // binary search).
-static int build_member_trees(trw_graph *g) {
+static int build_member_tables(trw_graph *g) {
     if (g->nrec || g->nv == 0 || !g->erec) return TRW_OK;
     hipStream_t st = g->ctx->stream;
     const size_t V = static_cast<size_t>(g->nv);
@@ -568,21 +568,21 @@ static int build_member_trees(trw_graph *g) {
         }
     } owned{root, lines, nrec};
     TRW_HIP(dmalloc(&root, V), "hipMalloc membership roots");
-    TRW_HIP(trw::launch_member_tree_starts(g->view(), root, nullptr, nullptr, temp_bytes, st), "membership scan size");
+    TRW_HIP(trw::launch_member_table_starts(g->view(), root, nullptr, nullptr, temp_bytes, st), "membership scan size");
     TRW_HIP(sc.get(&start, V + 1), "hipMalloc membership starts");
     TRW_HIP(sc.get(&temp, temp_bytes), "hipMalloc membership scan storage");
-    TRW_HIP(trw::launch_member_tree_starts(g->view(), root, start, temp, temp_bytes, st), "membership starts");
+    TRW_HIP(trw::launch_member_table_starts(g->view(), root, start, temp, temp_bytes, st), "membership starts");
     uint64_t total = 0;
     TRW_HIP(hipMemcpyAsync(&total, start + V, sizeof(total), hipMemcpyDeviceToHost, st), "membership size");
     TRW_HIP(hipStreamSynchronize(st), "membership starts");
     if (total >= trw::kNoRange) return TRW_OK;  // line indices would overflow 32 bits: binary search
-    if (total) TRW_HIP(dmalloc(&lines, static_cast<size_t>(total) * 8), "hipMalloc membership structures");
+    if (total) TRW_HIP(dmalloc(&lines, static_cast<size_t>(total) * 8), "hipMalloc membership tables");
     TRW_HIP(dmalloc(&nrec, static_cast<size_t>(g->ne) + 1), "hipMalloc Node2Vec neighbour records");
     uint32_t *fill_count = nullptr;
     TRW_HIP(sc.get(&fill_count, static_cast<size_t>(total)), "hipMalloc membership bucket counters");
-    TRW_HIP(trw::launch_member_tree_fill(g->view(), start, total, lines, fill_count, st), "membership table fill");
+    TRW_HIP(trw::launch_member_table_fill(g->view(), start, total, lines, fill_count, st), "membership table fill");
     TRW_HIP(trw::launch_member_records(g->view(), root, nrec, st), "Node2Vec neighbour records");
-    TRW_HIP(hipStreamSynchronize(st), "membership structures");
+    TRW_HIP(hipStreamSynchronize(st), "membership tables");
     g->mroot = root;
     g->mlines = lines;
     g->nrec = nrec;
@@ -595,7 +595,7 @@ static int build_member_trees(trw_graph *g) {
 int trw_build_membership(trw_graph *g) {
     if (!g) return fail(TRW_E_INVALID, "graph is null");
     TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
-    return build_member_trees(g);
+    return build_member_tables(g);
 }
 
 int trw_build_tables(trw_graph *g, int32_t sampling, int32_t weights) {
@@ -894,7 +894,7 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
         if (!(p->p > 0.0) || !(p->q > 0.0)) return fail(TRW_E_INVALID, "Node2Vec p and q must be > 0");
         k.mode = trw::kModeNode2Vec;
         if (allow_build) {
-            const int rc = build_member_trees(g);
+            const int rc = build_member_tables(g);
             if (rc != TRW_OK) return rc;
         }
         k.n2v_p = p->p;  // Node2Vec::p_ (node2vec.cpp:18, after the inversion at :69)

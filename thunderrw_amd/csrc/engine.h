@@ -68,9 +68,9 @@ struct DevGraph {
                                //       {degree of adj[e], its maximum weight lo, hi, 0}, or null
     const uint4 *grec;         // ITS guide records (2 uint4 per bucket), or null; see its_guide below
     const uint2 *gvrec;        // nv: {first bucket, bucket count} per vertex ({kNoRange, 0}: weights sum to zero)
-    const uint4 *nrec;         // ne: neighbour records with .w = the neighbour's membership structure root
-    const uint4 *mlines;       // membership structures (128-byte lines), or null
-    const uint32_t *mroot;     // nv: first line of v's membership structure, kNoRange if v has none
+    const uint4 *nrec;         // ne: neighbour records with .w = the neighbour's membership table root
+    const uint4 *mlines;       // membership tables (128-byte buckets), or null
+    const uint32_t *mroot;     // nv: first line of v's membership table, kNoRange if v has none
 };
 
 // ITS search structure of one vertex, built from its prefix sums when they are
@@ -175,9 +175,9 @@ struct WalkerState {
                     //   brought the walker here (pdeg == kNoRange: not known)
     uint32_t gid_lo;  // global walker id: the Philox counter's walker words
     uint32_t gid_hi;
-    uint32_t cidx;    // current vertex: root of its ITS search tree (ITS) or membership structure
+    uint32_t cidx;    // current vertex: root of its ITS search tree (ITS) or membership table
                       // (Node2Vec); kNoRange: not known / none
-    uint32_t pidx;    // previous vertex: root of its membership structure (Node2Vec)
+    uint32_t pidx;    // previous vertex: root of its membership table (Node2Vec)
     double cmax;      // rejection: the current vertex's maximum weight (from a rejection record), < 0: not known
 };
 constexpr uint32_t kNoRange = 0xffffffffu;
@@ -260,9 +260,9 @@ hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *r
 // Membership tables: bucket counts into root, then offsets by a scan (as the ITS structures),
 // then the insertion of every neighbour (fill_count: one zeroed counter per bucket, scratch),
 // plus the neighbour records carrying each neighbour's root.
-hipError_t launch_member_tree_starts(const DevGraph &g, uint32_t *root, uint64_t *start, void *temp,
+hipError_t launch_member_table_starts(const DevGraph &g, uint32_t *root, uint64_t *start, void *temp,
                                      size_t &temp_bytes, hipStream_t stream);
-hipError_t launch_member_tree_fill(const DevGraph &g, const uint64_t *start, uint64_t lines_total, uint4 *lines,
+hipError_t launch_member_table_fill(const DevGraph &g, const uint64_t *start, uint64_t lines_total, uint4 *lines,
                                    uint32_t *fill_count, hipStream_t stream);
 // ITS guide records (tables.hip): flag and size pass plus scan (temp == null: scan storage
 // size only), then ranges and records.  start has nv + 1 entries.

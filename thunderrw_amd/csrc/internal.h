@@ -36,8 +36,8 @@ struct trw_graph {
     uint4 *grec = nullptr;         // ITS guide records (2 per bucket), with the ITS table when its sums are monotone
     uint2 *gvrec = nullptr;        // nv: {first bucket, bucket count}
     uint4 *nrec = nullptr;         // neighbour records with membership roots (built by the first Node2Vec walk)
-    uint4 *mlines = nullptr;       // membership structures
-    uint32_t *mroot = nullptr;     // nv: first line of each vertex's membership structure
+    uint4 *mlines = nullptr;       // membership tables (hashed neighbour sets, 128-byte buckets)
+    uint32_t *mroot = nullptr;     // nv: first line of each vertex's membership table
     bool weights_nonneg = true;    // every edge weight is finite and >= 0
     int32_t build_hubs = 0;        // vertices of degree >= kWaveBuildMin (a prefix of order)
     int64_t tune[4] = {0, 0, 0, 0};  // TRW_TUNE_* knobs

@@ -670,7 +670,7 @@ __global__ void its_tree_roots_kernel(const int32_t *adj, int64_t ne, const uint
 
 // Membership tables.  Sizes: one thread per vertex writes its bucket count (none for degree <=
 // kMemberMin, whose tests binary-search the adjacency itself).
-__global__ void member_tree_sizes_kernel(const int64_t *offset, int32_t nv, uint32_t *count, uint64_t *start) {
+__global__ void member_table_sizes_kernel(const int64_t *offset, int32_t nv, uint32_t *count, uint64_t *start) {
     const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v == 0) start[0] = 0;
     if (v >= nv) return;
@@ -683,7 +683,7 @@ __global__ void member_tree_sizes_kernel(const int64_t *offset, int32_t nv, uint
 // [0, min(count, 32)) end up holding keys; the slots were set to kMemberEmpty beforehand).  The
 // order of the keys inside a bucket depends on the order of the claims; which keys a bucket holds
 // does not matter to a test, which looks at all of them.
-__global__ void member_tree_fill_kernel(const int64_t *offset, const uint64_t *start, int32_t nv, const int32_t *adj,
+__global__ void member_table_fill_kernel(const int64_t *offset, const uint64_t *start, int32_t nv, const int32_t *adj,
                                         int64_t ne, uint32_t *slots, uint32_t *fill_count) {
     for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
          e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -831,7 +831,7 @@ hipError_t launch_its_tree_roots_to_records(const DevGraph &g, const uint32_t *r
     return hipGetLastError();
 }
 
-hipError_t launch_member_tree_starts(const DevGraph &g, uint32_t *root, uint64_t *start, void *temp,
+hipError_t launch_member_table_starts(const DevGraph &g, uint32_t *root, uint64_t *start, void *temp,
                                      size_t &temp_bytes, hipStream_t stream) {
     if (g.nv <= 0) return hipSuccess;
     const size_t n = static_cast<size_t>(g.nv);
@@ -839,7 +839,7 @@ hipError_t launch_member_tree_starts(const DevGraph &g, uint32_t *root, uint64_t
         return rocprim::inclusive_scan<rocprim::default_config, const uint32_t *, uint64_t *, rocprim::plus<uint64_t>,
                                        uint64_t>(nullptr, temp_bytes, root, start + 1, n, rocprim::plus<uint64_t>(),
                                                  stream);
-    hipLaunchKernelGGL(member_tree_sizes_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv,
+    hipLaunchKernelGGL(member_table_sizes_kernel, dim3(grid_for(g.nv)), dim3(kBuildBlock), 0, stream, g.offset, g.nv,
                        root, start);
     hipError_t e = rocprim::inclusive_scan<rocprim::default_config, const uint32_t *, uint64_t *,
                                            rocprim::plus<uint64_t>, uint64_t>(temp, temp_bytes, root, start + 1, n,
@@ -850,7 +850,7 @@ hipError_t launch_member_tree_starts(const DevGraph &g, uint32_t *root, uint64_t
     return hipGetLastError();
 }
 
-hipError_t launch_member_tree_fill(const DevGraph &g, const uint64_t *start, uint64_t lines_total, uint4 *lines,
+hipError_t launch_member_table_fill(const DevGraph &g, const uint64_t *start, uint64_t lines_total, uint4 *lines,
                                    uint32_t *fill_count, hipStream_t stream) {
     if (g.nv <= 0 || lines_total == 0) return hipSuccess;
     uint32_t *slots = reinterpret_cast<uint32_t *>(lines);
@@ -858,7 +858,7 @@ hipError_t launch_member_tree_fill(const DevGraph &g, const uint64_t *start, uin
     if (e != hipSuccess) return e;
     e = launch_fill_u32(fill_count, 0u, static_cast<int64_t>(lines_total), stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(member_tree_fill_kernel, dim3(8192), dim3(256), 0, stream, g.offset, start, g.nv, g.adj, g.ne,
+    hipLaunchKernelGGL(member_table_fill_kernel, dim3(8192), dim3(256), 0, stream, g.offset, start, g.nv, g.adj, g.ne,
                        slots, fill_count);
     return hipGetLastError();
 }

@@ -171,9 +171,9 @@ __device__ __forceinline__ bool is_neighbor(const DevGraph &g, int32_t u, int64_
 
 // Graph::is_neighbor (util/graph/graph.h:180-199) with both endpoints' ranges known: the
 // larger-degree endpoint searched in the smaller-degree endpoint's sorted adjacency -- through
-// that endpoint's membership structure when it has one (root carried by the walker or the
+// that endpoint's membership table when it has one (root carried by the walker or the
 // neighbour record, else looked up), by binary search otherwise.  The answer is the same.
-__device__ __forceinline__ bool is_neighbor_tree(const DevGraph &g, int32_t u, int64_t uoff, int32_t udeg,
+__device__ __forceinline__ bool is_neighbor_member(const DevGraph &g, int32_t u, int64_t uoff, int32_t udeg,
                                                  uint32_t uroot, int32_t v, int64_t voff, int32_t vdeg,
                                                  uint32_t vroot) {
     if (udeg < vdeg) {
@@ -265,7 +265,7 @@ __device__ __forceinline__ int32_t read_neighbor_its(const DevGraph &g, int64_t 
     return g.adj[e];
 }
 
-// As read_neighbor, also returning the neighbour's membership structure root (Node2Vec).
+// As read_neighbor, also returning the neighbour's membership table root (Node2Vec).
 __device__ __forceinline__ int32_t read_neighbor_member(const DevGraph &g, int64_t e, uint32_t &noff, uint32_t &ndeg,
                                                         uint32_t &nroot) {
     if (g.nrec) {
@@ -609,7 +609,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
                 int32_t pd, cd;
                 known_range(g, s.prev, s.poff, s.pdeg, poff, pd);
                 known_range(g, c, noff, ndeg, coff, cd);
-                accept = (y <= (is_neighbor_tree(g, s.prev, poff, pd, s.pidx, c, coff, cd, croot) ? 1.0 : P.n2v_q));
+                accept = (y <= (is_neighbor_member(g, s.prev, poff, pd, s.pidx, c, coff, cd, croot) ? 1.0 : P.n2v_q));
             }
             if (accept) break;
         }

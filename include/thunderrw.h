@@ -244,7 +244,7 @@ int trw_graph_device_view(const trw_graph *g, trw_device_view *out);
 
 /* Membership tables for is_neighbor on the device (no reference counterpart: graph.h:180-199
  * binary-searches the smaller endpoint's sorted list; these answer the same question in one
- * 128-byte read): a hash set of the neighbours of every vertex of degree > 32, in 128-byte
+ * 128-byte read): a hash set of the neighbours of every vertex with neighbours, in 128-byte
  * buckets, layout in thunderrw_device.h.  Built once per graph (the first Node2Vec walk builds
  * them too); a graph with 2^32 edges or more gets none and its is_neighbor keeps the search. */
 int trw_build_membership(trw_graph *g);

@@ -64,23 +64,23 @@ __device__ __forceinline__ int32_t upper_bound_branchless(const double *src, int
     return static_cast<int32_t>((*base <= target) + (base - src));
 }
 
-// Membership table of one vertex's adjacency, for vertices of degree d > kMemberMin: a hash
-// set of its neighbours in nb = 2^member_bucket_bits(d) buckets of 128 bytes (32 int32 slots,
-// about 8 to 16 keys per bucket).  Neighbour u lives in bucket member_hash(u) or, when that one
+// Membership table of one vertex's adjacency, for every vertex of degree d > kMemberMin (0: all
+// with neighbours): a hash set of its neighbours in nb = 2^member_bucket_bits(d) buckets of 128
+// bytes (32 int32 slots, up to 16 keys per bucket on average; one bucket up to degree 16).  Neighbour u lives in bucket member_hash(u) or, when that one
 // is full, the next bucket (cyclically) with room; a bucket's keys fill its slots from the first,
 // the other slots hold kMemberEmpty.  A vertex's table starts at line root[v] (kNoRoot: none).
 constexpr int kMemberSlots = 32;
-constexpr int kMemberMin = 32;
+constexpr int kMemberMin = 0;
 constexpr uint32_t kMemberEmpty = 0xffffffffu;
 constexpr uint32_t kNoRoot = 0xffffffffu;
 
 __host__ __device__ inline uint32_t member_bucket_bits(int32_t d) {
-    const uint32_t need = static_cast<uint32_t>((d + 15) / 16);  // >= 3 for d > kMemberMin
+    const uint32_t need = static_cast<uint32_t>((d + 15) / 16);
     return need <= 1u ? 0u : 32u - static_cast<uint32_t>(__builtin_clz(need - 1u));
 }
 
 __host__ __device__ inline uint32_t member_hash(int32_t u, uint32_t bits) {
-    return (static_cast<uint32_t>(u) * 0x9E3779B1u) >> (32u - bits);  // bits >= 2
+    return bits == 0u ? 0u : (static_cast<uint32_t>(u) * 0x9E3779B1u) >> (32u - bits);
 }
 
 // Half of a bucket: 16 keys in 4 16-byte loads issued together.  A search loads the second

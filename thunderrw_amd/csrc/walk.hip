@@ -1030,6 +1030,159 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
     flush_counters(io, steps, gterm, dead, err);
 }
 
+// Node2Vec, fused form, with membership tables (max-weight rejection, rejection_sampling.h:368-402,
+// Node2Vec::weight node2vec.cpp:24-38): a walker per lane as a state machine that issues one
+// read per iteration -- the candidate record of a trial, or one half of a membership bucket -- and
+// then takes the decision that read allows.  In the step-per-iteration kernel a wavefront's step
+// lasts as long as its slowest lane's chain of trials and tests (5.5 reads against a lane mean of
+// 1.7 on the C3 graph) while the other lanes issue nothing; here no lane waits for another.  The
+// draws, comparisons and moves are the Move's: only their schedule across lanes differs.  Paths:
+// each lane stages 16 positions of its own row in LDS and writes them when the chunk is complete.
+constexpr int kN2vBlock = 256;
+constexpr int kN2vChunk = 16;
+
+template <bool REPLAY>
+__global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g, KindParams P, BatchIO io) {
+    __shared__ int32_t chunk[kN2vBlock * (kN2vChunk + 1)];
+    int32_t *mine = chunk + threadIdx.x * (kN2vChunk + 1);
+    const int32_t w = static_cast<int32_t>(blockIdx.x) * kN2vBlock + static_cast<int32_t>(threadIdx.x);
+    const int32_t L = P.length;
+    const double lo = fmin(1.0, P.n2v_q), hi = fmax(1.0, P.n2v_q);
+    WalkerState s = empty_walker();
+    if (w < io.n) init_walker(g, io, w, s);
+    int64_t off = 0;
+    int32_t d = 0;
+    if (s.st == kAlive) {
+        known_range(g, s.cur, s.coff, s.cdeg, off, d);
+        s.cidx = g.mroot[s.cur];
+    }
+    int32_t *row = (io.paths != nullptr && w < io.n) ? io.paths + static_cast<int64_t>(w) * io.path_stride : nullptr;
+    int32_t staged = 0;  // positions [len - staged, len) of the row sit in `mine`
+    if (row != nullptr) {
+        mine[0] = s.cur;
+        staged = 1;
+    }
+    uint32_t trial = 0;
+    bool testing = false;  // the next read is a membership bucket half, not a candidate
+    int32_t c = -1;        // the trial's candidate and its range, table root and draw y
+    uint32_t cnoff = 0u, cndeg = 0u, croot = kNoRange;
+    double y = 0.0;
+    uint32_t mline = 0u, mb = 0u, mmask = 0u, mleft = 0u, mhalf = 0u;  // the bucket cursor of a test
+    int32_t mkey = 0;
+    unsigned long long steps = 0;
+
+    for (;;) {
+        if (__ballot(s.st == kAlive) == 0) break;
+        bool decided = false, accept = false;
+        if (s.st == kAlive && !testing) {
+            if (d <= 0) {
+                s.st = kDeadEnd;
+            } else if (P.max_trials > 0 && trial >= static_cast<uint32_t>(P.max_trials)) {
+                s.st = kTrialCap;
+            } else {
+                uint32_t ua = 0u, ub = 0u;
+                bool ok = true;
+                if (REPLAY) {
+                    ok = replay_take(io, w, s, ua) && replay_take(io, w, s, ub);
+                } else {
+                    const uint4 x = philox_block(io, s, static_cast<uint32_t>(s.len - 1), trial);
+                    ua = x.x;
+                    ub = x.y;
+                }
+                if (ok) {
+                    const uint4 r = g.nrec[off + (ua % static_cast<uint32_t>(d))];
+                    c = static_cast<int32_t>(r.x);
+                    cnoff = r.y;
+                    cndeg = r.z;
+                    croot = r.w;
+                    y = real2(ub) * P.n2v_max;
+                    if (s.len == 1) accept = (y <= P.n2v_max), decided = true;
+                    else if (c == s.prev) accept = (y <= P.n2v_p), decided = true;
+                    else if (y <= lo) accept = true, decided = true;
+                    else if (y > hi) accept = false, decided = true;
+                    else {
+                        // is_neighbor(prev, c): the larger-degree endpoint looked up in the smaller
+                        // one's table (graph.h:180-199; equal degrees: prev in c's).
+                        const bool in_prev = static_cast<int32_t>(s.pdeg) < static_cast<int32_t>(cndeg);
+                        const int32_t md = in_prev ? static_cast<int32_t>(s.pdeg) : static_cast<int32_t>(cndeg);
+                        mline = in_prev ? s.pidx : croot;
+                        mkey = in_prev ? c : s.prev;
+                        if (md <= 0 || mline == kNoRange) {
+                            // An empty list (a directed graph's sink) holds nothing: not a neighbour.
+                            accept = y <= P.n2v_q;
+                            decided = true;
+                        } else {
+                            const uint32_t bits = member_bucket_bits(md);
+                            mb = member_hash(mkey, bits);
+                            mmask = (1u << bits) - 1u;
+                            mleft = 1u << bits;
+                            mhalf = 0u;
+                            testing = true;
+                        }
+                    }
+                }
+            }
+        } else if (s.st == kAlive) {
+            trw_dev::MemberHalf h;
+            h.load(g.mlines + static_cast<uint64_t>(mline + mb) * 8 + mhalf * 4);
+            bool found = h.contains(static_cast<uint32_t>(mkey)), done = found || !h.full();
+            if (!done) {
+                if (mhalf == 0u) {
+                    mhalf = 1u;
+                } else {
+                    mhalf = 0u;
+                    mb = (mb + 1u) & mmask;
+                    done = --mleft == 0u;
+                }
+            }
+            if (done) {
+                testing = false;
+                accept = y <= (found ? 1.0 : P.n2v_q);
+                decided = true;
+            }
+        }
+        if (decided) {
+            if (accept) {
+                s.prev = s.cur;
+                s.poff = static_cast<uint32_t>(off);
+                s.pdeg = static_cast<uint32_t>(d);
+                s.pidx = s.cidx;
+                s.cur = c;
+                s.coff = cnoff;
+                s.cdeg = cndeg;
+                s.cidx = croot;
+                s.len += 1;
+                ++steps;
+                off = static_cast<int64_t>(cnoff);
+                d = static_cast<int32_t>(cndeg);
+                trial = 0u;
+                if (s.len == L) s.st = kDone;
+                if (row != nullptr) {
+                    mine[staged++] = c;
+                    if (staged == kN2vChunk || s.len == L) {
+                        const int32_t at = s.len - staged;
+                        for (int32_t j = 0; j < staged; ++j) row[at + j] = mine[j];
+                        staged = 0;
+                    }
+                }
+            } else {
+                ++trial;
+            }
+        }
+    }
+    if (w < io.n) {
+        if (row != nullptr) {  // positions the walk did not reach hold -1, as the other forms leave them
+            const int32_t at = s.len - staged;
+            for (int32_t j = 0; j < staged; ++j) row[at + j] = mine[j];
+            for (int32_t j = s.len; j < L; ++j) row[j] = -1;
+        }
+        write_outputs(io, w, s);
+    }
+    unsigned long long gterm = 0, dead = 0, err = 0;
+    if (w < io.n) classify(s.st, gterm, dead, err);
+    flush_counters(io, steps, gterm, dead, err);
+}
+
 // ---------------------------------------------------------------- queue form
 template <int MODE, bool REPLAY>
 __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, KindParams A, KindParams B,
@@ -1360,7 +1513,14 @@ hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO 
     case kModeAlias: hipLaunchKernelGGL((walk_fused_kernel<kModeAlias, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     case kModeIts: hipLaunchKernelGGL((walk_fused_kernel<kModeIts, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     case kModeRejection: hipLaunchKernelGGL((walk_fused_kernel<kModeRejection, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
-    case kModeNode2Vec: hipLaunchKernelGGL((walk_fused_node2vec_kernel<REPLAY>), grid, block, lds, stream, g, p, io, T); break;
+    case kModeNode2Vec:
+        if (g.nrec && g.mlines && g.mroot) {
+            hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY>), dim3(grid_for(io.n, kN2vBlock)), dim3(kN2vBlock), 0,
+                               stream, g, p, io);
+        } else {
+            hipLaunchKernelGGL((walk_fused_node2vec_kernel<REPLAY>), grid, block, lds, stream, g, p, io, T);
+        }
+        break;
     default: hipLaunchKernelGGL((walk_fused_kernel<kModeMetaPath, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     }
     return hipGetLastError();

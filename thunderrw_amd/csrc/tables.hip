@@ -669,7 +669,7 @@ __global__ void its_tree_roots_kernel(const int32_t *adj, int64_t ne, const uint
 }
 
 // Membership tables.  Sizes: one thread per vertex writes its bucket count (none for degree <=
-// kMemberMin, whose tests binary-search the adjacency itself).
+// kMemberMin, i.e. a vertex without neighbours).
 __global__ void member_table_sizes_kernel(const int64_t *offset, int32_t nv, uint32_t *count, uint64_t *start) {
     const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v == 0) start[0] = 0;

@@ -60,6 +60,10 @@ __device__ __forceinline__ uint4 philox_block(const BatchIO &io, const WalkerSta
     return philox4x32_10(s.gid_lo, s.gid_hi, step, trial, io.key0, io.key1);
 }
 
+// Holds v as computed at this point of the program (an empty volatile asm takes it as an
+// operand), so the compiler cannot sink its computation below a following wait.
+__device__ __forceinline__ void pin(uint4 &v) { asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w)); }
+
 __device__ __forceinline__ bool replay_take(const BatchIO &io, int32_t w, WalkerState &s, uint32_t &u) {
     if (static_cast<int64_t>(s.k) >= io.K) {
         s.st = kReplayExhausted;
@@ -1041,6 +1045,40 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
 constexpr int kN2vBlock = 256;
 constexpr int kN2vChunk = 16;
 
+// A lane's own path row for the state-machine kernel: positions [len - staged, len) wait in
+// `mine` (LDS, kN2vChunk + 1 ints per lane) and go out when kN2vChunk have gathered or the walk
+// ends; at the end, positions the walk did not reach are -1, as the other forms leave them.
+struct LaneRow {
+    int32_t *row;   // the walker's row, or null when paths are not written
+    int32_t *mine;
+    int32_t staged;
+    __device__ __forceinline__ void put(int32_t v, int32_t len, int32_t L) {
+        if (row == nullptr) return;
+        mine[staged++] = v;
+        if (staged == kN2vChunk || len == L) flush(len);
+    }
+    __device__ __forceinline__ void flush(int32_t len) {
+        const int32_t at = len - staged;
+        int32_t *dst = row + at;
+        if (staged == kN2vChunk && (reinterpret_cast<uintptr_t>(dst) & 15u) == 0u) {
+            // A whole chunk at a 16-byte boundary: four 16-byte stores instead of sixteen 4-byte ones.
+            uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+            for (int32_t q = 0; q < kN2vChunk / 4; ++q)
+                d4[q] = make_uint4(static_cast<uint32_t>(mine[4 * q]), static_cast<uint32_t>(mine[4 * q + 1]),
+                                   static_cast<uint32_t>(mine[4 * q + 2]), static_cast<uint32_t>(mine[4 * q + 3]));
+        } else {
+            for (int32_t j = 0; j < staged; ++j) dst[j] = mine[j];
+        }
+        staged = 0;
+    }
+    __device__ __forceinline__ void finish(int32_t len, int32_t L) {
+        if (row == nullptr) return;
+        flush(len);
+        for (int32_t j = len; j < L; ++j) row[j] = -1;
+    }
+};
+
 template <bool REPLAY>
 __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g, KindParams P, BatchIO io) {
     __shared__ int32_t chunk[kN2vBlock * (kN2vChunk + 1)];
@@ -1056,13 +1094,11 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
         known_range(g, s.cur, s.coff, s.cdeg, off, d);
         s.cidx = g.mroot[s.cur];
     }
-    int32_t *row = (io.paths != nullptr && w < io.n) ? io.paths + static_cast<int64_t>(w) * io.path_stride : nullptr;
-    int32_t staged = 0;  // positions [len - staged, len) of the row sit in `mine`
-    if (row != nullptr) {
-        mine[0] = s.cur;
-        staged = 1;
-    }
+    LaneRow out{(io.paths != nullptr && w < io.n) ? io.paths + static_cast<int64_t>(w) * io.path_stride : nullptr, mine, 0};
+    out.put(s.cur, 1, L);
     uint32_t trial = 0;
+    uint4 blk = first_block<REPLAY>(io, w, s);  // free-running: the current trial's block
+    uint4 rej = blk, acc = blk;                 // the next trial's and the next step's
     bool testing = false;  // the next read is a membership bucket half, not a candidate
     int32_t c = -1;        // the trial's candidate and its range, table root and draw y
     uint32_t cnoff = 0u, cndeg = 0u, croot = kNoRange;
@@ -1085,12 +1121,17 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
                 if (REPLAY) {
                     ok = replay_take(io, w, s, ua) && replay_take(io, w, s, ub);
                 } else {
-                    const uint4 x = philox_block(io, s, static_cast<uint32_t>(s.len - 1), trial);
-                    ua = x.x;
-                    ub = x.y;
+                    ua = blk.x;
+                    ub = blk.y;
                 }
                 if (ok) {
                     const uint4 r = g.nrec[off + (ua % static_cast<uint32_t>(d))];
+                    if (!REPLAY) {  // both possible next blocks, in the candidate read's shadow
+                        rej = philox_block(io, s, static_cast<uint32_t>(s.len - 1), trial + 1u);
+                        acc = philox_block(io, s, static_cast<uint32_t>(s.len), 0u);
+                        pin(rej);
+                        pin(acc);
+                    }
                     c = static_cast<int32_t>(r.x);
                     cnoff = r.y;
                     cndeg = r.z;
@@ -1157,25 +1198,16 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
                 d = static_cast<int32_t>(cndeg);
                 trial = 0u;
                 if (s.len == L) s.st = kDone;
-                if (row != nullptr) {
-                    mine[staged++] = c;
-                    if (staged == kN2vChunk || s.len == L) {
-                        const int32_t at = s.len - staged;
-                        for (int32_t j = 0; j < staged; ++j) row[at + j] = mine[j];
-                        staged = 0;
-                    }
-                }
+                out.put(c, s.len, L);
+                blk = acc;
             } else {
                 ++trial;
+                blk = rej;
             }
         }
     }
     if (w < io.n) {
-        if (row != nullptr) {  // positions the walk did not reach hold -1, as the other forms leave them
-            const int32_t at = s.len - staged;
-            for (int32_t j = 0; j < staged; ++j) row[at + j] = mine[j];
-            for (int32_t j = s.len; j < L; ++j) row[j] = -1;
-        }
+        out.finish(s.len, L);
         write_outputs(io, w, s);
     }
     unsigned long long gterm = 0, dead = 0, err = 0;
@@ -1514,7 +1546,7 @@ hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO 
     case kModeIts: hipLaunchKernelGGL((walk_fused_kernel<kModeIts, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     case kModeRejection: hipLaunchKernelGGL((walk_fused_kernel<kModeRejection, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     case kModeNode2Vec:
-        if (g.nrec && g.mlines && g.mroot) {
+        if (g.nrec && g.mlines && g.mroot && !p.ppr) {
             hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY>), dim3(grid_for(io.n, kN2vBlock)), dim3(kN2vBlock), 0,
                                stream, g, p, io);
         } else {

@@ -273,17 +273,19 @@ def test_replay_exhaustion_and_bad_sources_are_reported(gpu_ctx):
     dg = dgraph(gpu_ctx, g)
     try:
         src = np.concatenate([src[:50], np.array([-1, g.nv, 10 ** 6], np.int32)]).astype(np.int32)
-        case = {"app": "deepwalk", "execution": 1, "sampling": 2, "length": 20, "extra": {}}
-        K = 11  # fewer than the 38 draws a 20-vertex alias walk needs
-        ref = oracle_walk(g, case, src, K, 1, 5, 6)
-        assert np.all(ref["status"][:50] == T._abi.WALKER_REPLAY_EXHAUSTED)
-        assert np.all(ref["status"][50:] == T._abi.WALKER_BAD_SOURCE)
-        draws = T.replay_fill(5, len(src), K)
-        for form in forms_for("deepwalk"):
-            r = T.walk(dg, case_params(case, form), src, draws, T.replay_fill(6, len(src), 1))
-            for key in ("paths", "lengths", "ends", "status", "draws_used"):
-                assert np.array_equal(r[key], ref[key]), (form, key)
-            assert r["errors"] == len(src)
+        K = 11  # fewer than the draws a 20-vertex walk needs with any of these Moves (19 or more)
+        for app, em, sm, extra in (("deepwalk", 1, 2, {}), ("deepwalk", 1, 1, {}), ("deepwalk", 1, 3, {}),
+                                   ("node2vec", 2, 4, {"-p": "2", "-q": "0.5"})):
+            case = {"app": app, "execution": em, "sampling": sm, "length": 20, "extra": extra}
+            ref = oracle_walk(g, case, src, K, 1, 5, 6)
+            assert np.all(ref["status"][:50] == T._abi.WALKER_REPLAY_EXHAUSTED), (app, sm)
+            assert np.all(ref["status"][50:] == T._abi.WALKER_BAD_SOURCE), (app, sm)
+            draws = T.replay_fill(5, len(src), K)
+            for form in forms_for(app):
+                r = T.walk(dg, case_params(case, form), src, draws, T.replay_fill(6, len(src), 1))
+                for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                    assert np.array_equal(r[key], ref[key]), (app, sm, form, key)
+                assert r["errors"] == len(src), (app, sm, form)
     finally:
         dg.close()
 
@@ -857,3 +859,29 @@ def test_its_guide_from_a_loaded_prefix_sum_file(gpu_ctx, tmp_path):
             assert np.array_equal(r[key], ref[key]), key
     finally:
         dg2.close()
+
+
+def test_node2vec_trial_cap_matches_oracle(gpu_ctx):
+    """A rejection-trial cap (TRW_WALKER_TRIAL_CAP) ends Node2Vec walkers where the oracle's
+    capped Move does, in every form (the fused form's per-lane state machine included), with
+    the draws consumed up to the cap."""
+    g = graphs.chung_lu(5000, 30000, 2.0, 2, 1500, 121)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        src = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        K = 4 * 30 + 64
+        for cap in (1, 2):
+            case = {"app": "node2vec", "execution": 2, "sampling": 4, "length": 30, "extra": {"-p": "2", "-q": "0.5"}}
+            a = golden_case_args(case)
+            og = ob.Graph(g)
+            d_wm = ob.sfmt_stream(122, len(src) * K).reshape(len(src), K)
+            ref = ob.walk(og, "node2vec", 2, 4, 30, src, d_wm, K, p=a["p"], q=a["q"], max_trials=cap)
+            assert (ref["status"] == T._abi.WALKER_TRIAL_CAP).sum() > 0, cap
+            for form in forms_for("node2vec"):
+                prm = T.make_params("node2vec", execution="dynamic", sampling="max_weight_rejection", length=30,
+                                    p=2.0, q=0.5, rng="replay", form=form, max_trials=cap)
+                r = T.walk(dg, prm, src, np.ascontiguousarray(d_wm.T))
+                for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                    assert np.array_equal(r[key], ref[key]), (cap, form, key)
+    finally:
+        dg.close()

@@ -1045,7 +1045,7 @@ __global__ __launch_bounds__(kMaxFusedBlock) void walk_fused_multi_kernel(DevGra
 constexpr int kN2vBlock = 256;
 constexpr int kN2vChunk = 16;
 
-// A lane's own path row for the state-machine kernel: positions [len - staged, len) wait in
+// A lane's own path row (the Node2Vec state machine, the queue form): positions [len - staged, len) wait in
 // `mine` (LDS, kN2vChunk + 1 ints per lane) and go out when kN2vChunk have gathered or the walk
 // ends; at the end, positions the walk did not reach are -1, as the other forms leave them.
 struct LaneRow {
@@ -1057,19 +1057,20 @@ struct LaneRow {
         mine[staged++] = v;
         if (staged == kN2vChunk || len == L) flush(len);
     }
+    // Entries before the first 16-byte boundary one by one, then 16-byte stores, then the rest:
+    // a chunk is a few store requests instead of one per entry.
     __device__ __forceinline__ void flush(int32_t len) {
-        const int32_t at = len - staged;
-        int32_t *dst = row + at;
-        if (staged == kN2vChunk && (reinterpret_cast<uintptr_t>(dst) & 15u) == 0u) {
-            // A whole chunk at a 16-byte boundary: four 16-byte stores instead of sixteen 4-byte ones.
-            uint4 *d4 = reinterpret_cast<uint4 *>(dst);
-#pragma unroll
-            for (int32_t q = 0; q < kN2vChunk / 4; ++q)
-                d4[q] = make_uint4(static_cast<uint32_t>(mine[4 * q]), static_cast<uint32_t>(mine[4 * q + 1]),
-                                   static_cast<uint32_t>(mine[4 * q + 2]), static_cast<uint32_t>(mine[4 * q + 3]));
-        } else {
-            for (int32_t j = 0; j < staged; ++j) dst[j] = mine[j];
+        int32_t *dst = row + (len - staged);
+        int32_t j = 0;
+        while (j < staged && (reinterpret_cast<uintptr_t>(dst + j) & 15u) != 0u) {
+            dst[j] = mine[j];
+            ++j;
         }
+        for (; j + 4 <= staged; j += 4)
+            *reinterpret_cast<uint4 *>(dst + j) =
+                make_uint4(static_cast<uint32_t>(mine[j]), static_cast<uint32_t>(mine[j + 1]),
+                           static_cast<uint32_t>(mine[j + 2]), static_cast<uint32_t>(mine[j + 3]));
+        for (; j < staged; ++j) dst[j] = mine[j];
         staged = 0;
     }
     __device__ __forceinline__ void finish(int32_t len, int32_t L) {
@@ -1233,6 +1234,10 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
     // rather than per refill keeps the single queue head off the critical path.
     unsigned int qnext = 0, qend = 0;
     bool queue_empty = false;
+    // The lane's walker's path entries gather here and go out in runs (the rows were set to -1
+    // beforehand), instead of one store request per entry.
+    __shared__ int32_t pchunk[kQueueBlock * (kN2vChunk + 1)];
+    LaneRow out{nullptr, pchunk + threadIdx.x * (kN2vChunk + 1), 0};
 
     for (;;) {
         const bool need = (s.st != kAlive) && !drained;
@@ -1269,10 +1274,12 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                     blk = first_block<REPLAY>(io, w, s);
                     const KindParams &P = kind_b ? B : A;
                     if (io.paths) {
-                        int32_t *rowp = io.paths + static_cast<int64_t>(w) * io.path_stride;
-                        rowp[0] = s.cur;
+                        out.row = io.paths + static_cast<int64_t>(w) * io.path_stride;
+                        out.staged = 0;
+                        out.put(s.cur, 1, P.length);
                     }
                     if (s.st != kAlive) {  // bad source: finished before the first step
+                        if (io.paths) out.flush(1);
                         write_outputs(io, w, s);
                         classify(s.st, gterm, dead, err);
                     }
@@ -1290,9 +1297,10 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                 kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
             if (moved) {
                 ++steps;
-                if (io.paths && s.len - 1 < P.length) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
+                if (io.paths && s.len - 1 < P.length) out.put(s.cur, s.len, P.length);
             }
             if (s.st != kAlive) {
+                if (io.paths) out.flush(s.len < P.length ? s.len : P.length);
                 write_outputs(io, w, s);
                 classify(s.st, gterm, dead, err);
             }

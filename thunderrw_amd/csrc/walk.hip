@@ -1579,12 +1579,14 @@ void fill_path_rows(int32_t *paths, int64_t rows, int32_t stride, int32_t L, hip
                            stride, L);
 }
 
+// Every form writes each walker's position 0 (its source, valid or not) itself, so rows of length
+// 1 (PPR's) need no -1 beforehand.
 void fill_paths(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io, hipStream_t stream) {
     if (!io.paths || io.n <= 0) return;
     const int32_t La = a.length > 0 ? a.length : 1, Lb = b.length > 0 ? b.length : 1;
     const int32_t sa = split < io.n ? split : io.n;
-    fill_path_rows(io.paths, sa, io.path_stride, La, stream);
-    fill_path_rows(io.paths + static_cast<int64_t>(sa) * io.path_stride, io.n - sa, io.path_stride, Lb, stream);
+    if (La > 1) fill_path_rows(io.paths, sa, io.path_stride, La, stream);
+    if (Lb > 1) fill_path_rows(io.paths + static_cast<int64_t>(sa) * io.path_stride, io.n - sa, io.path_stride, Lb, stream);
 }
 
 template <int MODE, bool REPLAY>

@@ -360,7 +360,18 @@ def test_free_running_ppr_matches_power_iteration(gpu_ctx):
         dg.close()
 
 
-def test_mixed_batch_queue_equals_step_and_single_kinds(gpu_ctx):
+# Kind pairs of a mixed batch (the kind's mode is read per walker at run time in every form).
+MIX_PAIRS = [
+    (("ppr", {}), ("metapath", {"schema": [0, 1, 2, 3, 4]})),
+    (("ppr", {"execution": "static", "sampling": "alias"}), ("deepwalk", {"execution": "static", "sampling": "rejection"})),
+    (("deepwalk", {"execution": "static", "sampling": "its"}), ("metapath", {"schema": [1, 0, 2]})),
+    (("ppr", {"execution": "static", "sampling": "alias"}), ("node2vec", {"p": 2.0, "q": 0.5})),
+]
+
+
+@pytest.mark.parametrize("pair", MIX_PAIRS, ids=["ppr+metapath", "alias+rejection", "its+metapath", "alias+node2vec"])
+def test_mixed_batch_queue_equals_step_and_single_kinds(gpu_ctx, pair):
+    (app_a, kw_a), (app_b, kw_b) = pair
     g = graphs.chung_lu(30000, 200000, 2.1, 2, 4000, 71)
     dg = dgraph(gpu_ctx, g)
     try:
@@ -371,16 +382,16 @@ def test_mixed_batch_queue_equals_step_and_single_kinds(gpu_ctx):
         split = n // 2
         res = {}
         for form in ("queue", "step", "graph"):
-            a = T.make_params("ppr", length=80, stop_probability=0.15, rng="philox", seed=5, form=form)
-            b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=5, form=form)
+            a = T.make_params(app_a, length=80, stop_probability=0.15, rng="philox", seed=5, form=form, **kw_a)
+            b = T.make_params(app_b, length=80, stop_probability=0.15, rng="philox", seed=5, form=form, **kw_b)
             res[form] = T.walk(dg, a, src, mix=(b, split))
         for key in ("paths", "lengths", "ends", "status"):
             assert np.array_equal(res["queue"][key], res["step"][key]), key
             assert np.array_equal(res["queue"][key], res["graph"][key]), key
         assert res["queue"]["steps"] == res["step"]["steps"] == res["graph"]["steps"]
         # Each kind alone (keyed by the same global walker ids) gives the same walks as in the mix.
-        a = T.make_params("ppr", length=80, stop_probability=0.15, rng="philox", seed=5)
-        b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=5)
+        a = T.make_params(app_a, length=80, stop_probability=0.15, rng="philox", seed=5, **kw_a)
+        b = T.make_params(app_b, length=80, stop_probability=0.15, rng="philox", seed=5, **kw_b)
         ra = T.walk(dg, a, src[:split])
         rb = T.walk(dg, b, src[split:], walker_id_base=split)
         for key in ("paths", "lengths", "ends", "status"):

@@ -1220,7 +1220,7 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
 template <int MODE, bool REPLAY>
 __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, KindParams A, KindParams B,
                                                                   int32_t split, BatchIO io, unsigned int *queue,
-                                                                  unsigned int chunk) {
+                                                                  unsigned int chunk, unsigned int rot) {
     const int lane = threadIdx.x & 63;
     WalkerState s = empty_walker();
     uint4 blk = make_uint4(0u, 0u, 0u, 0u);
@@ -1268,7 +1268,9 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
             if (queue_empty && qnext >= qend) drained = true;
             if (need) {
                 if (idx < n) {
-                    w = static_cast<int32_t>(idx);
+                    // Queue position idx runs walker idx + rot (mod n): the kind with the longer
+                    // walks starts first, so the batch ends on short walks.
+                    w = static_cast<int32_t>(idx < n - rot ? idx + rot : idx - (n - rot));
                     kind_b = w >= split;
                     init_walker(g, io, w, s);
                     blk = first_block<REPLAY>(io, w, s);
@@ -1597,8 +1599,16 @@ void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, i
     const int64_t waves = static_cast<int64_t>(grid) * (kQueueBlock / 64);
     int64_t chunk = a.queue_chunk > 0 ? a.queue_chunk : static_cast<int64_t>(io.n) / (waves * 2);
     chunk = chunk < 64 ? 64 : (chunk > 4096 ? 4096 : chunk);
+    // Expected walk length of a kind: its length, or min(length, 1 / stop probability) for a walk
+    // that stops by probability.  The kind whose walks run longer takes the queue's first positions.
+    const auto expected = [](const KindParams &p) {
+        const double L = p.length > 0 ? p.length : 1.0;
+        return p.ppr && p.stop_prob > 0.0 ? std::min(L, 1.0 / p.stop_prob) : L;
+    };
+    const unsigned int rot =
+        (split > 0 && split < io.n && expected(b) > expected(a)) ? static_cast<unsigned int>(split) : 0u;
     hipLaunchKernelGGL((walk_queue_kernel<MODE, REPLAY>), dim3(grid), dim3(kQueueBlock), 0, stream, g, a, b, split, io, q,
-                       static_cast<unsigned int>(chunk));
+                       static_cast<unsigned int>(chunk), rot);
 }
 
 template <bool REPLAY>

@@ -290,6 +290,18 @@ int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_wal
  * are built lazily by the first walk that needs them unless trw_build_tables ran. */
 int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the walk
+ * trw_walk_async runs, with the same kernels and results, that calls no hipMalloc, hipFree
+ * or other allocating routine.  The caller passes inputs and outputs as for trw_walk_async;
+ * the tables the walk reads must already be built (else TRW_E_STATE: nothing is built here).
+ * Forms AUTO, FUSED, QUEUE and STEP (GRAPH: TRW_E_UNSUPPORTED).  The QUEUE form's work-queue
+ * head and the STEP form's live flag and walker state live in a fixed device array of
+ * 16 MiB + 256 bytes that each device holds until the process ends; the STEP form therefore
+ * takes at most 262,144 walkers (more: TRW_E_INVALID).  Every call on one device shares that
+ * array, so calls that overlap in time on one device race. */
+int trw_walk_noalloc(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream);
+
 /* Whether the library has a Move kernel for params' (app, execution, sampling): TRW_OK, or
  * TRW_E_UNSUPPORTED (TRW_E_INVALID for values outside the reference's enums).  The pairs
  * the reference's apps accept (-em/-sm of apps/<app>.cpp) that the library has no kernel for --

@@ -7,8 +7,8 @@ call does, and raises if it has not been built.
 from ._abi import TrwError, exported_names, lib  # noqa: F401
 from .engine import (ALIAS_DTYPE, APP_DEFAULTS, Context, DeviceBuffer, DeviceGraph, DeviceTimer,  # noqa: F401
                      HostGraph, Server, device_count, make_io, make_params, replay_fill, sources_all, sources_random,
-                     sources_single, synchronize, walk, walk_async, walk_mix_async)
+                     sources_single, synchronize, walk, walk_async, walk_mix_async, walk_noalloc)
 
 __all__ = ["TrwError", "Context", "DeviceBuffer", "DeviceGraph", "DeviceTimer", "HostGraph", "Server", "device_count",
            "make_io", "make_params", "replay_fill", "sources_all", "sources_random", "sources_single", "synchronize",
-           "walk", "walk_async", "walk_mix_async", "ALIAS_DTYPE", "APP_DEFAULTS", "exported_names", "lib"]
+           "walk", "walk_async", "walk_mix_async", "walk_noalloc", "ALIAS_DTYPE", "APP_DEFAULTS", "exported_names", "lib"]

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -77,6 +78,34 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+// trw_walk_noalloc's scratch memory, a file-scope device array so that the entry point never
+// allocates: the first kNoAllocControlBytes hold the QUEUE form's work-queue head (word 0) and
+// the STEP form's live flag (word 1), the rest the STEP form's walker state for at most
+// kNoAllocMaxStepWalkers walkers.  Every device the library runs on holds its own copy until
+// the process ends.  In host code `scratch` is a stand-in, not a device address: host code
+// reaches the array only through the per-device address scratch_address() resolves.
+constexpr int32_t kNoAllocMaxStepWalkers = 1 << 18;
+constexpr size_t kNoAllocControlBytes = 256;
+constexpr size_t kNoAllocScratchBytes = kNoAllocControlBytes + sizeof(WalkerState) * kNoAllocMaxStepWalkers;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // per device: its copy of `scratch`, resolved once
+
+// The address of `scratch` on `device`, which must be the current device.
+hipError_t scratch_address(int device, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr.size() <= static_cast<size_t>(device)) g_scratch_ptr.resize(static_cast<size_t>(device) + 1, nullptr);
+    if (!g_scratch_ptr[static_cast<size_t>(device)]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[static_cast<size_t>(device)] = p;
+    }
+    *out = g_scratch_ptr[static_cast<size_t>(device)];
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct trw_host_graph {
@@ -123,6 +152,7 @@ int trw_context_create(int32_t device, trw_context **out) {
     TRW_HIP(hipSetDevice(device), "hipSetDevice");
     std::unique_ptr<trw_context> ctx(new trw_context);
     ctx->device = device;
+    TRW_HIP(scratch_address(device, &ctx->scratch), "hipGetSymbolAddress (trw_walk_noalloc scratch)");
     TRW_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     *out = ctx.release();
     return TRW_OK;
@@ -982,25 +1012,32 @@ static int ensure_state(trw_graph *g, int64_t n) {
     return TRW_OK;
 }
 
+// Plain per-step form over walker state in `state` (io.n entries) with a live flag at `alive_flag`.
+static int run_step_form_on(trw_graph *g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                            hipStream_t st, WalkerState *state, unsigned int *alive_flag) {
+    if (io.n == 0) return TRW_OK;
+    const DevGraph dv = g->view();
+    TRW_HIP(trw::launch_walk_step_init(dv, a, b, split, io, state, st), "step-form init");
+    unsigned int alive = 1;
+    const int kChunk = 8;
+    while (alive) {
+        for (int i = 0; i < kChunk; ++i) {
+            TRW_HIP(trw::launch_fill_u32(alive_flag, 0u, 1, st), "step-form reset");
+            TRW_HIP(trw::launch_walk_step(dv, a, b, split, io, state, alive_flag, st), "step-form step");
+        }
+        TRW_HIP(hipMemcpyAsync(&alive, alive_flag, sizeof(alive), hipMemcpyDeviceToHost, st), "read alive");
+        TRW_HIP(hipStreamSynchronize(st), "step-form synchronize");
+    }
+    TRW_HIP(trw::launch_walk_step_finish(a, b, split, io, state, st), "step-form finish");
+    return TRW_OK;
+}
+
 static int run_step_form(trw_graph *g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                          hipStream_t st) {
     if (io.n == 0) return TRW_OK;
     int rc = ensure_state(g, io.n);
     if (rc != TRW_OK) return rc;
-    const DevGraph dv = g->view();
-    TRW_HIP(trw::launch_walk_step_init(dv, a, b, split, io, g->state, st), "step-form init");
-    unsigned int alive = 1;
-    const int kChunk = 8;
-    while (alive) {
-        for (int i = 0; i < kChunk; ++i) {
-            TRW_HIP(trw::launch_fill_u32(g->counters + 1, 0u, 1, st), "step-form reset");
-            TRW_HIP(trw::launch_walk_step(dv, a, b, split, io, g->state, g->counters + 1, st), "step-form step");
-        }
-        TRW_HIP(hipMemcpyAsync(&alive, g->counters + 1, sizeof(alive), hipMemcpyDeviceToHost, st), "read alive");
-        TRW_HIP(hipStreamSynchronize(st), "step-form synchronize");
-    }
-    TRW_HIP(trw::launch_walk_step_finish(a, b, split, io, g->state, st), "step-form finish");
-    return TRW_OK;
+    return run_step_form_on(g, a, b, split, io, st, g->state, g->counters + 1);
 }
 
 namespace {
@@ -1101,6 +1138,50 @@ int trw::walk_check(trw_graph *g, const trw_walk_params *params) {
 
 int trw_walk_async(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream) {
     return trw::walk_launch(g, params, io, reinterpret_cast<hipStream_t>(stream), nullptr, true);
+}
+
+// trw_walk_async without allocation: tables are never built here, and the QUEUE form's queue
+// head and the STEP form's live flag and walker state live in this device's `scratch` array
+// (through the address resolved at context creation, never the array's name).  The kernels are
+// trw_walk_async's, so the walks are identical.  Calls that overlap on one device share that
+// array and race.
+int trw_walk_noalloc(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, void *stream) {
+    if (!g || !params || !io) return fail(TRW_E_INVALID, "null argument");
+    if (!g->ctx->scratch) return fail(TRW_E_STATE, "the context holds no scratch address for its device");
+    TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    KindParams k;
+    int rc = resolve_kind(g, params, k, false);
+    if (rc != TRW_OK) return rc;
+    BatchIO bio;
+    rc = make_io(g, params, io, k, k, bio);
+    if (rc != TRW_OK) return rc;
+    hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : g->ctx->stream;
+    unsigned int *control = static_cast<unsigned int *>(g->ctx->scratch);
+    WalkerState *state = reinterpret_cast<WalkerState *>(static_cast<unsigned char *>(g->ctx->scratch) +
+                                                         kNoAllocControlBytes);
+    int32_t form = params->form;
+    if (form == TRW_FORM_AUTO) form = (k.ppr || k.mode == trw::kModeMetaPath) ? TRW_FORM_QUEUE : TRW_FORM_FUSED;
+    switch (form) {
+    case TRW_FORM_FUSED:
+        if (k.ppr) return fail(TRW_E_UNSUPPORTED, "the fused form runs fixed-length walks; use QUEUE or STEP for PPR");
+        TRW_HIP(trw::launch_walk_fused(g->view(), k, bio, st), "fused walk launch");
+        return TRW_OK;
+    case TRW_FORM_QUEUE:
+        TRW_HIP(trw::launch_walk_queue(g->view(), k, k, bio.n, bio, control, st), "queue walk launch");
+        return TRW_OK;
+    case TRW_FORM_STEP:
+        if (bio.n > kNoAllocMaxStepWalkers)
+            return fail(TRW_E_INVALID,
+                        "trw_walk_noalloc's STEP form takes at most %d walkers (its walker state is a fixed device "
+                        "array); got %d",
+                        kNoAllocMaxStepWalkers, bio.n);
+        return run_step_form_on(g, k, k, bio.n, bio, st, state, control + 1);
+    case TRW_FORM_GRAPH:
+        return fail(TRW_E_UNSUPPORTED, "trw_walk_noalloc runs the AUTO, FUSED, QUEUE and STEP forms (instantiating "
+                                       "the GRAPH form's hipGraph allocates)");
+    default:
+        return fail(TRW_E_INVALID, "unknown form %d", params->form);
+    }
 }
 
 int trw_walk_mix_async(trw_graph *g, const trw_walk_params *pa, const trw_walk_params *pb, int32_t split,

@@ -10,6 +10,7 @@ struct trw_context {
     int32_t device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t timer[2] = {nullptr, nullptr};
+    void *scratch = nullptr;  // this device's copy of trw_walk_noalloc's scratch array (api.cpp)
 };
 
 struct trw_graph {

@@ -276,6 +276,12 @@ def walk_async(graph, params, io, stream=None):
     check(lib().trw_walk_async(graph.h, C.byref(params), C.byref(io), stream))
 
 
+def walk_noalloc(graph, params, io, stream=None):
+    """trw_walk_noalloc: trw_walk_async that allocates nothing (tables must be built; STEP form
+    at most 2^18 walkers; calls on one device must not overlap)."""
+    check(lib().trw_walk_noalloc(graph.h, C.byref(params), C.byref(io), stream))
+
+
 def walk_mix_async(graph, params_a, params_b, split, io, stream=None):
     check(lib().trw_walk_mix_async(graph.h, C.byref(params_a), C.byref(params_b), int(split), C.byref(io), stream))
 
@@ -373,12 +379,13 @@ class DeviceTimer:
 
 
 def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, walker_id_base=0, stream=None,
-         mix=None, walker_ids=None):
+         mix=None, walker_ids=None, noalloc=False):
     """Runs one batch in HBM buffers and returns numpy results.
 
     draws / term_draws: replay streams in the [k][walker] layout, or None (Philox).
     mix: optional (params_b, split) for a two-kind batch.
     walker_ids: optional per-walker global ids keying the Philox stream (uint64).
+    noalloc: run the batch through trw_walk_noalloc instead of trw_walk_async.
     """
     ctx = graph.ctx
     src_h = np.ascontiguousarray(sources, dtype=np.int32)
@@ -408,8 +415,10 @@ def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, w
         io = make_io(src, n, L, walker_id_base, d_draws, K, d_tdraws, KT, paths, lengths, ends, status, used,
                      counters, walker_ids=ids)
         if mix is None:
-            walk_async(graph, params, io, stream)
+            (walk_noalloc if noalloc else walk_async)(graph, params, io, stream)
         else:
+            if noalloc:
+                raise ValueError("trw_walk_noalloc runs one walker kind per batch")
             walk_mix_async(graph, params, mix[0], mix[1], io, stream)
         synchronize(ctx, stream)
         c = counters.to_numpy(np.uint64, 4)

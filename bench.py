@@ -149,8 +149,11 @@ def functor_engine(ctx, dg, hg, deg, args):
         b = Batch(ctx, src, 80)
         prm = T.make_params("node2vec", length=80, rng="philox", seed=11)
         ms = time_launches(ctx, dg, prm, b.io(), 1, 3)
+        lib_steps = b.steps() // 4
         b.free()
-        out["node2vec_library_steps_per_s"] = steps / (ms / 1e3 / 3)
+        if lib_steps != steps:
+            raise RuntimeError("library Node2Vec walked %d steps, the functor %d" % (lib_steps, steps))
+        out["node2vec_library_steps_per_s"] = lib_steps / (ms / 1e3 / 3)
         steps, secs = run(["-kind", "3", "-em", "2", "-sm", "1", "-wn", str(1 << 14)])
         out["dynamic_its_sample_steps_per_s"] = steps / secs
         steps, secs = run(["-kind", "1", "-em", "2", "-sm", "2", "-wn", str(1 << 14)])
@@ -262,7 +265,9 @@ def c1_ppr(ctx, args):
 
 def c3_node2vec(ctx, args):
     """Node2Vec p=2, q=0.5 (max-weight rejection) on a 2^22-vertex Chung-Lu graph (exponent 2.0,
-    average degree 32, dmin 2, dmax 65536), length 80, one walker per non-isolated vertex."""
+    average degree 32, dmin 2, dmax 65536, FP64 U[1,5) weights mirrored), length 80, one walker per
+    non-isolated vertex: the reference's functor (node2vec.cpp, which ignores the weights) and
+    Node2Vec over the edge weights (weight w_e times the factor, bound max w_e times max_)."""
     t0 = time.time()
     hg = T.HostGraph.chung_lu(1 << 22, 16 << 22, 2.0, 2.0, 65536.0, 3, True, 0)
     deg, adj, w, _ = hg.arrays()
@@ -270,20 +275,36 @@ def c3_node2vec(ctx, args):
     dg = T.DeviceGraph(ctx, deg, adj, w)
     src = T.sources_all(deg)
     b = Batch(ctx, src, 80)
-    prm = T.make_params("node2vec", length=80, p=2.0, q=0.5, rng="philox", seed=3)
-    ms = time_launches(ctx, dg, prm, b.io(), 1, max(1, min(args.steps, 3)))
     reps = max(1, min(args.steps, 3))
-    steps = 79 * len(src)
-    out = {"graph": "chung-lu 2^22 V, avg deg 32, gamma 2.0, dmin 2, dmax 65536", "directed_edges": int(len(adj)),
-           "walkers": len(src), "steps_per_launch": steps, "steps_per_s": steps / (ms / 1e3 / reps)}
+    per_launch = 79 * len(src)
+    out = {"graph": "chung-lu 2^22 V, avg deg 32, gamma 2.0, dmin 2, dmax 65536, FP64 U[1,5) weights",
+           "directed_edges": int(len(adj)), "walkers": len(src), "steps_per_launch": per_launch}
+    for label, weights in (("", "app"), ("weighted_", "edge")):
+        prm = T.make_params("node2vec", length=80, p=2.0, q=0.5, rng="philox", seed=3, weights=weights)
+        b.reset()
+        ms = time_launches(ctx, dg, prm, b.io(), 1, reps)
+        if b.steps() != per_launch * (reps + 1):
+            raise RuntimeError("C3 %s: step count %d != %d launches x %d" % (weights, b.steps(), reps + 1, per_launch))
+        out[label + "steps_per_s"] = per_launch / (ms / 1e3 / reps)
+        out[label + "ms"] = ms / reps
     b.free()
     dg.close()
-    ref = None if args.no_cpu_baseline else reference_run(hg, "node2vec", ["-l", "80", "-p", "2", "-q", "0.5"])
-    if ref:
-        out["reference_steps_per_s"] = ref["value"]
-        out["reference_cores"] = ref["cores"]
+    if not args.no_cpu_baseline:
+        ref = reference_run(hg, "node2vec", ["-l", "80", "-p", "2", "-q", "0.5"])
+        if ref:
+            out["reference_steps_per_s"] = ref["value"]
+            out["reference_cores"] = ref["cores"]
+        # The reference's engine (compute<F> with its ring interleaving) running the weighted functor
+        # of tests/dropin_apps/customwalk.cpp (kind 7), built against the reference's sources.
+        ref = reference_run(hg, "customwalk", ["-ew", "-l", "80", "-kind", "7", "-em", "2", "-sm", "4", "-p", "2",
+                                                "-q", "0.5"])
+        if ref:
+            out["reference_weighted_steps_per_s"] = ref["value"]
     hg.close()
-    log("C3 Node2Vec: %.3e steps/s, reference %s" % (out["steps_per_s"], "%.3e" % ref["value"] if ref else "n/a"))
+    log("C3 Node2Vec: %.3e steps/s (weighted %.3e), reference %s (weighted %s)"
+        % (out["steps_per_s"], out["weighted_steps_per_s"],
+           "%.3e" % out["reference_steps_per_s"] if "reference_steps_per_s" in out else "n/a",
+           "%.3e" % out["reference_weighted_steps_per_s"] if "reference_weighted_steps_per_s" in out else "n/a"))
     return out
 
 

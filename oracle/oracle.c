@@ -364,6 +364,7 @@ typedef struct {
     const orc_graph *g;
     const orc_params *p;
     double n2v_p, n2v_q, n2v_max;   /* Node2Vec p_, q_, max_ (node2vec.cpp:18-20, 69, 76) */
+    double n2v_mw;                  /* weighted Node2Vec: the largest edge weight (0 if none is positive) */
 } walk_ctx;
 
 /* F::weight of apps/{deepwalk,ppr,node2vec,metapath,genericwalk}.cpp */
@@ -372,6 +373,14 @@ static double app_weight(const walk_ctx *c, int32_t len, int32_t prev, int32_t n
     case ORC_APP_PPR:
         return 1.0;                                   /* ppr.cpp:23-25 */
     case ORC_APP_NODE2VEC:                            /* node2vec.cpp:24-38 */
+        if (c->p->n2v_weighted) {
+            /* The same factor times the edge weight (customwalk.cpp kind 7); w * 1.0 == w. */
+            const double ew = c->g->edge_weight[eid];
+            if (len == 1) return ew * c->n2v_max;
+            if (nbr == prev) return ew * c->n2v_p;
+            if (orc_is_neighbor(c->g, prev, nbr)) return ew;
+            return ew * c->n2v_q;
+        }
         if (len == 1) return c->n2v_max;
         if (nbr == prev) return c->n2v_p;
         if (orc_is_neighbor(c->g, prev, nbr)) return 1.0;
@@ -389,7 +398,7 @@ static double app_weight(const walk_ctx *c, int32_t len, int32_t prev, int32_t n
 static double app_max_weight(const walk_ctx *c) {
     switch (c->p->app) {
     case ORC_APP_PPR: return 1.0;
-    case ORC_APP_NODE2VEC: return c->n2v_max;
+    case ORC_APP_NODE2VEC: return c->p->n2v_weighted ? c->n2v_mw * c->n2v_max : c->n2v_max;
     case ORC_APP_METAPATH: return 1.0;
     default: return 5.0;
     }
@@ -422,6 +431,12 @@ int32_t orc_walk(const orc_graph *g, const orc_params *p, const int32_t *sources
     c.n2v_max = 1.0;
     if (c.n2v_q > c.n2v_max) c.n2v_max = c.n2v_q;
     if (c.n2v_p > c.n2v_max) c.n2v_max = c.n2v_p;
+    c.n2v_mw = 0.0;
+    if (p->app == ORC_APP_NODE2VEC && p->n2v_weighted) {
+        if (!g->edge_weight) return -1;
+        for (int64_t e = 0; e < g->ne; ++e)
+            if (g->edge_weight[e] > c.n2v_mw) c.n2v_mw = g->edge_weight[e];
+    }
 
     const int32_t exec = p->execution, sm = p->sampling, L = p->length;
     if (exec == ORC_EXEC_STATIC && !(sm == ORC_SM_ALIAS || sm == ORC_SM_ITS || sm == ORC_SM_REJECTION)) return -1;

@@ -65,6 +65,9 @@ typedef struct {
     const int32_t *schema;   /* MetaPath -s */
     int32_t schema_length;
     int32_t max_trials;      /* rejection trial cap per step (0 = unlimited) */
+    int32_t n2v_weighted;    /* Node2Vec over edge weights: weight w_e * factor, bound W * max_ with W
+                                the largest edge weight (no reference app; the functor of kind 7 in
+                                tests/dropin_apps/customwalk.cpp) */
 } orc_params;
 
 /* ---- SFMT-19937 (dependency/SFMT-src-1.5.1) ---- */

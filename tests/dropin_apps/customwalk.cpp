@@ -23,6 +23,9 @@
 // kind 5: DeepWalk's weight with an update() that never ends a walk (the reference would run
 //         forever; the functor engine stops it at its move cap).
 // kind 6: every weight 0 (max-weight rejection never accepts; the engine's trial cap ends it).
+// kind 7: Node2Vec over edge weights: the edge weight times Node2Vec's factor (x max(1, p_, q_) on
+//         the first move, x p_ back to the previous vertex, x1 to a common neighbour, x q_
+//         elsewhere), bounded by the largest edge weight times max(1, p_, q_).
 #include "creeper_main.h"
 
 HyperParameter g_para = {ExecutionMode::Dynamic, SamplingMethod::InverseTransformationSampling, 0, {nullptr, 0}};
@@ -33,15 +36,23 @@ struct BiasedWalk {
     double p_;
     double q_;
     double max_;
+    double n2v_;  // max(1, p_, q_): Node2Vec's bound (apps/node2vec.cpp:19)
+    double mw_;   // kind 7: the largest edge weight
     Graph* g_;
 
     BiasedWalk() {}
 
     BiasedWalk(int l, int kind, double p, double q, Graph* g) : l_(l), kind_(kind), p_(p), q_(q), g_(g) {
-        max_ = kind == 0 ? 2.5 : kind == 1 ? 10.0 : kind == 4 ? (q > p ? (q > 1.0 ? q : 1.0) : (p > 1.0 ? p : 1.0)) : 5.0;
+        n2v_ = q > p ? (q > 1.0 ? q : 1.0) : (p > 1.0 ? p : 1.0);
+        mw_ = 0.0;
+        if (kind == 7)
+            for (int64_t e = 0; e < g->num_edges(); ++e)
+                if (g->edge_weight_[e] > mw_) mw_ = g->edge_weight_[e];
+        max_ = kind == 0 ? 2.5 : kind == 1 ? 10.0 : kind == 4 ? n2v_ : kind == 7 ? mw_ * n2v_ : 5.0;
     }
 
-    BiasedWalk(const BiasedWalk& b) : BiasedWalk(b.l_, b.kind_, b.p_, b.q_, b.g_) {}
+    BiasedWalk(const BiasedWalk& b)
+        : l_(b.l_), kind_(b.kind_), p_(b.p_), q_(b.q_), max_(b.max_), n2v_(b.n2v_), mw_(b.mw_), g_(b.g_) {}
 
     inline double weight(WalkerMeta& w, intT begin, intT end, int64_t eid) {
         const double ew = g_->edge_weight_[eid];
@@ -55,6 +66,12 @@ struct BiasedWalk {
         if (kind_ == 2) return ((end + w.length_) % 4 == 0) ? ew : 0.0;
         if (kind_ == 3 || kind_ == 5) return ew;
         if (kind_ == 6) return 0.0;
+        if (kind_ == 7) {
+            if (w.length_ == 1) return ew * n2v_;
+            const int prev = w.seq_[w.length_ - 2];
+            if (end == prev) return ew * p_;
+            return g_->is_neighbor(prev, end) ? ew : ew * q_;
+        }
         if (w.length_ == 1) return max_;
         const int prev = w.seq_[w.length_ - 2];
         if (end == prev) return p_;

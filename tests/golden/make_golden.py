@@ -106,6 +106,9 @@ FUNCTOR_CASES = [
     ("custom2_uniform_small", "small", "customwalk", 0, 0, 20, {"-kind": "2"}, 64),
     ("custom2_uniform_hub", "hub", "customwalk", 0, 0, 20, {"-kind": "2"}, 64),
     ("custom4_dyn_maxrs_small", "small", "customwalk", 2, 4, 20, {"-kind": "4", "-p": "2", "-q": "0.5"}, 1024),
+    ("custom7_dyn_maxrs_small", "small", "customwalk", 2, 4, 20, {"-kind": "7", "-p": "2", "-q": "0.5"}, 2048),
+    ("custom7_dyn_maxrs_hub", "hub", "customwalk", 2, 4, 20, {"-kind": "7", "-p": "2", "-q": "0.5"}, 2048),
+    ("custom7_inv_dyn_maxrs_small", "small", "customwalk", 2, 4, 20, {"-kind": "7", "-p": "0.5", "-q": "2"}, 4096),
 ]
 
 

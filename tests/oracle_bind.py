@@ -26,7 +26,7 @@ class OrcGraph(C.Structure):
 class OrcParams(C.Structure):
     _fields_ = [("app", C.c_int32), ("execution", C.c_int32), ("sampling", C.c_int32), ("length", C.c_int32),
                 ("stop_prob", C.c_double), ("p", C.c_double), ("q", C.c_double), ("schema", C.c_void_p),
-                ("schema_length", C.c_int32), ("max_trials", C.c_int32)]
+                ("schema_length", C.c_int32), ("max_trials", C.c_int32), ("n2v_weighted", C.c_int32)]
 
 
 def lib():
@@ -130,8 +130,9 @@ def rs_build(g, w):
 
 
 def walk(graph, app, execution, sampling, length, sources, draws, K, term_draws=None, KT=0, stop_prob=0.2,
-         p=2.0, q=0.5, schema=None, max_trials=0):
-    """draws: walker-major (n_walkers, K) uint32; returns dict of outputs."""
+         p=2.0, q=0.5, schema=None, max_trials=0, weighted=False):
+    """draws: walker-major (n_walkers, K) uint32; returns dict of outputs.
+    weighted: Node2Vec over edge weights (factor times w_e, bound max w times max_)."""
     sources = np.ascontiguousarray(sources, dtype=np.int32)
     n = len(sources)
     L = max(length, 1)
@@ -143,7 +144,7 @@ def walk(graph, app, execution, sampling, length, sources, draws, K, term_draws=
     steps = np.zeros(1, np.uint64)
     sch = None if schema is None else np.ascontiguousarray(schema, dtype=np.int32)
     prm = OrcParams(APP[app] if isinstance(app, str) else app, execution, sampling, length, stop_prob, p, q,
-                    _p(sch), 0 if sch is None else len(sch), max_trials)
+                    _p(sch), 0 if sch is None else len(sch), max_trials, int(bool(weighted)))
     d = np.ascontiguousarray(draws, dtype=np.uint32)
     t = None if term_draws is None else np.ascontiguousarray(term_draws, dtype=np.uint32)
     rc = lib().orc_walk(C.addressof(graph.s), C.addressof(prm), _p(sources), n, _p(d), K, _p(t), KT, _p(paths),

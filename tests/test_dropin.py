@@ -127,10 +127,12 @@ def test_functor_engine_matches_the_reference_harness(case):
     assert r["steps"] == int(ref["steps"][0])
 
 
-# customwalk kinds 3 and 4 are DeepWalk and Node2Vec under other names: in free-running
-# mode the functor engine must walk exactly as the library's kernels for those apps.
+# customwalk kinds 3 and 4 are DeepWalk and Node2Vec under other names, kind 7 Node2Vec over edge
+# weights: in free-running mode the functor engine must walk exactly as the library's kernels for
+# those apps (kind 7: the Node2Vec kernel with weights EDGE).
 TWINS = [(3, 1, 2, "deepwalk"), (3, 1, 1, "deepwalk"), (3, 1, 3, "deepwalk"), (3, 2, 1, "deepwalk"),
-         (3, 2, 2, "deepwalk"), (3, 2, 3, "deepwalk"), (3, 0, 0, "deepwalk"), (4, 2, 4, "node2vec")]
+         (3, 2, 2, "deepwalk"), (3, 2, 3, "deepwalk"), (3, 0, 0, "deepwalk"), (4, 2, 4, "node2vec"),
+         (7, 2, 4, "node2vec")]
 
 
 @pytest.mark.parametrize("gname", ["small", "hub"])
@@ -146,7 +148,8 @@ def test_functor_engine_walks_as_the_library_in_free_running_mode(gpu_ctx, gname
     src = T.sources_all(g.degree)
     dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight, g.label)
     try:
-        lib = T.walk(dg, T.make_params(app, execution=em, sampling=sm, length=20, rng="philox", seed=77), src)
+        lib = T.walk(dg, T.make_params(app, execution=em, sampling=sm, length=20, rng="philox", seed=77,
+                                       weights="edge" if kind == 7 else "app"), src)
     finally:
         dg.close()
     n = len(src)

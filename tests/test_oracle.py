@@ -243,3 +243,94 @@ def test_oracle_matches_reference_on_fresh_graphs(seed):
             r = run_golden_walk(case, g, src, seed, seed + 100)
             assert np.array_equal(r["paths"], ref_paths), (app, em, sm)
             assert np.array_equal(r["draws_used"], ref_used), (app, em, sm)
+
+
+def _custom7_cases():
+    import json
+    with open(os.path.join(GOLDEN, "index.json")) as f:
+        return [c for c in json.load(f)["functor_walks"] if c["case"].startswith("custom7")]
+
+
+@pytest.mark.parametrize("case", _custom7_cases(), ids=lambda c: c["case"])
+def test_weighted_node2vec_matches_the_reference_engine(case):
+    """Node2Vec over edge weights (factor times w_e, bound max_e w_e times max(1, p_, q_)): the
+    oracle equals the reference's own dynamic_max_weight_rs_move (rejection_sampling.h:368-402)
+    driving the weighted functor of tests/dropin_apps/customwalk.cpp (kind 7), bit for bit."""
+    import thunderrw_amd as T
+
+    g, _ = load_golden_graph(case["graph"])
+    z = np.load(os.path.join(GOLDEN, "functor_%s.npz" % case["case"]))
+    src = T.sources_all(g.degree)  # the fixture's walkers: generate_all_source order
+    K = case["K"]
+    d = ob.sfmt_stream(1234, len(src) * K).reshape(len(src), K)
+    p, q = float(case["extra"]["-p"]), float(case["extra"]["-q"])
+    r = ob.walk(ob.Graph(g), "node2vec", 2, 4, case["length"], src, d, K, p=p, q=q, weighted=True)
+    assert np.array_equal(r["paths"], z["paths"])
+    assert np.array_equal(r["lengths"], z["lengths"])
+    assert r["steps"] == int(z["steps"][0])
+    assert np.array_equal(r["draws_used"], z["draws_used"])
+    # The unweighted walk from the same draws is a different one (the fixture pins the weights).
+    u = ob.walk(ob.Graph(g), "node2vec", 2, 4, case["length"], src, d, K, p=p, q=q)
+    assert not np.array_equal(u["paths"], z["paths"])
+
+
+def _tiny_weighted_graph():
+    pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (2, 3), (3, 4), (1, 4), (4, 5), (2, 5)]
+    w = [1.0, 4.0, 2.0, 3.0, 1.5, 4.5, 2.5, 1.0, 3.5]
+    g = graphs.csr_from_undirected(6, pairs, weights=False, labels=False)
+    wt = {}
+    for (a, b), x in zip(pairs, w):
+        wt[(a, b)] = wt[(b, a)] = x
+    weight = np.array([wt[(int(v), int(u))] for v in range(6) for u in g.neighbors(v)])
+    return graphs.CSR(g.degree, g.adj, weight), wt
+
+
+def _weighted_n2v_law(g, wt, prev, cur, p_, q_):
+    """P(next | prev, cur) of Node2Vec over edge weights: w(cur, x) * {p_: x == prev, 1: x a neighbour
+    of prev, q_: otherwise}, normalised (the law max-weight rejection samples, node2vec.cpp:24-38)."""
+    nb = [int(x) for x in g.neighbors(cur)]
+    pn = set(int(x) for x in g.neighbors(prev))
+    a = np.array([wt[(cur, x)] * (p_ if x == prev else 1.0 if x in pn else q_) for x in nb])
+    return nb, a / a.sum()
+
+
+@pytest.mark.parametrize("pq", [(2.0, 0.5), (0.5, 2.0)])
+def test_weighted_node2vec_second_order_law_brute_force(pq):
+    """Tiny graph, exact second-order law by enumeration: the oracle's move from (source, first)
+    follows it (chi-square, every (prev, cur) pair with enough visits), and its first move follows
+    w / sum w.  A planted mistake -- the unweighted functor -- fails the same test."""
+    from scipy import stats
+
+    g, wt = _tiny_weighted_graph()
+    p, q = pq
+    p_, q_ = 1.0 / p, 1.0 / q
+    n, K = 240000, 96
+    src = np.repeat(np.arange(6, dtype=np.int32), n // 6)
+    d = ob.sfmt_stream(77, n * K).reshape(n, K)
+
+    def pvalues(weighted):
+        r = ob.walk(ob.Graph(g), "node2vec", 2, 4, 3, src, d, K, p=p, q=q, weighted=weighted)
+        assert np.all(r["lengths"] == 3)
+        path = r["paths"]
+        out = []
+        for s in range(6):
+            first = path[path[:, 0] == s, 1]
+            nb = [int(x) for x in g.neighbors(s)]
+            ws = np.array([wt[(s, x)] for x in nb])
+            obs = np.array([(first == x).sum() for x in nb])
+            out.append(stats.chisquare(obs, ws / ws.sum() * obs.sum()).pvalue)
+            for a in nb:
+                sel = path[(path[:, 0] == s) & (path[:, 1] == a), 2]
+                if len(sel) < 500:
+                    continue
+                nb2, law = _weighted_n2v_law(g, wt, s, a, p_, q_)
+                obs = np.array([(sel == x).sum() for x in nb2])
+                out.append(stats.chisquare(obs, law * obs.sum()).pvalue)
+        return np.maximum(np.array(out), 1e-300)
+
+    good = pvalues(True)
+    assert len(good) > 20
+    assert stats.combine_pvalues(good).pvalue > 1e-3
+    assert good.min() > 1e-5
+    bad = pvalues(False)
+    assert stats.combine_pvalues(bad).pvalue < 1e-12

@@ -287,6 +287,7 @@ static void graph_release(trw_graph *g) {
     dfree(g->gvrec);
     dfree(g->mlines);
     dfree(g->nrec);
+    dfree(g->wnrec);
     dfree(g->mroot);
     dfree(g->order);
     dfree(g->vrec);
@@ -380,6 +381,8 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
     if (edge_weight) {
         for (size_t e = 0; e < E && g->weights_nonneg; ++e)
             g->weights_nonneg = edge_weight[e] >= 0.0 && edge_weight[e] < HUGE_VAL;  // NaN fails both
+        for (size_t e = 0; e < E; ++e)
+            if (edge_weight[e] > g->max_weight) g->max_weight = edge_weight[e];
         TRW_HIP(dmalloc(&g->weight, E + 16), "hipMalloc weight");
         if (E) TRW_HIP(hipMemcpy(g->weight, edge_weight, sizeof(double) * E, hipMemcpyHostToDevice), "copy weight");
     }
@@ -619,6 +622,22 @@ static int build_member_tables(trw_graph *g) {
     root = nullptr;
     lines = nullptr;
     nrec = nullptr;
+    return TRW_OK;
+}
+
+// Weighted Node2Vec records (two per edge) from the neighbour records with roots and the edge
+// weights: built by the first weighted Node2Vec walk once the membership tables exist.
+static int build_member_weight_records(trw_graph *g) {
+    if (g->wnrec || !g->nrec || !g->weight || g->ne == 0) return TRW_OK;
+    uint4 *w = nullptr;
+    TRW_HIP(dmalloc(&w, 2 * static_cast<size_t>(g->ne) + 2), "hipMalloc weighted Node2Vec records");
+    hipError_t e = trw::launch_member_weight_records(g->view(), w, g->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+    if (e != hipSuccess) {
+        dfree(w);
+        return hip_fail(e, "weighted Node2Vec records");
+    }
+    g->wnrec = w;
     return TRW_OK;
 }
 
@@ -922,14 +941,21 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
     }
     if (p->app == TRW_APP_NODE2VEC) {
         if (!(p->p > 0.0) || !(p->q > 0.0)) return fail(TRW_E_INVALID, "Node2Vec p and q must be > 0");
+        // weights EDGE: Node2Vec over edge weights -- a candidate's weight is w_e times Node2Vec's
+        // factor, the bound the largest edge weight times max_; APP and UNIT: the reference's functor.
+        const bool weighted = p->weights == TRW_WALK_WEIGHTS_EDGE;
+        if (weighted && !g->weight) return fail(TRW_E_STATE, "graph was uploaded without edge weights");
         k.mode = trw::kModeNode2Vec;
         if (allow_build) {
-            const int rc = build_member_tables(g);
+            int rc = build_member_tables(g);
+            if (rc == TRW_OK && weighted) rc = build_member_weight_records(g);
             if (rc != TRW_OK) return rc;
         }
         k.n2v_p = p->p;  // Node2Vec::p_ (node2vec.cpp:18, after the inversion at :69)
         k.n2v_q = p->q;  // Node2Vec::q_ (:76)
         k.n2v_max = std::max(1.0, std::max(k.n2v_q, k.n2v_p));  // node2vec.cpp:19
+        k.n2v_weighted = weighted;
+        k.n2v_bound = weighted ? g->max_weight * k.n2v_max : k.n2v_max;
         return TRW_OK;
     }
     if (p->app == TRW_APP_METAPATH) {

@@ -69,6 +69,8 @@ struct DevGraph {
     const uint4 *grec;         // ITS guide records (2 uint4 per bucket), or null; see its_guide below
     const uint2 *gvrec;        // nv: {first bucket, bucket count} per vertex ({kNoRange, 0}: weights sum to zero)
     const uint4 *nrec;         // ne: neighbour records with .w = the neighbour's membership table root
+    const uint4 *wnrec;        // 2*ne: weighted Node2Vec records {weight lo, weight hi, adj[e], first edge of adj[e]}
+                               //       {degree of adj[e], its membership root, 0, 0}, or null
     const uint4 *mlines;       // membership tables (128-byte buckets), or null
     const uint32_t *mroot;     // nv: first line of v's membership table, kNoRange if v has none
 };
@@ -133,6 +135,8 @@ struct KindParams {
     int32_t schema_length;
     double stop_prob;
     double n2v_p, n2v_q, n2v_max;  // Node2Vec p_, q_, max_ (node2vec.cpp:18-20)
+    double n2v_bound;       // Node2Vec rejection bound: max_, or max_ times the largest edge weight (weighted)
+    int32_t n2v_weighted;   // Node2Vec over edge weights: a candidate's weight is w_e times its factor
     int32_t schema[16];
     int32_t its_tree_min;   // ITS: vertices of at least this degree search their tree (when built)
     int32_t queue_chunk;    // QUEUE: walker indices a wavefront takes per atomic (0 = from the batch size)
@@ -271,6 +275,8 @@ hipError_t launch_its_guide_starts(const DevGraph &g, const double *prefix, uint
 hipError_t launch_its_guide_fill(const DevGraph &g, const double *prefix, const uint64_t *start, uint64_t buckets,
                                  uint2 *gvrec, uint4 *grec, hipStream_t stream);
 hipError_t launch_member_records(const DevGraph &g, const uint32_t *root, uint4 *nrec, hipStream_t stream);
+// Weighted Node2Vec records from the neighbour records with roots (g.nrec) and the edge weights.
+hipError_t launch_member_weight_records(const DevGraph &g, uint4 *wnrec, hipStream_t stream);
 hipError_t launch_its_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,
                             double *out, hipStream_t stream);
 hipError_t launch_rs_build(const DevGraph &g, const double *w, int32_t unit, const int32_t *order, int32_t hubs,

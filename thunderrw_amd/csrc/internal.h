@@ -37,6 +37,8 @@ struct trw_graph {
     uint4 *grec = nullptr;         // ITS guide records (2 per bucket), with the ITS table when its sums are monotone
     uint2 *gvrec = nullptr;        // nv: {first bucket, bucket count}
     uint4 *nrec = nullptr;         // neighbour records with membership roots (built by the first Node2Vec walk)
+    uint4 *wnrec = nullptr;        // weighted Node2Vec records (2 per edge; built by the first weighted Node2Vec walk)
+    double max_weight = 0.0;       // the largest edge weight (0 when none is positive), weighted Node2Vec's bound
     uint4 *mlines = nullptr;       // membership tables (hashed neighbour sets, 128-byte buckets)
     uint32_t *mroot = nullptr;     // nv: first line of each vertex's membership table
     bool weights_nonneg = true;    // every edge weight is finite and >= 0
@@ -78,6 +80,7 @@ struct trw_graph {
         g.grec = grec;
         g.gvrec = gvrec;
         g.nrec = nrec;
+        g.wnrec = wnrec;
         g.mlines = mlines;
         g.mroot = mroot;
         return g;

@@ -718,6 +718,18 @@ __global__ void member_records_kernel(const uint4 *erec, int64_t ne, const uint3
     }
 }
 
+// Weighted Node2Vec records: a candidate's edge weight beside its neighbour record with root, so
+// a trial of the weighted Move is one 32-byte read.
+__global__ void member_weight_records_kernel(const uint4 *nrec, const double *w, int64_t ne, uint4 *wnrec) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const uint4 r = nrec[e];
+        const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(w[e]));
+        wnrec[2 * e] = make_uint4(static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32), r.x, r.y);
+        wnrec[2 * e + 1] = make_uint4(r.z, r.w, 0u, 0u);
+    }
+}
+
 // Alias walk records: the reference AliasSlot plus both candidates' ranges.
 __global__ void alias_records_kernel(const AliasSlot *alias, const uint2 *vrec, int64_t ne, uint4 *arec) {
     for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
@@ -866,6 +878,12 @@ hipError_t launch_member_table_fill(const DevGraph &g, const uint64_t *start, ui
 hipError_t launch_member_records(const DevGraph &g, const uint32_t *root, uint4 *nrec, hipStream_t stream) {
     if (g.ne <= 0 || !g.erec) return hipSuccess;
     hipLaunchKernelGGL(member_records_kernel, dim3(8192), dim3(256), 0, stream, g.erec, g.ne, root, nrec);
+    return hipGetLastError();
+}
+
+hipError_t launch_member_weight_records(const DevGraph &g, uint4 *wnrec, hipStream_t stream) {
+    if (g.ne <= 0 || !g.nrec || !g.weight) return hipSuccess;
+    hipLaunchKernelGGL(member_weight_records_kernel, dim3(8192), dim3(256), 0, stream, g.nrec, g.weight, g.ne, wnrec);
     return hipGetLastError();
 }
 

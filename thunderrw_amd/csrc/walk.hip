@@ -283,6 +283,22 @@ __device__ __forceinline__ int32_t read_neighbor_member(const DevGraph &g, int64
     return read_neighbor(g, e, noff, ndeg);
 }
 
+// Weighted Node2Vec: the candidate's edge weight with its neighbour record and membership root,
+// one 32-byte record when the graph has them.
+__device__ __forceinline__ int32_t read_neighbor_member_weighted(const DevGraph &g, int64_t e, uint32_t &noff,
+                                                                 uint32_t &ndeg, uint32_t &nroot, double &ew) {
+    if (g.wnrec) {
+        const uint4 a = g.wnrec[2 * e], b = g.wnrec[2 * e + 1];
+        ew = __hiloint2double(static_cast<int>(a.y), static_cast<int>(a.x));
+        noff = a.w;
+        ndeg = b.x;
+        nroot = b.y;
+        return static_cast<int32_t>(a.z);
+    }
+    ew = g.weight[e];
+    return read_neighbor_member(g, e, noff, ndeg, nroot);
+}
+
 // ITS guide range of the walker's current vertex: {first bucket, bucket count}, read once
 // per walk (the records carry the next vertex's).  A count of 0 is degree 0, or weights that
 // sum to zero when the first bucket is kNoRange.
@@ -588,10 +604,10 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         }
         if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
     } else if (MODE == kModeNode2Vec) {
-        // rejection_sampling.h:368-402 with Node2Vec::weight (node2vec.cpp:24-38):
-        // accept iff real2(ub) * max_ <= weight(candidate).  The membership search
-        // runs only when the draw falls between the two possible weights.
-        const double lo = fmin(1.0, P.n2v_q), hi = fmax(1.0, P.n2v_q);
+        // rejection_sampling.h:368-402 with Node2Vec::weight (node2vec.cpp:24-38): a trial is
+        // accepted unless real2(ub) * bound > weight(candidate), the weight being max_ on the first
+        // move, p_ back to the previous vertex, 1 to one of its neighbours and q_ elsewhere (times
+        // the edge weight when weighted).  The membership search runs only when its answer decides.
         int32_t c = -1;
         uint32_t croot = kNoRange;
         for (uint32_t trial = 0;; ++trial) {
@@ -601,19 +617,23 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             }
             uint32_t ua, ub;
             if (!draw_pair<REPLAY>(io, w, s, step, trial, blk, ua, ub)) return false;
-            c = read_neighbor_member(g, off + (ua % ud), noff, ndeg, croot);
-            const double y = real2(ub) * P.n2v_max;
+            double ew = 1.0;
+            if (P.n2v_weighted) c = read_neighbor_member_weighted(g, off + (ua % ud), noff, ndeg, croot, ew);
+            else c = read_neighbor_member(g, off + (ua % ud), noff, ndeg, croot);
+            const double y = real2(ub) * P.n2v_bound;
             bool accept;
-            if (s.len == 1) accept = (y <= P.n2v_max);
-            else if (c == s.prev) accept = (y <= P.n2v_p);
-            else if (y <= lo) accept = true;
-            else if (y > hi) accept = false;
+            if (s.len == 1) accept = !(y > ew * P.n2v_max);
+            else if (c == s.prev) accept = !(y > ew * P.n2v_p);
             else {
-                int64_t poff, coff;
-                int32_t pd, cd;
-                known_range(g, s.prev, s.poff, s.pdeg, poff, pd);
-                known_range(g, c, noff, ndeg, coff, cd);
-                accept = (y <= (is_neighbor_member(g, s.prev, poff, pd, s.pidx, c, coff, cd, croot) ? 1.0 : P.n2v_q));
+                const bool acc_nb = !(y > ew), acc_far = !(y > ew * P.n2v_q);
+                accept = acc_nb;
+                if (acc_nb != acc_far) {
+                    int64_t poff, coff;
+                    int32_t pd, cd;
+                    known_range(g, s.prev, s.poff, s.pdeg, poff, pd);
+                    known_range(g, c, noff, ndeg, coff, cd);
+                    accept = is_neighbor_member(g, s.prev, poff, pd, s.pidx, c, coff, cd, croot) ? acc_nb : acc_far;
+                }
             }
             if (accept) break;
         }
@@ -1080,13 +1100,14 @@ struct LaneRow {
     }
 };
 
-template <bool REPLAY>
+// WEIGHTED: Node2Vec over edge weights, a trial reading the candidate's 32-byte weighted record
+// (edge weight beside the neighbour record) instead of its 16-byte neighbour record.
+template <bool REPLAY, bool WEIGHTED>
 __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g, KindParams P, BatchIO io) {
     __shared__ int32_t chunk[kN2vBlock * (kN2vChunk + 1)];
     int32_t *mine = chunk + threadIdx.x * (kN2vChunk + 1);
     const int32_t w = static_cast<int32_t>(blockIdx.x) * kN2vBlock + static_cast<int32_t>(threadIdx.x);
     const int32_t L = P.length;
-    const double lo = fmin(1.0, P.n2v_q), hi = fmax(1.0, P.n2v_q);
     WalkerState s = empty_walker();
     if (w < io.n) init_walker(g, io, w, s);
     int64_t off = 0;
@@ -1101,9 +1122,9 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
     uint4 blk = first_block<REPLAY>(io, w, s);  // free-running: the current trial's block
     uint4 rej = blk, acc = blk;                 // the next trial's and the next step's
     bool testing = false;  // the next read is a membership bucket half, not a candidate
-    int32_t c = -1;        // the trial's candidate and its range, table root and draw y
+    int32_t c = -1;        // the trial's candidate and its range and table root
     uint32_t cnoff = 0u, cndeg = 0u, croot = kNoRange;
-    double y = 0.0;
+    bool acc_nb = false, acc_far = false;  // the trial's decision if c is / is not a neighbour of prev
     uint32_t mline = 0u, mb = 0u, mmask = 0u, mleft = 0u, mhalf = 0u;  // the bucket cursor of a test
     int32_t mkey = 0;
     unsigned long long steps = 0;
@@ -1126,22 +1147,40 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
                     ub = blk.y;
                 }
                 if (ok) {
-                    const uint4 r = g.nrec[off + (ua % static_cast<uint32_t>(d))];
+                    const int64_t e = off + (ua % static_cast<uint32_t>(d));
+                    uint4 r, rw = make_uint4(0u, 0u, 0u, 0u);
+                    if (WEIGHTED) {
+                        r = g.wnrec[2 * e];
+                        rw = g.wnrec[2 * e + 1];
+                    } else {
+                        r = g.nrec[e];
+                    }
                     if (!REPLAY) {  // both possible next blocks, in the candidate read's shadow
                         rej = philox_block(io, s, static_cast<uint32_t>(s.len - 1), trial + 1u);
                         acc = philox_block(io, s, static_cast<uint32_t>(s.len), 0u);
                         pin(rej);
                         pin(acc);
                     }
-                    c = static_cast<int32_t>(r.x);
-                    cnoff = r.y;
-                    cndeg = r.z;
-                    croot = r.w;
-                    y = real2(ub) * P.n2v_max;
-                    if (s.len == 1) accept = (y <= P.n2v_max), decided = true;
-                    else if (c == s.prev) accept = (y <= P.n2v_p), decided = true;
-                    else if (y <= lo) accept = true, decided = true;
-                    else if (y > hi) accept = false, decided = true;
+                    double ew = 1.0;
+                    if (WEIGHTED) {
+                        ew = __hiloint2double(static_cast<int>(r.y), static_cast<int>(r.x));
+                        c = static_cast<int32_t>(r.z);
+                        cnoff = r.w;
+                        cndeg = rw.x;
+                        croot = rw.y;
+                    } else {
+                        c = static_cast<int32_t>(r.x);
+                        cnoff = r.y;
+                        cndeg = r.z;
+                        croot = r.w;
+                    }
+                    // Accepted unless y > weight (the reference's loop condition).
+                    const double y = real2(ub) * P.n2v_bound;
+                    acc_nb = !(y > ew);
+                    acc_far = !(y > ew * P.n2v_q);
+                    if (s.len == 1) accept = !(y > ew * P.n2v_max), decided = true;
+                    else if (c == s.prev) accept = !(y > ew * P.n2v_p), decided = true;
+                    else if (acc_nb == acc_far) accept = acc_nb, decided = true;
                     else {
                         // is_neighbor(prev, c): the larger-degree endpoint looked up in the smaller
                         // one's table (graph.h:180-199; equal degrees: prev in c's).
@@ -1151,7 +1190,7 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
                         mkey = in_prev ? c : s.prev;
                         if (md <= 0 || mline == kNoRange) {
                             // An empty list (a directed graph's sink) holds nothing: not a neighbour.
-                            accept = y <= P.n2v_q;
+                            accept = acc_far;
                             decided = true;
                         } else {
                             const uint32_t bits = member_bucket_bits(md);
@@ -1179,7 +1218,7 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
             }
             if (done) {
                 testing = false;
-                accept = y <= (found ? 1.0 : P.n2v_q);
+                accept = found ? acc_nb : acc_far;
                 decided = true;
             }
         }
@@ -1556,9 +1595,13 @@ hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO 
     case kModeIts: hipLaunchKernelGGL((walk_fused_kernel<kModeIts, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     case kModeRejection: hipLaunchKernelGGL((walk_fused_kernel<kModeRejection, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     case kModeNode2Vec:
-        if (g.nrec && g.mlines && g.mroot && !p.ppr) {
-            hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY>), dim3(grid_for(io.n, kN2vBlock)), dim3(kN2vBlock), 0,
-                               stream, g, p, io);
+        if (g.nrec && g.mlines && g.mroot && !p.ppr && (!p.n2v_weighted || g.wnrec)) {
+            if (p.n2v_weighted)
+                hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY, true>), dim3(grid_for(io.n, kN2vBlock)),
+                                   dim3(kN2vBlock), 0, stream, g, p, io);
+            else
+                hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY, false>), dim3(grid_for(io.n, kN2vBlock)),
+                                   dim3(kN2vBlock), 0, stream, g, p, io);
         } else {
             hipLaunchKernelGGL((walk_fused_node2vec_kernel<REPLAY>), grid, block, lds, stream, g, p, io, T);
         }

@@ -139,6 +139,25 @@ int main(int argc, char **argv) {
         return 0;
     }
 
+    if (mode == "sources") {
+        // walker_generator.cpp:9-63: every non-isolated vertex shuffled (generate_all_source), -wn
+        // random sources of seed -wseed (generate_random_source) and -wn copies of the max-degree vertex
+        // (generate_single_source), the WalkerMeta::source_ of each walker in order.
+        const int wn = static_cast<int>(opt_l(cmd, "-wn", 1000));
+        const int wseed = static_cast<int>(opt_l(cmd, "-wseed", 0));
+        const char *names[3] = {"all", "random", "single"};
+        for (int k = 0; k < 3; ++k) {
+            std::vector<WalkerMeta> walkers;
+            if (k == 0) generate_all_source(graph, walkers);
+            else if (k == 1) generate_random_source(graph, walkers, wn, wseed);
+            else generate_single_source(graph, walkers, wn, wseed);
+            std::vector<int32_t> s;
+            for (const WalkerMeta &w : walkers) s.push_back(w.source_);
+            dump(out + "/sources_" + names[k] + ".bin", s.data(), s.size());
+        }
+        return 0;
+    }
+
     // mode == "walk"
     if (g_para.execution_ == Static) static_initialization(&graph, f);
 

@@ -190,19 +190,45 @@ def functor_fixtures(index):
                                            "length": length, "extra": extra, "K": K})
 
 
+# Walker sources (walker_generator.cpp:9-63) on the golden graphs: generate_all_source's shuffle
+# (std::shuffle with std::mt19937(0), libstdc++'s algorithm), generate_random_source and
+# generate_single_source, written by the reference's own functions.
+SOURCE_ARGS = {"wn": 5000, "wseed": 7}
+
+
+def source_fixtures(index):
+    index["sources"] = dict(SOURCE_ARGS, graphs=list(GRAPHS))
+    with tempfile.TemporaryDirectory() as tmp:
+        for name in GRAPHS:
+            z = np.load(os.path.join(HERE, f"graph_{name}.npz"))
+            g = graphs.CSR(z["degree"], z["adj"], z["weight"], z["label"])
+            d = os.path.join(tmp, "graph_" + name)
+            graphs.write_reference_csr(g, d)
+            out = os.path.join(tmp, "out_" + name)
+            os.makedirs(out)
+            run([os.path.join(REF_BIN, "harness_deepwalk"), "-f", d, "-ew", "-o", out, "-mode", "sources",
+                 "-wn", str(SOURCE_ARGS["wn"]), "-wseed", str(SOURCE_ARGS["wseed"])])
+            np.savez_compressed(os.path.join(HERE, f"sources_{name}.npz"),
+                                **{k: load(os.path.join(out, f"sources_{k}.bin"), np.int32)
+                                   for k in ("all", "random", "single")})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
-    ap.add_argument("--only", choices=["functor"], help="write only these fixtures (index entries kept)")
+    ap.add_argument("--only", choices=["functor", "sources"], help="write only these fixtures (index entries kept)")
     args = ap.parse_args()
 
-    if args.only == "functor":
+    if args.only:
         with open(os.path.join(HERE, "index.json")) as f:
             index = json.load(f)
-        functor_fixtures(index)
+        if args.only == "functor":
+            functor_fixtures(index)
+        else:
+            source_fixtures(index)
         with open(os.path.join(HERE, "index.json"), "w") as f:
             json.dump(index, f, indent=1, sort_keys=True)
-        print("wrote", len(index["functor_walks"]), "functor-engine walk fixtures")
+        print("wrote", args.only, "fixtures")
         return
 
     gen, arr = parse_kat(os.path.join(args.ref, "dependency/SFMT-src-1.5.1/SFMT.19937.out.txt"))
@@ -242,6 +268,7 @@ def main():
                                    "length": length, "extra": extra, "K": K, "KT": KT})
 
     functor_fixtures(index)
+    source_fixtures(index)
     with open(os.path.join(HERE, "index.json"), "w") as f:
         json.dump(index, f, indent=1, sort_keys=True)
     print("wrote", len(index["walks"]), "walk fixtures,", len(index["functor_walks"]), "functor-engine walk fixtures and",

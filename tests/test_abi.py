@@ -96,6 +96,38 @@ def test_sources_all_is_a_permutation_of_live_vertices():
     assert np.array_equal(s, T.sources_all(deg))
 
 
+@pytest.mark.parametrize("gname", ["small", "hub"])
+def test_sources_equal_the_reference_walker_generator(gname, golden_index):
+    """tests/golden/sources_<graph>.npz holds the walkers the reference's own walker_generator.cpp
+    (generate_all_source's std::shuffle with std::mt19937(0), generate_random_source,
+    generate_single_source) made on the golden graphs."""
+    from conftest import load_golden_graph
+
+    g, _ = load_golden_graph(gname)
+    z = np.load(os.path.join(ROOT, "tests", "golden", "sources_%s.npz" % gname))
+    args = golden_index["sources"]
+    assert np.array_equal(T.sources_all(g.degree), z["all"])
+    assert np.array_equal(T.sources_random(g.degree, args["wn"], seed=args["wseed"]), z["random"])
+    assert np.array_equal(T.sources_single(g.degree, args["wn"]), z["single"])
+
+
+def test_dropin_sfmt_header_known_answers():
+    """thunderrw_amd/dropin/SFMT.h (the SFMT names the reference's functors call, e.g. PPR::sfmt_)
+    against the reference-held KAT: init_gen_rand(1234), first 1000 words; and real1/real2."""
+    kat = np.load(os.path.join(ROOT, "tests", "golden", "sfmt_kat.npz"))["init_gen_rand_1234"]
+    src = ('#include <stdio.h>\n#include "SFMT.h"\nint main(void) {\n  sfmt_t s;\n  sfmt_init_gen_rand(&s, 1234);\n'
+           '  for (int i = 0; i < 1000; ++i) printf("%u\\n", sfmt_genrand_uint32(&s));\n'
+           '  printf("%.17g %.17g\\n", sfmt_to_real1(4294967295u), sfmt_to_real2(4294967295u));\n  return 0;\n}\n')
+    with tempfile.TemporaryDirectory() as tmp:
+        c = os.path.join(tmp, "kat.c")
+        open(c, "w").write(src)
+        exe = os.path.join(tmp, "kat")
+        subprocess.run(["gcc", "-O2", "-I", os.path.join(ROOT, "thunderrw_amd", "dropin"), "-o", exe, c], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()
+    assert np.array_equal(np.array([int(x) for x in out[:1000]], np.uint32), kat)
+    assert float(out[1000]) == 1.0 and float(out[1001]) == (2.0 ** 32 - 1) / 2.0 ** 32
+
+
 def test_generator_is_a_simple_symmetric_sorted_graph():
     h = T.HostGraph.chung_lu(20000, 80000, 2.1, 2, 500, 7, True, 5)
     deg, adj, w, lab = h.arrays()

@@ -67,7 +67,9 @@ def test_edge_list_csr_equals_reference_converter(seed):
 
 def test_edge_list_rejects_what_the_reference_cannot_parse():
     with tempfile.TemporaryDirectory() as tmp:
-        for bad in (["1 2", "3"], ["1 2", "x 4"], ["1 2", ""], ["1 -2"], ["1 99999999999"]):
+        # 2147483647 parses as an int32 but leaves no int32 vertex count (largest id + 1).
+        for bad in (["1 2", "3"], ["1 2", "x 4"], ["1 2", ""], ["1 -2"], ["1 99999999999"], ["2147483647 3"],
+                    ["0 2147483647"]):
             write_lines(tmp, bad)
             with pytest.raises(T.TrwError) as e:
                 T.HostGraph.load_edge_list(tmp, "#")

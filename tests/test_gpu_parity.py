@@ -402,6 +402,35 @@ def test_mixed_batch_queue_equals_step_and_single_kinds(gpu_ctx, pair):
         dg.close()
 
 
+def test_mixed_kinds_on_one_static_table_from_different_weights_are_refused(gpu_ctx):
+    """A graph holds one table per sampler: PPR's unit-weight alias table and DeepWalk's
+    edge-weight one cannot serve one batch (the second build would replace the first under the
+    first kind's walkers).  The mix is refused; the same kinds on one weight source walk as alone."""
+    g = graphs.chung_lu(3000, 12000, 2.1, 2, 400, 57)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        src = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        a = T.make_params("ppr", execution="static", sampling="alias", stop_probability=0.2, rng="philox", seed=3,
+                          form="queue")
+        b = T.make_params("deepwalk", execution="static", sampling="alias", length=12, rng="philox", seed=3,
+                          form="queue")
+        for form in ("queue", "step", "graph"):
+            a.form = b.form = T.engine._FORM[form]
+            with pytest.raises(T.TrwError) as e:
+                T.walk(dg, a, src, mix=(b, len(src) // 2))
+            assert e.value.code == T._abi.E_UNSUPPORTED
+        a_edge = T.make_params("ppr", execution="static", sampling="alias", stop_probability=0.2, rng="philox",
+                               seed=3, form="queue", weights="edge")
+        split = len(src) // 2
+        mixed = T.walk(dg, a_edge, src, mix=(b, split))
+        alone_a = T.walk(dg, a_edge, src[:split], walker_id_base=0)
+        alone_b = T.walk(dg, b, src[split:], walker_id_base=split)
+        assert np.array_equal(mixed["ends"][:split], alone_a["ends"])
+        assert np.array_equal(mixed["paths"][split:, :12], alone_b["paths"])
+    finally:
+        dg.close()
+
+
 def test_walker_ids_key_the_free_running_stream(gpu_ctx):
     """A walker's Philox stream depends only on its global id: a permuted batch with
     per-walker ids gives each walker the walk it has in the ordered batch."""

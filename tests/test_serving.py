@@ -135,6 +135,10 @@ def test_admission_defers_and_rejects(served):
             srv.submit(T.make_params("ppr", rng="replay"), live[:2], 10)
         with pytest.raises(T.TrwError):  # sources are checked at submit
             srv.submit(params_for("ppr"), np.array([g.nv], np.int32), 10)
+        for form in ("step", "graph"):  # forms whose walker state is graph-wide scratch
+            with pytest.raises(T.TrwError) as e:
+                srv.submit(T.make_params("ppr", **dict(PPR, form=form)), live[:2], 10)
+            assert e.value.code == A.E_UNSUPPORTED
 
 
 def test_open_loop_reports_latency_percentiles(served):

@@ -50,7 +50,8 @@ def test_stored_tables_match_oracle_and_reference_tools(gpu_ctx, gname):
             for sm, tool in TOOLS.items():
                 exe = os.path.join(REF, tool)
                 if not os.path.exists(exe):
-                    continue
+                    pytest.skip("reference tool %s not built (make -C oracle ref): the files are checked "
+                                "against the oracle only" % tool)
                 r = subprocess.run([exe, "-f", ref_dir], capture_output=True, text=True, timeout=120)
                 assert r.returncode == 0, r.stderr[-1000:]
                 ref_b, ours_b = read_bytes(os.path.join(ref_dir, FILES[sm])), read_bytes(os.path.join(tmp, FILES[sm]))

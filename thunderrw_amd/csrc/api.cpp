@@ -899,6 +899,21 @@ int trw_walk_supported(const trw_walk_params *p) {
     return TRW_OK;
 }
 
+// The weights (TRW_WEIGHTS_EDGE / UNIT) the static table a walk of DeepWalk, GenericWalk or PPR
+// samples from is built from: the app's F::weight unless params.weights overrides it.
+static int32_t table_weights(const trw_walk_params *p) {
+    return p->weights == TRW_WALK_WEIGHTS_EDGE   ? TRW_WEIGHTS_EDGE
+           : p->weights == TRW_WALK_WEIGHTS_UNIT ? TRW_WEIGHTS_UNIT
+           : p->app == TRW_APP_PPR               ? TRW_WEIGHTS_UNIT
+                                                 : TRW_WEIGHTS_EDGE;
+}
+
+// Whether a walk samples from a graph-wide static table (alias, ITS or rejection), and which.
+static bool reads_static_table(const trw_walk_params *p) {
+    return p->execution != TRW_EXEC_UNIFORM && p->app != TRW_APP_NODE2VEC && p->app != TRW_APP_METAPATH &&
+           (p->sampling == TRW_SM_ALIAS || p->sampling == TRW_SM_ITS || p->sampling == TRW_SM_REJECTION);
+}
+
 // Maps an app + g_para to a device mode, building the static tables the mode
 // reads when they are missing.
 static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, bool allow_build = true) {
@@ -968,10 +983,7 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
     }
     // DeepWalk, GenericWalk (edge weights) and PPR (unit weights): walker-independent
     // weights, so the Dynamic gather rebuilds exactly the Static tables every step.
-    const int32_t wsrc = p->weights == TRW_WALK_WEIGHTS_EDGE   ? TRW_WEIGHTS_EDGE
-                         : p->weights == TRW_WALK_WEIGHTS_UNIT ? TRW_WEIGHTS_UNIT
-                         : ppr                                  ? TRW_WEIGHTS_UNIT
-                                                                : TRW_WEIGHTS_EDGE;
+    const int32_t wsrc = table_weights(p);
     if (wsrc == TRW_WEIGHTS_EDGE && !g->weight) return fail(TRW_E_STATE, "graph was uploaded without edge weights");
     int32_t *built = p->sampling == TRW_SM_ALIAS ? &g->alias_weights
                      : p->sampling == TRW_SM_ITS ? &g->its_weights
@@ -1216,6 +1228,12 @@ int trw_walk_mix_async(trw_graph *g, const trw_walk_params *pa, const trw_walk_p
     if (pa->rng != pb->rng || pa->seed != pb->seed) return fail(TRW_E_INVALID, "both kinds must share rng and seed");
     if (split < 0 || split > io->n_walkers) return fail(TRW_E_INVALID, "split outside [0, n_walkers]");
     TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
+    // A graph holds one table per sampler: two kinds sampling the same static table from different
+    // weights (e.g. PPR's unit-weight alias and DeepWalk's edge-weight alias) cannot share a batch.
+    if (reads_static_table(pa) && reads_static_table(pb) && pa->sampling == pb->sampling &&
+        table_weights(pa) != table_weights(pb))
+        return fail(TRW_E_UNSUPPORTED, "both kinds sample the static table of sampler %d, from different weights; "
+                                       "a graph holds one table per sampler", pa->sampling);
     KindParams a, b;
     int rc = resolve_kind(g, pa, a);
     if (rc != TRW_OK) return rc;

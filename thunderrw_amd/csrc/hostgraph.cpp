@@ -357,6 +357,10 @@ bool load_edge_list(const std::string &dir, char skip, HostGraph &out, std::stri
                 err = "negative vertex id on line " + std::to_string(line_no);
                 return false;
             }
+            if (a == INT32_MAX || b == INT32_MAX) {  // the vertex count (largest id + 1) must fit int32
+                err = "vertex id 2147483647 on line " + std::to_string(line_no) + " leaves no int32 vertex count";
+                return false;
+            }
             if (a > b) std::swap(a, b);
             keys.push_back((static_cast<uint64_t>(a) << 32) | static_cast<uint32_t>(b));
             max_id = std::max(max_id, b);

@@ -517,8 +517,11 @@ int trw_server::submit(const trw_query *q, Clock::time_point arrival, uint64_t *
         return trw::set_error(TRW_E_INVALID, "unknown result kind %d", q->result);
     if (q->params.rng != TRW_RNG_PHILOX)
         return trw::set_error(TRW_E_UNSUPPORTED, "served queries draw from the Philox stream (TRW_RNG_PHILOX)");
-    if (q->params.form == TRW_FORM_STEP)
-        return trw::set_error(TRW_E_UNSUPPORTED, "the server runs the FUSED or QUEUE form");
+    // Slots run side by side on their own streams; the STEP and GRAPH forms keep walker state in
+    // graph-wide scratch, so only the forms whose state stays in registers are served.
+    if (q->params.form != TRW_FORM_AUTO && q->params.form != TRW_FORM_FUSED && q->params.form != TRW_FORM_QUEUE)
+        return trw::set_error(TRW_E_UNSUPPORTED, "the server runs the AUTO, FUSED or QUEUE form (got %d)",
+                              q->params.form);
     int rc = trw::walk_check(g, &q->params);
     if (rc != TRW_OK) return rc;
     const int32_t L = std::max(q->params.length, 1);

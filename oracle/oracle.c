@@ -596,6 +596,19 @@ int32_t orc_ppr_power(const orc_graph *g, int32_t weighted, double alpha, const 
     if (!x || !y) return -1;
     for (int32_t i = 0; i < n; ++i) out[i] = 0.0;
     for (int32_t i = 0; i < n_sources; ++i) x[sources[i]] += 1.0 / n_sources;
+    /* Weighted: each vertex's total weight, summed left to right once (the same value every
+     * iteration would recompute). */
+    double *strength = NULL;
+    if (weighted) {
+        strength = (double *)calloc((size_t)n, sizeof(double));
+        if (!strength) {
+            free(x);
+            free(y);
+            return -1;
+        }
+        for (int32_t u = 0; u < n; ++u)
+            for (int64_t j = g->offset[u]; j < g->offset[u + 1]; ++j) strength[u] += g->edge_weight[j];
+    }
     double coef = alpha, remaining = 1.0;
     int32_t iters = 0;
     while (remaining > tol && iters < 100000) {
@@ -610,8 +623,7 @@ int32_t orc_ppr_power(const orc_graph *g, int32_t weighted, double alpha, const 
                 continue;
             }
             if (weighted) {
-                double s = 0.0;
-                for (int64_t j = 0; j < d; ++j) s += g->edge_weight[off + j];
+                const double s = strength[u];
                 for (int64_t j = 0; j < d; ++j) y[g->adj[off + j]] += xu * (g->edge_weight[off + j] / s);
             } else {
                 double share = xu / (double)d;
@@ -628,5 +640,6 @@ int32_t orc_ppr_power(const orc_graph *g, int32_t weighted, double alpha, const 
     }
     free(x);
     free(y);
+    free(strength);
     return iters;
 }

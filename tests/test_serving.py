@@ -187,3 +187,46 @@ def test_concurrent_clients_get_their_own_results(served):
     assert not errors, errors
     for i, (a, b) in enumerate(zip(alone, results)):
         assert same(a, b), i
+
+
+def test_served_ppr_l1_on_a_2p20_vertex_graph(gpu_ctx):
+    """The serving query mix (|S| ~ logU[1, 4096] sources uniform over the non-isolated vertices,
+    N ~ logU[1e3, 1e5] walkers, termination 0.2, weighted alias) on a 2^20-vertex Chung-Lu graph
+    (exponent 2.1, average degree 16, dmin 2, dmax 65536, FP64 U[1,5) weights): each query's
+    estimate count / N against the oracle's FP64 power iteration started from the same walker
+    sources (ppr.cpp:27-29 semantics), L1 printed and below 1.5 x the summed per-vertex standard
+    errors sqrt(pi (1 - pi) / N)."""
+    hg = T.HostGraph.chung_lu(1 << 20, 8 << 20, 2.1, 2.0, 65536.0, 51, True, 0)
+    deg, adj, w, _ = hg.arrays()
+    g = graphs.CSR(deg.copy(), adj.copy(), w.copy())
+    hg.close()
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight)
+    try:
+        dg.build_tables("alias", "edge")
+        rng = np.random.default_rng(52)
+        live = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        qs = []
+        for i in range(4):
+            k = int(np.exp(rng.uniform(0.0, np.log(4096.0))))
+            nw = int(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
+            qs.append((rng.choice(live, k).astype(np.int32), nw))
+        with T.Server(dg) as srv:
+            ids = [srv.submit(T.make_params("ppr", **PPR), s, nw, (i + 1) << 32) for i, (s, nw) in enumerate(qs)]
+            results = []
+            for qid in ids:
+                assert srv.wait(qid)["state"] == A.QUERY_DONE
+                results.append(srv.ppr(qid))
+                srv.release(qid)
+        for (s, nw), (v, c) in zip(qs, results):
+            assert c.sum() == nw
+            start = s[np.arange(nw) % len(s)]  # walker i starts at sources[i % |S|]
+            pi, _ = ob.ppr_power(g, 0.2, start, weighted=True, tol=1e-10)
+            est = np.zeros(g.nv)
+            est[v] = c / nw
+            l1 = np.abs(est - pi).sum()
+            bound = np.sqrt(pi * (1 - pi) / nw).sum()
+            print("served PPR: |S| %d, N %d, L1 %.4f, summed standard errors %.4f" % (len(s), nw, l1, bound),
+                  flush=True)
+            assert l1 < 1.5 * bound
+    finally:
+        dg.close()

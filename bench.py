@@ -84,8 +84,9 @@ class Batch:
         self.paths = T.DeviceBuffer(ctx, self.n * length * 4) if paths else None
         self.counters = T.DeviceBuffer.from_array(ctx, np.zeros(4, np.uint64))
 
-    def io(self, draws=None, K=0):
-        return T.make_io(self.src, self.n, self.length, 0, draws, K, None, 0, self.paths, counters=self.counters)
+    def io(self, draws=None, K=0, unpadded=False):
+        return T.make_io(self.src, self.n, self.length, 0, draws, K, None, 0, self.paths, counters=self.counters,
+                         unpadded=unpadded)
 
     def steps(self):
         return int(self.counters.to_numpy(np.uint64, 4)[0])
@@ -343,10 +344,13 @@ def mix_forms(ctx, dg, deg, args):
     tune = dg.autotune(T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form="queue"),
                        sample_walkers=1 << 21, seed=4, with_paths=False)
     out["queue_autotune"] = tune["chosen"]
-    for form in ("queue", "graph", "step"):
-        a = T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form=form)
-        b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=4, form=form)
-        io = batch.io()
+    # "queue_unpadded": the same walks with TRW_PATHS_UNPADDED, each row holding its walker's
+    # length entries as the reference's LOG_SEQUENCE output does, with no -1 padding written.
+    for form in ("queue", "queue_unpadded", "graph", "step"):
+        kf = "queue" if form == "queue_unpadded" else form
+        a = T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form=kf)
+        b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=4, form=kf)
+        io = batch.io(unpadded=form == "queue_unpadded")
         T.walk_mix_async(dg, a, b, split, io)
         T.synchronize(ctx)
         batch.reset()

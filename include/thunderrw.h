@@ -127,7 +127,16 @@ typedef struct {
     int32_t *draws_used;           /* nullable: REPLAY move draws consumed */
     uint64_t *counters;            /* nullable: [0] steps [1] gather-terminated [2] dead ends [3] errors; accumulated */
     const uint64_t *walker_ids;    /* nullable: per-walker global ids keying Philox (else walker_id_base + index) */
+    int32_t path_flags;            /* TRW_PATHS_*: 0 = rows padded with -1 past the walk's length */
+    int32_t reserved;
 } trw_walk_io;
+
+/* trw_walk_io.path_flags.  UNPADDED: row i holds its walk's lengths[i] vertices (at most the
+ * kind's length) and nothing past them is written -- the reference's LOG_SEQUENCE output, where
+ * a walker's sequence is its length_ entries (creeper.h:310-317), and no -1 padding to write;
+ * the caller reads lengths.  The fixed-length FUSED form writes whole rows either way. */
+#define TRW_PATHS_PADDED 0
+#define TRW_PATHS_UNPADDED 1
 
 typedef struct trw_context trw_context;
 typedef struct trw_graph trw_graph;

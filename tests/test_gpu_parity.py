@@ -431,6 +431,42 @@ def test_mixed_kinds_on_one_static_table_from_different_weights_are_refused(gpu_
         dg.close()
 
 
+def test_unpadded_rows_hold_exactly_the_walk(gpu_ctx):
+    """TRW_PATHS_UNPADDED: each row's first lengths[i] entries are the padded walk's, and nothing
+    past them is written (the buffer's -2 stays), in every form and in a mixed batch."""
+    g = graphs.chung_lu(6000, 30000, 2.1, 2, 1500, 58)
+    lbl = np.random.default_rng(3).integers(0, 5, g.ne).astype(np.int32)
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight, lbl)
+    try:
+        src = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        cases = [("metapath", {"schema": [0, 1, 2, 3, 4], "length": 40}, ["fused", "queue", "step", "graph"]),
+                 ("ppr", {"stop_probability": 0.15}, ["queue", "step", "graph"]),
+                 ("node2vec", {"length": 30}, ["fused", "queue", "step"]),
+                 ("deepwalk", {"length": 30}, ["fused", "queue"])]
+        for app, kw, forms in cases:
+            for form in forms:
+                prm = T.make_params(app, rng="philox", seed=5, form=form, **kw)
+                a = T.walk(dg, prm, src)
+                b = T.walk(dg, prm, src, unpadded=True)
+                L = a["paths"].shape[1]
+                rec = np.minimum(a["lengths"], L)
+                inside = np.arange(L)[None, :] < rec[:, None]
+                assert np.array_equal(a["paths"][inside], b["paths"][inside]), (app, form)
+                if form != "fused":
+                    assert np.all(b["paths"][~inside] == -2), (app, form)
+        a_p = T.make_params("ppr", stop_probability=0.15, rng="philox", seed=6, form="queue")
+        b_p = T.make_params("metapath", length=40, schema=[0, 1, 2, 3, 4], rng="philox", seed=6, form="queue")
+        split = len(src) // 2
+        a = T.walk(dg, a_p, src, mix=(b_p, split))
+        b = T.walk(dg, a_p, src, mix=(b_p, split), unpadded=True)
+        rec = np.where(np.arange(len(src)) < split, 1, np.minimum(a["lengths"], 40))
+        inside = np.arange(40)[None, :] < rec[:, None]
+        assert np.array_equal(a["paths"][inside], b["paths"][inside])
+        assert np.all(b["paths"][~inside] == -2)
+    finally:
+        dg.close()
+
+
 def test_walker_ids_key_the_free_running_stream(gpu_ctx):
     """A walker's Philox stream depends only on its global id: a permuted batch with
     per-walker ids gives each walker the walk it has in the ordered batch."""

@@ -17,6 +17,7 @@ WEIGHTS_EDGE, WEIGHTS_UNIT, WEIGHTS_CUSTOM = 0, 1, 2
 WALK_WEIGHTS_APP, WALK_WEIGHTS_EDGE, WALK_WEIGHTS_UNIT = 0, 1, 2
 TUNE_WALKERS_PER_THREAD, TUNE_ITS_TREE_MIN_DEGREE, TUNE_QUEUE_CHUNK, TUNE_ITS_GUIDE_OFF = 0, 1, 2, 3
 RESULT_PPR, RESULT_PATHS = 0, 1
+PATHS_PADDED, PATHS_UNPADDED = 0, 1
 QUERY_PENDING, QUERY_RUNNING, QUERY_DONE, QUERY_FAILED = 0, 1, 2, 3
 WALKER_DONE, WALKER_GATHER_TERMINATED, WALKER_DEAD_END, WALKER_REPLAY_EXHAUSTED, WALKER_TRIAL_CAP, \
     WALKER_BAD_SOURCE, WALKER_STEP_CAP = 0, 1, 2, 3, 4, 5, 6
@@ -57,7 +58,8 @@ class WalkIO(C.Structure):
                 ("walker_id_base", C.c_int64), ("draws", C.c_void_p), ("draws_per_walker", C.c_int64),
                 ("term_draws", C.c_void_p), ("term_draws_per_walker", C.c_int64), ("paths", C.c_void_p),
                 ("lengths", C.c_void_p), ("ends", C.c_void_p), ("status", C.c_void_p), ("draws_used", C.c_void_p),
-                ("counters", C.c_void_p), ("walker_ids", C.c_void_p)]
+                ("counters", C.c_void_p), ("walker_ids", C.c_void_p), ("path_flags", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class TuneReport(C.Structure):

@@ -978,7 +978,11 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
         if (p->schema_length <= 0 || p->schema_length > 16) return fail(TRW_E_INVALID, "schema length must be 1..16");
         k.mode = trw::kModeMetaPath;
         k.schema_length = p->schema_length;
-        for (int i = 0; i < p->schema_length; ++i) k.schema[i] = p->schema[i];
+        for (int i = 0; i < p->schema_length; ++i) {
+            k.schema[i] = p->schema[i];
+            const uint64_t nib = (p->schema[i] >= 0 && p->schema[i] < 8) ? static_cast<uint64_t>(p->schema[i]) : 8u;
+            k.schema_nibbles |= nib << (4 * i);
+        }
         return TRW_OK;
     }
     // DeepWalk, GenericWalk (edge weights) and PPR (unit weights): walker-independent
@@ -1022,6 +1026,9 @@ static int make_io(const trw_graph *g, const trw_walk_params *p, const trw_walk_
     out.path_stride = io->path_stride;
     out.gid_base = static_cast<uint64_t>(io->walker_id_base);
     out.walker_ids = io->walker_ids;
+    if (io->path_flags != TRW_PATHS_PADDED && io->path_flags != TRW_PATHS_UNPADDED)
+        return fail(TRW_E_INVALID, "unknown path_flags %d", io->path_flags);
+    out.unpadded = io->path_flags == TRW_PATHS_UNPADDED;
     out.draws = replay ? io->draws : nullptr;
     out.K = replay ? io->draws_per_walker : 0;
     out.tdraws = replay ? io->term_draws : nullptr;

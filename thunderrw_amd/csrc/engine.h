@@ -138,6 +138,7 @@ struct KindParams {
     double n2v_bound;       // Node2Vec rejection bound: max_, or max_ times the largest edge weight (weighted)
     int32_t n2v_weighted;   // Node2Vec over edge weights: a candidate's weight is w_e times its factor
     int32_t schema[16];
+    uint64_t schema_nibbles;  // schema[i] as nibble i when in [0, 8), else 8 (no neighbour carries it)
     int32_t its_tree_min;   // ITS: vertices of at least this degree search their tree (when built)
     int32_t queue_chunk;    // QUEUE: walker indices a wavefront takes per atomic (0 = from the batch size)
     int32_t its_guide;      // ITS: read the guide records when the graph has them
@@ -150,6 +151,7 @@ struct BatchIO {
     int32_t path_stride;
     uint64_t gid_base;
     const uint64_t *walker_ids;  // per-walker global ids (null: gid_base + index)
+    int32_t unpadded;            // path rows are not padded with -1 past the walk (TRW_PATHS_UNPADDED)
     const uint32_t *draws;
     int64_t K;
     const uint32_t *tdraws;
@@ -180,7 +182,7 @@ struct WalkerState {
     uint32_t gid_lo;  // global walker id: the Philox counter's walker words
     uint32_t gid_hi;
     uint32_t cidx;    // current vertex: root of its ITS search tree (ITS) or membership table
-                      // (Node2Vec); kNoRange: not known / none
+                      // (Node2Vec); MetaPath: this step's schema position; kNoRange: not known / none
     uint32_t pidx;    // previous vertex: root of its membership table (Node2Vec)
     double cmax;      // rejection: the current vertex's maximum weight (from a rejection record), < 0: not known
 };

@@ -437,9 +437,10 @@ __device__ __forceinline__ void finish_move(const DevGraph &g, const KindParams 
 // One Gather + Move + Update for walker w.  Returns true when the walker moved
 // (and counted a step).  On return s.st is kAlive or the walker's final status.
 //
-// blk holds this step's trial-0 Philox block on entry; after the step's gather
-// has been issued the next step's block is computed into it (free-running only).
-template <int MODE, bool REPLAY>
+// blk holds this step's trial-0 Philox block on entry; with AHEAD, after the step's gather has been
+// issued the next step's block is computed into it (free-running only); without, the caller
+// computes each step's block itself.
+template <int MODE, bool REPLAY, bool AHEAD = true>
 __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams &P, const BatchIO &io, int32_t w,
                                              WalkerState &s, uint4 &blk) {
     const int32_t cur = s.cur;
@@ -461,6 +462,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     uint32_t noff = 0u, ndeg = kNoRange, nidx = kNoRange;
     double nmax = -1.0;  // rejection: the next vertex's maximum weight, when a record gave it
     uint32_t nlo = 0u, nhi = kNoRange;  // MetaPath: the next step's label range at the next vertex
+    uint32_t npos_out = kNoRange;       // MetaPath: the next step's schema position
     uint4 nblk = blk;
 
     if (MODE == kModeUniform) {
@@ -468,13 +470,13 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         uint32_t u;
         if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
         next = read_neighbor(g, off + (u % ud), noff, ndeg);
-        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+        if (!REPLAY && AHEAD) nblk = philox_block(io, s, step + 1, 0u);
     } else if (MODE == kModeAlias) {
         // alias_sampling.h:147-172: column u0 % d, then real2(u1) <= prob ? first : second.
         uint32_t u0, u1;
         if (!draw_pair<REPLAY>(io, w, s, step, 0u, blk, u0, u1)) return false;
         const bool ok = read_alias(g, off + (u0 % ud), u1, next, noff, ndeg);
-        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+        if (!REPLAY && AHEAD) nblk = philox_block(io, s, step + 1, 0u);
         if (!ok) {
             if (REPLAY) s.k -= 2;  // the vertex's weights sum to zero: terminate without consuming draws
             s.st = kGatherTerminated;
@@ -486,7 +488,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
         const int64_t e = off + static_cast<int64_t>((static_cast<uint64_t>(u) * ud) >> 32);
         const uint4 ra = g.grec[2 * e], rb = g.grec[2 * e + 1];
-        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+        if (!REPLAY && AHEAD) nblk = philox_block(io, s, step + 1, 0u);
         if (ra.w != 0u) {
             next = guide_search(g, cur, u, noff, ndeg);
         } else {
@@ -519,7 +521,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             }
             uint32_t u;
             if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
-            if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+            if (!REPLAY && AHEAD) nblk = philox_block(io, s, step + 1, 0u);
             const double target = real2(u) * total;
             int64_t leaf = 0;
             bool beyond = false;  // target >= total: the pick clamps to the last edge
@@ -558,7 +560,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             }
             uint32_t u;
             if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
-            if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+            if (!REPLAY && AHEAD) nblk = philox_block(io, s, step + 1, 0u);
             // The count is < d whenever the prefix sums are non-decreasing (non-negative
             // weights); the clamp only keeps negative-weight inputs inside the segment.
             const int32_t pos = min(count_le(P_, d, real2(u) * total), d - 1);
@@ -602,7 +604,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         } else {
             next = read_neighbor(g, off + pos, noff, ndeg);
         }
-        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+        if (!REPLAY && AHEAD) nblk = philox_block(io, s, step + 1, 0u);
     } else if (MODE == kModeNode2Vec) {
         // rejection_sampling.h:368-402 with Node2Vec::weight (node2vec.cpp:24-38): a trial is
         // accepted unless real2(ub) * bound > weight(candidate), the weight being max_ on the first
@@ -639,12 +641,18 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         }
         next = c;
         nidx = croot;
-        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+        if (!REPLAY && AHEAD) nblk = philox_block(io, s, step + 1, 0u);
     } else {  // kModeMetaPath
         // dynamic_gather (creeper.h:178-296) with MetaPath::weight (metapath.cpp:22-30)
         // then dynamic_its_move: the prefix sums are match counts, so the pick is
         // the (floor(real2(u) * K) + 1)-th neighbour whose label matches.
-        const int32_t target = P.schema[(s.len - 1) % P.schema_length];
+        // The schema position (length - 1) % |schema|, carried by the walker in cidx after its
+        // first move (a modulo per step otherwise); with label records the schema's labels come as
+        // nibbles (8: a label outside [0, 8), which no neighbour carries).
+        const uint32_t pos = s.cidx != kNoRange ? s.cidx : static_cast<uint32_t>((s.len - 1) % P.schema_length);
+        const uint32_t npos = pos + 1u == static_cast<uint32_t>(P.schema_length) ? 0u : pos + 1u;
+        const int32_t target =
+            g.lerec ? static_cast<int32_t>((P.schema_nibbles >> (4u * pos)) & 15u) : P.schema[pos];
         int32_t a, K;
         if (g.lerec) {
             // Matches of `target` are [lstart[target], lstart[target + 1]) in label order; after
@@ -653,7 +661,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             if (s.pdeg != kNoRange) {
                 lo = s.poff;
                 hi = s.pdeg;
-            } else if (target >= 0 && target < 8) {
+            } else if (target < 8) {
                 const uint32_t *L = g.lstart + 8 * static_cast<int64_t>(cur);
                 lo = L[target];
                 hi = target < 7 ? L[target + 1] : ud;
@@ -679,9 +687,9 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             noff = r.y;
             ndeg = r.z;
             // The next step's label range at `next`, from its label starts in the same record.
-            const int32_t nt = P.schema[s.len % P.schema_length];
+            const int32_t nt = static_cast<int32_t>((P.schema_nibbles >> (4u * npos)) & 15u);
             nlo = nhi = 0u;
-            if (nt >= 0 && nt < 8) {
+            if (nt < 8) {
                 const uint32_t *NL = reinterpret_cast<const uint32_t *>(R + 1);
                 nlo = NL[nt];
                 nhi = nt < 7 ? NL[nt + 1] : ndeg;
@@ -689,14 +697,16 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         } else {
             next = read_neighbor(g, off + g.lbl_pos[off + a + t], noff, ndeg);
         }
-        if (!REPLAY) nblk = philox_block(io, s, step + 1, 0u);
+        npos_out = npos;
+        if (!REPLAY && AHEAD) nblk = philox_block(io, s, step + 1, 0u);
     }
 
     finish_move(g, P, s, cur, off, d, next, noff, ndeg, nidx);
     if (MODE == kModeRejection) s.cmax = nmax;
-    if (MODE == kModeMetaPath) {  // the previous vertex's range is Node2Vec's alone
+    if (MODE == kModeMetaPath) {  // the previous vertex's range and table root are Node2Vec's alone
         s.poff = nlo;
         s.pdeg = nhi;
+        s.cidx = npos_out;
     }
     const uint4 this_blk = blk;
     blk = nblk;
@@ -710,18 +720,30 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     return true;
 }
 
-// MODE < 0: the walker kind's mode is read at run time (mixed batches).
-template <int MODE, bool REPLAY>
+// Mixed batches of two walker kinds: MODE == -1 compiles every Move and reads the kind's at run
+// time; pair_mode(a, b) (<= -2) compiles only Moves a and b, so a kernel for a given pair holds
+// the registers of those two alone (every Move: 117 VGPRs, three wavefronts per SIMD in the queue
+// form; uniform + MetaPath alone: far fewer).
+constexpr int pair_mode(int a, int b) { return -2 - (a < b ? a * 8 + b : b * 8 + a); }
+constexpr int pair_first(int mode) { return (-2 - mode) / 8; }
+constexpr int pair_second(int mode) { return (-2 - mode) % 8; }
+
+template <int MODE, bool REPLAY, bool AHEAD = true>
 __device__ __forceinline__ bool advance(const DevGraph &g, const KindParams &P, const BatchIO &io, int32_t w,
                                         WalkerState &s, uint4 &blk) {
-    if (MODE >= 0) return advance_mode<MODE, REPLAY>(g, P, io, w, s, blk);
+    if (MODE >= 0) return advance_mode<MODE, REPLAY, AHEAD>(g, P, io, w, s, blk);
+    if (MODE <= -2) {
+        constexpr int M1 = MODE <= -2 ? pair_first(MODE) : 0, M2 = MODE <= -2 ? pair_second(MODE) : 0;
+        return P.mode == M1 ? advance_mode<M1, REPLAY, AHEAD>(g, P, io, w, s, blk)
+                            : advance_mode<M2, REPLAY, AHEAD>(g, P, io, w, s, blk);
+    }
     switch (P.mode) {
-    case kModeUniform: return advance_mode<kModeUniform, REPLAY>(g, P, io, w, s, blk);
-    case kModeAlias: return advance_mode<kModeAlias, REPLAY>(g, P, io, w, s, blk);
-    case kModeIts: return advance_mode<kModeIts, REPLAY>(g, P, io, w, s, blk);
-    case kModeRejection: return advance_mode<kModeRejection, REPLAY>(g, P, io, w, s, blk);
-    case kModeNode2Vec: return advance_mode<kModeNode2Vec, REPLAY>(g, P, io, w, s, blk);
-    default: return advance_mode<kModeMetaPath, REPLAY>(g, P, io, w, s, blk);
+    case kModeUniform: return advance_mode<kModeUniform, REPLAY, AHEAD>(g, P, io, w, s, blk);
+    case kModeAlias: return advance_mode<kModeAlias, REPLAY, AHEAD>(g, P, io, w, s, blk);
+    case kModeIts: return advance_mode<kModeIts, REPLAY, AHEAD>(g, P, io, w, s, blk);
+    case kModeRejection: return advance_mode<kModeRejection, REPLAY, AHEAD>(g, P, io, w, s, blk);
+    case kModeNode2Vec: return advance_mode<kModeNode2Vec, REPLAY, AHEAD>(g, P, io, w, s, blk);
+    default: return advance_mode<kModeMetaPath, REPLAY, AHEAD>(g, P, io, w, s, blk);
     }
 }
 
@@ -1247,7 +1269,7 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
         }
     }
     if (w < io.n) {
-        out.finish(s.len, L);
+        out.finish(s.len, io.unpadded ? s.len : L);
         write_outputs(io, w, s);
     }
     unsigned long long gterm = 0, dead = 0, err = 0;
@@ -1256,6 +1278,105 @@ __global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g,
 }
 
 // ---------------------------------------------------------------- queue form
+//
+// Persistent: each lane walks one walker at a time and, while it walks, prepares the next one --
+// its index from the wavefront's chunk of the shared queue, then its source (and global id), then
+// what its first step reads before the gather (the source's vertex or guide range, MetaPath's
+// label range, the rejection maximum) -- one load per stage, issued beside the step's gather.  A
+// walker that ends is replaced in the same iteration by the prepared one, whose first step is then
+// one gather like any other.  (Loading a new walker on demand put a chain of three dependent reads
+// -- source, vertex record, first gather -- into the iteration, and a wavefront waits for its
+// slowest lane: 89% of iterations carried a refill and took 4.3 times as long as the others.)
+
+// What a kind's first step reads before its gather, for a walker starting at src (walk.hip's
+// advance_mode reads the same when the walker carries nothing): .x/.y the range it carries in
+// coff/cdeg (vertex record, or ITS guide range), .z/.w MetaPath's label range for schema[0]
+// (poff/pdeg) or the rejection maximum's bits (cmax); kNoRange where the step reads it itself.
+// (The kind's mode, ITS guide switch and first schema label come as values: a reference to one of
+// two kernel-argument structures chosen at run time puts both in scratch memory.)
+__device__ __forceinline__ uint4 first_step_inputs(const DevGraph &g, int32_t mode, int32_t its_guide,
+                                                   int32_t schema0, int32_t src) {
+    uint4 r = make_uint4(0u, kNoRange, 0u, kNoRange);
+    if (mode == kModeIts && g.grec != nullptr && its_guide) {
+        const uint2 v = g.gvrec[src];
+        r.x = v.x;
+        r.y = v.y;
+        return r;
+    }
+    if (g.vrec) {
+        const uint2 v = g.vrec[src];
+        r.x = v.x;
+        r.y = v.y;
+    }
+    if (mode == kModeMetaPath && g.lerec) {
+        const int32_t t = schema0;  // the first schema label as a nibble (8: outside [0, 8))
+        if (t < 8) {
+            const uint32_t *L = g.lstart + 8 * static_cast<int64_t>(src);
+            r.z = L[t];
+            r.w = t < 7 ? L[t + 1] : kNoRange;  // label 7 ends at the degree (the vertex record's .y)
+        } else {
+            r.z = 0u;
+            r.w = 0u;  // no neighbour carries a label outside [0, 8)
+        }
+    } else if (mode == kModeRejection && g.rrec) {
+        const double m = g.rs_max[src];
+        const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(m));
+        r.z = static_cast<uint32_t>(b);
+        r.w = static_cast<uint32_t>(b >> 32);
+    }
+    return r;
+}
+
+// The path rows of queue positions [p0, p1), filled by the wavefront that has just taken them:
+// -1 in [0, L) of a row of length L > 1 (its walker writes its source and moves over them), the
+// source alone in a row of length 1 (PPR's, which its walker then leaves alone).  Consecutive
+// positions are consecutive rows (position p runs walker p + rot mod n), so consecutive lanes
+// store consecutive 16-byte pieces: about 1 KB per store instruction, where a separate pass over
+// every row before the walk cost 12% of the PPR + MetaPath batch.  A walker's own stores come at
+// least two iterations later, after waits that the counter, covering stores in issue order, makes
+// wait for these too.
+__device__ __forceinline__ void fill_chunk_rows(const BatchIO &io, int32_t split, int32_t La, int32_t Lb,
+                                                unsigned int p0, unsigned int p1, unsigned int rot, int lane) {
+    const unsigned int n = static_cast<unsigned int>(io.n);
+    const auto walker = [&](unsigned int pos) {
+        return static_cast<int32_t>(pos < n - rot ? pos + rot : pos - (n - rot));
+    };
+    const bool vec = (io.path_stride & 3) == 0 && (reinterpret_cast<uintptr_t>(io.paths) & 15) == 0;
+    // Pieces per row: L / 4 16-byte pieces (L > 1, L % 4 == 0), L entries (other L > 1), or the
+    // source entry (L == 1).
+    const int32_t qa = La == 1 ? 1 : io.unpadded ? 0 : ((La & 3) == 0 && vec ? La >> 2 : La);
+    const int32_t qb = Lb == 1 ? 1 : io.unpadded ? 0 : ((Lb & 3) == 0 && vec ? Lb >> 2 : Lb);
+    // Kinds come in runs of positions; a chunk holds both only if it crosses the position of
+    // walker `split` or of walker 0.
+    const unsigned int split_pos = static_cast<unsigned int>(split) >= n ? n : (static_cast<unsigned int>(split) + n - rot) % n;
+    const unsigned int zero_pos = (n - rot) % n;
+    const bool both = (split_pos > p0 && split_pos < p1) || (zero_pos > p0 && zero_pos < p1);
+    const int32_t qmax = both ? (qa > qb ? qa : qb) : (walker(p0) >= split ? qb : qa);
+    if (qmax == 0) return;  // unpadded rows of length > 1: nothing to write before the walk
+    const unsigned int rows = p1 - p0;
+    const auto piece = [&](unsigned int r, int32_t v) {
+        const int32_t w = walker(p0 + r);
+        const bool kb = w >= split;
+        const int32_t L = kb ? Lb : La;
+        if (v >= (kb ? qb : qa)) return;
+        int32_t *row = io.paths + static_cast<int64_t>(w) * io.path_stride;
+        if (L == 1) row[0] = io.sources[w];
+        else if (io.unpadded) return;  // the walker writes its source and moves, nothing else
+        else if (vec && (L & 3) == 0) reinterpret_cast<int4 *>(row)[v] = make_int4(-1, -1, -1, -1);
+        else row[v] = -1;
+    };
+    if (qmax <= 64) {
+        // 64 / qmax rows per pass, lane i on piece i % qmax of row i / qmax.
+        const int32_t per = 64 / qmax, r_in = lane / qmax, v = lane - r_in * qmax;
+        if (r_in < per)
+            for (unsigned int r = static_cast<unsigned int>(r_in); r < rows; r += static_cast<unsigned int>(per))
+                piece(r, v);
+    } else {
+        for (unsigned int r = 0; r < rows; ++r)
+            for (int32_t v = lane; v < qmax; v += 64) piece(r, v);
+    }
+}
+
 template <int MODE, bool REPLAY>
 __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, KindParams A, KindParams B,
                                                                   int32_t split, BatchIO io, unsigned int *queue,
@@ -1265,35 +1386,82 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
     uint4 blk = make_uint4(0u, 0u, 0u, 0u);
     int32_t w = -1;
     bool kind_b = false;
-    bool drained = false;
     unsigned long long steps = 0, gterm = 0, dead = 0, err = 0;
     const unsigned int n = static_cast<unsigned int>(io.n);
-    // The wavefront takes walker indices from the shared queue `chunk` at a time and
-    // hands them to its idle lanes from [qnext, qend): one returning atomic per chunk
-    // rather than per refill keeps the single queue head off the critical path.
+    // The wavefront takes walker indices from the shared queue `chunk` at a time and hands them to
+    // its lanes from [qnext, qend): one returning atomic per chunk rather than per walker keeps the
+    // single queue head off the critical path.
     unsigned int qnext = 0, qend = 0;
     bool queue_empty = false;
-    // The lane's walker's path entries gather here and go out in runs (the rows were set to -1
-    // beforehand), instead of one store request per entry.
+    // The lane's next walker: stage 0 none, 1 index held (source and id loads issued), 2 source
+    // held (first-step loads issued); its inputs arrive by the next iteration.
+    int32_t stage = 0, nw = -1, nsrc = 0;
+    uint32_t ngid_lo = 0u, ngid_hi = 0u;
+    uint4 nin = make_uint4(0u, kNoRange, 0u, kNoRange);
+    // The lane's walker's path entries gather here and go out in runs (its row was set to -1 when
+    // the wavefront took its chunk), instead of one store request per entry.
     __shared__ int32_t pchunk[kQueueBlock * (kN2vChunk + 1)];
     LaneRow out{nullptr, pchunk + threadIdx.x * (kN2vChunk + 1), 0};
 
     for (;;) {
-        const bool need = (s.st != kAlive) && !drained;
-        const unsigned long long need_mask = __ballot(need);
-        if (need_mask) {
-            const unsigned int cnt = static_cast<unsigned int>(__popcll(need_mask));
+        // 1. A lane whose walker has ended takes the prepared one.
+        if (s.st != kAlive && stage == 2) {
+            w = nw;
+            kind_b = w >= split;
+            const int32_t mode = kind_b ? B.mode : A.mode, length = kind_b ? B.length : A.length;
+            s = empty_walker();
+            s.gid_lo = ngid_lo;
+            s.gid_hi = ngid_hi;
+            s.cur = nsrc;
+            s.st = (nsrc >= 0 && nsrc < g.nv) ? kAlive : kBadSource;
+            if (s.st == kAlive) {
+                s.coff = nin.x;
+                s.cdeg = nin.y;
+                if (mode == kModeMetaPath && g.lerec) {
+                    s.poff = nin.z;
+                    s.pdeg = nin.w != kNoRange ? nin.w : nin.y;
+                } else if (mode == kModeRejection && g.rrec) {
+                    s.cmax = __hiloint2double(static_cast<int>(nin.w), static_cast<int>(nin.z));
+                }
+            }
+            stage = 0;
+            if (io.paths) {
+                // Position 0 already holds the source: a length-1 row (PPR's) is complete; a longer
+                // one restates it so that its first run of stores starts on the row's alignment.
+                out.row = length > 1 ? io.paths + static_cast<int64_t>(w) * io.path_stride : nullptr;
+                out.staged = 0;
+                out.put(s.cur, 1, length);
+            }
+            if (s.st != kAlive) {  // bad source: finished before the first step
+                if (out.row) out.flush(1);
+                write_outputs(io, w, s);
+                classify(s.st, gterm, dead, err);
+            }
+        }
+
+        // 2. Preparation, one stage per iteration; each stage's loads are issued before this
+        // iteration's gather and have arrived when it has.
+        if (stage == 1) {
+            const bool nb = nw >= split;
+            if (nsrc >= 0 && nsrc < g.nv)
+                nin = first_step_inputs(g, nb ? B.mode : A.mode, nb ? B.its_guide : A.its_guide,
+                                        static_cast<int32_t>((nb ? B.schema_nibbles : A.schema_nibbles) & 15u), nsrc);
+            stage = 2;
+        }
+        const bool want = stage == 0 && !queue_empty;
+        const unsigned long long want_mask = __ballot(want);
+        if (want_mask) {
+            const unsigned int cnt = static_cast<unsigned int>(__popcll(want_mask));
             const unsigned int avail = qend - qnext;
             unsigned int fresh = n;  // first index of a newly taken chunk (n = none)
-            if (avail < cnt && !queue_empty) {
-                const int leader = __ffsll(static_cast<long long>(need_mask)) - 1;
+            if (avail < cnt) {
+                const int leader = __ffsll(static_cast<long long>(want_mask)) - 1;
                 unsigned int base = 0;
                 if (lane == leader) base = atomicAdd(queue, chunk);
                 base = __shfl(base, leader, 64);
-                if (base >= n) queue_empty = true;
-                else fresh = base;
+                if (base < n) fresh = base;
             }
-            const unsigned int rank = static_cast<unsigned int>(__popcll(need_mask & ((1ull << lane) - 1ull)));
+            const unsigned int rank = static_cast<unsigned int>(__popcll(want_mask & ((1ull << lane) - 1ull)));
             unsigned int idx = n;
             if (rank < avail) idx = qnext + rank;
             else if (fresh < n && fresh + (rank - avail) < min(fresh + chunk, n)) idx = fresh + (rank - avail);
@@ -1301,47 +1469,43 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                 const unsigned int used = cnt > avail ? cnt - avail : 0u;
                 qend = min(fresh + chunk, n);
                 qnext = min(fresh + used, qend);
+                if (io.paths) fill_chunk_rows(io, split, A.length > 0 ? A.length : 1, B.length > 0 ? B.length : 1, fresh,
+                                              qend, rot, lane);
             } else {
                 qnext += min(cnt, avail);
+                // No chunk was left: the queue is empty once this wavefront's own chunk is.
+                if (avail < cnt) queue_empty = true;
             }
-            if (queue_empty && qnext >= qend) drained = true;
-            if (need) {
-                if (idx < n) {
-                    // Queue position idx runs walker idx + rot (mod n): the kind with the longer
-                    // walks starts first, so the batch ends on short walks.
-                    w = static_cast<int32_t>(idx < n - rot ? idx + rot : idx - (n - rot));
-                    kind_b = w >= split;
-                    init_walker(g, io, w, s);
-                    blk = first_block<REPLAY>(io, w, s);
-                    const KindParams &P = kind_b ? B : A;
-                    if (io.paths) {
-                        out.row = io.paths + static_cast<int64_t>(w) * io.path_stride;
-                        out.staged = 0;
-                        out.put(s.cur, 1, P.length);
-                    }
-                    if (s.st != kAlive) {  // bad source: finished before the first step
-                        if (io.paths) out.flush(1);
-                        write_outputs(io, w, s);
-                        classify(s.st, gterm, dead, err);
-                    }
-                }
+            if (want && idx < n) {
+                // Queue position idx runs walker idx + rot (mod n): the kind with the longer walks
+                // starts first, so the batch ends on short walks.
+                nw = static_cast<int32_t>(idx < n - rot ? idx + rot : idx - (n - rot));
+                nsrc = io.sources[nw];
+                const uint64_t gid = io.walker_ids ? io.walker_ids[nw] : io.gid_base + static_cast<uint64_t>(nw);
+                ngid_lo = static_cast<uint32_t>(gid);
+                ngid_hi = static_cast<uint32_t>(gid >> 32);
+                nin = make_uint4(0u, kNoRange, 0u, kNoRange);
+                stage = 1;
             }
         }
+
+        // 3. One step of the lane's walker.
         const bool active = s.st == kAlive;
-        if (__ballot(active) == 0) {
-            if (drained) break;
-            continue;
-        }
+        if (__ballot(active || stage != 0) == 0) break;  // nothing walks, nothing is prepared: drained
         if (active) {
-            const KindParams &P = kind_b ? B : A;
-            const bool moved =
-                kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
+            const int32_t length = kind_b ? B.length : A.length;
+            // This step's Philox block, computed here by every walking lane alike: a lane that has
+            // just taken a new walker needs no block of its own (in a wavefront every path any lane
+            // takes costs every lane).
+            blk = first_block<REPLAY>(io, w, s);
+            const bool moved = kind_b ? advance<MODE, REPLAY, false>(g, B, io, w, s, blk)
+                                      : advance<MODE, REPLAY, false>(g, A, io, w, s, blk);
             if (moved) {
                 ++steps;
-                if (io.paths && s.len - 1 < P.length) out.put(s.cur, s.len, P.length);
+                if (out.row && s.len - 1 < length) out.put(s.cur, s.len, length);
             }
             if (s.st != kAlive) {
-                if (io.paths) out.flush(s.len < P.length ? s.len : P.length);
+                if (out.row) out.flush(s.len < length ? s.len : length);
                 write_outputs(io, w, s);
                 classify(s.st, gterm, dead, err);
             }
@@ -1484,30 +1648,34 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_finish_kernel(BatchIO io
     }
 }
 
-// -1 in entries [0, length) of every path row (length of the walker's kind), written
-// before the QUEUE, STEP and GRAPH forms run, so that their walkers write only the
-// vertices they visit and never pad rows one entry at a time.  Rows whose length equals
-// the stride form one contiguous range, filled with 16-byte stores; other rows take a
-// thread per (row, entry).
-__global__ void fill_range_kernel(int32_t *p, int64_t n) {
-    const int64_t head = (-(reinterpret_cast<uintptr_t>(p) >> 2)) & 3;  // entries before 16-byte alignment
-    const int64_t h = head < n ? head : n;
-    const int64_t vec = (n - h) >> 2;
-    int4 *pv = reinterpret_cast<int4 *>(p + h);
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (int64_t i = t; i < vec; i += stride) pv[i] = make_int4(-1, -1, -1, -1);
-    if (t < h) p[t] = -1;
-    const int64_t tail = h + 4 * vec;
-    if (t < n - tail) p[tail + t] = -1;
+// Path rows before the STEP and GRAPH forms run: the walker's source at position 0 and -1
+// in [1, L), so that a walk writes only the vertices it moves to (measured: padding each row with
+// 16-byte stores as its walk ends made the PPR + MetaPath mix 11% slower and MetaPath alone 30%,
+// since stores share the wait counter with the gathers the walk waits on).  Rows that are 16-byte aligned
+// with L a multiple of 4 are written as 16-byte pieces, a block covering 256 / (L / 4) consecutive
+// rows at a time (consecutive threads, consecutive pieces); other rows an entry per thread.
+__global__ void fill_rows_source_vec_kernel(int32_t *p, const int32_t *src, int64_t rows, int32_t stride, int32_t L) {
+    const int32_t q = L >> 2;  // pieces per row
+    const int32_t per = q <= static_cast<int32_t>(blockDim.x) ? static_cast<int32_t>(blockDim.x) / q : 1;
+    const int32_t r_in = static_cast<int32_t>(threadIdx.x) / (q <= static_cast<int32_t>(blockDim.x) ? q : 1 << 30);
+    const int32_t v0 = q <= static_cast<int32_t>(blockDim.x) ? static_cast<int32_t>(threadIdx.x) - r_in * q
+                                                             : static_cast<int32_t>(threadIdx.x);
+    if (r_in >= per) return;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * per + r_in; row < rows;
+         row += static_cast<int64_t>(gridDim.x) * per) {
+        int4 *r = reinterpret_cast<int4 *>(p + row * stride);
+        for (int32_t v = v0; v < q; v += (q <= static_cast<int32_t>(blockDim.x) ? q : static_cast<int32_t>(blockDim.x)))
+            r[v] = make_int4(v == 0 ? src[row] : -1, -1, -1, -1);
+    }
 }
 
-__global__ void fill_rows_kernel(int32_t *p, int64_t rows, int32_t stride, int32_t L) {
+__global__ void fill_rows_source_kernel(int32_t *p, const int32_t *src, int64_t rows, int32_t stride, int32_t L) {
     const int64_t total = rows * L;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const int64_t row = L == 1 ? i : i / L;
-        p[row * stride + (i - row * L)] = -1;
+        const int64_t col = i - row * L;
+        p[row * stride + col] = col == 0 ? src[row] : -1;
     }
 }
 
@@ -1611,36 +1779,53 @@ hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO 
     return hipGetLastError();
 }
 
-int queue_grid(int device_cus) { return device_cus * 8; }
-
-void fill_path_rows(int32_t *paths, int64_t rows, int32_t stride, int32_t L, hipStream_t stream) {
-    if (rows <= 0) return;
-    const int64_t total = rows * L;
-    const int64_t blocks = std::min<int64_t>(grid_for(L == stride ? (total + 3) / 4 : total, 256), 256 * 64);
-    if (L == stride)
-        hipLaunchKernelGGL(fill_range_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, paths, total);
-    else
-        hipLaunchKernelGGL(fill_rows_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, paths, rows,
-                           stride, L);
+// Blocks of a persistent kernel that one CU holds at once.  The grid is that times the CUs: blocks
+// beyond it would start only when the resident ones have drained the queue, and find it empty.
+template <class K>
+int blocks_per_cu(K kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kQueueBlock, 0) != hipSuccess || per_cu < 1)
+        per_cu = 2;
+    return per_cu;
 }
 
-// Every form writes each walker's position 0 (its source, valid or not) itself, so rows of length
-// 1 (PPR's) need no -1 beforehand.
+// Rows [0, rows) of length L: each walker's source at position 0 (valid or not), -1 after it.
+void fill_source_rows(int32_t *paths, const int32_t *src, int64_t rows, int32_t stride, int32_t L,
+                      hipStream_t stream) {
+    if (rows <= 0) return;
+    if ((L & 3) == 0 && (stride & 3) == 0 && (reinterpret_cast<uintptr_t>(paths) & 15) == 0) {
+        const int32_t q = L >> 2, per = q <= 256 ? 256 / q : 1;
+        const int64_t blocks = std::min<int64_t>((rows + per - 1) / per, 256 * 64);
+        hipLaunchKernelGGL(fill_rows_source_vec_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, paths,
+                           src, rows, stride, L);
+    } else {
+        const int64_t blocks = std::min<int64_t>(grid_for(rows * L, 256), 256 * 64);
+        hipLaunchKernelGGL(fill_rows_source_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, paths,
+                           src, rows, stride, L);
+    }
+}
+
+// Every walker's row holds its source and -1 before a STEP or GRAPH walk starts (the QUEUE form
+// fills each chunk's rows as a wavefront takes it, fill_chunk_rows).
 void fill_paths(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io, hipStream_t stream) {
-    if (!io.paths || io.n <= 0) return;
+    if (!io.paths || io.n <= 0 || io.unpadded) return;  // unpadded: the walk writes its source itself
     const int32_t La = a.length > 0 ? a.length : 1, Lb = b.length > 0 ? b.length : 1;
     const int32_t sa = split < io.n ? split : io.n;
-    if (La > 1) fill_path_rows(io.paths, sa, io.path_stride, La, stream);
-    if (Lb > 1) fill_path_rows(io.paths + static_cast<int64_t>(sa) * io.path_stride, io.n - sa, io.path_stride, Lb, stream);
+    fill_source_rows(io.paths, io.sources, sa, io.path_stride, La, stream);
+    fill_source_rows(io.paths + static_cast<int64_t>(sa) * io.path_stride, io.sources + sa, io.n - sa, io.path_stride,
+                     Lb, stream);
 }
 
 template <int MODE, bool REPLAY>
 void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
-                  unsigned int *q, hipStream_t stream, unsigned grid) {
-    // About two chunks per wavefront of the grid, between 64 and 4096 walkers each,
-    // unless the graph's tuning fixes it.
+                  unsigned int *q, hipStream_t stream, int cus) {
+    static const int per_cu = blocks_per_cu(walk_queue_kernel<MODE, REPLAY>);  // one kernel, one answer
+    const int64_t resident = static_cast<int64_t>(per_cu) * cus, needed = grid_for(io.n, kQueueBlock);
+    const unsigned grid = static_cast<unsigned>(resident < needed ? resident : needed);
+    // About eight chunks per wavefront of the grid, between 64 and 4096 walkers each, unless the
+    // graph's tuning fixes it: a wavefront that takes its last chunk late ends the batch late.
     const int64_t waves = static_cast<int64_t>(grid) * (kQueueBlock / 64);
-    int64_t chunk = a.queue_chunk > 0 ? a.queue_chunk : static_cast<int64_t>(io.n) / (waves * 2);
+    int64_t chunk = a.queue_chunk > 0 ? a.queue_chunk : static_cast<int64_t>(io.n) / (waves * 8);
     chunk = chunk < 64 ? 64 : (chunk > 4096 ? 4096 : chunk);
     // Expected walk length of a kind: its length, or min(length, 1 / stop probability) for a walk
     // that stops by probability.  The kind whose walks run longer takes the queue's first positions.
@@ -1656,19 +1841,43 @@ void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, i
 
 template <bool REPLAY>
 void queue_dispatch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
-                    unsigned int *q, hipStream_t stream, unsigned grid) {
+                    unsigned int *q, hipStream_t stream, int cus) {
     const bool single = (split >= io.n) || (a.mode == b.mode);
     if (!single) {
-        queue_launch<-1, REPLAY>(g, a, b, split, io, q, stream, grid);
+        // Two kinds: the kernel compiled for exactly their two Moves.
+#define TRW_PAIR(x, y)                                                                                \
+    case pair_mode(x, y):                                                                             \
+        queue_launch<pair_mode(x, y), REPLAY>(g, a, b, split, io, q, stream, cus);                    \
         return;
+        switch (pair_mode(a.mode, b.mode)) {
+            TRW_PAIR(kModeUniform, kModeAlias)
+            TRW_PAIR(kModeUniform, kModeIts)
+            TRW_PAIR(kModeUniform, kModeRejection)
+            TRW_PAIR(kModeUniform, kModeNode2Vec)
+            TRW_PAIR(kModeUniform, kModeMetaPath)
+            TRW_PAIR(kModeAlias, kModeIts)
+            TRW_PAIR(kModeAlias, kModeRejection)
+            TRW_PAIR(kModeAlias, kModeNode2Vec)
+            TRW_PAIR(kModeAlias, kModeMetaPath)
+            TRW_PAIR(kModeIts, kModeRejection)
+            TRW_PAIR(kModeIts, kModeNode2Vec)
+            TRW_PAIR(kModeIts, kModeMetaPath)
+            TRW_PAIR(kModeRejection, kModeNode2Vec)
+            TRW_PAIR(kModeRejection, kModeMetaPath)
+            TRW_PAIR(kModeNode2Vec, kModeMetaPath)
+        default:
+            queue_launch<-1, REPLAY>(g, a, b, split, io, q, stream, cus);
+            return;
+        }
+#undef TRW_PAIR
     }
     switch (split >= io.n ? a.mode : b.mode) {
-    case kModeUniform: queue_launch<kModeUniform, REPLAY>(g, a, b, split, io, q, stream, grid); break;
-    case kModeAlias: queue_launch<kModeAlias, REPLAY>(g, a, b, split, io, q, stream, grid); break;
-    case kModeIts: queue_launch<kModeIts, REPLAY>(g, a, b, split, io, q, stream, grid); break;
-    case kModeRejection: queue_launch<kModeRejection, REPLAY>(g, a, b, split, io, q, stream, grid); break;
-    case kModeNode2Vec: queue_launch<kModeNode2Vec, REPLAY>(g, a, b, split, io, q, stream, grid); break;
-    default: queue_launch<kModeMetaPath, REPLAY>(g, a, b, split, io, q, stream, grid); break;
+    case kModeUniform: queue_launch<kModeUniform, REPLAY>(g, a, b, split, io, q, stream, cus); break;
+    case kModeAlias: queue_launch<kModeAlias, REPLAY>(g, a, b, split, io, q, stream, cus); break;
+    case kModeIts: queue_launch<kModeIts, REPLAY>(g, a, b, split, io, q, stream, cus); break;
+    case kModeRejection: queue_launch<kModeRejection, REPLAY>(g, a, b, split, io, q, stream, cus); break;
+    case kModeNode2Vec: queue_launch<kModeNode2Vec, REPLAY>(g, a, b, split, io, q, stream, cus); break;
+    default: queue_launch<kModeMetaPath, REPLAY>(g, a, b, split, io, q, stream, cus); break;
     }
 }
 
@@ -1706,12 +1915,9 @@ hipError_t launch_walk_queue(const DevGraph &g, const KindParams &a, const KindP
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
     }
-    const int64_t needed = grid_for(io.n, kQueueBlock);
-    const unsigned grid = static_cast<unsigned>(needed < queue_grid(cus) ? needed : queue_grid(cus));
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, queue_counter, 0u, int64_t(1));
-    fill_paths(a, b, split, io, stream);
-    if (io.replay) queue_dispatch<true>(g, a, b, split, io, queue_counter, stream, grid);
-    else queue_dispatch<false>(g, a, b, split, io, queue_counter, stream, grid);
+    if (io.replay) queue_dispatch<true>(g, a, b, split, io, queue_counter, stream, cus);
+    else queue_dispatch<false>(g, a, b, split, io, queue_counter, stream, cus);
     return hipGetLastError();
 }
 

@@ -256,7 +256,7 @@ def make_params(app, execution=None, sampling=None, length=None, stop_probabilit
 
 def make_io(sources, n_walkers, path_stride=0, walker_id_base=0, draws=None, draws_per_walker=0, term_draws=None,
             term_draws_per_walker=0, paths=None, lengths=None, ends=None, status=None, draws_used=None,
-            counters=None, walker_ids=None):
+            counters=None, walker_ids=None, unpadded=False):
     io = A.WalkIO()
     io.sources = _ptr(sources)
     io.n_walkers = int(n_walkers)
@@ -269,6 +269,7 @@ def make_io(sources, n_walkers, path_stride=0, walker_id_base=0, draws=None, dra
     io.paths, io.lengths, io.ends = _ptr(paths), _ptr(lengths), _ptr(ends)
     io.status, io.draws_used, io.counters = _ptr(status), _ptr(draws_used), _ptr(counters)
     io.walker_ids = _ptr(walker_ids)
+    io.path_flags = A.PATHS_UNPADDED if unpadded else A.PATHS_PADDED
     return io
 
 
@@ -379,13 +380,15 @@ class DeviceTimer:
 
 
 def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, walker_id_base=0, stream=None,
-         mix=None, walker_ids=None, noalloc=False):
+         mix=None, walker_ids=None, noalloc=False, unpadded=False):
     """Runs one batch in HBM buffers and returns numpy results.
 
     draws / term_draws: replay streams in the [k][walker] layout, or None (Philox).
     mix: optional (params_b, split) for a two-kind batch.
     walker_ids: optional per-walker global ids keying the Philox stream (uint64).
     noalloc: run the batch through trw_walk_noalloc instead of trw_walk_async.
+    unpadded: TRW_PATHS_UNPADDED -- entries past a walker's length are left as they were (the
+    buffer starts filled with -2 here).
     """
     ctx = graph.ctx
     src_h = np.ascontiguousarray(sources, dtype=np.int32)
@@ -413,7 +416,7 @@ def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, w
         counters = up(np.zeros(4, np.uint64))
         ids = None if walker_ids is None else up(np.ascontiguousarray(walker_ids, dtype=np.uint64))
         io = make_io(src, n, L, walker_id_base, d_draws, K, d_tdraws, KT, paths, lengths, ends, status, used,
-                     counters, walker_ids=ids)
+                     counters, walker_ids=ids, unpadded=unpadded)
         if mix is None:
             (walk_noalloc if noalloc else walk_async)(graph, params, io, stream)
         else:

@@ -271,6 +271,8 @@ int trw_context_stream(trw_context *ctx, void **stream) {
     return TRW_OK;
 }
 
+void release_graph_chunk(trw_graph *g);
+
 static void graph_release(trw_graph *g) {
     dfree(g->offset);
     dfree(g->adj);
@@ -301,6 +303,7 @@ static void graph_release(trw_graph *g) {
     dfree(g->counters);
     dfree(g->state);
     dfree(g->active);
+    release_graph_chunk(g);
 }
 
 int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *degree, const int32_t *adj,
@@ -1052,7 +1055,7 @@ static int ensure_state(trw_graph *g, int64_t n) {
     dfree(g->active);
     g->state_cap = 0;
     TRW_HIP(dmalloc(&g->state, static_cast<size_t>(n)), "hipMalloc walker state");
-    TRW_HIP(dmalloc(&g->active, 2 * static_cast<size_t>(n)), "hipMalloc live-walker lists");
+    TRW_HIP(dmalloc(&g->active, 3 * static_cast<size_t>(n)), "hipMalloc live-walker lists");
     g->state_cap = n;
     return TRW_OK;
 }
@@ -1085,53 +1088,84 @@ static int run_step_form(trw_graph *g, const KindParams &a, const KindParams &b,
     return run_step_form_on(g, a, b, split, io, st, g->state, g->counters + 1);
 }
 
-namespace {
-struct GraphExec {
+// The GRAPH form's chunk: six launches captured once and replayed while walkers remain.  Kept per
+// graph with the bytes of every argument the launches were captured with; a call with other
+// arguments (other kinds, buffers or batch) captures a new one.
+struct GraphChunk {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    ~GraphExec() {
+    std::vector<unsigned char> key;
+    ~GraphChunk() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
     }
 };
+
+void release_graph_chunk(trw_graph *g) {
+    delete g->graph_chunk;
+    g->graph_chunk = nullptr;
+}
+
+namespace {
+constexpr int kGraphBurst = 4;   // steps per walker per launch
+constexpr int kGraphChunk = 6;   // launches per captured chunk (a multiple of the three rotating lists)
+
+template <class T>
+void append_bytes(std::vector<unsigned char> &k, const T &v) {
+    const unsigned char *p = reinterpret_cast<const unsigned char *>(&v);
+    k.insert(k.end(), p, p + sizeof(T));
+}
 }  // namespace
 
-// Live-walker list in ping-pong buffers: step k reads list[k % 2] and appends the
-// survivors to list[(k + 1) % 2].  A chunk of an even number of steps (counter
-// resets are fill kernels, so the chunk holds kernel nodes only) is captured once
-// and replayed until the list is empty; after each replay the live list is list[0].
+// GRAPH form: live walkers on three rotating lists (launch k reads list k % 3, appends the walkers
+// still walking to list (k + 1) % 3 and zeroes count (k + 2) % 3, the one launch k - 1 read).  The
+// first launch starts every walker from its source and writes list 1; a chunk of six launches
+// (k = 1..6) ends with the live list again in list 1, so it is captured once, kept, and replayed
+// until the list is empty.  Each launch takes a walker up to kGraphBurst steps.
 static int run_graph_form(trw_graph *g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                           hipStream_t st) {
     if (io.n == 0) return TRW_OK;
     int rc = ensure_state(g, io.n);
     if (rc != TRW_OK) return rc;
     const DevGraph dv = g->view();
-    int32_t *list[2] = {g->active, g->active + g->state_cap};
-    unsigned int *count[2] = {g->counters + 4, g->counters + 5};
-    TRW_HIP(trw::launch_walk_step_init(dv, a, b, split, io, g->state, st), "graph-form init");
-    TRW_HIP(trw::launch_fill_u32(count[0], 0u, 1, st), "graph-form reset");
-    TRW_HIP(trw::launch_walk_list_init(g->state, io.n, list[0], count[0], st), "graph-form list");
-    const int kChunk = 8;
-    GraphExec ge;
-    TRW_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
-    hipError_t e = hipSuccess;
-    for (int k = 0; k < kChunk && e == hipSuccess; ++k) {
-        const int i = k & 1, o = i ^ 1;
-        e = trw::launch_fill_u32(count[o], 0u, 1, st);
-        if (e == hipSuccess)
-            e = trw::launch_walk_step_compact(dv, a, b, split, io, g->state, list[i], count[i], list[o], count[o], st);
+    int32_t *list[3] = {g->active, g->active + g->state_cap, g->active + 2 * g->state_cap};
+    unsigned int *count[3] = {g->counters + 4, g->counters + 5, g->counters + 6};
+    TRW_HIP(trw::launch_fill_paths(a, b, split, io, st), "graph-form path rows");
+    TRW_HIP(trw::launch_fill_u32(count[1], 0u, 1, st), "graph-form reset");
+    TRW_HIP(trw::launch_walk_graph(dv, a, b, split, io, g->state, nullptr, nullptr, list[1], count[1], count[2], 1,
+                                   kGraphBurst, st),
+            "graph-form first launch");
+    std::vector<unsigned char> key;
+    append_bytes(key, dv);
+    append_bytes(key, a);
+    append_bytes(key, b);
+    append_bytes(key, split);
+    append_bytes(key, io);
+    append_bytes(key, g->state);
+    append_bytes(key, g->active);
+    if (!g->graph_chunk || g->graph_chunk->key != key) {
+        release_graph_chunk(g);
+        std::unique_ptr<GraphChunk> c(new GraphChunk);
+        TRW_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+        hipError_t e = hipSuccess;
+        for (int k = 1; k <= kGraphChunk && e == hipSuccess; ++k)
+            e = trw::launch_walk_graph(dv, a, b, split, io, g->state, list[k % 3], count[k % 3], list[(k + 1) % 3],
+                                       count[(k + 1) % 3], count[(k + 2) % 3], 0, kGraphBurst, st);
+        const hipError_t ec = hipStreamEndCapture(st, &c->graph);
+        TRW_HIP(e, "capturing the step chunk");
+        TRW_HIP(ec, "end capture");
+        TRW_HIP(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0), "instantiate the step chunk");
+        c->key = std::move(key);
+        g->graph_chunk = c.release();
     }
-    const hipError_t ec = hipStreamEndCapture(st, &ge.graph);
-    TRW_HIP(e, "capturing the step chunk");
-    TRW_HIP(ec, "end capture");
-    TRW_HIP(hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0), "instantiate the step chunk");
-    unsigned int live = 1;
+    unsigned int live = 0;
+    TRW_HIP(hipMemcpyAsync(&live, count[1], sizeof(live), hipMemcpyDeviceToHost, st), "read live count");
+    TRW_HIP(hipStreamSynchronize(st), "graph-form synchronize");
     while (live) {
-        TRW_HIP(hipGraphLaunch(ge.exec, st), "replay the step chunk");
-        TRW_HIP(hipMemcpyAsync(&live, count[0], sizeof(live), hipMemcpyDeviceToHost, st), "read live count");
+        TRW_HIP(hipGraphLaunch(g->graph_chunk->exec, st), "replay the step chunk");
+        TRW_HIP(hipMemcpyAsync(&live, count[1], sizeof(live), hipMemcpyDeviceToHost, st), "read live count");
         TRW_HIP(hipStreamSynchronize(st), "graph-form synchronize");
     }
-    TRW_HIP(trw::launch_walk_step_finish(a, b, split, io, g->state, st), "graph-form finish");
     return TRW_OK;
 }
 

@@ -198,15 +198,15 @@ hipError_t launch_walk_step(const DevGraph &g, const KindParams &a, const KindPa
                             const BatchIO &io, WalkerState *state, unsigned int *alive, hipStream_t stream);
 hipError_t launch_walk_step_finish(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                                    const WalkerState *state, hipStream_t stream);
-// Compacted step form: list_init appends the walkers alive after step_init to list;
-// step_compact advances list_in[0, *count_in) one step and appends survivors to
-// list_out / *count_out (which the caller zeroes first).
-hipError_t launch_walk_list_init(const WalkerState *state, int32_t n, int32_t *list, unsigned int *count,
-                                 hipStream_t stream);
-hipError_t launch_walk_step_compact(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
-                                    const BatchIO &io, WalkerState *state, const int32_t *list_in,
-                                    const unsigned int *count_in, int32_t *list_out, unsigned int *count_out,
-                                    hipStream_t stream);
+// Path rows of a STEP or GRAPH batch: source at position 0, -1 after (nothing when unpadded).
+hipError_t launch_fill_paths(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                             hipStream_t stream);
+// GRAPH form: advance the live list (first: every walker of the batch, from its source) by up to
+// `burst` steps per walker, append the walkers still walking to list_out, zero *count_zero.
+hipError_t launch_walk_graph(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
+                             const BatchIO &io, WalkerState *state, const int32_t *list_in,
+                             const unsigned int *count_in, int32_t *list_out, unsigned int *count_out,
+                             unsigned int *count_zero, int32_t first, int32_t burst, hipStream_t stream);
 hipError_t launch_fill_u32(unsigned int *p, unsigned int value, int64_t n, hipStream_t stream);
 
 // One query chunk of a serving batch: walkers [batch_offset, batch_offset + count)

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "../../include/thunderrw.h"
 #include "engine.h"
 
@@ -51,12 +53,14 @@ struct trw_graph {
     int32_t alias_weights = -1, its_weights = -1, rs_weights = -1;  // TRW_WEIGHTS_* the table was built from
     unsigned int *counters = nullptr;                                 // [0] queue head [1] alive [2] zero-sum count
     trw::WalkerState *state = nullptr;
-    int32_t *active = nullptr;  // two live-walker lists of state_cap entries (GRAPH form)
+    int32_t *active = nullptr;  // three live-walker lists of state_cap entries (GRAPH form)
+    struct GraphChunk *graph_chunk = nullptr;  // the GRAPH form's instantiated step chunk (api.cpp)
     int64_t state_cap = 0;
     double last_build_ms = 0.0;
 
     trw::DevGraph view() const {
         trw::DevGraph g;
+        std::memset(&g, 0, sizeof(g));  // padding too: the GRAPH form keys its cached chunk on these bytes
         g.nv = nv;
         g.ne = ne;
         g.offset = offset;

@@ -1550,11 +1550,6 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_kernel(DevGraph g, KindP
     if (__ballot(still) && (threadIdx.x & 63) == 0) *alive = 1u;
 }
 
-// Compacted step form: each launch advances only the walkers on the active list
-// list_in[0, *count_in) by one step and appends the ones still alive to list_out
-// (one atomic per wavefront), so a step's cost follows the live walkers rather than
-// the batch.  The grid is fixed (grid-stride over the list), which lets a chunk of
-// these launches be captured once in a hipGraph and replayed.
 // Appends the lanes with keep set to list_out: one returning atomic per block (the
 // block's wavefront counts are combined in LDS first), so the shared counter is not
 // the bottleneck.  Every thread of the block must call it.
@@ -1578,41 +1573,60 @@ __device__ __forceinline__ void block_append(bool keep, int32_t w, int32_t *list
     __syncthreads();  // wave_count and block_base are reused by the next call
 }
 
-__global__ __launch_bounds__(kStepBlock) void walk_list_init_kernel(const WalkerState *state, int32_t n,
-                                                                     int32_t *list, unsigned int *count) {
-    const int32_t w = blockIdx.x * kStepBlock + threadIdx.x;
-    const bool keep = w < n && state[w].st == kAlive;
-    block_append(keep, w, list, count);
-}
-
+// GRAPH form: each launch advances every walker on the live list by up to `burst` steps, keeping
+// its state in registers between them, then stores the state of the ones still walking and appends
+// them to list_out.  The first launch (first != 0) starts every walker of the batch from its source
+// instead of reading a list.  A walker that ends writes its outputs there and then, so no pass over
+// the batch follows the walk.  Counters rotate over three words: a launch reads count_in, appends to
+// count_out and zeroes count_zero, the one the launch before it read, so a chunk of launches holds
+// kernel nodes only (no memset) and can be replayed from a hipGraph.  Walks equal the other forms'.
 template <int MODE, bool REPLAY>
-__global__ __launch_bounds__(kStepBlock) void walk_step_compact_kernel(DevGraph g, KindParams A, KindParams B,
-                                                                        int32_t split, BatchIO io, WalkerState *state,
-                                                                        const int32_t *list_in,
-                                                                        const unsigned int *count_in,
-                                                                        int32_t *list_out, unsigned int *count_out) {
-    const uint32_t count = *count_in;
+__global__ __launch_bounds__(kStepBlock) void walk_graph_kernel(DevGraph g, KindParams A, KindParams B, int32_t split,
+                                                                BatchIO io, WalkerState *state, const int32_t *list_in,
+                                                                const unsigned int *count_in, int32_t *list_out,
+                                                                unsigned int *count_out, unsigned int *count_zero,
+                                                                int32_t first, int32_t burst) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count_zero = 0u;
+    const uint32_t count = first ? static_cast<uint32_t>(io.n) : *count_in;
     const uint32_t stride = gridDim.x * kStepBlock;
+    unsigned long long steps = 0, gterm = 0, dead = 0, err = 0;
     // Block-uniform trip count: every thread of the block reaches each block_append.
     for (uint32_t base = blockIdx.x * kStepBlock; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
         bool keep = false;
         int32_t w = 0;
         if (i < count) {
-            w = list_in[i];
-            WalkerState s = state[w];
+            WalkerState s;
+            if (first) {
+                w = static_cast<int32_t>(i);
+                init_walker(g, io, w, s);
+                if (io.paths && io.unpadded) io.paths[static_cast<int64_t>(w) * io.path_stride] = s.cur;
+            } else {
+                w = list_in[i];
+                s = state[w];
+            }
             const bool kind_b = w >= split;
             const int32_t plen = kind_b ? B.length : A.length;  // a value, not a reference to either kind
-            uint4 blk = first_block<REPLAY>(io, w, s);
-            const bool moved =
-                kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
-            if (moved && io.paths && s.len - 1 < plen)
-                io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
+            for (int32_t k = 0; k < burst && s.st == kAlive; ++k) {
+                uint4 blk = first_block<REPLAY>(io, w, s);
+                const bool moved = kind_b ? advance<MODE, REPLAY, false>(g, B, io, w, s, blk)
+                                          : advance<MODE, REPLAY, false>(g, A, io, w, s, blk);
+                if (moved) {
+                    ++steps;
+                    if (io.paths && s.len - 1 < plen) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
+                }
+            }
             keep = s.st == kAlive;
-            state[w] = s;
+            if (keep) {
+                state[w] = s;
+            } else {
+                write_outputs(io, w, s);
+                classify(s.st, gterm, dead, err);
+            }
         }
         block_append(keep, w, list_out, count_out);
     }
+    flush_counters(io, steps, gterm, dead, err);
 }
 
 // Every move adds one to a walker's length, so a batch's steps are the sum of
@@ -1921,6 +1935,13 @@ hipError_t launch_walk_queue(const DevGraph &g, const KindParams &a, const KindP
     return hipGetLastError();
 }
 
+hipError_t launch_fill_paths(const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                             hipStream_t stream) {
+    if (io.n <= 0) return hipSuccess;
+    fill_paths(a, b, split, io, stream);
+    return hipGetLastError();
+}
+
 hipError_t launch_walk_step_init(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
                                  const BatchIO &io, WalkerState *state, hipStream_t stream) {
     if (io.n <= 0) return hipSuccess;
@@ -1940,44 +1961,40 @@ hipError_t launch_walk_step(const DevGraph &g, const KindParams &a, const KindPa
 }
 
 template <int MODE, bool REPLAY>
-void compact_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
-                    WalkerState *state, const int32_t *list_in, const unsigned int *count_in, int32_t *list_out,
-                    unsigned int *count_out, unsigned grid, hipStream_t stream) {
-    hipLaunchKernelGGL((walk_step_compact_kernel<MODE, REPLAY>), dim3(grid), dim3(kStepBlock), 0, stream, g, a, b, split,
-                       io, state, list_in, count_in, list_out, count_out);
+void graph_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                  WalkerState *state, const int32_t *list_in, const unsigned int *count_in, int32_t *list_out,
+                  unsigned int *count_out, unsigned int *count_zero, int32_t first, int32_t burst, unsigned grid,
+                  hipStream_t stream) {
+    hipLaunchKernelGGL((walk_graph_kernel<MODE, REPLAY>), dim3(grid), dim3(kStepBlock), 0, stream, g, a, b, split, io,
+                       state, list_in, count_in, list_out, count_out, count_zero, first, burst);
 }
 
 template <bool REPLAY>
-void compact_dispatch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
-                      WalkerState *state, const int32_t *list_in, const unsigned int *count_in, int32_t *list_out,
-                      unsigned int *count_out, unsigned grid, hipStream_t stream) {
+void graph_dispatch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
+                    WalkerState *state, const int32_t *list_in, const unsigned int *count_in, int32_t *list_out,
+                    unsigned int *count_out, unsigned int *count_zero, int32_t first, int32_t burst, unsigned grid,
+                    hipStream_t stream) {
     const bool single = (split >= io.n) || (a.mode == b.mode);
+#define TRW_ARGS g, a, b, split, io, state, list_in, count_in, list_out, count_out, count_zero, first, burst, grid, stream
     if (!single) {
-        compact_launch<-1, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream);
+        graph_launch<-1, REPLAY>(TRW_ARGS);
         return;
     }
     switch (split >= io.n ? a.mode : b.mode) {
-    case kModeUniform: compact_launch<kModeUniform, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
-    case kModeAlias: compact_launch<kModeAlias, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
-    case kModeIts: compact_launch<kModeIts, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
-    case kModeRejection: compact_launch<kModeRejection, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
-    case kModeNode2Vec: compact_launch<kModeNode2Vec, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
-    default: compact_launch<kModeMetaPath, REPLAY>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream); break;
+    case kModeUniform: graph_launch<kModeUniform, REPLAY>(TRW_ARGS); break;
+    case kModeAlias: graph_launch<kModeAlias, REPLAY>(TRW_ARGS); break;
+    case kModeIts: graph_launch<kModeIts, REPLAY>(TRW_ARGS); break;
+    case kModeRejection: graph_launch<kModeRejection, REPLAY>(TRW_ARGS); break;
+    case kModeNode2Vec: graph_launch<kModeNode2Vec, REPLAY>(TRW_ARGS); break;
+    default: graph_launch<kModeMetaPath, REPLAY>(TRW_ARGS); break;
     }
+#undef TRW_ARGS
 }
 
-hipError_t launch_walk_list_init(const WalkerState *state, int32_t n, int32_t *list, unsigned int *count,
-                                 hipStream_t stream) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(walk_list_init_kernel, dim3(grid_for(n, kStepBlock)), dim3(kStepBlock), 0, stream, state, n,
-                       list, count);
-    return hipGetLastError();
-}
-
-hipError_t launch_walk_step_compact(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
-                                    const BatchIO &io, WalkerState *state, const int32_t *list_in,
-                                    const unsigned int *count_in, int32_t *list_out, unsigned int *count_out,
-                                    hipStream_t stream) {
+hipError_t launch_walk_graph(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
+                             const BatchIO &io, WalkerState *state, const int32_t *list_in,
+                             const unsigned int *count_in, int32_t *list_out, unsigned int *count_out,
+                             unsigned int *count_zero, int32_t first, int32_t burst, hipStream_t stream) {
     if (io.n <= 0) return hipSuccess;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
@@ -1987,8 +2004,12 @@ hipError_t launch_walk_step_compact(const DevGraph &g, const KindParams &a, cons
     // Full occupancy (8 wavefronts of 256-lane blocks per CU) or the batch, whichever is smaller.
     const int64_t needed = grid_for(io.n, kStepBlock);
     const unsigned grid = static_cast<unsigned>(needed < int64_t(cus) * 8 ? needed : int64_t(cus) * 8);
-    if (io.replay) compact_dispatch<true>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream);
-    else compact_dispatch<false>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, grid, stream);
+    if (io.replay)
+        graph_dispatch<true>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, count_zero, first,
+                             burst, grid, stream);
+    else
+        graph_dispatch<false>(g, a, b, split, io, state, list_in, count_in, list_out, count_out, count_zero, first,
+                              burst, grid, stream);
     return hipGetLastError();
 }
 

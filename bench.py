@@ -66,6 +66,8 @@ def parse_args():
     ap.add_argument("--qps", type=float, default=10000.0, help="offered load of the PPR serving run")
     ap.add_argument("--queries", type=int, default=10000, help="queries in the PPR serving run")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--sweep", type=float, nargs="*", default=[10000.0, 25000.0, 50000.0, 100000.0, 200000.0, 400000.0],
+                    help="offered loads (queries/s) of the serving sweep")
     ap.add_argument("--only", choices=["its", "mix", "serving", "c1", "c3", "functor"],
                     help="run one secondary measurement alone and print its figures (for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -391,6 +393,8 @@ def serving(ctx, args):
                         rng="philox", seed=5)
     out = {"graph": "chung-lu 2^24 V, avg deg 16, gamma 2.1, dmin 2, dmax 65536, FP64 U[1,5) weights",
            "queries": args.queries, "offered_qps": args.qps, "alias_table_build_ms": round(dg.last_build_ms(), 3)}
+    keys = ("achieved_qps", "p50_ms", "p90_ms", "p99_ms", "p999_ms", "mean_ms", "max_ms", "steps_per_s", "steps",
+            "batches", "mean_batch_queries", "rejected")
     with T.Server(dg, streams=4, completion_threads=4) as srv:
         qs = []
         for i in range(args.queries):
@@ -403,10 +407,25 @@ def serving(ctx, args):
         st = srv.stats()
         for q in ids:
             srv.release(int(q))
+        # Offered-load sweep: the same queries at rising Poisson rates; where the server saturates the
+        # achieved rate falls below the offered one and latency is queueing.
+        sweep = []
+        for qps in args.sweep:
+            r, ids, _ = srv.run_open_loop(qs, qps, seed=3)
+            for q in ids:
+                srv.release(int(q))
+            sweep.append({k: r[k] for k in ("offered_qps",) + keys})
+            log("serving sweep: offered %.0f qps -> achieved %.0f qps, p50 %.3f ms, p99 %.3f ms, %.3e steps/s, "
+                "%.1f queries per batch" % (qps, r["achieved_qps"], r["p50_ms"], r["p99_ms"], r["steps_per_s"],
+                                            r["mean_batch_queries"]))
     dg.close()
-    out.update({k: rep[k] for k in ("achieved_qps", "p50_ms", "p90_ms", "p99_ms", "p999_ms", "mean_ms", "max_ms",
-                                    "steps_per_s", "steps", "batches", "mean_batch_queries", "rejected")})
+    out.update({k: rep[k] for k in keys})
     out["mean_batch_walkers"] = st["mean_batch_walkers"]
+    out["sweep"] = sweep
+    if sweep:
+        best = max(sweep, key=lambda x: x["steps_per_s"])
+        out["saturation"] = {"offered_qps": best["offered_qps"], "achieved_qps": best["achieved_qps"],
+                             "steps_per_s": best["steps_per_s"], "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"]}
     log("serving at %.0f qps: p50 %.3f ms, p99 %.3f ms, %.3e steps/s, %.1f queries per batch"
         % (args.qps, rep["p50_ms"], rep["p99_ms"], rep["steps_per_s"], rep["mean_batch_queries"]))
     return out

@@ -230,3 +230,38 @@ def test_served_ppr_l1_on_a_2p20_vertex_graph(gpu_ctx):
             assert l1 < 1.5 * bound
     finally:
         dg.close()
+
+
+@pytest.mark.parametrize("nv", [37, 4101, 2 * 131072 + 77, 5 * 131072])
+def test_ppr_counts_at_every_bucket_shape(gpu_ctx, nv):
+    """PPR batches count end vertices per (segment, chunk of 2^17 vertex ids) bucket in LDS:
+    a bucket of at most 2048 walkers in a hash, a larger one by rank counters in passes.
+    One chunk holding every walker (so large buckets), one ragged chunk, three and five
+    chunks: many small batches on one slot and on four slots give, per query, exactly
+    np.unique of a direct walk's ends."""
+    g = graphs.hub_graph(nv, min(nv - 1, 30), 3 * nv, 90 + nv)
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight)
+    try:
+        dg.build_tables("alias", "edge")
+        rng = np.random.default_rng(nv)
+        live = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        qs = []
+        for i in range(12):
+            k = int(rng.integers(1, 9))
+            qs.append((rng.choice(live, k).astype(np.int32), int(rng.integers(1, 9000)), (i + 3) << 32))
+        want = []
+        for s, n, base in qs:
+            r = T.walk(dg, params_for("ppr"), s[np.arange(n) % len(s)], walker_ids=base + np.arange(n, dtype=np.uint64))
+            v, c = np.unique(r["ends"], return_counts=True)
+            want.append((v, c, r["steps"]))
+        for streams, cap in ((1, 4096), (4, 2500)):
+            with T.Server(dg, streams=streams, max_batch_walkers=cap, max_queries_per_batch=5) as srv:
+                ids = [srv.submit(params_for("ppr"), s, n, base) for s, n, base in qs]
+                for qid, (v, c, steps) in zip(ids, want):
+                    info = srv.wait(qid)
+                    got_v, got_c = srv.ppr(qid)
+                    srv.release(qid)
+                    assert info["errors"] == 0 and info["steps"] == steps and info["ppr_entries"] == len(v)
+                    assert np.array_equal(got_v, v) and np.array_equal(got_c, c)
+    finally:
+        dg.close()

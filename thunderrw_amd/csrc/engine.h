@@ -211,26 +211,48 @@ hipError_t launch_fill_u32(unsigned int *p, unsigned int value, int64_t n, hipSt
 
 // One query chunk of a serving batch: walkers [batch_offset, batch_offset + count)
 // of the batch are walkers [first_walker, first_walker + count) of the query, whose
-// source set is pool[pool_offset, pool_offset + n_sources).
+// source set is pool[pool_offset, pool_offset + n_sources).  A PPR batch cuts each
+// segment into tiles of 2048 walkers; first_tile is the segment's first.
 struct ServeSegment {
     int32_t batch_offset;
     int32_t count;
     int32_t pool_offset;
     int32_t n_sources;
     int32_t first_walker;
-    int32_t pad;
+    int32_t first_tile;
     uint64_t id_base;
 };
+// Expands the segment table into per-walker sources and Philox ids, and zeroes
+// zero[0, n_zero) (the batch's walk counters).
 hipError_t launch_serve_expand(const ServeSegment *segs, int32_t n_segs, const int32_t *pool, int32_t n,
-                               int32_t *sources, uint64_t *ids, hipStream_t stream);
-// PPR batch aggregation on the device: sorted (segment << 32 | end vertex) runs with
-// walker counts (end vertex 0xffffffff: walkers that ended in error) and per-segment steps.
-hipError_t serve_ppr_temp_bytes(int32_t max_walkers, int32_t max_segments, size_t *bytes);
-hipError_t launch_serve_ppr_reduce(const ServeSegment *segs, const int32_t *seg_offsets, int32_t n_segs, int32_t n,
-                                   const int32_t *ends, const int32_t *lengths, const int32_t *status,
-                                   uint64_t *keys_in, uint64_t *keys_out, uint64_t *walker_steps, uint64_t *run_keys,
-                                   uint32_t *run_counts, uint32_t *num_runs, uint64_t *seg_steps, void *temp,
-                                   size_t temp_bytes, hipStream_t stream);
+                               int32_t *sources, uint64_t *ids, uint32_t *zero, int32_t n_zero, hipStream_t stream);
+
+// Totals of one segment (or tile) of a PPR batch.
+struct PprSegmentStats {
+    uint64_t steps;   // sum over the walkers of (length - 1)
+    uint64_t errors;  // walkers that did not end at a vertex
+};
+// A slot's PPR aggregation buffers (C = chunks of 2^17 vertex ids).
+struct PprScratch {
+    uint32_t *tile_hist;         // [tiles][C]: walkers per chunk, then their first bucket position
+    PprSegmentStats *tile_stats; // [tiles]
+    int32_t *bucketed;           // [walkers]: end vertices grouped by (segment, chunk)
+    int2 *pairs;                 // [walkers]: (vertex, count) runs at their bucket's positions
+    uint2 *bucket;               // [segments][C]: {first position, walkers}
+    uint32_t *bucket_runs;       // [segments][C]
+    int32_t nv;
+    int32_t C;
+};
+int32_t ppr_chunks(int32_t nv);
+int32_t ppr_tiles(int32_t walkers);  // tiles of one segment
+// PPR batch aggregation on the device, into host memory the device can write: each
+// segment's totals (host_stats[s]), each (segment, chunk) bucket's {first position,
+// runs} (host_buckets[s * C + c]), and its ascending (vertex, count) runs at
+// host_pairs[first position ...].
+hipError_t launch_serve_ppr_reduce(const ServeSegment *segs, int32_t n_segs, int32_t n_tiles, const int32_t *ends,
+                                   const int32_t *lengths, const int32_t *status, const PprScratch &p,
+                                   PprSegmentStats *host_stats, uint2 *host_buckets, int2 *host_pairs,
+                                   hipStream_t stream);
 hipError_t launch_copy_u128(uint4 *dst, const uint4 *src, int64_t n, hipStream_t stream);
 
 hipError_t run_gather_probe(uint64_t table_bytes, int32_t record_bytes, int32_t chains, int32_t steps, int32_t cus,

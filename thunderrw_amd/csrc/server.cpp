@@ -11,9 +11,11 @@
 //     expanded on the device into per-walker start vertices and Philox ids, walks
 //     on the slot's own stream, and its per-walker outputs come back into the
 //     slot's pinned staging;
-//   * a PPR batch is reduced on the device to each segment's sorted end vertices
-//     with walker counts (radix sort + run-length encoding) and its step count, so
-//     only those runs come back; a walk batch returns its per-walker outputs;
+//   * a PPR batch is reduced on the device to each segment's ascending end vertices
+//     with walker counts (per-segment bitmaps over the vertex ids, no sort), laid
+//     end to end; the device writes the segments' step, error and run totals into
+//     the slot's pinned staging, and the completion thread copies back exactly the
+//     runs by DMA; a walk batch copies back its per-walker outputs;
 //   * the dispatcher polls the slots' completion events; a finished slot is handed
 //     to a completion thread, which hands each segment's results to its query
 //     (merging the runs of a query split over several batches) and frees the slot.
@@ -49,6 +51,18 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 }
 
 constexpr int32_t kPoolInts = 1 << 22;  // source-set pool of one batch (16 MiB)
+// A slot's control block in u32 words: the walk's counters (64 bytes).
+constexpr int32_t kCtlWords = 16;
+// Bound on (segments of a PPR batch) x (chunks of 2^17 vertex ids): the per-bucket
+// arrays of a slot (8 MiB each at most).
+constexpr int64_t kPprBuckets = int64_t(1) << 20;
+
+// One PPR run: an end vertex and its walker count (the device's pair layout).
+struct Run {
+    int32_t vertex;
+    uint32_t count;
+};
+static_assert(sizeof(Run) == sizeof(int2), "runs are copied from the device's (vertex, count) pairs");
 
 struct Query {
     uint64_t id = 0;
@@ -66,8 +80,7 @@ struct Query {
     std::atomic<uint64_t> steps{0}, errors{0};
     std::vector<int32_t> ends, lengths, status, paths;  // TRW_RESULT_PATHS queries
     std::mutex merge_mu;                                // PPR runs of a query split over batches
-    std::vector<int32_t> ppr_vertices;
-    std::vector<uint32_t> ppr_counts;
+    std::vector<Run> ppr;                               // ascending end vertices with walker counts
     // Guarded by the server mutex.
     Clock::time_point arrival, dispatched, finished;
     bool was_dispatched = false;
@@ -90,27 +103,23 @@ bool same_walk(const Query &a, const Query &b) {
     return std::equal(x.schema, x.schema + std::max(0, std::min(x.schema_length, 16)), y.schema);
 }
 
-// Adds the ascending (vertex, count) runs b into the ascending runs a.
-void merge_runs(std::vector<int32_t> &av, std::vector<uint32_t> &ac, const int32_t *bv, const uint32_t *bc, size_t nb) {
-    std::vector<int32_t> v;
-    std::vector<uint32_t> c;
-    v.reserve(av.size() + nb);
-    c.reserve(av.size() + nb);
+// Adds the ascending runs b[0, nb) into the ascending runs a.
+void merge_runs(std::vector<Run> &a, const Run *b, size_t nb) {
+    std::vector<Run> m;
+    m.reserve(a.size() + nb);
     size_t i = 0, j = 0;
-    while (i < av.size() || j < nb) {
-        if (j == nb || (i < av.size() && av[i] < bv[j])) {
-            v.push_back(av[i]);
-            c.push_back(ac[i++]);
-        } else if (i == av.size() || bv[j] < av[i]) {
-            v.push_back(bv[j]);
-            c.push_back(bc[j++]);
+    while (i < a.size() || j < nb) {
+        if (j == nb || (i < a.size() && a[i].vertex < b[j].vertex)) {
+            m.push_back(a[i++]);
+        } else if (i == a.size() || b[j].vertex < a[i].vertex) {
+            m.push_back(b[j++]);
         } else {
-            v.push_back(av[i]);
-            c.push_back(ac[i++] + bc[j++]);
+            m.push_back({a[i].vertex, a[i].count + b[j].count});
+            ++i;
+            ++j;
         }
     }
-    av.swap(v);
-    ac.swap(c);
+    a.swap(m);
 }
 
 struct Segment {
@@ -130,17 +139,16 @@ struct Slot {
     int32_t *d_sources = nullptr;
     uint64_t *d_ids = nullptr;
     int32_t *d_ends = nullptr, *d_lengths = nullptr, *d_status = nullptr, *d_paths = nullptr;
-    uint64_t *d_counters = nullptr;
+    // Control block zeroed by the expand launch: the walk's counters.
+    uint32_t *d_ctl = nullptr;
     unsigned int *d_queue_head = nullptr;
     int32_t *h_ends = nullptr, *h_lengths = nullptr, *h_status = nullptr, *h_paths = nullptr;
-    // PPR aggregation (d_ids doubles as the unsorted keys once the walk has read them).
-    int32_t *d_seg_off = nullptr, *h_seg_off = nullptr;
-    uint64_t *d_keys_out = nullptr, *d_wsteps = nullptr, *d_run_keys = nullptr, *d_seg_steps = nullptr;
-    uint32_t *d_run_counts = nullptr, *d_num_runs = nullptr;
-    void *d_temp = nullptr;
-    size_t temp_bytes = 0;
-    uint64_t *h_run_keys = nullptr, *h_seg_steps = nullptr;
-    uint32_t *h_run_counts = nullptr, *h_num_runs = nullptr;
+    // PPR aggregation: device scratch, and the pinned memory the device writes each
+    // segment's totals, each bucket's {first position, runs} and the runs into.
+    trw::PprScratch agg{};
+    trw::PprSegmentStats *h_stats = nullptr;
+    uint2 *h_buckets = nullptr;
+    int2 *h_pairs = nullptr;
     std::vector<Segment> segs;
     int32_t n = 0;
     int32_t path_stride = 0;
@@ -203,6 +211,8 @@ struct trw_server {
     int64_t path_ints = 0;
     int32_t max_pending = 0;
     int32_t max_segments = 0;
+    int32_t ppr_segments = 0;  // segments of a PPR batch
+    int32_t ppr_chunks = 0;    // chunks of 2^17 vertex ids the PPR aggregation buckets by
     std::vector<Slot> slots;
 
     std::mutex mu;
@@ -233,6 +243,7 @@ bool trw_server::form_batch(Slot &sl, std::unique_lock<std::mutex> &) {
     // Called with mu held and a non-empty pending queue.
     const std::shared_ptr<Query> head = pending.front();
     const int32_t L = head->result == TRW_RESULT_PATHS ? head->path_len : 0;
+    const int32_t seg_cap = head->result == TRW_RESULT_PPR ? ppr_segments : max_segments;
     int64_t room_paths = L > 0 ? path_ints / L : INT64_MAX;
     int32_t n = 0, pool = 0;
     sl.segs.clear();
@@ -243,7 +254,7 @@ bool trw_server::form_batch(Slot &sl, std::unique_lock<std::mutex> &) {
             ++it;
             continue;
         }
-        if (static_cast<int32_t>(sl.segs.size()) == max_segments) break;
+        if (static_cast<int32_t>(sl.segs.size()) == seg_cap) break;
         if (pool + static_cast<int32_t>(q->sources.size()) > kPoolInts) break;
         const int64_t room = std::min<int64_t>(max_walkers - n, room_paths - n);
         if (room <= 0) break;
@@ -278,7 +289,7 @@ bool trw_server::form_batch(Slot &sl, std::unique_lock<std::mutex> &) {
 void trw_server::launch(Slot &sl) {
     // Runs without mu: the slot is InFlight and owned by the dispatcher until its
     // completion event fires.
-    int32_t pool = 0;
+    int32_t pool = 0, tiles = 0;
     for (size_t i = 0; i < sl.segs.size(); ++i) {
         const Segment &s = sl.segs[i];
         trw::ServeSegment d;
@@ -287,15 +298,14 @@ void trw_server::launch(Slot &sl) {
         d.pool_offset = pool;
         d.n_sources = static_cast<int32_t>(s.q->sources.size());
         d.first_walker = s.first;
-        d.pad = 0;
+        d.first_tile = tiles;
+        if (sl.ppr) tiles += trw::ppr_tiles(s.count);
         d.id_base = s.q->id_base;
         sl.h_segs[i] = d;
-        sl.h_seg_off[i] = s.batch_offset;
         std::memcpy(sl.h_pool + pool, s.q->sources.data(), sizeof(int32_t) * s.q->sources.size());
         pool += d.n_sources;
     }
     const int32_t nseg = static_cast<int32_t>(sl.segs.size());
-    sl.h_seg_off[nseg] = sl.n;
     const trw_walk_params params = sl.segs.front().q->params;
     const size_t N = static_cast<size_t>(sl.n);
     hipStream_t st = sl.stream;
@@ -304,13 +314,9 @@ void trw_server::launch(Slot &sl) {
                                   hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(sl.d_pool, sl.h_pool, sizeof(int32_t) * static_cast<size_t>(pool), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && sl.ppr)
-        e = hipMemcpyAsync(sl.d_seg_off, sl.h_seg_off, sizeof(int32_t) * static_cast<size_t>(nseg + 1),
-                           hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = trw::launch_serve_expand(sl.d_segs, static_cast<int32_t>(sl.segs.size()), sl.d_pool, sl.n, sl.d_sources,
-                                     sl.d_ids, st);
-    if (e == hipSuccess) e = trw::launch_fill_u32(reinterpret_cast<unsigned int *>(sl.d_counters), 0u, 8, st);
+        e = trw::launch_serve_expand(sl.d_segs, nseg, sl.d_pool, sl.n, sl.d_sources, sl.d_ids, sl.d_ctl, kCtlWords,
+                                     st);
     if (e == hipSuccess) {
         trw_walk_io io;
         std::memset(&io, 0, sizeof(io));
@@ -322,7 +328,7 @@ void trw_server::launch(Slot &sl) {
         io.lengths = sl.d_lengths;
         io.ends = sl.d_ends;
         io.status = sl.d_status;
-        io.counters = sl.d_counters;
+        io.counters = reinterpret_cast<uint64_t *>(sl.d_ctl);
         const int rc = trw::walk_launch(g, &params, &io, st, sl.d_queue_head, false);
         if (rc != TRW_OK) {
             e = hipErrorLaunchFailure;
@@ -330,20 +336,10 @@ void trw_server::launch(Slot &sl) {
         }
     }
     if (sl.ppr) {
+        // The device writes the totals, bucket extents and runs straight into the pinned staging.
         if (e == hipSuccess)
-            e = trw::launch_serve_ppr_reduce(sl.d_segs, sl.d_seg_off, nseg, sl.n, sl.d_ends, sl.d_lengths, sl.d_status,
-                                             sl.d_ids, sl.d_keys_out, sl.d_wsteps, sl.d_run_keys, sl.d_run_counts,
-                                             sl.d_num_runs, sl.d_seg_steps, sl.d_temp, sl.temp_bytes, st);
-        // Runs are at most one per walker; the copy is bounded by the batch, the drain reads num_runs.
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(sl.h_num_runs, sl.d_num_runs, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(sl.h_run_keys, sl.d_run_keys, sizeof(uint64_t) * N, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(sl.h_run_counts, sl.d_run_counts, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(sl.h_seg_steps, sl.d_seg_steps, sizeof(uint64_t) * static_cast<size_t>(nseg),
-                               hipMemcpyDeviceToHost, st);
+            e = trw::launch_serve_ppr_reduce(sl.d_segs, nseg, tiles, sl.d_ends, sl.d_lengths, sl.d_status, sl.agg,
+                                             sl.h_stats, sl.h_buckets, sl.h_pairs, st);
     } else {
         if (e == hipSuccess) e = hipMemcpyAsync(sl.h_ends, sl.d_ends, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess)
@@ -422,36 +418,47 @@ void trw_server::dispatch_loop() {
 // ---------------------------------------------------------------- completion
 
 void trw_server::drain(Slot *sl, hipError_t err, std::string msg) {
-    const bool ok = err == hipSuccess && msg.empty();
-    const uint32_t runs = ok && sl->ppr ? *sl->h_num_runs : 0u;
-    uint32_t r = 0;  // runs are sorted by segment, then vertex
+    bool ok = err == hipSuccess && msg.empty();
+    const size_t C = static_cast<size_t>(ppr_chunks);
+    if (ok && sl->ppr) {
+        // A segment's buckets lie inside its walker positions, one after another.
+        for (size_t j = 0; j < sl->segs.size() && ok; ++j) {
+            const uint64_t lo = static_cast<uint64_t>(sl->segs[j].batch_offset);
+            const uint64_t hi = lo + static_cast<uint64_t>(sl->segs[j].count);
+            uint64_t at = lo;
+            for (size_t c = 0; c < C; ++c) {
+                const uint2 b = sl->h_buckets[j * C + c];
+                if (b.x < at || static_cast<uint64_t>(b.x) + b.y > hi) {
+                    ok = false;
+                    msg = "serving batch: inconsistent PPR buckets";
+                    break;
+                }
+                at = static_cast<uint64_t>(b.x) + b.y;
+            }
+        }
+    }
     for (size_t j = 0; j < sl->segs.size(); ++j) {
         const Segment &s = sl->segs[j];
         Query &q = *s.q;
         if (ok && sl->ppr) {
-            std::vector<int32_t> v;
-            std::vector<uint32_t> c;
-            uint64_t errors = 0;
-            for (; r < runs && (sl->h_run_keys[r] >> 32) == j; ++r) {
-                const uint32_t vert = static_cast<uint32_t>(sl->h_run_keys[r]);
-                if (vert == 0xffffffffu) {
-                    errors += sl->h_run_counts[r];
-                } else {
-                    v.push_back(static_cast<int32_t>(vert));
-                    c.push_back(sl->h_run_counts[r]);
-                }
+            // Segment j's runs, ascending: its buckets' runs in chunk order.
+            const trw::PprSegmentStats &st = sl->h_stats[j];
+            const uint2 *bk = sl->h_buckets + j * C;
+            size_t nr = 0;
+            for (size_t c = 0; c < C; ++c) nr += bk[c].y;
+            std::vector<Run> runs(nr);
+            size_t k = 0;
+            for (size_t c = 0; c < C; ++c) {
+                std::memcpy(runs.data() + k, sl->h_pairs + bk[c].x, sizeof(Run) * bk[c].y);
+                k += bk[c].y;
             }
             {
                 std::lock_guard<std::mutex> lk(q.merge_mu);
-                if (q.ppr_vertices.empty()) {
-                    q.ppr_vertices.swap(v);
-                    q.ppr_counts.swap(c);
-                } else {
-                    merge_runs(q.ppr_vertices, q.ppr_counts, v.data(), c.data(), v.size());
-                }
+                if (q.ppr.empty()) q.ppr.swap(runs);
+                else merge_runs(q.ppr, runs.data(), nr);
             }
-            q.steps += sl->h_seg_steps[j];
-            q.errors += errors;
+            q.steps += st.steps;
+            q.errors += st.errors;
         } else if (ok) {
             const size_t o = static_cast<size_t>(s.batch_offset), f = static_cast<size_t>(s.first),
                          c = static_cast<size_t>(s.count);
@@ -580,7 +587,7 @@ void trw_server::free_slots() {
         (void)hipFree(sl.d_lengths);
         (void)hipFree(sl.d_status);
         (void)hipFree(sl.d_paths);
-        (void)hipFree(sl.d_counters);
+        (void)hipFree(sl.d_ctl);
         (void)hipFree(sl.d_queue_head);
         (void)hipHostFree(sl.h_segs);
         (void)hipHostFree(sl.h_pool);
@@ -588,19 +595,15 @@ void trw_server::free_slots() {
         (void)hipHostFree(sl.h_lengths);
         (void)hipHostFree(sl.h_status);
         (void)hipHostFree(sl.h_paths);
-        (void)hipFree(sl.d_seg_off);
-        (void)hipFree(sl.d_keys_out);
-        (void)hipFree(sl.d_wsteps);
-        (void)hipFree(sl.d_run_keys);
-        (void)hipFree(sl.d_seg_steps);
-        (void)hipFree(sl.d_run_counts);
-        (void)hipFree(sl.d_num_runs);
-        (void)hipFree(sl.d_temp);
-        (void)hipHostFree(sl.h_seg_off);
-        (void)hipHostFree(sl.h_run_keys);
-        (void)hipHostFree(sl.h_seg_steps);
-        (void)hipHostFree(sl.h_run_counts);
-        (void)hipHostFree(sl.h_num_runs);
+        (void)hipFree(sl.agg.tile_hist);
+        (void)hipFree(sl.agg.tile_stats);
+        (void)hipFree(sl.agg.bucketed);
+        (void)hipFree(sl.agg.pairs);
+        (void)hipFree(sl.agg.bucket);
+        (void)hipFree(sl.agg.bucket_runs);
+        (void)hipHostFree(sl.h_stats);
+        (void)hipHostFree(sl.h_buckets);
+        (void)hipHostFree(sl.h_pairs);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         sl = Slot();
@@ -631,7 +634,7 @@ void fill_info(const Query &q, trw_query_info *info) {
     const bool finished = q.state == TRW_QUERY_DONE || q.state == TRW_QUERY_FAILED;
     info->steps = finished ? q.steps.load() : 0;
     info->errors = finished ? q.errors.load() : 0;
-    info->ppr_entries = finished ? static_cast<int64_t>(q.ppr_vertices.size()) : 0;
+    info->ppr_entries = finished ? static_cast<int64_t>(q.ppr.size()) : 0;
     info->queue_ms = q.was_dispatched || finished ? ms_between(q.arrival, q.dispatched) : -1.0;
     info->latency_ms = finished ? ms_between(q.arrival, q.finished) : -1.0;
 }
@@ -672,6 +675,12 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
     s->slots.resize(static_cast<size_t>(c.streams));
     const size_t W = static_cast<size_t>(c.max_batch_walkers), P = static_cast<size_t>(s->path_ints),
                  S = static_cast<size_t>(c.max_queries_per_batch);
+    // PPR batches: buckets of (segment, chunk of 2^17 vertex ids), kPprBuckets at most.
+    s->ppr_chunks = trw::ppr_chunks(g->nv);
+    s->ppr_segments = static_cast<int32_t>(std::max<int64_t>(
+        1, std::min<int64_t>(static_cast<int64_t>(S), std::min<int64_t>(kPprBuckets / s->ppr_chunks, 65535))));
+    const size_t PC = static_cast<size_t>(s->ppr_chunks), PS = static_cast<size_t>(s->ppr_segments),
+                 PT = static_cast<size_t>(trw::ppr_tiles(c.max_batch_walkers)) + PS;
     for (Slot &sl : s->slots) {
         // Default stream priority: every stream of the library runs at the same one.
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
@@ -684,7 +693,7 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
         if (e == hipSuccess) e = dev(&sl.d_lengths, W);
         if (e == hipSuccess) e = dev(&sl.d_status, W);
         if (e == hipSuccess) e = dev(&sl.d_paths, P);
-        if (e == hipSuccess) e = dev(&sl.d_counters, 4);
+        if (e == hipSuccess) e = dev(&sl.d_ctl, static_cast<size_t>(kCtlWords));
         if (e == hipSuccess) e = dev(&sl.d_queue_head, 4);
         if (e == hipSuccess) e = pinned(&sl.h_segs, S);
         if (e == hipSuccess) e = pinned(&sl.h_pool, kPoolInts);
@@ -692,20 +701,18 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
         if (e == hipSuccess) e = pinned(&sl.h_lengths, W);
         if (e == hipSuccess) e = pinned(&sl.h_status, W);
         if (e == hipSuccess) e = pinned(&sl.h_paths, P);
-        if (e == hipSuccess) e = dev(&sl.d_seg_off, S + 1);
-        if (e == hipSuccess) e = dev(&sl.d_keys_out, W);
-        if (e == hipSuccess) e = dev(&sl.d_wsteps, W);
-        if (e == hipSuccess) e = dev(&sl.d_run_keys, W);
-        if (e == hipSuccess) e = dev(&sl.d_seg_steps, S);
-        if (e == hipSuccess) e = dev(&sl.d_run_counts, W);
-        if (e == hipSuccess) e = dev(&sl.d_num_runs, 1);
-        if (e == hipSuccess) e = trw::serve_ppr_temp_bytes(c.max_batch_walkers, c.max_queries_per_batch, &sl.temp_bytes);
-        if (e == hipSuccess) e = hipMalloc(&sl.d_temp, std::max<size_t>(sl.temp_bytes, 256));
-        if (e == hipSuccess) e = pinned(&sl.h_seg_off, S + 1);
-        if (e == hipSuccess) e = pinned(&sl.h_run_keys, W);
-        if (e == hipSuccess) e = pinned(&sl.h_seg_steps, S);
-        if (e == hipSuccess) e = pinned(&sl.h_run_counts, W);
-        if (e == hipSuccess) e = pinned(&sl.h_num_runs, 1);
+        trw::PprScratch &x = sl.agg;
+        x.nv = g->nv;
+        x.C = s->ppr_chunks;
+        if (e == hipSuccess) e = dev(&x.tile_hist, PT * PC);
+        if (e == hipSuccess) e = dev(&x.tile_stats, PT);
+        if (e == hipSuccess) e = dev(&x.bucketed, W);
+        if (e == hipSuccess) e = dev(&x.pairs, W);
+        if (e == hipSuccess) e = dev(&x.bucket, PS * PC);
+        if (e == hipSuccess) e = dev(&x.bucket_runs, PS * PC);
+        if (e == hipSuccess) e = pinned(&sl.h_stats, PS);
+        if (e == hipSuccess) e = pinned(&sl.h_buckets, PS * PC);
+        if (e == hipSuccess) e = pinned(&sl.h_pairs, W);
     }
     if (e != hipSuccess) {
         s->free_slots();
@@ -771,11 +778,13 @@ int trw_server_ppr_result(trw_server *s, uint64_t query_id, int32_t *vertices, u
     if (!q) return trw::set_error(TRW_E_INVALID, "unknown query %llu", static_cast<unsigned long long>(query_id));
     if (q->state != TRW_QUERY_DONE) return trw::set_error(TRW_E_STATE, "query is not done");
     if (q->result != TRW_RESULT_PPR) return trw::set_error(TRW_E_STATE, "query was not a PPR query");
-    const int64_t n = static_cast<int64_t>(q->ppr_vertices.size());
+    const int64_t n = static_cast<int64_t>(q->ppr.size());
     if (capacity < n) return trw::set_error(TRW_E_INVALID, "capacity %lld < %lld entries", (long long)capacity, (long long)n);
     if (n && (!vertices || !counts)) return trw::set_error(TRW_E_INVALID, "null output");
-    std::copy(q->ppr_vertices.begin(), q->ppr_vertices.end(), vertices);
-    std::copy(q->ppr_counts.begin(), q->ppr_counts.end(), counts);
+    for (int64_t i = 0; i < n; ++i) {
+        vertices[i] = q->ppr[static_cast<size_t>(i)].vertex;
+        counts[i] = q->ppr[static_cast<size_t>(i)].count;
+    }
     return TRW_OK;
 }
 

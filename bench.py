@@ -64,7 +64,7 @@ def parse_args():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline measurement only")
     ap.add_argument("--qps", type=float, default=10000.0, help="offered load of the PPR serving run")
-    ap.add_argument("--queries", type=int, default=10000, help="queries in the PPR serving run")
+    ap.add_argument("--queries", type=int, default=20000, help="queries in the PPR serving run")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--sweep", type=float, nargs="*", default=[10000.0, 25000.0, 50000.0, 100000.0, 200000.0, 400000.0],
                     help="offered loads (queries/s) of the serving sweep")
@@ -378,7 +378,8 @@ def serving(ctx, args):
     """PPR query serving on a 2^24-vertex Chung-Lu graph (exponent 2.1, average degree 16,
     dmin 2, dmax 65536, FP64 U[1,5) weights, weighted alias tables resident).  Poisson
     arrivals at a fixed rate; each query has |S| ~ logU[1, 4096] sources drawn uniformly
-    from the non-isolated vertices and N ~ logU[1e3, 1e5] walkers; termination 0.2."""
+    from the non-isolated vertices and N ~ logU[1e3, 1e5] walkers; termination 0.2.  The
+    server runs 8 slots (batches in flight) of up to 2^21 walkers with 8 completion threads."""
     t0 = time.time()
     hg = T.HostGraph.chung_lu(1 << 24, 8 << 24, 2.1, 2.0, 65536.0, 5, True, 0)
     deg, adj, w, _ = hg.arrays()
@@ -395,7 +396,8 @@ def serving(ctx, args):
            "queries": args.queries, "offered_qps": args.qps, "alias_table_build_ms": round(dg.last_build_ms(), 3)}
     keys = ("achieved_qps", "p50_ms", "p90_ms", "p99_ms", "p999_ms", "mean_ms", "max_ms", "steps_per_s", "steps",
             "batches", "mean_batch_queries", "rejected")
-    with T.Server(dg, streams=4, completion_threads=4) as srv:
+    out["server"] = {"streams": 8, "completion_threads": 8, "max_batch_walkers": 1 << 21}
+    with T.Server(dg, **out["server"]) as srv:
         qs = []
         for i in range(args.queries):
             k = int(np.exp(rng.uniform(0.0, np.log(4096.0))))

@@ -20,7 +20,11 @@ binary (oracle/_ref/deepwalk.out, built from the reference sources) on all
 host cores on the same graph.
 """
 import argparse
+import collections
+import csv
 import ctypes
+import glob
+import hashlib
 import json
 import os
 import re
@@ -48,6 +52,11 @@ B_STEP = {
 }
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "deepwalk.out")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_walk_fused_alias_deepwalk_2p22.json")
+# The sources the headline kernel's HBM traffic depends on (the kernel, its records, the
+# table builds that lay them out); a traffic record taken on other sources is not used.
+TRAFFIC_SOURCES = ("thunderrw_amd/csrc/walk.hip", "thunderrw_amd/csrc/engine.h", "thunderrw_amd/csrc/tables.hip")
+HEADLINE_KERNEL = "walk_fused_multi_kernel"
+COPY_KERNEL = "copy_u128_kernel"
 
 
 def log(*a):
@@ -72,6 +81,10 @@ def parse_args():
                     help="run one secondary measurement alone and print its figures (for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also launch the copy kernel over this many MiB (byte-counter calibration)")
+    ap.add_argument("--record-traffic", metavar="DIR",
+                    help="write the headline traffic record from DIR/fetch and DIR/write (rocprofv3 --pmc "
+                         "FETCH_SIZE / WRITE_SIZE -f csv outputs of a --quick --calibrate-copy run) and "
+                         "DIR/bench.json (that run's output line); no GPU is used")
     return ap.parse_args()
 
 
@@ -433,17 +446,85 @@ def serving(ctx, args):
     return out
 
 
+def kernel_source_hash():
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def load_traffic():
+    """HBM bytes per headline launch from the counter record, or None when the record is
+    missing or was taken on other kernel sources."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
-        return float(t["fetch_bytes_per_launch"]) + float(t["write_bytes_per_launch"])
+        total = float(t["fetch_bytes_per_launch"]) + float(t["write_bytes_per_launch"])
     except (OSError, KeyError, ValueError):
         return None
+    if t.get("source_sha256") != kernel_source_hash():
+        log("traffic record %s was taken on other kernel sources; roofline.traffic is null" % TRAFFIC_FILE)
+        return None
+    return total
+
+
+def read_pmc(pass_dir, counter):
+    """{kernel name: [value per dispatch, in dispatch order]} of one rocprofv3 --pmc csv pass."""
+    files = glob.glob(os.path.join(pass_dir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection.csv under %s" % pass_dir)
+    per = collections.defaultdict(float)
+    names = {}
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                key = (fn, int(row["Dispatch_Id"]))
+                per[key] += float(row["Counter_Value"])
+                names[key] = row["Kernel_Name"]
+    out = collections.defaultdict(list)
+    for key in sorted(per):
+        out[names[key]].append(per[key])
+    return out
+
+
+def record_traffic(d):
+    """Headline HBM traffic per launch from two counter passes, each scaled by what the same
+    counter reads for the copy kernel over a known size (bytes moved = counter x scale)."""
+    with open(os.path.join(d, "bench.json")) as f:
+        line = json.loads(f.read().strip().splitlines()[-1])
+    steps = int(line["details"]["steps_per_launch"])
+    fetch, write = read_pmc(os.path.join(d, "fetch"), "FETCH_SIZE"), read_pmc(os.path.join(d, "write"), "WRITE_SIZE")
+
+    def one(table, name):
+        hits = [v for k, v in table.items() if name in k]
+        if len(hits) != 1 or not hits[0]:
+            raise SystemExit("expected one kernel named %s in the pass, got %d" % (name, len(hits)))
+        return hits[0][-1]  # the last dispatch: the timed launch
+
+    nbytes = int(line["details"]["calibrate_copy_bytes"])
+    f_walk, w_walk = one(fetch, HEADLINE_KERNEL), one(write, HEADLINE_KERNEL)
+    f_copy, w_copy = one(fetch, COPY_KERNEL), one(write, COPY_KERNEL)
+    fs, ws = nbytes / (f_copy * 1024.0), nbytes / (w_copy * 1024.0)
+    rec = {"kernel": line["config"]["kernel"], "workload": line["config"]["workload"],
+           "source_sha256": kernel_source_hash(), "sources": list(TRAFFIC_SOURCES),
+           "steps_per_launch": steps, "fetch_size_kib_raw": f_walk, "write_size_kib_raw": w_walk,
+           "copy_kernel_bytes": nbytes, "copy_fetch_kib_raw": f_copy, "copy_write_kib_raw": w_copy,
+           "fetch_scale": fs, "write_scale": ws,
+           "fetch_bytes_per_launch": f_walk * 1024.0 * fs, "write_bytes_per_launch": w_walk * 1024.0 * ws}
+    rec["bytes_per_step"] = (rec["fetch_bytes_per_launch"] + rec["write_bytes_per_launch"]) / steps
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(json.dumps(rec))
 
 
 def main():
     args = parse_args()
+    if args.record_traffic:
+        record_traffic(args.record_traffic)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path (walkers would shard by "
               "contiguous ranges with the graph replicated, see DESIGN.md)", file=sys.stderr)
@@ -565,6 +646,7 @@ def main():
         a.zero()
         T.lib().trw_device_copy(ctx.h, b.ptr, a.ptr, nbytes, None)
         T.synchronize(ctx)
+        details["calibrate_copy_bytes"] = nbytes
         a.free()
         b.free()
 

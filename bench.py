@@ -297,6 +297,11 @@ def c3_node2vec(ctx, args):
            "directed_edges": int(len(adj)), "walkers": len(src), "steps_per_launch": per_launch}
     for label, weights in (("", "app"), ("weighted_", "edge")):
         prm = T.make_params("node2vec", length=80, p=2.0, q=0.5, rng="philox", seed=3, weights=weights)
+        # One walk builds the membership tables (and the weighted records), then the Move's launch
+        # shapes are tuned on a 2^21-walker sample before the timed launches.
+        T.walk_async(dg, prm, b.io())
+        T.synchronize(ctx)
+        out[label + "autotune"] = tune_record(dg.autotune(prm, sample_walkers=1 << 21, seed=3))
         b.reset()
         ms = time_launches(ctx, dg, prm, b.io(), 1, reps)
         if b.steps() != per_launch * (reps + 1):
@@ -324,6 +329,13 @@ def c3_node2vec(ctx, args):
     return out
 
 
+def tune_record(tune):
+    """What the autotuner kept for a walk: its Move, the chosen knobs, and the sample rates."""
+    return {"move": tune["move"], "chosen": tune["chosen"],
+            "default_steps_per_s": round(tune["default_steps_per_s"], 1),
+            "tuned_steps_per_s": round(tune["tuned_steps_per_s"], 1)}
+
+
 def its_walk(ctx, dg, batch, length, per_launch, args):
     """DeepWalk with static ITS and with static rejection sampling on the same graph and walkers
     (paths written).  ITS: prefix sums, search trees and guide records built on the device (a
@@ -336,7 +348,7 @@ def its_walk(ctx, dg, batch, length, per_launch, args):
         prm = T.make_params("deepwalk", execution="static", sampling=sm, length=length, rng="philox", seed=7,
                             form="fused")
         tune = dg.autotune(prm, sample_walkers=1 << 21, seed=7)
-        out[key + "_autotune"] = tune["chosen"]
+        out[key + "_autotune"] = tune_record(tune)
         batch.reset()
         ms = time_launches(ctx, dg, prm, batch.io(), 1, args.steps)
         out[key + "_steps_per_s"] = per_launch / (ms / 1e3 / args.steps)
@@ -356,9 +368,13 @@ def mix_forms(ctx, dg, deg, args):
     src = T.sources_random(deg, n, 4)
     batch = Batch(ctx, src, 80)
     out = {"walkers": n, "split": split}
-    tune = dg.autotune(T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form="queue"),
-                       sample_walkers=1 << 21, seed=4, with_paths=False)
-    out["queue_autotune"] = tune["chosen"]
+    # MetaPath's own launch shapes first, then the mix's queue chunk and blocks per CU on a mixed
+    # sample the size of the batch (the chunk is per graph, the shapes per Move; the queue kernel of a
+    # mix takes the first kind's shapes).
+    pa = T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form="queue")
+    pb = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=4, form="queue")
+    out["metapath_queue_autotune"] = tune_record(dg.autotune(pb, sample_walkers=1 << 20, seed=4, with_paths=True))
+    out["queue_autotune"] = tune_record(dg.autotune_mix(pa, pb, split, sample_walkers=n, seed=4, with_paths=True))
     # "queue_unpadded": the same walks with TRW_PATHS_UNPADDED, each row holding its walker's
     # length entries as the reference's LOG_SEQUENCE output does, with no -1 padding written.
     for form in ("queue", "queue_unpadded", "graph", "step"):
@@ -409,6 +425,10 @@ def serving(ctx, args):
            "queries": args.queries, "offered_qps": args.qps, "alias_table_build_ms": round(dg.last_build_ms(), 3)}
     keys = ("achieved_qps", "p50_ms", "p90_ms", "p99_ms", "p999_ms", "mean_ms", "max_ms", "steps_per_s", "steps",
             "batches", "mean_batch_queries", "rejected")
+    # The served walk's launch shapes (queue chunk, blocks per CU) tuned on a 2^20-walker sample.
+    out["autotune"] = tune_record(dg.autotune(T.make_params("ppr", execution="static", sampling="alias", weights="edge",
+                                                            stop_probability=0.2, rng="philox", seed=5, form="queue"),
+                                              sample_walkers=1 << 20, seed=5, with_paths=False))
     out["server"] = {"streams": 8, "completion_threads": 8, "max_batch_walkers": 1 << 21}
     with T.Server(dg, **out["server"]) as srv:
         qs = []
@@ -569,6 +589,10 @@ def main():
         return
     prm = T.make_params("deepwalk", execution="static", sampling="alias", length=length, rng="philox", seed=2024,
                         form="fused")
+    # Launch shapes for this graph and Move (walkers per lane, threads per block, path tile), timed on a
+    # 2^21-walker sample; the walks are the same under every choice.
+    head_tune = tune_record(dg.autotune(prm, sample_walkers=1 << 21, seed=2024))
+    log("autotune (alias): %s" % head_tune)
     ms = time_launches(ctx, dg, prm, batch.io(), args.warmup, args.steps)
     total = batch.steps()
     per_launch = (length - 1) * n
@@ -580,7 +604,7 @@ def main():
     log("fused alias (philox): %.3f ms/launch, %.3e steps/s, %.1f GB/s algorithmic" % (avg_s * 1e3, value, achieved))
 
     details = {"alias_table_build_ms": round(alias_build_ms, 3), "walkers": n, "steps_per_launch": per_launch,
-               "b_step_bytes": B_STEP["alias"]}
+               "b_step_bytes": B_STEP["alias"], "autotune": {"alias": head_tune}}
     if not args.quick:
         # Replay mode: the same walk consuming the host SFMT stream (bit-exact with the oracle).
         K = 2 * (length - 1)
@@ -653,6 +677,23 @@ def main():
     cpu = None if args.no_cpu_baseline else reference_cpu_baseline(hg, length)
     if cpu:
         log("reference CPU baseline: %.3e steps/s on %d cores" % (cpu["value"], cpu["cores"]))
+
+    # The autotuner's choices for every Move the run walked, in one place.
+    at = details["autotune"]
+    for key in ("its_autotune", "rejection_autotune"):
+        if key in details:
+            at[details[key]["move"]] = details[key]
+    for key in ("metapath_queue_autotune", "queue_autotune"):
+        rec = (details.get("mix_ppr_metapath") or {}).get(key)
+        if rec:
+            at[rec["move"] + "_queue"] = rec
+    rec = (details.get("ppr_serving") or {}).get("autotune")
+    if rec:
+        at[rec["move"] + "_queue_serving"] = rec
+    for key, name in (("autotune", "node2vec"), ("weighted_autotune", "node2vec_weighted")):
+        rec = (details.get("c3_node2vec") or {}).get(key)
+        if rec:
+            at[name] = rec
 
     traffic = load_traffic()
     line = {

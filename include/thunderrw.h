@@ -269,24 +269,53 @@ int trw_build_membership(trw_graph *g);
 #define TRW_TUNE_ITS_GUIDE_OFF 3       /* ITS: 0 = steps read the guide records where the table has them
                                           (one gather), 1 = they search the prefix-sum structures (the
                                           picks are identical; not autotuned) */
+/* Launch shapes, one value per Move (TRW_MOVE_*): */
+#define TRW_TUNE_FUSED_BLOCK 4         /* FUSED: threads per block of the Move's kernel: 64, 128 or 256 */
+#define TRW_TUNE_FUSED_TILE 5          /* FUSED: path positions a lane stages in LDS before writing them
+                                          out: 16, 32 or 64 (0: the whole row when the LDS budget holds it) */
+#define TRW_TUNE_QUEUE_BLOCKS_PER_CU 6 /* QUEUE: blocks of the persistent grid per CU, 1..16 (0: as many as
+                                          are resident at once) */
+#define TRW_TUNE_N2V_BLOCK 7           /* FUSED Node2Vec over membership tables: threads per block, 128 or 256 */
+#define TRW_TUNE_KNOBS 8
+/* The Moves the shape knobs are kept for (the walk kernels' Move of a walk). */
+#define TRW_MOVE_UNIFORM 0
+#define TRW_MOVE_ALIAS 1
+#define TRW_MOVE_ITS 2
+#define TRW_MOVE_REJECTION 3
+#define TRW_MOVE_NODE2VEC 4
+#define TRW_MOVE_METAPATH 5
+#define TRW_MOVES 6
+/* knobs 0-3 are per graph; a shape knob (4-7) set here applies to every Move, and reads
+ * back the uniform Move's value. */
 int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value);
 int trw_graph_get_tuning(const trw_graph *g, int32_t knob, int64_t *value);
+/* A shape knob (4-7) of one Move. */
+int trw_graph_set_move_tuning(trw_graph *g, int32_t move, int32_t knob, int64_t value);
+int trw_graph_get_move_tuning(const trw_graph *g, int32_t move, int32_t knob, int64_t *value);
 
 /* Autotuner: for the walk `params` describes (rng must be PHILOX), times a sample of
  * sample_walkers walkers from random non-isolated sources (seeded) for each candidate
- * of every knob that applies -- walkers per lane for FUSED uniform/alias, the ITS tree
- * degree threshold, the QUEUE chunk -- one knob at a time, and keeps the fastest on the
- * graph.  Walks are identical under every setting, so tuning never changes results. */
+ * of every knob that applies -- walkers per lane for FUSED uniform/alias/ITS records,
+ * the ITS tree degree threshold, the QUEUE chunk, and the launch shapes of the walk's
+ * Move: FUSED threads per block and path tile, QUEUE blocks per CU, Node2Vec threads per
+ * block -- one knob at a time, and keeps the fastest on the graph (shape knobs for the
+ * walk's Move).  Walks are identical under every setting, so tuning never changes results. */
 typedef struct {
     int32_t knobs;                 /* entries used below */
-    int32_t knob[3];               /* TRW_TUNE_* */
-    int64_t chosen[3];
-    int32_t candidates[3];         /* values tried per knob */
+    int32_t move;                  /* TRW_MOVE_* of the walk (the shape knobs were kept for it) */
+    int32_t knob[TRW_TUNE_KNOBS];  /* TRW_TUNE_* */
+    int32_t candidates[TRW_TUNE_KNOBS]; /* values tried per knob */
+    int64_t chosen[TRW_TUNE_KNOBS];
     double default_steps_per_s;    /* all knobs at their defaults */
     double tuned_steps_per_s;
 } trw_tune_report;
 int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_walkers, uint64_t seed,
                  int32_t with_paths, trw_tune_report *report);
+/* The same for a mixed batch as trw_walk_mix_async runs it (QUEUE form): the sample's walkers
+ * [0, split) walk as params_a, the rest as params_b; the queue knobs are tuned, and the launch
+ * shapes kept for params_a's Move (the queue kernel of a mix takes the first kind's). */
+int trw_autotune_mix(trw_graph *g, const trw_walk_params *params_a, const trw_walk_params *params_b, int32_t split,
+                     int32_t sample_walkers, uint64_t seed, int32_t with_paths, trw_tune_report *report);
 
 /* ---- walks ----
  * compute<F>(graph, walkers, f) (creeper.h:689-831) for a built-in app, on a

@@ -220,10 +220,18 @@ def test_walk_supported_names_the_library_moves():
 
 
 def test_tuning_knob_ids_match_the_header():
-    """Every TRW_TUNE_* knob the header defines has the same id in the Python mirror, and the
-    mirror's knob names cover exactly those ids."""
-    defs = dict(re.findall(r"^#define TRW_TUNE_(\w+)\s+(\d+)", open(HEADER).read(), re.M))
+    """Every TRW_TUNE_* knob the header defines has the same id in the Python mirror, the
+    mirror's knob names (per-graph knobs and per-Move launch shapes) cover exactly those ids,
+    and TRW_MOVE_* index the mirror's Move names in order."""
+    text = open(HEADER).read()
+    defs = dict(re.findall(r"^#define TRW_TUNE_(\w+)\s+(\d+)", text, re.M))
     assert defs, "no TRW_TUNE_* defines found"
     for name, val in defs.items():
         assert getattr(A, "TUNE_" + name) == int(val), name
-    assert set(T.DeviceGraph.TUNING.values()) == {int(v) for v in defs.values()}
+    knobs = {int(v) for k, v in defs.items() if k != "KNOBS"}
+    assert set(T.DeviceGraph.TUNING.values()) | set(T.DeviceGraph.SHAPES.values()) == knobs
+    assert not set(T.DeviceGraph.TUNING.values()) & set(T.DeviceGraph.SHAPES.values())
+    assert len(knobs) == int(defs["KNOBS"])
+    moves = dict(re.findall(r"^#define TRW_MOVE_(\w+)\s+(\d+)", text, re.M))
+    assert int(re.search(r"^#define TRW_MOVES\s+(\d+)", text, re.M).group(1)) == len(A.MOVES) == len(moves)
+    assert {A.MOVES[int(v)]: int(v) for v in moves.values()} == {k.lower(): int(v) for k, v in moves.items()}

@@ -205,8 +205,9 @@ def test_metapath_label_alphabets_match_oracle(gpu_ctx, n_labels, schema):
 def test_tuning_changes_no_walk(gpu_ctx):
     """ITS through guide records, and with them switched off through the search tree on every
     vertex, on hubs only, and off; the queue form with several chunk sizes; the fused form with
-    1, 2 and 4 walkers per lane: identical replay walks, equal to the oracle's.  Weights with a
-    negative entry build no tree and no guide."""
+    1, 2 and 4 walkers per lane; the launch shapes (fused threads per block and path tile, queue
+    blocks per CU, Node2Vec threads per block): identical replay walks, equal to the oracle's.
+    Weights with a negative entry build no tree and no guide."""
     g = graphs.hub_graph(5000, 4500, 20000, 71)
     dg = dgraph(gpu_ctx, g)
     try:
@@ -220,7 +221,21 @@ def test_tuning_changes_no_walk(gpu_ctx):
                  ({"app": "deepwalk", "execution": 1, "sampling": 2, "length": 40, "extra": {}},
                   [dict(walkers_per_thread=v) for v in (1, 2, 4)], ["fused"]),
                  ({"app": "metapath", "execution": 2, "sampling": 1, "length": 40, "extra": {"-s": "0,1,2,3,4"}},
-                  [dict(queue_chunk=v) for v in (64, 256, 4096)], ["queue"])]
+                  [dict(queue_chunk=v) for v in (64, 256, 4096)], ["queue"]),
+                 # Launch shapes: fused threads per block and path tile, queue blocks per CU,
+                 # Node2Vec threads per block.
+                 ({"app": "deepwalk", "execution": 1, "sampling": 2, "length": 40, "extra": {}},
+                  [dict(walkers_per_thread=w, fused_block=b, fused_tile=t)
+                   for w, b, t in ((1, 64, 0), (1, 256, 16), (2, 128, 32), (4, 256, 0), (4, 64, 16))], ["fused"]),
+                 ({"app": "deepwalk", "execution": 1, "sampling": 3, "length": 40, "extra": {}},
+                  [dict(walkers_per_thread=0, fused_block=b, fused_tile=t) for b, t in ((64, 0), (128, 16), (256, 32))],
+                  ["fused"]),
+                 ({"app": "node2vec", "execution": 2, "sampling": 4, "length": 40, "extra": {"-p": "2", "-q": "0.5"}},
+                  [dict(fused_block=0, fused_tile=0, n2v_block=v) for v in (128, 256)], ["fused"]),
+                 ({"app": "metapath", "execution": 2, "sampling": 1, "length": 40, "extra": {"-s": "0,1,2,3,4"}},
+                  [dict(queue_chunk=0, queue_blocks_per_cu=v) for v in (1, 2)] +
+                  [dict(queue_blocks_per_cu=0, fused_block=b, fused_tile=t) for b, t in ((64, 16), (256, 0))],
+                  ["queue", "fused"])]
         for case, knobs, forms in cases:
             ref = oracle_walk(g, case, src, K, 1, 72, 73)
             for kn in knobs:
@@ -229,7 +244,8 @@ def test_tuning_changes_no_walk(gpu_ctx):
                     r = T.walk(dg, case_params(case, form), src, draws, T.replay_fill(73, len(src), 1))
                     for key in ("paths", "lengths", "ends", "status", "draws_used"):
                         assert np.array_equal(r[key], ref[key]), (case["app"], kn, form, key)
-            dg.set_tuning(walkers_per_thread=0, its_tree_min_degree=0, queue_chunk=0, its_guide_off=0)
+            dg.set_tuning(walkers_per_thread=0, its_tree_min_degree=0, queue_chunk=0, its_guide_off=0, fused_block=0,
+                          fused_tile=0, queue_blocks_per_cu=0, n2v_block=0)
     finally:
         dg.close()
     # Negative weights: no tree; the walk still equals the oracle's (binary search, clamped pick).
@@ -253,17 +269,50 @@ def test_autotune_picks_knobs_and_keeps_walks(gpu_ctx):
     dg = dgraph(gpu_ctx, g)
     try:
         src = np.nonzero(g.degree > 0)[0].astype(np.int32)
-        for app, kw, knob in [("deepwalk", dict(execution="static", sampling="its", length=30), "its_tree_min_degree"),
-                              ("deepwalk", dict(execution="static", sampling="alias", length=30), "walkers_per_thread"),
-                              ("ppr", dict(stop_probability=0.2), "queue_chunk")]:
+        for app, kw, knobs, move in [
+                ("deepwalk", dict(execution="static", sampling="its", length=30),
+                 ("its_tree_min_degree", "fused_block", "fused_tile"), "its"),
+                ("deepwalk", dict(execution="static", sampling="alias", length=30),
+                 ("walkers_per_thread", "fused_block", "fused_tile"), "alias"),
+                ("node2vec", dict(length=30, p=2.0, q=0.5),
+                 ("n2v_block", "fused_block", "fused_tile"), "node2vec"),
+                ("ppr", dict(stop_probability=0.2), ("queue_chunk", "queue_blocks_per_cu"), "uniform")]:
             prm = T.make_params(app, rng="philox", seed=3, **kw)
             before = T.walk(dg, prm, src)
             rep = dg.autotune(prm, sample_walkers=1 << 16, seed=1)
-            assert knob in rep["chosen"] and rep["tuned_steps_per_s"] >= rep["default_steps_per_s"] > 0
-            assert dg.tuning()[knob] == rep["chosen"][knob]
+            assert rep["move"] == move, (app, rep["move"])
+            assert set(knobs) <= set(rep["chosen"]) and rep["tuned_steps_per_s"] >= rep["default_steps_per_s"] > 0
+            tuned = dg.tuning(move)
+            for knob in knobs:
+                assert tuned[knob] == rep["chosen"][knob], (app, knob)
             after = T.walk(dg, prm, src)
             for key in ("paths", "lengths", "ends", "status"):
                 assert np.array_equal(before[key], after[key]), (app, key)
+    finally:
+        dg.close()
+
+
+def test_autotune_mix_picks_queue_knobs_and_keeps_walks(gpu_ctx):
+    """trw_autotune_mix on a PPR + MetaPath sample: it tunes the queue chunk and blocks per CU
+    (kept for PPR's Move, the first kind), and the mixed batch walks the same after as before."""
+    g = graphs.chung_lu(30000, 200000, 2.1, 2, 4000, 78)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        live = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        src = np.random.default_rng(5).choice(live, 20000).astype(np.int32)
+        a = T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form="queue")
+        b = T.make_params("metapath", length=30, schema=[0, 1, 2, 3, 4], rng="philox", seed=4, form="queue")
+        before = T.walk(dg, a, src, mix=(b, 10000))
+        rep = dg.autotune_mix(a, b, 1 << 15, sample_walkers=1 << 16, seed=2)
+        assert rep["move"] == "uniform"
+        assert set(rep["chosen"]) == {"queue_chunk", "queue_blocks_per_cu"}
+        assert rep["tuned_steps_per_s"] >= rep["default_steps_per_s"] > 0
+        assert dg.tuning("uniform")["queue_blocks_per_cu"] == rep["chosen"]["queue_blocks_per_cu"]
+        after = T.walk(dg, a, src, mix=(b, 10000))
+        for key in ("paths", "lengths", "ends", "status"):
+            assert np.array_equal(before[key], after[key]), key
+        with pytest.raises(T.TrwError):  # the sample split must lie inside the sample
+            dg.autotune_mix(a, b, (1 << 16) + 1, sample_walkers=1 << 16)
     finally:
         dg.close()
 

@@ -16,6 +16,7 @@ FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP, FORM_GRAPH = 0, 1, 2, 3, 4
 WEIGHTS_EDGE, WEIGHTS_UNIT, WEIGHTS_CUSTOM = 0, 1, 2
 WALK_WEIGHTS_APP, WALK_WEIGHTS_EDGE, WALK_WEIGHTS_UNIT = 0, 1, 2
 TUNE_WALKERS_PER_THREAD, TUNE_ITS_TREE_MIN_DEGREE, TUNE_QUEUE_CHUNK, TUNE_ITS_GUIDE_OFF = 0, 1, 2, 3
+TUNE_FUSED_BLOCK, TUNE_FUSED_TILE, TUNE_QUEUE_BLOCKS_PER_CU, TUNE_N2V_BLOCK = 4, 5, 6, 7
 RESULT_PPR, RESULT_PATHS = 0, 1
 PATHS_PADDED, PATHS_UNPADDED = 0, 1
 QUERY_PENDING, QUERY_RUNNING, QUERY_DONE, QUERY_FAILED = 0, 1, 2, 3
@@ -62,9 +63,14 @@ class WalkIO(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+TUNE_KNOBS = 8
+MOVES = ("uniform", "alias", "its", "rejection", "node2vec", "metapath")  # TRW_MOVE_* order
+
+
 class TuneReport(C.Structure):
-    _fields_ = [("knobs", C.c_int32), ("knob", C.c_int32 * 3), ("chosen", C.c_int64 * 3),
-                ("candidates", C.c_int32 * 3), ("default_steps_per_s", C.c_double), ("tuned_steps_per_s", C.c_double)]
+    _fields_ = [("knobs", C.c_int32), ("move", C.c_int32), ("knob", C.c_int32 * TUNE_KNOBS),
+                ("candidates", C.c_int32 * TUNE_KNOBS), ("chosen", C.c_int64 * TUNE_KNOBS),
+                ("default_steps_per_s", C.c_double), ("tuned_steps_per_s", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -130,7 +136,11 @@ _SIGS = {
     "trw_graph_load_tables": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32]),
     "trw_graph_set_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "trw_graph_get_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "trw_graph_set_move_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int64]),
+    "trw_graph_get_move_tuning": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
     "trw_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64, C.c_int32, C.c_void_p]),
+    "trw_autotune_mix": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, C.c_int32,
+                                   C.c_void_p]),
     "trw_walk_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "trw_walk_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "trw_walk_supported": (C.c_int, [C.c_void_p]),

@@ -815,6 +815,31 @@ int trw_graph_load_tables(trw_graph *g, const char *dir, int32_t sampling, int32
     return TRW_OK;
 }
 
+namespace {
+
+int check_shape(int32_t knob, int64_t value) {
+    switch (knob) {
+    case TRW_TUNE_FUSED_BLOCK:
+        if (value != 0 && value != 64 && value != 128 && value != 256)
+            return fail(TRW_E_INVALID, "fused threads per block: 0, 64, 128 or 256");
+        return TRW_OK;
+    case TRW_TUNE_FUSED_TILE:
+        if (value != 0 && (value < 4 || value > 4096)) return fail(TRW_E_INVALID, "fused path tile: 0 or 4..4096");
+        return TRW_OK;
+    case TRW_TUNE_QUEUE_BLOCKS_PER_CU:
+        if (value < 0 || value > 16) return fail(TRW_E_INVALID, "queue blocks per CU outside [0, 16]");
+        return TRW_OK;
+    case TRW_TUNE_N2V_BLOCK:
+        if (value != 0 && value != 128 && value != 256)
+            return fail(TRW_E_INVALID, "Node2Vec threads per block: 0, 128 or 256");
+        return TRW_OK;
+    default:
+        return fail(TRW_E_INVALID, "unknown tuning knob %d", knob);
+    }
+}
+
+}  // namespace
+
 int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value) {
     if (!g) return fail(TRW_E_INVALID, "graph is null");
     switch (knob) {
@@ -830,8 +855,12 @@ int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value) {
     case TRW_TUNE_ITS_GUIDE_OFF:
         if (value != 0 && value != 1) return fail(TRW_E_INVALID, "ITS guide switch: 0 or 1");
         break;
-    default:
-        return fail(TRW_E_INVALID, "unknown tuning knob %d", knob);
+    default: {
+        const int rc = check_shape(knob, value);
+        if (rc != TRW_OK) return rc;
+        for (int m = 0; m < TRW_MOVES; ++m) g->shape[m][knob - 4] = value;
+        return TRW_OK;
+    }
     }
     g->tune[knob] = value;
     return TRW_OK;
@@ -839,8 +868,26 @@ int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value) {
 
 int trw_graph_get_tuning(const trw_graph *g, int32_t knob, int64_t *value) {
     if (!g || !value) return fail(TRW_E_INVALID, "null argument");
-    if (knob < 0 || knob > TRW_TUNE_ITS_GUIDE_OFF) return fail(TRW_E_INVALID, "unknown tuning knob %d", knob);
-    *value = g->tune[knob];
+    if (knob < 0 || knob >= TRW_TUNE_KNOBS) return fail(TRW_E_INVALID, "unknown tuning knob %d", knob);
+    *value = knob < 4 ? g->tune[knob] : g->shape[TRW_MOVE_UNIFORM][knob - 4];
+    return TRW_OK;
+}
+
+int trw_graph_set_move_tuning(trw_graph *g, int32_t move, int32_t knob, int64_t value) {
+    if (!g) return fail(TRW_E_INVALID, "graph is null");
+    if (move < 0 || move >= TRW_MOVES) return fail(TRW_E_INVALID, "unknown Move %d", move);
+    if (knob < 4 || knob >= TRW_TUNE_KNOBS) return fail(TRW_E_INVALID, "knob %d is not a launch shape (4..7)", knob);
+    const int rc = check_shape(knob, value);
+    if (rc != TRW_OK) return rc;
+    g->shape[move][knob - 4] = value;
+    return TRW_OK;
+}
+
+int trw_graph_get_move_tuning(const trw_graph *g, int32_t move, int32_t knob, int64_t *value) {
+    if (!g || !value) return fail(TRW_E_INVALID, "null argument");
+    if (move < 0 || move >= TRW_MOVES) return fail(TRW_E_INVALID, "unknown Move %d", move);
+    if (knob < 4 || knob >= TRW_TUNE_KNOBS) return fail(TRW_E_INVALID, "knob %d is not a launch shape (4..7)", knob);
+    *value = g->shape[move][knob - 4];
     return TRW_OK;
 }
 
@@ -919,7 +966,7 @@ static bool reads_static_table(const trw_walk_params *p) {
 
 // Maps an app + g_para to a device mode, building the static tables the mode
 // reads when they are missing.
-static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, bool allow_build = true) {
+static int resolve_move(trw_graph *g, const trw_walk_params *p, KindParams &k, bool allow_build) {
     std::memset(&k, 0, sizeof(k));
     if (p->app < TRW_APP_DEEPWALK || p->app > TRW_APP_GENERIC) return fail(TRW_E_INVALID, "unknown app %d", p->app);
     if (p->execution < TRW_EXEC_UNIFORM || p->execution > TRW_EXEC_DYNAMIC)
@@ -1002,6 +1049,25 @@ static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, b
     }
     k.mode = p->sampling == TRW_SM_ALIAS ? trw::kModeAlias : p->sampling == TRW_SM_ITS ? trw::kModeIts : trw::kModeRejection;
     k.unit_weights = wsrc == TRW_WEIGHTS_UNIT;
+    return TRW_OK;
+}
+
+static_assert(trw::kModeUniform == TRW_MOVE_UNIFORM && trw::kModeAlias == TRW_MOVE_ALIAS &&
+                  trw::kModeIts == TRW_MOVE_ITS && trw::kModeRejection == TRW_MOVE_REJECTION &&
+                  trw::kModeNode2Vec == TRW_MOVE_NODE2VEC && trw::kModeMetaPath == TRW_MOVE_METAPATH,
+              "TRW_MOVE_* index the graph's per-Move shapes by the walk's Move");
+
+// The walk's kind, then the launch shapes the graph keeps for its Move.
+static int resolve_kind(trw_graph *g, const trw_walk_params *p, KindParams &k, bool allow_build = true) {
+    const int rc = resolve_move(g, p, k, allow_build);
+    if (rc != TRW_OK) return rc;
+    if (k.mode >= 0 && k.mode < TRW_MOVES) {
+        const int64_t *sh = g->shape[k.mode];
+        k.fused_block = static_cast<int32_t>(sh[TRW_TUNE_FUSED_BLOCK - 4]);
+        k.fused_tile = static_cast<int32_t>(sh[TRW_TUNE_FUSED_TILE - 4]);
+        k.queue_blocks_per_cu = static_cast<int32_t>(sh[TRW_TUNE_QUEUE_BLOCKS_PER_CU - 4]);
+        k.n2v_block = static_cast<int32_t>(sh[TRW_TUNE_N2V_BLOCK - 4]);
+    }
     return TRW_OK;
 }
 

@@ -3,8 +3,10 @@
 // The knobs change how walkers are laid onto lanes and how a Move searches, never
 // which draws a walker consumes or what it compares, so every candidate produces the
 // same walks; only device time differs.  Knobs are tuned one at a time (walkers per
-// lane, then the ITS tree threshold, then the queue chunk), each with the others at
-// their current best.
+// lane, the ITS tree threshold, the queue chunk and blocks per CU, then the launch shapes
+// of the walk's Move: Node2Vec threads per block, fused threads per block, fused path
+// tile), each with the others at their current best.  The tuned knobs all start from their
+// defaults; the ITS guide switch is left as the caller set it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,8 +30,10 @@ struct Sample {
     }
 };
 
-// Device time of one walk over the sample (median of three after one warm-up).
-int time_walk(trw_graph *g, const trw_walk_params &p, Sample &s, double *steps_per_s) {
+// Device time of one walk over the sample (median of three after one warm-up); with pb, the
+// sample's walkers [0, split) walk as pa and the rest as pb, in one mixed batch.
+int time_walk(trw_graph *g, const trw_walk_params &p, const trw_walk_params *pb, int32_t split, Sample &s,
+              double *steps_per_s) {
     trw_walk_io io;
     std::memset(&io, 0, sizeof(io));
     io.sources = s.sources;
@@ -54,7 +58,7 @@ int time_walk(trw_graph *g, const trw_walk_params &p, Sample &s, double *steps_p
             break;
         }
         (void)hipEventRecord(a, st);
-        rc = trw::walk_launch(g, &p, &io, st, nullptr, false);
+        rc = pb ? trw_walk_mix_async(g, &p, pb, split, &io, st) : trw::walk_launch(g, &p, &io, st, nullptr, false);
         (void)hipEventRecord(b, st);
         if (rc != TRW_OK) break;
         if (hipStreamSynchronize(st) != hipSuccess) {
@@ -78,15 +82,23 @@ int time_walk(trw_graph *g, const trw_walk_params &p, Sample &s, double *steps_p
 
 }  // namespace
 
-int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_walkers, uint64_t seed,
-                 int32_t with_paths, trw_tune_report *report) {
+namespace {
+
+// pb null: one kind; else a mixed batch, walkers [0, split) of kind params, the rest of pb
+// (QUEUE form; the queue kernel of a mix takes the first kind's launch shapes).
+int tune(trw_graph *g, const trw_walk_params *params, const trw_walk_params *pb, int32_t split,
+         int32_t sample_walkers, uint64_t seed, int32_t with_paths, trw_tune_report *report) {
     if (!g || !params || !report) return trw::set_error(TRW_E_INVALID, "null argument");
-    if (params->rng != TRW_RNG_PHILOX) return trw::set_error(TRW_E_INVALID, "tune with the Philox stream");
+    if (params->rng != TRW_RNG_PHILOX || (pb && pb->rng != TRW_RNG_PHILOX))
+        return trw::set_error(TRW_E_INVALID, "tune with the Philox stream");
     if (sample_walkers < 1 || sample_walkers > (1 << 26))
         return trw::set_error(TRW_E_INVALID, "sample_walkers must be in [1, 2^26]");
+    if (pb && (split < 0 || split > sample_walkers))
+        return trw::set_error(TRW_E_INVALID, "split outside [0, sample_walkers]");
     std::memset(report, 0, sizeof(*report));
     if (hipSetDevice(g->ctx->device) != hipSuccess) return trw::set_error(TRW_E_HIP, "hipSetDevice");
     int rc = trw::walk_check(g, params);
+    if (rc == TRW_OK && pb) rc = trw::walk_check(g, pb);
     if (rc != TRW_OK) return rc;
 
     // Sources: non-isolated vertices drawn by a seeded generator (splitmix64).
@@ -110,7 +122,7 @@ int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_wal
     }
     Sample s;
     s.n = sample_walkers;
-    s.stride = with_paths ? std::max(params->length, 1) : 0;
+    s.stride = with_paths ? std::max(std::max(params->length, pb ? pb->length : 1), 1) : 0;
     const size_t N = static_cast<size_t>(s.n);
     if (hipMalloc(reinterpret_cast<void **>(&s.sources), sizeof(int32_t) * N) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&s.lengths), sizeof(int32_t) * N) != hipSuccess ||
@@ -123,13 +135,22 @@ int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_wal
     // Knobs that apply to this walk, with their candidates.
     const bool ppr = params->app == TRW_APP_PPR;
     const bool metapath = params->app == TRW_APP_METAPATH;
+    const bool node2vec = params->app == TRW_APP_NODE2VEC && params->execution != TRW_EXEC_UNIFORM;
     int32_t form = params->form;
-    if (form == TRW_FORM_AUTO) form = (ppr || metapath) ? TRW_FORM_QUEUE : TRW_FORM_FUSED;
+    if (form == TRW_FORM_AUTO) form = (ppr || metapath || pb) ? TRW_FORM_QUEUE : TRW_FORM_FUSED;
+    if (pb && form != TRW_FORM_QUEUE) return trw::set_error(TRW_E_UNSUPPORTED, "mixed batches tune in the QUEUE form");
     const bool fused_records = form == TRW_FORM_FUSED && params->walkers_per_thread == 0 &&
                                (params->execution == TRW_EXEC_UNIFORM ||
                                 (params->execution != TRW_EXEC_DYNAMIC &&
                                  (params->sampling == TRW_SM_ALIAS || params->sampling == TRW_SM_ITS)));
     const bool its = params->execution != TRW_EXEC_UNIFORM && params->sampling == TRW_SM_ITS && !metapath;
+    // The walk's Move (the shape knobs are kept per Move).
+    const int32_t move = params->execution == TRW_EXEC_UNIFORM ? TRW_MOVE_UNIFORM
+                         : node2vec                            ? TRW_MOVE_NODE2VEC
+                         : metapath                            ? TRW_MOVE_METAPATH
+                         : params->sampling == TRW_SM_ALIAS    ? TRW_MOVE_ALIAS
+                         : params->sampling == TRW_SM_ITS      ? TRW_MOVE_ITS
+                                                               : TRW_MOVE_REJECTION;
     struct Knob {
         int32_t id;
         std::vector<int64_t> values;
@@ -137,24 +158,39 @@ int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_wal
     std::vector<Knob> knobs;
     if (fused_records) knobs.push_back({TRW_TUNE_WALKERS_PER_THREAD, {1, 2, 4}});
     if (its) knobs.push_back({TRW_TUNE_ITS_TREE_MIN_DEGREE, {0, 32, 128, 512, INT32_MAX}});
-    if (form == TRW_FORM_QUEUE) knobs.push_back({TRW_TUNE_QUEUE_CHUNK, {0, 64, 256, 1024, 4096}});
+    if (form == TRW_FORM_QUEUE) {
+        knobs.push_back({TRW_TUNE_QUEUE_CHUNK, {0, 64, 256, 1024, 4096}});
+        knobs.push_back({TRW_TUNE_QUEUE_BLOCKS_PER_CU, {0, 1, 2, 4}});
+    }
+    if (form == TRW_FORM_FUSED) {
+        if (node2vec) knobs.push_back({TRW_TUNE_N2V_BLOCK, {256, 128}});
+        knobs.push_back({TRW_TUNE_FUSED_BLOCK, {0, 64, 128, 256}});
+        if (with_paths && params->length > 16) knobs.push_back({TRW_TUNE_FUSED_TILE, {0, 16, 32, 64}});
+    }
+    const auto get = [&](int32_t id) -> int64_t & { return id < 4 ? g->tune[id] : g->shape[move][id - 4]; };
 
-    const int64_t saved[3] = {g->tune[0], g->tune[1], g->tune[2]};
-    for (int k = 0; k < 3; ++k) g->tune[k] = 0;
-    rc = time_walk(g, *params, s, &report->default_steps_per_s);
+    int64_t saved[TRW_TUNE_KNOBS];
+    for (int k = 0; k < TRW_TUNE_KNOBS; ++k) saved[k] = get(k);
+    const auto restore = [&] {
+        for (int k = 0; k < TRW_TUNE_KNOBS; ++k) get(k) = saved[k];
+    };
+    for (int k = 0; k < TRW_TUNE_KNOBS; ++k)
+        if (k != TRW_TUNE_ITS_GUIDE_OFF) get(k) = 0;  // the guide switch is the caller's, not tuned
+    rc = time_walk(g, *params, pb, split, s, &report->default_steps_per_s);
     if (rc != TRW_OK) {
-        std::memcpy(g->tune, saved, sizeof(saved));
+        restore();
         return rc;
     }
+    report->move = move;
     double best = report->default_steps_per_s;
     for (const Knob &kn : knobs) {
-        int64_t best_v = g->tune[kn.id];
+        int64_t best_v = get(kn.id);
         for (int64_t v : kn.values) {
-            g->tune[kn.id] = v;
+            get(kn.id) = v;
             double sps = 0.0;
-            rc = time_walk(g, *params, s, &sps);
+            rc = time_walk(g, *params, pb, split, s, &sps);
             if (rc != TRW_OK) {
-                std::memcpy(g->tune, saved, sizeof(saved));
+                restore();
                 return rc;
             }
             if (sps > best * 1.01) {  // keep the earlier (default-first) value unless clearly faster
@@ -162,7 +198,7 @@ int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_wal
                 best_v = v;
             }
         }
-        g->tune[kn.id] = best_v;
+        get(kn.id) = best_v;
         const int i = report->knobs++;
         report->knob[i] = kn.id;
         report->chosen[i] = best_v;
@@ -170,4 +206,17 @@ int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_wal
     }
     report->tuned_steps_per_s = best;
     return TRW_OK;
+}
+
+}  // namespace
+
+int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_walkers, uint64_t seed,
+                 int32_t with_paths, trw_tune_report *report) {
+    return tune(g, params, nullptr, 0, sample_walkers, seed, with_paths, report);
+}
+
+int trw_autotune_mix(trw_graph *g, const trw_walk_params *params_a, const trw_walk_params *params_b, int32_t split,
+                     int32_t sample_walkers, uint64_t seed, int32_t with_paths, trw_tune_report *report) {
+    if (!params_b) return trw::set_error(TRW_E_INVALID, "null argument");
+    return tune(g, params_a, params_b, split, sample_walkers, seed, with_paths, report);
 }

@@ -142,6 +142,11 @@ struct KindParams {
     int32_t its_tree_min;   // ITS: vertices of at least this degree search their tree (when built)
     int32_t queue_chunk;    // QUEUE: walker indices a wavefront takes per atomic (0 = from the batch size)
     int32_t its_guide;      // ITS: read the guide records when the graph has them
+    // Launch shapes of the Move (0: the built-in choice).
+    int32_t fused_block;          // FUSED: threads per block
+    int32_t fused_tile;           // FUSED: most path positions a lane stages in LDS per write-out
+    int32_t queue_blocks_per_cu;  // QUEUE: resident blocks per CU of the persistent grid
+    int32_t n2v_block;            // Node2Vec state machine: threads per block (128 or 256)
 };
 
 // Walker batch (device pointers).

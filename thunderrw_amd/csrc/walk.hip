@@ -1124,11 +1124,12 @@ struct LaneRow {
 
 // WEIGHTED: Node2Vec over edge weights, a trial reading the candidate's 32-byte weighted record
 // (edge weight beside the neighbour record) instead of its 16-byte neighbour record.
-template <bool REPLAY, bool WEIGHTED>
-__global__ __launch_bounds__(kN2vBlock) void walk_node2vec_sm_kernel(DevGraph g, KindParams P, BatchIO io) {
-    __shared__ int32_t chunk[kN2vBlock * (kN2vChunk + 1)];
+// BLOCK: threads per block (a launch shape the graph's tuning picks, 128 or 256).
+template <bool REPLAY, bool WEIGHTED, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void walk_node2vec_sm_kernel(DevGraph g, KindParams P, BatchIO io) {
+    __shared__ int32_t chunk[BLOCK * (kN2vChunk + 1)];
     int32_t *mine = chunk + threadIdx.x * (kN2vChunk + 1);
-    const int32_t w = static_cast<int32_t>(blockIdx.x) * kN2vBlock + static_cast<int32_t>(threadIdx.x);
+    const int32_t w = static_cast<int32_t>(blockIdx.x) * BLOCK + static_cast<int32_t>(threadIdx.x);
     const int32_t L = P.length;
     WalkerState s = empty_walker();
     if (w < io.n) init_walker(g, io, w, s);
@@ -1718,8 +1719,12 @@ int32_t fused_tile(int32_t L, int32_t rows, int32_t budget) {
 template <int MODE, bool REPLAY>
 void fused_multi_launch(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream) {
     const int32_t wpt = p.walkers_per_thread == 4 ? 4 : p.walkers_per_thread == 2 ? 2 : 1;
-    const int32_t block = wpt == 4 ? TRW_MULTI_BLOCK4 : wpt == 2 ? TRW_MULTI_BLOCK2 : TRW_MULTI_BLOCK1;
-    const int32_t T = fused_tile(p.length, wpt * block, kTileBudget);
+    const int32_t block = p.fused_block > 0 ? p.fused_block  // the graph's tuning for this Move
+                          : wpt == 4        ? TRW_MULTI_BLOCK4
+                          : wpt == 2        ? TRW_MULTI_BLOCK2
+                                            : TRW_MULTI_BLOCK1;
+    int32_t T = fused_tile(p.length, wpt * block, kTileBudget);
+    if (p.fused_tile > 0 && p.fused_tile < T) T = p.fused_tile;
     const size_t lds = static_cast<size_t>(wpt) * block * (T + 1) * sizeof(int32_t);
     const dim3 grid(grid_for(io.n, wpt * block));
     switch (wpt) {
@@ -1767,8 +1772,10 @@ hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO 
         else fused_multi_launch<kModeIts, REPLAY>(g, p, io, stream);
         return hipGetLastError();
     }
-    const FusedShape sh = fused_shape(p.mode);
-    const int32_t T = fused_tile(p.length, sh.block, sh.budget);
+    FusedShape sh = fused_shape(p.mode);
+    if (p.fused_block > 0) sh.block = p.fused_block;  // the graph's tuning for this Move
+    int32_t T = fused_tile(p.length, sh.block, sh.budget);
+    if (p.fused_tile > 0 && p.fused_tile < T) T = p.fused_tile;
     const size_t lds = static_cast<size_t>(sh.block) * (T + 1) * sizeof(int32_t);
     const dim3 grid(grid_for(io.n, sh.block)), block(sh.block);
     switch (p.mode) {
@@ -1778,12 +1785,20 @@ hipError_t fused_dispatch(const DevGraph &g, const KindParams &p, const BatchIO 
     case kModeRejection: hipLaunchKernelGGL((walk_fused_kernel<kModeRejection, REPLAY>), grid, block, lds, stream, g, p, io, T); break;
     case kModeNode2Vec:
         if (g.nrec && g.mlines && g.mroot && !p.ppr && (!p.n2v_weighted || g.wnrec)) {
-            if (p.n2v_weighted)
-                hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY, true>), dim3(grid_for(io.n, kN2vBlock)),
+            if (p.n2v_block == 128) {
+                if (p.n2v_weighted)
+                    hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY, true, 128>), dim3(grid_for(io.n, 128)),
+                                       dim3(128), 0, stream, g, p, io);
+                else
+                    hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY, false, 128>), dim3(grid_for(io.n, 128)),
+                                       dim3(128), 0, stream, g, p, io);
+            } else if (p.n2v_weighted) {
+                hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY, true, kN2vBlock>), dim3(grid_for(io.n, kN2vBlock)),
                                    dim3(kN2vBlock), 0, stream, g, p, io);
-            else
-                hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY, false>), dim3(grid_for(io.n, kN2vBlock)),
+            } else {
+                hipLaunchKernelGGL((walk_node2vec_sm_kernel<REPLAY, false, kN2vBlock>), dim3(grid_for(io.n, kN2vBlock)),
                                    dim3(kN2vBlock), 0, stream, g, p, io);
+            }
         } else {
             hipLaunchKernelGGL((walk_fused_node2vec_kernel<REPLAY>), grid, block, lds, stream, g, p, io, T);
         }
@@ -1834,7 +1849,9 @@ template <int MODE, bool REPLAY>
 void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split, const BatchIO &io,
                   unsigned int *q, hipStream_t stream, int cus) {
     static const int per_cu = blocks_per_cu(walk_queue_kernel<MODE, REPLAY>);  // one kernel, one answer
-    const int64_t resident = static_cast<int64_t>(per_cu) * cus, needed = grid_for(io.n, kQueueBlock);
+    // The graph's tuning may keep fewer blocks per CU than fit (first kind's Move).
+    const int use = a.queue_blocks_per_cu > 0 && a.queue_blocks_per_cu < per_cu ? a.queue_blocks_per_cu : per_cu;
+    const int64_t resident = static_cast<int64_t>(use) * cus, needed = grid_for(io.n, kQueueBlock);
     const unsigned grid = static_cast<unsigned>(resident < needed ? resident : needed);
     // About eight chunks per wavefront of the grid, between 64 and 4096 walkers each, unless the
     // graph's tuning fixes it: a wavefront that takes its last chunk late ends the batch late.

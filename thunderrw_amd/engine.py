@@ -178,11 +178,17 @@ class DeviceGraph:
     TUNING = {"walkers_per_thread": A.TUNE_WALKERS_PER_THREAD, "its_tree_min_degree": A.TUNE_ITS_TREE_MIN_DEGREE,
               "its_guide_off": A.TUNE_ITS_GUIDE_OFF,
               "queue_chunk": A.TUNE_QUEUE_CHUNK}
+    SHAPES = {"fused_block": A.TUNE_FUSED_BLOCK, "fused_tile": A.TUNE_FUSED_TILE,
+              "queue_blocks_per_cu": A.TUNE_QUEUE_BLOCKS_PER_CU, "n2v_block": A.TUNE_N2V_BLOCK}
 
-    def set_tuning(self, **knobs):
-        """Launch choices that change speed, never results (trw_graph_set_tuning)."""
+    def set_tuning(self, move=None, **knobs):
+        """Launch choices that change speed, never results (trw_graph_set_tuning); the shape
+        knobs (SHAPES) for every Move, or for one Move ("alias", ...) when move is given."""
         for k, v in knobs.items():
-            check(lib().trw_graph_set_tuning(self.h, self.TUNING[k], int(v)))
+            if k in self.SHAPES and move is not None:
+                check(lib().trw_graph_set_move_tuning(self.h, A.MOVES.index(move), self.SHAPES[k], int(v)))
+            else:
+                check(lib().trw_graph_set_tuning(self.h, {**self.TUNING, **self.SHAPES}[k], int(v)))
 
     def autotune(self, params, sample_walkers=1 << 20, seed=0, with_paths=True):
         """trw_autotune: times the applicable knobs' candidates on a sample batch and keeps the
@@ -190,16 +196,33 @@ class DeviceGraph:
         rep = A.TuneReport()
         check(lib().trw_autotune(self.h, C.byref(params), int(sample_walkers), int(seed), int(bool(with_paths)),
                                  C.byref(rep)))
-        names = {v: k for k, v in self.TUNING.items()}
-        return {"chosen": {names[rep.knob[i]]: rep.chosen[i] for i in range(rep.knobs)},
+        return self._report(rep)
+
+    def autotune_mix(self, params_a, params_b, split, sample_walkers=1 << 20, seed=0, with_paths=True):
+        """trw_autotune_mix: the same for a mixed QUEUE-form batch, walkers [0, split) of the
+        sample as params_a and the rest as params_b."""
+        rep = A.TuneReport()
+        check(lib().trw_autotune_mix(self.h, C.byref(params_a), C.byref(params_b), int(split), int(sample_walkers),
+                                     int(seed), int(bool(with_paths)), C.byref(rep)))
+        return self._report(rep)
+
+    def _report(self, rep):
+        names = {v: k for k, v in {**self.TUNING, **self.SHAPES}.items()}
+        return {"move": A.MOVES[rep.move], "chosen": {names[rep.knob[i]]: rep.chosen[i] for i in range(rep.knobs)},
                 "default_steps_per_s": rep.default_steps_per_s, "tuned_steps_per_s": rep.tuned_steps_per_s}
 
-    def tuning(self):
+    def tuning(self, move=None):
+        """The per-graph knobs, and the shape knobs of one Move when move is given."""
         out = {}
         for k, knob in self.TUNING.items():
             v = C.c_int64()
             check(lib().trw_graph_get_tuning(self.h, knob, C.byref(v)))
             out[k] = v.value
+        if move is not None:
+            for k, knob in self.SHAPES.items():
+                v = C.c_int64()
+                check(lib().trw_graph_get_move_tuning(self.h, A.MOVES.index(move), knob, C.byref(v)))
+                out[k] = v.value
         return out
 
     def alias_table(self):

@@ -265,7 +265,8 @@ int trw_build_membership(trw_graph *g);
 #define TRW_TUNE_WALKERS_PER_THREAD 0  /* FUSED uniform/alias/static ITS walkers per lane when params say 0: 1, 2, 4 */
 #define TRW_TUNE_ITS_TREE_MIN_DEGREE 1 /* ITS: vertices of at least this degree search their 128-byte-node
                                           tree, the rest binary-search their prefix sums */
-#define TRW_TUNE_QUEUE_CHUNK 2         /* QUEUE: walker indices a wavefront takes per atomic (64..4096) */
+#define TRW_TUNE_QUEUE_CHUNK 2         /* QUEUE: walker indices a wavefront takes per atomic (64..4096; at most
+                                          the batch's walkers per wavefront of the grid) */
 #define TRW_TUNE_ITS_GUIDE_OFF 3       /* ITS: 0 = steps read the guide records where the table has them
                                           (one gather), 1 = they search the prefix-sum structures (the
                                           picks are identical; not autotuned) */

@@ -1857,6 +1857,9 @@ void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, i
     // graph's tuning fixes it: a wavefront that takes its last chunk late ends the batch late.
     const int64_t waves = static_cast<int64_t>(grid) * (kQueueBlock / 64);
     int64_t chunk = a.queue_chunk > 0 ? a.queue_chunk : static_cast<int64_t>(io.n) / (waves * 8);
+    // A tuned chunk is at most the batch's share per wavefront, so a small batch still spreads
+    // over every wavefront of the grid.
+    if (a.queue_chunk > 0) chunk = std::min<int64_t>(chunk, static_cast<int64_t>(io.n) / waves);
     chunk = chunk < 64 ? 64 : (chunk > 4096 ? 4096 : chunk);
     // Expected walk length of a kind: its length, or min(length, 1 / stop probability) for a walk
     // that stops by probability.  The kind whose walks run longer takes the queue's first positions.

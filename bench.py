@@ -170,14 +170,29 @@ def functor_engine(ctx, dg, hg, deg, args):
         if lib_steps != steps:
             raise RuntimeError("library Node2Vec walked %d steps, the functor %d" % (lib_steps, steps))
         out["node2vec_library_steps_per_s"] = lib_steps / (ms / 1e3 / 3)
-        steps, secs = run(["-kind", "3", "-em", "2", "-sm", "1", "-wn", str(1 << 14)])
-        out["dynamic_its_sample_steps_per_s"] = steps / secs
-        steps, secs = run(["-kind", "1", "-em", "2", "-sm", "2", "-wn", str(1 << 14)])
-        out["second_order_dynamic_alias_sample_steps_per_s"] = steps / secs
-        log("functor engine: Node2Vec functor %.3e steps/s (library kernel %.3e); dynamic ITS %.3e, "
-            "second-order dynamic alias %.3e steps/s (2^14-walker samples)"
+        # Dynamic mode: DeepWalk's weights by dynamic ITS (kind 3) and a second-order functor by
+        # dynamic alias (kind 2 sampler, kind 1 functor), on 2^14 and 2^17 walkers from random
+        # sources; beside them the reference's own engine (customwalk.out, the same functor built
+        # against the reference's sources) on the same graph, walkers and flags on the host cores.
+        for key, flags in (("dynamic_its", ["-kind", "3", "-em", "2", "-sm", "1"]),
+                           ("second_order_dynamic_alias", ["-kind", "1", "-em", "2", "-sm", "2"])):
+            for wn, tag in ((1 << 14, "sample"), (1 << 17, "2p17")):
+                steps, secs = run(flags + ["-wn", str(wn)])
+                out["%s_%s_steps_per_s" % (key, tag)] = steps / secs
+            if not args.no_cpu_baseline:
+                ref = reference_run(hg, "customwalk", ["-ew", "-l", "80"] + flags + ["-wn", str(1 << 14)])
+                if ref:
+                    out["reference_%s_sample_steps_per_s" % key] = ref["value"]
+                    out["reference_cores"] = ref["cores"]
+        log("functor engine: Node2Vec functor %.3e steps/s (library kernel %.3e); dynamic ITS %.3e (2^14 walkers), "
+            "%.3e (2^17), reference %s; second-order dynamic alias %.3e (2^14), %.3e (2^17), reference %s"
             % (out["node2vec_functor_steps_per_s"], out["node2vec_library_steps_per_s"],
-               out["dynamic_its_sample_steps_per_s"], out["second_order_dynamic_alias_sample_steps_per_s"]))
+               out["dynamic_its_sample_steps_per_s"], out["dynamic_its_2p17_steps_per_s"],
+               "%.3e" % out["reference_dynamic_its_sample_steps_per_s"]
+               if "reference_dynamic_its_sample_steps_per_s" in out else "n/a",
+               out["second_order_dynamic_alias_sample_steps_per_s"], out["second_order_dynamic_alias_2p17_steps_per_s"],
+               "%.3e" % out["reference_second_order_dynamic_alias_sample_steps_per_s"]
+               if "reference_second_order_dynamic_alias_sample_steps_per_s" in out else "n/a"))
         return out
     finally:
         shutil.rmtree(tmp, ignore_errors=True)

@@ -367,20 +367,21 @@ TRW_DROPIN_MEMBER_TRAIT(schema_length_)
 TRW_DROPIN_MEMBER_TRAIT(sfmt_)
 #undef TRW_DROPIN_MEMBER_TRAIT
 
-// Whether F's type is named `want` (the compiler's function signature spells "F = <type>").
+// Whether F's type is named `want`: the compiler's function signature spells "F = <type>", and
+// the type's last component -- after any namespace, named or anonymous, or enclosing class --
+// must equal `want` (an alias names the type it aliases).
 template <class F>
 bool functor_named(const char *want) {
-    const char *p = __PRETTY_FUNCTION__;
-    for (; *p; ++p) {
-        if (p[0] == 'F' && p[1] == ' ' && p[2] == '=' && p[3] == ' ') {
-            const char *q = p + 4;
-            size_t i = 0;
-            for (; want[i]; ++i)
-                if (q[i] != want[i]) return false;
-            return q[i] == ']' || q[i] == ';';
-        }
-    }
-    return false;
+    const char *p = std::strstr(__PRETTY_FUNCTION__, "F = ");
+    if (!p) return false;
+    p += 4;
+    const char *end = p;
+    while (*end && *end != ']' && *end != ';' && *end != ',') ++end;
+    const char *last = p;
+    for (const char *q = p; q + 1 < end; ++q)
+        if (q[0] == ':' && q[1] == ':') last = q + 2;
+    const size_t n = std::strlen(want);
+    return static_cast<size_t>(end - last) == n && std::strncmp(last, want, n) == 0;
 }
 
 // The reference's built-in functors, recognised by name and members, as library apps with

@@ -199,13 +199,22 @@ def functor_engine(ctx, dg, hg, deg, args):
 
 
 def host_cpu():
+    """The host CPU as /proc/cpuinfo describes it: model name (which a virtualised host may cut
+    short), CPUID family / model / stepping, and the logical CPUs of the host and of this
+    process's affinity (the cores the reference is given)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "vendor_id") and k not in info:
+                    info[k] = v
     except OSError:
         pass
-    return "unknown"
+    return ("%s (%s, CPUID family %s, model %s, stepping %s; %d logical CPUs on the host, %d in this "
+            "process's affinity)" % (info.get("model name", "unknown"), info.get("vendor_id", "?"),
+                                     info.get("cpu family", "?"), info.get("model", "?"), info.get("stepping", "?"),
+                                     os.cpu_count() or 0, len(os.sched_getaffinity(0))))
 
 
 def reference_run(hg, app, flags):

@@ -55,7 +55,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_walk_fused_alias_deepwalk
 # The sources the headline kernel's HBM traffic depends on (the kernel, its records, the
 # table builds that lay them out); a traffic record taken on other sources is not used.
 TRAFFIC_SOURCES = ("thunderrw_amd/csrc/walk.hip", "thunderrw_amd/csrc/engine.h", "thunderrw_amd/csrc/tables.hip")
-HEADLINE_KERNEL = "walk_fused_multi_kernel"
+# The headline kernel's instantiation: walk_fused_multi_kernel<alias Move (1), philox, walkers per lane>.
+HEADLINE_KERNEL = "walk_fused_multi_kernel<1, false, %d>"
 COPY_KERNEL = "copy_u128_kernel"
 
 
@@ -353,6 +354,11 @@ def c3_node2vec(ctx, args):
     return out
 
 
+def headline_walkers_per_lane(tune):
+    """Walkers per lane of the timed headline launches (the tuner's choice; 0 is the default, 1)."""
+    return int(tune["chosen"].get("walkers_per_thread", 0)) or 1
+
+
 def tune_record(tune):
     """What the autotuner kept for a walk: its Move, the chosen knobs, and the sample rates."""
     return {"move": tune["move"], "chosen": tune["chosen"],
@@ -549,7 +555,8 @@ def record_traffic(d):
         return hits[0][-1]  # the last dispatch: the timed launch
 
     nbytes = int(line["details"]["calibrate_copy_bytes"])
-    f_walk, w_walk = one(fetch, HEADLINE_KERNEL), one(write, HEADLINE_KERNEL)
+    kernel = HEADLINE_KERNEL % headline_walkers_per_lane(line["details"]["autotune"]["alias"])
+    f_walk, w_walk = one(fetch, kernel), one(write, kernel)
     f_copy, w_copy = one(fetch, COPY_KERNEL), one(write, COPY_KERNEL)
     fs, ws = nbytes / (f_copy * 1024.0), nbytes / (w_copy * 1024.0)
     rec = {"kernel": line["config"]["kernel"], "workload": line["config"]["workload"],
@@ -737,7 +744,8 @@ def main():
                                "FP64 U[1,5) weights, length 80, one walker per non-isolated vertex, philox"
                                % args.scale,
                    "vertices": nv, "directed_edges": int(len(adj)), "walkers": n, "walk_length": length,
-                   "kernel": "walk_fused_multi_kernel<alias, philox, 1 walker per lane>"},
+                   "kernel": "walk_fused_multi_kernel<alias, philox, %d walker(s) per lane>"
+                             % headline_walkers_per_lane(head_tune)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,

@@ -100,7 +100,14 @@ class Batch:
         self.paths = T.DeviceBuffer(ctx, self.n * length * 4) if paths else None
         self.counters = T.DeviceBuffer.from_array(ctx, np.zeros(4, np.uint64))
 
-    def io(self, draws=None, K=0, unpadded=False):
+    def io(self, draws=None, K=0, unpadded=False, kind_rows=None):
+        """kind_rows=(split, La, Lb): walkers [0, split) in rows of La ints, the rest in rows of
+        Lb ints after them (io.path_stride_b), within the same buffer."""
+        if kind_rows:
+            split, La, Lb = kind_rows
+            assert split * La + (self.n - split) * Lb <= self.n * self.length
+            return T.make_io(self.src, self.n, La, 0, draws, K, None, 0, self.paths, counters=self.counters,
+                             unpadded=unpadded, path_stride_b=Lb)
         return T.make_io(self.src, self.n, self.length, 0, draws, K, None, 0, self.paths, counters=self.counters,
                          unpadded=unpadded)
 
@@ -266,6 +273,41 @@ def reference_cpu_baseline(hg, length):
                       % (r["cores"], length, r["steps"], r["walk_s"], r["pre_s"], host_cpu())}
 
 
+def c1_longest_walk(ctx, dg, prm, src):
+    """The batch's longest walk timed alone: walks are keyed by walker id, so the walker with the
+    most steps, launched by itself under its own id, repeats its walk.  Its time is the floor of a
+    batch launch (a chain of dependent gathers that no other walker can shorten)."""
+    n = len(src)
+    lengths = T.DeviceBuffer(ctx, n * 4)
+    all_src = T.DeviceBuffer.from_array(ctx, src)
+    io = T.make_io(all_src, n, lengths=lengths)
+    T.walk_async(dg, prm, io)
+    T.synchronize(ctx)
+    lens = lengths.to_numpy(np.int32, n)
+    w = int(np.argmax(lens))
+    one_src = T.DeviceBuffer.from_array(ctx, src[w:w + 1].copy())
+    one_id = T.DeviceBuffer.from_array(ctx, np.array([w], np.uint64))
+    one_len = T.DeviceBuffer(ctx, 4)
+    counters = T.DeviceBuffer.from_array(ctx, np.zeros(4, np.uint64))
+    io1 = T.make_io(one_src, 1, lengths=one_len, counters=counters, walker_ids=one_id)
+    T.walk_async(dg, prm, io1)
+    T.synchronize(ctx)
+    if int(one_len.to_numpy(np.int32, 1)[0]) != int(lens[w]):
+        raise RuntimeError("the longest walker alone walked %d positions, in the batch %d"
+                           % (int(one_len.to_numpy(np.int32, 1)[0]), int(lens[w])))
+    reps = 20
+    timer = T.DeviceTimer(ctx)
+    timer.begin()
+    for _ in range(reps):
+        T.walk_async(dg, prm, io1)
+    ms = timer.end() / reps
+    T.synchronize(ctx)
+    for b in (lengths, all_src, one_src, one_id, one_len, counters):
+        b.free()
+    return {"walker": w, "steps": int(lens[w]) - 1, "alone_ms": ms,
+            "us_per_step_alone": ms * 1e3 / max(1, int(lens[w]) - 1)}
+
+
 def c1_ppr(ctx, args):
     """PPR with uniform Moves and termination 0.2 on a 2^18-vertex Chung-Lu graph (exponent 2.1,
     dmin 2, dmax 4096, ~1.5M undirected edges, unweighted): one walker per non-isolated vertex,
@@ -292,6 +334,7 @@ def c1_ppr(ctx, args):
         out[label + "_steps_per_s"] = b.steps() / reps / (ms / 1e3)
         out[label + "_ms"] = ms
         b.free()
+        out[label + "_longest_walk"] = c1_longest_walk(ctx, dg, prm, src)
     dg.close()
     ref = None if args.no_cpu_baseline else reference_run(hg, "ppr", ["-sp", "0.2"])
     if ref:
@@ -390,9 +433,11 @@ def mix_forms(ctx, dg, deg, args):
     """Uneven PPR + MetaPath mix: 2^23 walkers from uniform random non-isolated sources
     (generate_random_source), walkers [0, 2^22) PPR with termination 0.15 and length 1
     (g_para.length_ = 1, ppr.cpp:51: the path row holds the source), the rest MetaPath on
-    schema 0,1,2,3,4 with length 80 (metapath.cpp:48), paths logged.  The same batch
-    runs in the persistent work-queue form, the compacted hipGraph step form and the
-    plain per-step-launch form; the forms give identical walks."""
+    schema 0,1,2,3,4 with length 80 (metapath.cpp:48), paths logged.  Each kind's rows are as
+    long as its walks (io.path_stride_b: PPR's 1-entry rows, then MetaPath's 80-entry rows), as
+    the two apps' own runs would write them; "*_one_stride" is the same batch in one matrix of
+    80-entry rows.  The same batch runs in the persistent work-queue form, the compacted
+    hipGraph step form and the plain per-step-launch form; the forms give identical walks."""
     n = 1 << 23
     split = n // 2
     src = T.sources_random(deg, n, 4)
@@ -407,11 +452,11 @@ def mix_forms(ctx, dg, deg, args):
     out["queue_autotune"] = tune_record(dg.autotune_mix(pa, pb, split, sample_walkers=n, seed=4, with_paths=True))
     # "queue_unpadded": the same walks with TRW_PATHS_UNPADDED, each row holding its walker's
     # length entries as the reference's LOG_SEQUENCE output does, with no -1 padding written.
-    for form in ("queue", "queue_unpadded", "graph", "step"):
-        kf = "queue" if form == "queue_unpadded" else form
+    for form in ("queue", "queue_unpadded", "graph", "step", "queue_one_stride", "queue_unpadded_one_stride"):
+        kf = form.split("_")[0]
         a = T.make_params("ppr", length=1, stop_probability=0.15, rng="philox", seed=4, form=kf)
         b = T.make_params("metapath", length=80, schema=[0, 1, 2, 3, 4], rng="philox", seed=4, form=kf)
-        io = batch.io(unpadded=form == "queue_unpadded")
+        io = batch.io(unpadded="unpadded" in form, kind_rows=None if "one_stride" in form else (split, 1, 80))
         T.walk_mix_async(dg, a, b, split, io)
         T.synchronize(ctx)
         batch.reset()

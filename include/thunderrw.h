@@ -128,7 +128,8 @@ typedef struct {
     uint64_t *counters;            /* nullable: [0] steps [1] gather-terminated [2] dead ends [3] errors; accumulated */
     const uint64_t *walker_ids;    /* nullable: per-walker global ids keying Philox (else walker_id_base + index) */
     int32_t path_flags;            /* TRW_PATHS_*: 0 = rows padded with -1 past the walk's length */
-    int32_t reserved;
+    int32_t path_stride_b;         /* mixed batches: ints per row of the walkers [split, n), whose rows
+                                      follow the first split rows; 0 = path_stride (one row matrix) */
 } trw_walk_io;
 
 /* trw_walk_io.path_flags.  UNPADDED: row i holds its walk's lengths[i] vertices (at most the
@@ -312,7 +313,8 @@ typedef struct {
 } trw_tune_report;
 int trw_autotune(trw_graph *g, const trw_walk_params *params, int32_t sample_walkers, uint64_t seed,
                  int32_t with_paths, trw_tune_report *report);
-/* The same for a mixed batch as trw_walk_mix_async runs it (QUEUE form): the sample's walkers
+/* The same for a mixed batch as trw_walk_mix_async runs it (QUEUE form, each kind's path rows at
+ * its own length when with_paths, io.path_stride_b): the sample's walkers
  * [0, split) walk as params_a, the rest as params_b; the queue knobs are tuned, and the launch
  * shapes kept for params_a's Move (the queue kernel of a mix takes the first kind's). */
 int trw_autotune_mix(trw_graph *g, const trw_walk_params *params_a, const trw_walk_params *params_b, int32_t split,
@@ -350,7 +352,12 @@ int trw_walk_noalloc(trw_graph *g, const trw_walk_params *params, const trw_walk
 int trw_walk_supported(const trw_walk_params *params);
 
 /* Two walker kinds in one batch: walkers [0, split) run params_a, [split, n) run
- * params_b (the uneven PPR + MetaPath mix).  form: QUEUE, STEP or GRAPH. */
+ * params_b (the uneven PPR + MetaPath mix).  form: QUEUE, STEP or GRAPH.  Path rows: with
+ * io.path_stride_b = 0 every walker's row has io.path_stride ints (>= both lengths); otherwise
+ * walker w < split has row paths + w * path_stride (>= params_a's length) and walker w >= split
+ * row paths + split * path_stride + (w - split) * path_stride_b (>= params_b's length), so each
+ * kind's rows are as long as its own walks, as two separate runs would write them (e.g. PPR's
+ * length-1 rows as one dense array of sources). */
 int trw_walk_mix_async(trw_graph *g, const trw_walk_params *params_a, const trw_walk_params *params_b,
                        int32_t split, const trw_walk_io *io, void *stream);
 

@@ -451,6 +451,55 @@ def test_mixed_batch_queue_equals_step_and_single_kinds(gpu_ctx, pair):
         dg.close()
 
 
+# Kind pairs with rows of their own lengths: PPR's length-1 rows beside MetaPath's 80 (the bench's
+# mix), rows at most one 16-position chunk long beside an odd length, and lengths either side of a
+# chunk boundary.  Odd splits leave the second kind's rows off 16-byte alignment.
+KIND_ROWS = [
+    (("ppr", {"length": 1}), ("metapath", {"length": 80, "schema": [0, 1, 2, 3, 4]})),
+    (("deepwalk", {"execution": "static", "sampling": "its", "length": 12}),
+     ("metapath", {"length": 37, "schema": [1, 0, 2]})),
+    (("ppr", {"length": 16}), ("metapath", {"length": 17, "schema": [0, 1, 2, 3, 4]})),
+]
+
+
+@pytest.mark.parametrize("unpadded", [False, True], ids=["padded", "unpadded"])
+@pytest.mark.parametrize("pair", KIND_ROWS, ids=["ppr1+metapath80", "its12+metapath37", "ppr16+metapath17"])
+def test_mixed_batch_rows_per_kind(gpu_ctx, pair, unpadded):
+    """io.path_stride_b: each kind's rows as long as its walks.  Every form writes the same rows
+    as with one stride for both kinds, padded or not."""
+    (app_a, kw_a), (app_b, kw_b) = pair
+    g = graphs.chung_lu(20000, 120000, 2.1, 2, 3000, 73)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        live = np.nonzero(g.degree > 0)[0]
+        n = 100003
+        src = np.random.default_rng(8).choice(live, n).astype(np.int32)
+        split = 50001
+        La, Lb = kw_a["length"], kw_b["length"]
+        first = None
+        for form in ("queue", "step", "graph"):
+            a = T.make_params(app_a, stop_probability=0.15, rng="philox", seed=6, form=form, **kw_a)
+            b = T.make_params(app_b, stop_probability=0.15, rng="philox", seed=6, form=form, **kw_b)
+            one = T.walk(dg, a, src, mix=(b, split), unpadded=unpadded)
+            per = T.walk(dg, a, src, mix=(b, split), unpadded=unpadded, kind_rows=True)
+            assert np.array_equal(per["paths_a"], one["paths"][:split, :La]), form
+            assert np.array_equal(per["paths_b"], one["paths"][split:, :Lb]), form
+            for key in ("lengths", "ends", "status"):
+                assert np.array_equal(per[key], one[key]), (form, key)
+            if not unpadded:
+                # Padded rows: the walk, then -1 to the kind's length.
+                for rows, lens in ((per["paths_a"], per["lengths"][:split]), (per["paths_b"], per["lengths"][split:])):
+                    pos = np.arange(rows.shape[1])[None, :]
+                    assert np.all((rows >= 0) == (pos < lens[:, None])), form
+            if first is None:
+                first = per
+            else:
+                for key in ("paths_a", "paths_b", "lengths", "ends", "status"):
+                    assert np.array_equal(per[key], first[key]), (form, key)
+    finally:
+        dg.close()
+
+
 def test_mixed_kinds_on_one_static_table_from_different_weights_are_refused(gpu_ctx):
     """A graph holds one table per sampler: PPR's unit-weight alias table and DeepWalk's
     edge-weight one cannot serve one batch (the second build would replace the first under the

@@ -60,7 +60,7 @@ class WalkIO(C.Structure):
                 ("term_draws", C.c_void_p), ("term_draws_per_walker", C.c_int64), ("paths", C.c_void_p),
                 ("lengths", C.c_void_p), ("ends", C.c_void_p), ("status", C.c_void_p), ("draws_used", C.c_void_p),
                 ("counters", C.c_void_p), ("walker_ids", C.c_void_p), ("path_flags", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("path_stride_b", C.c_int32)]
 
 
 TUNE_KNOBS = 8

@@ -1077,8 +1077,13 @@ static int make_io(const trw_graph *g, const trw_walk_params *p, const trw_walk_
     if (io->n_walkers < 0) return fail(TRW_E_INVALID, "n_walkers must be >= 0");
     if (io->n_walkers > 0 && !io->sources) return fail(TRW_E_INVALID, "sources is null");
     if (io->n_walkers > (1 << 30)) return fail(TRW_E_INVALID, "at most 2^30 walkers per call");
-    const int32_t L = std::max(std::max(a.length, b.length), 1);
-    if (io->paths && io->path_stride < L) return fail(TRW_E_INVALID, "path_stride %d < length %d", io->path_stride, L);
+    const int32_t La = std::max(a.length, 1), Lb = std::max(b.length, 1);
+    if (io->path_stride_b < 0) return fail(TRW_E_INVALID, "path_stride_b must be >= 0");
+    const int32_t stride_b = io->path_stride_b > 0 ? io->path_stride_b : io->path_stride;
+    if (io->paths && io->path_stride_b == 0 && io->path_stride < std::max(La, Lb))
+        return fail(TRW_E_INVALID, "path_stride %d < length %d", io->path_stride, std::max(La, Lb));
+    if (io->paths && io->path_stride < La) return fail(TRW_E_INVALID, "path_stride %d < length %d", io->path_stride, La);
+    if (io->paths && stride_b < Lb) return fail(TRW_E_INVALID, "path_stride_b %d < length %d", stride_b, Lb);
     if (p->rng != TRW_RNG_REPLAY && p->rng != TRW_RNG_PHILOX) return fail(TRW_E_INVALID, "unknown rng %d", p->rng);
     const bool replay = p->rng == TRW_RNG_REPLAY;
     if (replay) {
@@ -1093,6 +1098,7 @@ static int make_io(const trw_graph *g, const trw_walk_params *p, const trw_walk_
     out.sources = io->sources;
     out.n = io->n_walkers;
     out.path_stride = io->path_stride;
+    out.path_stride_b = stride_b;
     out.gid_base = static_cast<uint64_t>(io->walker_id_base);
     out.walker_ids = io->walker_ids;
     if (io->path_flags != TRW_PATHS_PADDED && io->path_flags != TRW_PATHS_UNPADDED)

@@ -21,7 +21,7 @@ namespace {
 struct Sample {
     int32_t *sources = nullptr, *paths = nullptr, *lengths = nullptr;
     uint64_t *counters = nullptr;
-    int32_t n = 0, stride = 0;
+    int32_t n = 0, stride = 0, stride_b = 0;  // mixed sample: each kind's rows at its own length
     ~Sample() {
         (void)hipFree(sources);
         (void)hipFree(paths);
@@ -39,6 +39,7 @@ int time_walk(trw_graph *g, const trw_walk_params &p, const trw_walk_params *pb,
     io.sources = s.sources;
     io.n_walkers = s.n;
     io.path_stride = s.stride;
+    io.path_stride_b = pb ? s.stride_b : 0;
     io.paths = s.paths;
     io.lengths = s.lengths;
     io.counters = s.counters;
@@ -122,12 +123,18 @@ int tune(trw_graph *g, const trw_walk_params *params, const trw_walk_params *pb,
     }
     Sample s;
     s.n = sample_walkers;
-    s.stride = with_paths ? std::max(std::max(params->length, pb ? pb->length : 1), 1) : 0;
+    // A mixed sample writes each kind's rows at its own length (io.path_stride_b), as the mixed
+    // batches trw_walk_mix_async runs with such rows; a single kind, rows of its length.
+    s.stride = with_paths ? std::max(params->length, 1) : 0;
+    s.stride_b = with_paths && pb ? std::max(pb->length, 1) : 0;
     const size_t N = static_cast<size_t>(s.n);
+    const size_t path_ints = pb ? static_cast<size_t>(split) * static_cast<size_t>(s.stride) +
+                                      (N - static_cast<size_t>(split)) * static_cast<size_t>(s.stride_b)
+                                : N * static_cast<size_t>(s.stride);
     if (hipMalloc(reinterpret_cast<void **>(&s.sources), sizeof(int32_t) * N) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&s.lengths), sizeof(int32_t) * N) != hipSuccess ||
         hipMalloc(reinterpret_cast<void **>(&s.counters), sizeof(uint64_t) * 4) != hipSuccess ||
-        (s.stride && hipMalloc(reinterpret_cast<void **>(&s.paths), sizeof(int32_t) * N * s.stride) != hipSuccess))
+        (s.stride && hipMalloc(reinterpret_cast<void **>(&s.paths), sizeof(int32_t) * path_ints) != hipSuccess))
         return trw::set_error(TRW_E_NOMEM, "autotune sample buffers");
     if (hipMemcpy(s.sources, src.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice) != hipSuccess)
         return trw::set_error(TRW_E_HIP, "autotune sources");

@@ -154,6 +154,7 @@ struct BatchIO {
     const int32_t *sources;
     int32_t n;
     int32_t path_stride;
+    int32_t path_stride_b;       // rows of a mixed batch's walkers [split, n) (follow the first split rows)
     uint64_t gid_base;
     const uint64_t *walker_ids;  // per-walker global ids (null: gid_base + index)
     int32_t unpadded;            // path rows are not padded with -1 past the walk (TRW_PATHS_UNPADDED)

@@ -790,6 +790,16 @@ __device__ __forceinline__ void write_outputs(const BatchIO &io, int32_t w, cons
     if (io.draws_used) io.draws_used[w] = static_cast<int32_t>(s.k);
 }
 
+// Path row of walker w in a batch whose walkers [0, split) are one kind and [split, n) another:
+// the second kind's rows (path_stride_b ints each) follow the first kind's split rows
+// (path_stride ints each); with equal strides this is row w of one matrix.
+__device__ __forceinline__ int32_t *path_row(const BatchIO &io, int32_t split, int32_t w) {
+    const int64_t r = w < split ? static_cast<int64_t>(w) * io.path_stride
+                                : static_cast<int64_t>(split) * io.path_stride +
+                                      static_cast<int64_t>(w - split) * io.path_stride_b;
+    return io.paths + r;
+}
+
 __device__ __forceinline__ WalkerState empty_walker() {
     return WalkerState{0, -1, 1, kDone, 0u, 0u, 0u, kNoRange, 0u, kNoRange, 0u, 0u, kNoRange, kNoRange, -1.0};
 }
@@ -1115,6 +1125,19 @@ struct LaneRow {
         for (; j < staged; ++j) dst[j] = mine[j];
         staged = 0;
     }
+    // Padded rows: the staged chunk filled with -1 up to its 16-position boundary (or the row's
+    // end), so that it leaves as one aligned run; the positions after it were set to -1 before the
+    // walk.  Returns the length to flush to.  Nothing staged: the walk ended on a chunk boundary,
+    // and the chunks from there on already hold -1.
+    __device__ __forceinline__ int32_t pad_chunk(int32_t len, int32_t L) {
+        if (staged == 0) return len;
+        const int32_t end = min(len - staged + kN2vChunk, L);
+        while (len < end) {
+            mine[staged++] = -1;
+            ++len;
+        }
+        return len;
+    }
     __device__ __forceinline__ void finish(int32_t len, int32_t L) {
         if (row == nullptr) return;
         flush(len);
@@ -1329,8 +1352,11 @@ __device__ __forceinline__ uint4 first_step_inputs(const DevGraph &g, int32_t mo
 }
 
 // The path rows of queue positions [p0, p1), filled by the wavefront that has just taken them:
-// -1 in [0, L) of a row of length L > 1 (its walker writes its source and moves over them), the
-// source alone in a row of length 1 (PPR's, which its walker then leaves alone).  Consecutive
+// -1 in [16, L) of a padded row of length L > 16 (its walker writes its source and moves over
+// them, and its first 16 positions itself, padded with -1: pad_chunk), nothing in a row of length
+// 2..16 (the walker writes it whole), the source alone in a row of length 1 (PPR's, which its
+// walker then leaves alone).  A walker's last chunk thus leaves as one aligned run instead of a
+// few entries into a line that was filled before (a partial-line write per walker).  Consecutive
 // positions are consecutive rows (position p runs walker p + rot mod n), so consecutive lanes
 // store consecutive 16-byte pieces: about 1 KB per store instruction, where a separate pass over
 // every row before the walk cost 12% of the PPR + MetaPath batch.  A walker's own stores come at
@@ -1342,11 +1368,15 @@ __device__ __forceinline__ void fill_chunk_rows(const BatchIO &io, int32_t split
     const auto walker = [&](unsigned int pos) {
         return static_cast<int32_t>(pos < n - rot ? pos + rot : pos - (n - rot));
     };
-    const bool vec = (io.path_stride & 3) == 0 && (reinterpret_cast<uintptr_t>(io.paths) & 15) == 0;
+    // Each kind's rows are 16-byte aligned when its stride and first row are.
+    const bool base16 = (reinterpret_cast<uintptr_t>(io.paths) & 15) == 0;
+    const bool vec_a = base16 && (io.path_stride & 3) == 0;
+    const bool vec_b = base16 && (io.path_stride_b & 3) == 0 &&
+                       ((static_cast<int64_t>(split) * io.path_stride) & 3) == 0;
     // Pieces per row: L / 4 16-byte pieces (L > 1, L % 4 == 0), L entries (other L > 1), or the
     // source entry (L == 1).
-    const int32_t qa = La == 1 ? 1 : io.unpadded ? 0 : ((La & 3) == 0 && vec ? La >> 2 : La);
-    const int32_t qb = Lb == 1 ? 1 : io.unpadded ? 0 : ((Lb & 3) == 0 && vec ? Lb >> 2 : Lb);
+    const int32_t qa = La == 1 ? 1 : io.unpadded || La <= kN2vChunk ? 0 : ((La & 3) == 0 && vec_a ? La >> 2 : La);
+    const int32_t qb = Lb == 1 ? 1 : io.unpadded || Lb <= kN2vChunk ? 0 : ((Lb & 3) == 0 && vec_b ? Lb >> 2 : Lb);
     // Kinds come in runs of positions; a chunk holds both only if it crosses the position of
     // walker `split` or of walker 0.
     const unsigned int split_pos = static_cast<unsigned int>(split) >= n ? n : (static_cast<unsigned int>(split) + n - rot) % n;
@@ -1360,11 +1390,14 @@ __device__ __forceinline__ void fill_chunk_rows(const BatchIO &io, int32_t split
         const bool kb = w >= split;
         const int32_t L = kb ? Lb : La;
         if (v >= (kb ? qb : qa)) return;
-        int32_t *row = io.paths + static_cast<int64_t>(w) * io.path_stride;
+        int32_t *row = path_row(io, split, w);
         if (L == 1) row[0] = io.sources[w];
         else if (io.unpadded) return;  // the walker writes its source and moves, nothing else
-        else if (vec && (L & 3) == 0) reinterpret_cast<int4 *>(row)[v] = make_int4(-1, -1, -1, -1);
-        else row[v] = -1;
+        else if ((kb ? vec_b : vec_a) && (L & 3) == 0) {
+            if (v >= kN2vChunk / 4) reinterpret_cast<int4 *>(row)[v] = make_int4(-1, -1, -1, -1);
+        } else if (v >= kN2vChunk) {
+            row[v] = -1;
+        }
     };
     if (qmax <= 64) {
         // 64 / qmax rows per pass, lane i on piece i % qmax of row i / qmax.
@@ -1429,12 +1462,12 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
             if (io.paths) {
                 // Position 0 already holds the source: a length-1 row (PPR's) is complete; a longer
                 // one restates it so that its first run of stores starts on the row's alignment.
-                out.row = length > 1 ? io.paths + static_cast<int64_t>(w) * io.path_stride : nullptr;
+                out.row = length > 1 ? path_row(io, split, w) : nullptr;
                 out.staged = 0;
                 out.put(s.cur, 1, length);
             }
             if (s.st != kAlive) {  // bad source: finished before the first step
-                if (out.row) out.flush(1);
+                if (out.row) out.flush(io.unpadded ? 1 : out.pad_chunk(1, length));
                 write_outputs(io, w, s);
                 classify(s.st, gterm, dead, err);
             }
@@ -1506,7 +1539,10 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                 if (out.row && s.len - 1 < length) out.put(s.cur, s.len, length);
             }
             if (s.st != kAlive) {
-                if (out.row) out.flush(s.len < length ? s.len : length);
+                if (out.row) {
+                    const int32_t e = s.len < length ? s.len : length;
+                    out.flush(io.unpadded ? e : out.pad_chunk(e, length));
+                }
                 write_outputs(io, w, s);
                 classify(s.st, gterm, dead, err);
             }
@@ -1524,7 +1560,7 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_init_kernel(DevGraph g, 
     WalkerState s;
     init_walker(g, io, w, s);
     state[w] = s;
-    if (io.paths) io.paths[static_cast<int64_t>(w) * io.path_stride] = s.cur;
+    if (io.paths) path_row(io, split, w)[0] = s.cur;
 }
 
 template <int MODE, bool REPLAY>
@@ -1540,8 +1576,7 @@ __global__ __launch_bounds__(kStepBlock) void walk_step_kernel(DevGraph g, KindP
             uint4 blk = first_block<REPLAY>(io, w, s);
             const bool moved =
                 kind_b ? advance<MODE, REPLAY>(g, B, io, w, s, blk) : advance<MODE, REPLAY>(g, A, io, w, s, blk);
-            if (moved && io.paths && s.len - 1 < plen)
-                io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
+            if (moved && io.paths && s.len - 1 < plen) path_row(io, split, w)[s.len - 1] = s.cur;
             still = s.st == kAlive;
             state[w] = s;
         }
@@ -1601,7 +1636,7 @@ __global__ __launch_bounds__(kStepBlock) void walk_graph_kernel(DevGraph g, Kind
             if (first) {
                 w = static_cast<int32_t>(i);
                 init_walker(g, io, w, s);
-                if (io.paths && io.unpadded) io.paths[static_cast<int64_t>(w) * io.path_stride] = s.cur;
+                if (io.paths && io.unpadded) path_row(io, split, w)[0] = s.cur;
             } else {
                 w = list_in[i];
                 s = state[w];
@@ -1614,7 +1649,7 @@ __global__ __launch_bounds__(kStepBlock) void walk_graph_kernel(DevGraph g, Kind
                                           : advance<MODE, REPLAY, false>(g, A, io, w, s, blk);
                 if (moved) {
                     ++steps;
-                    if (io.paths && s.len - 1 < plen) io.paths[static_cast<int64_t>(w) * io.path_stride + s.len - 1] = s.cur;
+                    if (io.paths && s.len - 1 < plen) path_row(io, split, w)[s.len - 1] = s.cur;
                 }
             }
             keep = s.st == kAlive;
@@ -1841,7 +1876,7 @@ void fill_paths(const KindParams &a, const KindParams &b, int32_t split, const B
     const int32_t La = a.length > 0 ? a.length : 1, Lb = b.length > 0 ? b.length : 1;
     const int32_t sa = split < io.n ? split : io.n;
     fill_source_rows(io.paths, io.sources, sa, io.path_stride, La, stream);
-    fill_source_rows(io.paths + static_cast<int64_t>(sa) * io.path_stride, io.sources + sa, io.n - sa, io.path_stride,
+    fill_source_rows(io.paths + static_cast<int64_t>(sa) * io.path_stride, io.sources + sa, io.n - sa, io.path_stride_b,
                      Lb, stream);
 }
 

@@ -279,7 +279,7 @@ def make_params(app, execution=None, sampling=None, length=None, stop_probabilit
 
 def make_io(sources, n_walkers, path_stride=0, walker_id_base=0, draws=None, draws_per_walker=0, term_draws=None,
             term_draws_per_walker=0, paths=None, lengths=None, ends=None, status=None, draws_used=None,
-            counters=None, walker_ids=None, unpadded=False):
+            counters=None, walker_ids=None, unpadded=False, path_stride_b=0):
     io = A.WalkIO()
     io.sources = _ptr(sources)
     io.n_walkers = int(n_walkers)
@@ -293,6 +293,7 @@ def make_io(sources, n_walkers, path_stride=0, walker_id_base=0, draws=None, dra
     io.status, io.draws_used, io.counters = _ptr(status), _ptr(draws_used), _ptr(counters)
     io.walker_ids = _ptr(walker_ids)
     io.path_flags = A.PATHS_UNPADDED if unpadded else A.PATHS_PADDED
+    io.path_stride_b = int(path_stride_b)
     return io
 
 
@@ -403,7 +404,7 @@ class DeviceTimer:
 
 
 def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, walker_id_base=0, stream=None,
-         mix=None, walker_ids=None, noalloc=False, unpadded=False):
+         mix=None, walker_ids=None, noalloc=False, unpadded=False, kind_rows=False):
     """Runs one batch in HBM buffers and returns numpy results.
 
     draws / term_draws: replay streams in the [k][walker] layout, or None (Philox).
@@ -412,6 +413,8 @@ def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, w
     noalloc: run the batch through trw_walk_noalloc instead of trw_walk_async.
     unpadded: TRW_PATHS_UNPADDED -- entries past a walker's length are left as they were (the
     buffer starts filled with -2 here).
+    kind_rows: with mix, each kind's rows as long as its own walks (io.path_stride_b); the
+    result then holds paths_a (split x length_a) and paths_b ((n - split) x length_b).
     """
     ctx = graph.ctx
     src_h = np.ascontiguousarray(sources, dtype=np.int32)
@@ -431,15 +434,21 @@ def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, w
         K = 0 if draws is None else draws.shape[0]
         KT = 0 if term_draws is None else term_draws.shape[0]
         m = max(n, 1)
-        paths = up(np.full((m, L), -2, np.int32)) if want_paths else None
+        per_kind = bool(mix) and kind_rows
+        if per_kind:
+            split = int(mix[1])
+            La, Lb = max(params.length, 1), max(mix[0].length, 1)
+            paths = up(np.full(max(split * La + (n - split) * Lb, 1), -2, np.int32)) if want_paths else None
+        else:
+            paths = up(np.full((m, L), -2, np.int32)) if want_paths else None
         lengths = up(np.zeros(m, np.int32))
         ends = up(np.zeros(m, np.int32))
         status = up(np.full(m, -9, np.int32))
         used = up(np.zeros(m, np.int32))
         counters = up(np.zeros(4, np.uint64))
         ids = None if walker_ids is None else up(np.ascontiguousarray(walker_ids, dtype=np.uint64))
-        io = make_io(src, n, L, walker_id_base, d_draws, K, d_tdraws, KT, paths, lengths, ends, status, used,
-                     counters, walker_ids=ids, unpadded=unpadded)
+        io = make_io(src, n, La if per_kind else L, walker_id_base, d_draws, K, d_tdraws, KT, paths, lengths, ends,
+                     status, used, counters, walker_ids=ids, unpadded=unpadded, path_stride_b=Lb if per_kind else 0)
         if mix is None:
             (walk_noalloc if noalloc else walk_async)(graph, params, io, stream)
         else:
@@ -448,7 +457,13 @@ def walk(graph, params, sources, draws=None, term_draws=None, want_paths=True, w
             walk_mix_async(graph, params, mix[0], mix[1], io, stream)
         synchronize(ctx, stream)
         c = counters.to_numpy(np.uint64, 4)
-        return dict(paths=None if paths is None else paths.to_numpy(np.int32, (n, L)),
+        if per_kind:
+            flat = None if paths is None else paths.to_numpy(np.int32, split * La + (n - split) * Lb)
+            extra = dict(paths_a=None if flat is None else flat[:split * La].reshape(split, La),
+                         paths_b=None if flat is None else flat[split * La:].reshape(n - split, Lb))
+        else:
+            extra = {}
+        return dict(paths=None if paths is None or per_kind else paths.to_numpy(np.int32, (n, L)), **extra,
                     lengths=lengths.to_numpy(np.int32, n), ends=ends.to_numpy(np.int32, n),
                     status=status.to_numpy(np.int32, n), draws_used=used.to_numpy(np.int32, n),
                     steps=int(c[0]), gather_terminated=int(c[1]), dead_ends=int(c[2]), errors=int(c[3]))

@@ -405,7 +405,7 @@ int trw_graph_upload(trw_context *ctx, int32_t nv, int64_t ne, const int32_t *de
         bool small = g->erec != nullptr;
         for (size_t e = 0; e < E && small; ++e) small = edge_label[e] >= 0 && edge_label[e] < 8;
         if (small) {
-            TRW_HIP(dmalloc(&g->lstart, 8 * V + 8), "hipMalloc label starts");
+            TRW_HIP(dmalloc(&g->lstart, static_cast<size_t>(trw::kLabelRecWords) * (V + 1)), "hipMalloc label records");
             TRW_HIP(dmalloc(&g->lerec, 4 * E + 4), "hipMalloc label records");
             TRW_HIP(trw::launch_label_records(g->view(), g->lstart, g->lerec, ctx->stream), "label records");
             TRW_HIP(hipStreamSynchronize(ctx->stream), "label records");

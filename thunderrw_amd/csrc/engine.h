@@ -57,8 +57,10 @@ struct DevGraph {
     const uint4 *erec;         // ne: neighbour records {adj[e], first edge of adj[e], degree of adj[e], 0}
     const uint4 *arec;         // 2*ne: alias walk records {prob lo, prob hi, first, second}
                                //       {first's edge, first's degree, second's edge, second's degree}
-    const uint32_t *lstart;    // 8*nv when every label is in [0, 8): lstart[8v + k] = neighbours of v
-                               //       with label < k (the start of label k in label order), else null
+    const uint32_t *lstart;    // kLabelRecWords*nv when every label is in [0, 8), else null: v's 64-byte
+                               //       label record, words k = 0..7 the neighbours of v with label < k
+                               //       (the start of label k in label order), word 8 v's first edge,
+                               //       word 9 its degree (a MetaPath walk's first step reads one line)
     const uint4 *lerec;        // 4*ne with lstart: 64-byte neighbour records in label order, entry j of v:
                                //       lerec[4 (off(v) + j)] = erec[off(v) + lbl_pos[off(v) + j]], then the
                                //       neighbour's 8 label starts (2 uint4), then 16 bytes of zero
@@ -193,6 +195,7 @@ struct WalkerState {
     double cmax;      // rejection: the current vertex's maximum weight (from a rejection record), < 0: not known
 };
 constexpr uint32_t kNoRange = 0xffffffffu;
+constexpr int64_t kLabelRecWords = 16;  // DevGraph::lstart words per vertex
 
 // Kernel launchers (walk.hip / tables.hip).  All asynchronous on `stream`.
 hipError_t launch_walk_fused(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream);

@@ -531,14 +531,16 @@ __global__ void edge_records_kernel(const int32_t *adj, const uint2 *vrec, int64
     }
 }
 
-// Label starts: for each vertex the number of neighbours with label < k, k = 0..7,
-// by binary search of its label-sorted segment.
+// Label records: for each vertex the number of neighbours with label < k, k = 0..7, by binary
+// search of its label-sorted segment, then its first edge and degree (num_edges < 2^32), padded
+// to 64 bytes: the first MetaPath step from v reads its range and label range from one line.
 __global__ void label_start_kernel(const int64_t *offset, const int32_t *lbl_sorted, int32_t nv, uint32_t *lstart) {
     const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (v >= nv) return;
     const int64_t off = offset[v];
     const int32_t d = static_cast<int32_t>(offset[v + 1] - off);
     const int32_t *ls = lbl_sorted + off;
+    uint32_t k8[8];
     for (int32_t k = 0; k < 8; ++k) {
         int32_t lo = 0, hi = d;
         while (lo < hi) {
@@ -546,8 +548,13 @@ __global__ void label_start_kernel(const int64_t *offset, const int32_t *lbl_sor
             if (ls[m] < k) lo = m + 1;
             else hi = m;
         }
-        lstart[8 * static_cast<int64_t>(v) + k] = static_cast<uint32_t>(lo);
+        k8[k] = static_cast<uint32_t>(lo);
     }
+    uint4 *rec = reinterpret_cast<uint4 *>(lstart + kLabelRecWords * static_cast<int64_t>(v));
+    rec[0] = make_uint4(k8[0], k8[1], k8[2], k8[3]);
+    rec[1] = make_uint4(k8[4], k8[5], k8[6], k8[7]);
+    rec[2] = make_uint4(static_cast<uint32_t>(off), static_cast<uint32_t>(d), 0u, 0u);
+    rec[3] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // Neighbour records in label order, 64 bytes each: entry j of v's label-sorted segment is the
@@ -565,7 +572,7 @@ __global__ void label_records_kernel(const int64_t *offset, int32_t nv, const in
             else hi = m;
         }
         const uint4 r = erec[offset[lo] + lbl_pos[e]];
-        const uint4 *ls = reinterpret_cast<const uint4 *>(lstart + 8 * static_cast<int64_t>(r.x));
+        const uint4 *ls = reinterpret_cast<const uint4 *>(lstart + kLabelRecWords * static_cast<int64_t>(r.x));
         lerec[4 * e] = r;
         lerec[4 * e + 1] = ls[0];
         lerec[4 * e + 2] = ls[1];

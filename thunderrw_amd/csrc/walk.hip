@@ -450,8 +450,18 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
     // no buckets at a vertex of positive degree (first bucket kNoRange) means weights summing to
     // zero, which ends the walk before a draw as the total check below does.
     const bool guide = MODE == kModeIts && g.grec != nullptr && P.its_guide;
-    if (guide) guide_range(g, cur, s.coff, s.cdeg, off, d);
-    else known_range(g, cur, s.coff, s.cdeg, off, d);
+    // MetaPath over label records: a walker that carries nothing reads its range from the current
+    // vertex's label record, the line its label range comes from below.
+    const bool label_rec = MODE == kModeMetaPath && g.lerec != nullptr && s.cdeg == kNoRange;
+    if (guide) {
+        guide_range(g, cur, s.coff, s.cdeg, off, d);
+    } else if (label_rec) {
+        const uint2 r = *reinterpret_cast<const uint2 *>(g.lstart + kLabelRecWords * static_cast<int64_t>(cur) + 8);
+        off = static_cast<int64_t>(r.x);
+        d = static_cast<int32_t>(r.y);
+    } else {
+        known_range(g, cur, s.coff, s.cdeg, off, d);
+    }
     if (d <= 0) {
         s.st = (guide && off == static_cast<int64_t>(kNoRange)) ? kGatherTerminated : kDeadEnd;
         return false;
@@ -662,7 +672,7 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
                 lo = s.poff;
                 hi = s.pdeg;
             } else if (target < 8) {
-                const uint32_t *L = g.lstart + 8 * static_cast<int64_t>(cur);
+                const uint32_t *L = g.lstart + kLabelRecWords * static_cast<int64_t>(cur);
                 lo = L[target];
                 hi = target < 7 ? L[target + 1] : ud;
             }
@@ -1327,22 +1337,28 @@ __device__ __forceinline__ uint4 first_step_inputs(const DevGraph &g, int32_t mo
         r.y = v.y;
         return r;
     }
+    if (mode == kModeMetaPath && g.lerec) {
+        // The source's label record: its range and the label range of schema[0], one line.
+        const uint32_t *L = g.lstart + kLabelRecWords * static_cast<int64_t>(src);
+        const uint2 v = *reinterpret_cast<const uint2 *>(L + 8);
+        r.x = v.x;
+        r.y = v.y;
+        const int32_t t = schema0;  // the first schema label as a nibble (8: outside [0, 8))
+        if (t < 8) {
+            r.z = L[t];
+            r.w = t < 7 ? L[t + 1] : kNoRange;  // label 7 ends at the degree (.y)
+        } else {
+            r.z = 0u;
+            r.w = 0u;  // no neighbour carries a label outside [0, 8)
+        }
+        return r;
+    }
     if (g.vrec) {
         const uint2 v = g.vrec[src];
         r.x = v.x;
         r.y = v.y;
     }
-    if (mode == kModeMetaPath && g.lerec) {
-        const int32_t t = schema0;  // the first schema label as a nibble (8: outside [0, 8))
-        if (t < 8) {
-            const uint32_t *L = g.lstart + 8 * static_cast<int64_t>(src);
-            r.z = L[t];
-            r.w = t < 7 ? L[t + 1] : kNoRange;  // label 7 ends at the degree (the vertex record's .y)
-        } else {
-            r.z = 0u;
-            r.w = 0u;  // no neighbour carries a label outside [0, 8)
-        }
-    } else if (mode == kModeRejection && g.rrec) {
+    if (mode == kModeRejection && g.rrec) {
         const double m = g.rs_max[src];
         const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(m));
         r.z = static_cast<uint32_t>(b);

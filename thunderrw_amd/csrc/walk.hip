@@ -698,12 +698,12 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
             ndeg = r.z;
             // The next step's label range at `next`, from its label starts in the same record.
             const int32_t nt = static_cast<int32_t>((P.schema_nibbles >> (4u * npos)) & 15u);
-            nlo = nhi = 0u;
-            if (nt < 8) {
-                const uint32_t *NL = reinterpret_cast<const uint32_t *>(R + 1);
-                nlo = NL[nt];
-                nhi = nt < 7 ? NL[nt + 1] : ndeg;
-            }
+            // Both label starts are loaded with the record (clamped indices, then value selects):
+            // a load that waits on the record's degree would be a second round trip.
+            const uint32_t *NL = reinterpret_cast<const uint32_t *>(R + 1);
+            const uint32_t l0 = NL[nt < 8 ? nt : 0], l1 = NL[nt < 7 ? nt + 1 : 7];
+            nlo = nt < 8 ? l0 : 0u;
+            nhi = nt < 7 ? l1 : (nt == 7 ? ndeg : 0u);
         } else {
             next = read_neighbor(g, off + g.lbl_pos[off + a + t], noff, ndeg);
         }
@@ -737,6 +737,10 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
 constexpr int pair_mode(int a, int b) { return -2 - (a < b ? a * 8 + b : b * 8 + a); }
 constexpr int pair_first(int mode) { return (-2 - mode) / 8; }
 constexpr int pair_second(int mode) { return (-2 - mode) % 8; }
+// Whether a kernel compiled for MODE (a Move, a pair of Moves, or -1: every Move) runs Move m.
+constexpr bool runs_move(int MODE, int m) {
+    return MODE == -1 || MODE == m || (MODE <= -2 && (pair_first(MODE) == m || pair_second(MODE) == m));
+}
 
 template <int MODE, bool REPLAY, bool AHEAD = true>
 __device__ __forceinline__ bool advance(const DevGraph &g, const KindParams &P, const BatchIO &io, int32_t w,
@@ -793,11 +797,16 @@ __device__ __forceinline__ void classify(int32_t st, unsigned long long &gterm, 
     err += (st == kReplayExhausted || st == kTrialCap || st == kBadSource);
 }
 
+__device__ __forceinline__ void write_outputs(const BatchIO &io, int32_t w, int32_t len, int32_t cur, int32_t st,
+                                              uint32_t k) {
+    if (io.lengths) io.lengths[w] = len;
+    if (io.ends) io.ends[w] = cur;
+    if (io.status) io.status[w] = st;
+    if (io.draws_used) io.draws_used[w] = static_cast<int32_t>(k);
+}
+
 __device__ __forceinline__ void write_outputs(const BatchIO &io, int32_t w, const WalkerState &s) {
-    if (io.lengths) io.lengths[w] = s.len;
-    if (io.ends) io.ends[w] = s.cur;
-    if (io.status) io.status[w] = s.st;
-    if (io.draws_used) io.draws_used[w] = static_cast<int32_t>(s.k);
+    write_outputs(io, w, s.len, s.cur, s.st, s.k);
 }
 
 // Path row of walker w in a batch whose walkers [0, split) are one kind and [split, n) another:
@@ -1328,29 +1337,30 @@ __global__ __launch_bounds__(BLOCK) void walk_node2vec_sm_kernel(DevGraph g, Kin
 // (poff/pdeg) or the rejection maximum's bits (cmax); kNoRange where the step reads it itself.
 // (The kind's mode, ITS guide switch and first schema label come as values: a reference to one of
 // two kernel-argument structures chosen at run time puts both in scratch memory.)
+// Only the Moves the kernel runs (MODE) are compiled in, so the graph arrays of the others hold
+// no registers.
+template <int MODE>
 __device__ __forceinline__ uint4 first_step_inputs(const DevGraph &g, int32_t mode, int32_t its_guide,
                                                    int32_t schema0, int32_t src) {
     uint4 r = make_uint4(0u, kNoRange, 0u, kNoRange);
-    if (mode == kModeIts && g.grec != nullptr && its_guide) {
+    if (runs_move(MODE, kModeIts) && mode == kModeIts && g.grec != nullptr && its_guide) {
         const uint2 v = g.gvrec[src];
         r.x = v.x;
         r.y = v.y;
         return r;
     }
-    if (mode == kModeMetaPath && g.lerec) {
+    if (runs_move(MODE, kModeMetaPath) && mode == kModeMetaPath && g.lerec) {
         // The source's label record: its range and the label range of schema[0], one line.
         const uint32_t *L = g.lstart + kLabelRecWords * static_cast<int64_t>(src);
         const uint2 v = *reinterpret_cast<const uint2 *>(L + 8);
         r.x = v.x;
         r.y = v.y;
-        const int32_t t = schema0;  // the first schema label as a nibble (8: outside [0, 8))
-        if (t < 8) {
-            r.z = L[t];
-            r.w = t < 7 ? L[t + 1] : kNoRange;  // label 7 ends at the degree (.y)
-        } else {
-            r.z = 0u;
-            r.w = 0u;  // no neighbour carries a label outside [0, 8)
-        }
+        // schema[0] as a nibble (8: outside [0, 8), which no neighbour carries: an empty range);
+        // label 7 ends at the degree (.y, kNoRange here).  Loads at clamped indices, then selects.
+        const int32_t t = schema0;
+        const uint32_t l0 = L[t < 8 ? t : 0], l1 = L[t < 7 ? t + 1 : 7];
+        r.z = t < 8 ? l0 : 0u;
+        r.w = t < 7 ? l1 : (t == 7 ? kNoRange : 0u);
         return r;
     }
     if (g.vrec) {
@@ -1358,7 +1368,7 @@ __device__ __forceinline__ uint4 first_step_inputs(const DevGraph &g, int32_t mo
         r.x = v.x;
         r.y = v.y;
     }
-    if (mode == kModeRejection && g.rrec) {
+    if (runs_move(MODE, kModeRejection) && mode == kModeRejection && g.rrec) {
         const double m = g.rs_max[src];
         const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(m));
         r.z = static_cast<uint32_t>(b);
@@ -1427,6 +1437,77 @@ __device__ __forceinline__ void fill_chunk_rows(const BatchIO &io, int32_t split
     }
 }
 
+// A lane's path row in the QUEUE form.  Entries gather in the lane's LDS row; a full 16-position
+// chunk, or a finished walk's last one, is handed over rather than stored, and goes out in the
+// lane's next iteration after that iteration's loads (first-step inputs, the refill's queue, source
+// and id reads) and just before its gather (drain).  Vector memory operations complete in issue
+// order and one counter tracks loads and stores alike, so every wait covers the stores issued
+// before it: stored at the end of an iteration, a run held up the next iteration's first wait
+// (for the prepared walker's inputs) by its write latency; drained before the gather, it completes
+// in the gather's shadow.
+struct QueueRow {
+    int32_t *row;    // the walker's row, or null when paths are not written
+    int32_t *mine;   // the lane's LDS staging row (kN2vChunk + 1 ints)
+    int32_t staged;
+    int32_t *pdst;   // the handed-over run: destination and entries (0: none)
+    int32_t pcnt;
+    __device__ __forceinline__ void hand_over(int32_t len) {
+        pdst = row + (len - staged);
+        pcnt = staged;
+        staged = 0;
+    }
+    __device__ __forceinline__ void put(int32_t v, int32_t len, int32_t L) {
+        if (row == nullptr) return;
+        mine[staged++] = v;
+        if (staged == kN2vChunk || len == L) hand_over(len);
+    }
+    // Padded rows: the staged chunk filled with -1 to its 16-position boundary (or the row's end),
+    // as LaneRow::pad_chunk.
+    __device__ __forceinline__ int32_t pad_chunk(int32_t len, int32_t L) {
+        if (staged == 0) return len;
+        const int32_t end = min(len - staged + kN2vChunk, L);
+        while (len < end) {
+            mine[staged++] = -1;
+            ++len;
+        }
+        return len;
+    }
+    // The handed-over run: entries before the first 16-byte boundary one by one, then 16-byte
+    // stores, then the rest.
+    __device__ __forceinline__ void drain() {
+        if (pcnt == 0) return;
+        int32_t j = 0;
+        while (j < pcnt && (reinterpret_cast<uintptr_t>(pdst + j) & 15u) != 0u) {
+            pdst[j] = mine[j];
+            ++j;
+        }
+        for (; j + 4 <= pcnt; j += 4)
+            *reinterpret_cast<uint4 *>(pdst + j) =
+                make_uint4(static_cast<uint32_t>(mine[j]), static_cast<uint32_t>(mine[j + 1]),
+                           static_cast<uint32_t>(mine[j + 2]), static_cast<uint32_t>(mine[j + 3]));
+        for (; j < pcnt; ++j) pdst[j] = mine[j];
+        pcnt = 0;
+    }
+};
+
+// A finished walker's outputs, stored with the lane's next drain (as its last path run).
+struct PendingOutputs {
+    int32_t w = -1, len = 0, cur = 0, st = 0;
+    uint32_t k = 0u;
+    __device__ __forceinline__ void hold(int32_t walker, const WalkerState &s) {
+        w = walker;
+        len = s.len;
+        cur = s.cur;
+        st = s.st;
+        k = s.k;
+    }
+    __device__ __forceinline__ void drain(const BatchIO &io) {
+        if (w < 0) return;
+        write_outputs(io, w, len, cur, st, k);
+        w = -1;
+    }
+};
+
 template <int MODE, bool REPLAY>
 __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, KindParams A, KindParams B,
                                                                   int32_t split, BatchIO io, unsigned int *queue,
@@ -1448,10 +1529,12 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
     int32_t stage = 0, nw = -1, nsrc = 0;
     uint32_t ngid_lo = 0u, ngid_hi = 0u;
     uint4 nin = make_uint4(0u, kNoRange, 0u, kNoRange);
-    // The lane's walker's path entries gather here and go out in runs (its row was set to -1 when
-    // the wavefront took its chunk), instead of one store request per entry.
+    // The lane's walker's path entries gather here and go out in runs (its row was set to -1 past
+    // its first chunk when the wavefront took its chunk), instead of one store request per entry.
     __shared__ int32_t pchunk[kQueueBlock * (kN2vChunk + 1)];
-    LaneRow out{nullptr, pchunk + threadIdx.x * (kN2vChunk + 1), 0};
+    QueueRow out{nullptr, pchunk + threadIdx.x * (kN2vChunk + 1), 0, nullptr, 0};
+    PendingOutputs done;
+    bool fresh = false;  // a walker taken this iteration: its source is staged after the drain
 
     for (;;) {
         // 1. A lane whose walker has ended takes the prepared one.
@@ -1467,24 +1550,26 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
             if (s.st == kAlive) {
                 s.coff = nin.x;
                 s.cdeg = nin.y;
-                if (mode == kModeMetaPath && g.lerec) {
+                if (runs_move(MODE, kModeMetaPath) && mode == kModeMetaPath && g.lerec) {
                     s.poff = nin.z;
                     s.pdeg = nin.w != kNoRange ? nin.w : nin.y;
-                } else if (mode == kModeRejection && g.rrec) {
+                } else if (runs_move(MODE, kModeRejection) && mode == kModeRejection && g.rrec) {
                     s.cmax = __hiloint2double(static_cast<int>(nin.w), static_cast<int>(nin.z));
                 }
             }
             stage = 0;
-            if (io.paths) {
-                // Position 0 already holds the source: a length-1 row (PPR's) is complete; a longer
-                // one restates it so that its first run of stores starts on the row's alignment.
-                out.row = length > 1 ? path_row(io, split, w) : nullptr;
-                out.staged = 0;
-                out.put(s.cur, 1, length);
-            }
+            // Position 0 already holds the source in a length-1 row (PPR's), which is then complete;
+            // a longer row restates it so that its first run starts on the row's alignment.
+            out.row = io.paths && length > 1 ? path_row(io, split, w) : nullptr;
+            out.staged = 0;
+            fresh = true;
             if (s.st != kAlive) {  // bad source: finished before the first step
-                if (out.row) out.flush(io.unpadded ? 1 : out.pad_chunk(1, length));
-                write_outputs(io, w, s);
+                out.drain();       // the previous walker's run and outputs first (the LDS row is reused)
+                done.drain(io);
+                out.put(s.cur, 1, length);
+                if (out.row && out.staged > 0) out.hand_over(io.unpadded ? 1 : out.pad_chunk(1, length));
+                fresh = false;
+                done.hold(w, s);
                 classify(s.st, gterm, dead, err);
             }
         }
@@ -1494,7 +1579,7 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
         if (stage == 1) {
             const bool nb = nw >= split;
             if (nsrc >= 0 && nsrc < g.nv)
-                nin = first_step_inputs(g, nb ? B.mode : A.mode, nb ? B.its_guide : A.its_guide,
+                nin = first_step_inputs<MODE>(g, nb ? B.mode : A.mode, nb ? B.its_guide : A.its_guide,
                                         static_cast<int32_t>((nb ? B.schema_nibbles : A.schema_nibbles) & 15u), nsrc);
             stage = 2;
         }
@@ -1539,7 +1624,16 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
             }
         }
 
-        // 3. One step of the lane's walker.
+        // 3. Last iteration's stores, issued after this iteration's loads; then a new walker's
+        // source into its staging row (which the drain has just read).
+        out.drain();
+        done.drain(io);
+        if (fresh) {
+            out.put(s.cur, 1, kind_b ? B.length : A.length);
+            fresh = false;
+        }
+
+        // 4. One step of the lane's walker.
         const bool active = s.st == kAlive;
         if (__ballot(active || stage != 0) == 0) break;  // nothing walks, nothing is prepared: drained
         if (active) {
@@ -1555,15 +1649,19 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                 if (out.row && s.len - 1 < length) out.put(s.cur, s.len, length);
             }
             if (s.st != kAlive) {
-                if (out.row) {
+                // The last chunk (unless put has just handed a full one over), padded to its
+                // boundary in padded rows; stored with the outputs at the next drain.
+                if (out.row && out.staged > 0) {
                     const int32_t e = s.len < length ? s.len : length;
-                    out.flush(io.unpadded ? e : out.pad_chunk(e, length));
+                    out.hand_over(io.unpadded ? e : out.pad_chunk(e, length));
                 }
-                write_outputs(io, w, s);
+                done.hold(w, s);
                 classify(s.st, gterm, dead, err);
             }
         }
     }
+    out.drain();
+    done.drain(io);
     flush_counters(io, steps, gterm, dead, err);
 }
 

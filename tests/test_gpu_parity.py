@@ -415,10 +415,13 @@ MIX_PAIRS = [
     (("ppr", {"execution": "static", "sampling": "alias"}), ("deepwalk", {"execution": "static", "sampling": "rejection"})),
     (("deepwalk", {"execution": "static", "sampling": "its"}), ("metapath", {"schema": [1, 0, 2]})),
     (("ppr", {"execution": "static", "sampling": "alias"}), ("node2vec", {"p": 2.0, "q": 0.5})),
+    # The first kind's Move after the second's: the pair kernel runs walkers [0, split) as its second Move.
+    (("metapath", {"schema": [0, 1, 2, 3, 4]}), ("ppr", {})),
 ]
 
 
-@pytest.mark.parametrize("pair", MIX_PAIRS, ids=["ppr+metapath", "alias+rejection", "its+metapath", "alias+node2vec"])
+@pytest.mark.parametrize("pair", MIX_PAIRS,
+                         ids=["ppr+metapath", "alias+rejection", "its+metapath", "alias+node2vec", "metapath+ppr"])
 def test_mixed_batch_queue_equals_step_and_single_kinds(gpu_ctx, pair):
     (app_a, kw_a), (app_b, kw_b) = pair
     g = graphs.chung_lu(30000, 200000, 2.1, 2, 4000, 71)
@@ -459,11 +462,13 @@ KIND_ROWS = [
     (("deepwalk", {"execution": "static", "sampling": "its", "length": 12}),
      ("metapath", {"length": 37, "schema": [1, 0, 2]})),
     (("ppr", {"length": 16}), ("metapath", {"length": 17, "schema": [0, 1, 2, 3, 4]})),
+    (("metapath", {"length": 80, "schema": [0, 1, 2, 3, 4]}), ("ppr", {"length": 1})),
 ]
 
 
 @pytest.mark.parametrize("unpadded", [False, True], ids=["padded", "unpadded"])
-@pytest.mark.parametrize("pair", KIND_ROWS, ids=["ppr1+metapath80", "its12+metapath37", "ppr16+metapath17"])
+@pytest.mark.parametrize("pair", KIND_ROWS,
+                         ids=["ppr1+metapath80", "its12+metapath37", "ppr16+metapath17", "metapath80+ppr1"])
 def test_mixed_batch_rows_per_kind(gpu_ctx, pair, unpadded):
     """io.path_stride_b: each kind's rows as long as its walks.  Every form writes the same rows
     as with one stride for both kinds, padded or not."""

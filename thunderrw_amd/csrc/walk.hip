@@ -1508,10 +1508,15 @@ struct PendingOutputs {
     }
 };
 
+// A and B are the batch's two kinds; walkers [0, split) run A unless `swapped`, when they run B.
+// A pair kernel (MODE <= -2) is launched with A the kind of the pair's first Move and B that of its
+// second, so each kind's Move is known at compile time: two Move paths instead of four, and only
+// the parameters those read held in registers.
 template <int MODE, bool REPLAY>
 __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, KindParams A, KindParams B,
-                                                                  int32_t split, BatchIO io, unsigned int *queue,
-                                                                  unsigned int chunk, unsigned int rot) {
+                                                                  int32_t split, int32_t swapped, BatchIO io,
+                                                                  unsigned int *queue, unsigned int chunk,
+                                                                  unsigned int rot) {
     const int lane = threadIdx.x & 63;
     WalkerState s = empty_walker();
     uint4 blk = make_uint4(0u, 0u, 0u, 0u);
@@ -1540,7 +1545,7 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
         // 1. A lane whose walker has ended takes the prepared one.
         if (s.st != kAlive && stage == 2) {
             w = nw;
-            kind_b = w >= split;
+            kind_b = (w >= split) != (swapped != 0);
             const int32_t mode = kind_b ? B.mode : A.mode, length = kind_b ? B.length : A.length;
             s = empty_walker();
             s.gid_lo = ngid_lo;
@@ -1577,7 +1582,7 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
         // 2. Preparation, one stage per iteration; each stage's loads are issued before this
         // iteration's gather and have arrived when it has.
         if (stage == 1) {
-            const bool nb = nw >= split;
+            const bool nb = (nw >= split) != (swapped != 0);
             if (nsrc >= 0 && nsrc < g.nv)
                 nin = first_step_inputs<MODE>(g, nb ? B.mode : A.mode, nb ? B.its_guide : A.its_guide,
                                         static_cast<int32_t>((nb ? B.schema_nibbles : A.schema_nibbles) & 15u), nsrc);
@@ -1604,8 +1609,11 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
                 const unsigned int used = cnt > avail ? cnt - avail : 0u;
                 qend = min(fresh + chunk, n);
                 qnext = min(fresh + used, qend);
-                if (io.paths) fill_chunk_rows(io, split, A.length > 0 ? A.length : 1, B.length > 0 ? B.length : 1, fresh,
-                                              qend, rot, lane);
+                if (io.paths) {
+                    // Row lengths of walkers [0, split) and [split, n).
+                    const int32_t la = A.length > 0 ? A.length : 1, lb = B.length > 0 ? B.length : 1;
+                    fill_chunk_rows(io, split, swapped ? lb : la, swapped ? la : lb, fresh, qend, rot, lane);
+                }
             } else {
                 qnext += min(cnt, avail);
                 // No chunk was left: the queue is empty once this wavefront's own chunk is.
@@ -1642,8 +1650,14 @@ __global__ __launch_bounds__(kQueueBlock) void walk_queue_kernel(DevGraph g, Kin
             // just taken a new walker needs no block of its own (in a wavefront every path any lane
             // takes costs every lane).
             blk = first_block<REPLAY>(io, w, s);
-            const bool moved = kind_b ? advance<MODE, REPLAY, false>(g, B, io, w, s, blk)
-                                      : advance<MODE, REPLAY, false>(g, A, io, w, s, blk);
+            bool moved;
+            if constexpr (MODE <= -2) {
+                moved = kind_b ? advance_mode<pair_second(MODE), REPLAY, false>(g, B, io, w, s, blk)
+                               : advance_mode<pair_first(MODE), REPLAY, false>(g, A, io, w, s, blk);
+            } else {
+                moved = kind_b ? advance<MODE, REPLAY, false>(g, B, io, w, s, blk)
+                               : advance<MODE, REPLAY, false>(g, A, io, w, s, blk);
+            }
             if (moved) {
                 ++steps;
                 if (out.row && s.len - 1 < length) out.put(s.cur, s.len, length);
@@ -2018,8 +2032,10 @@ void queue_launch(const DevGraph &g, const KindParams &a, const KindParams &b, i
     };
     const unsigned int rot =
         (split > 0 && split < io.n && expected(b) > expected(a)) ? static_cast<unsigned int>(split) : 0u;
-    hipLaunchKernelGGL((walk_queue_kernel<MODE, REPLAY>), dim3(grid), dim3(kQueueBlock), 0, stream, g, a, b, split, io, q,
-                       static_cast<unsigned int>(chunk), rot);
+    // A pair kernel takes the kind of its first Move as A (walkers [0, split) then run B).
+    const bool swap = MODE <= -2 && a.mode != pair_first(MODE);
+    hipLaunchKernelGGL((walk_queue_kernel<MODE, REPLAY>), dim3(grid), dim3(kQueueBlock), 0, stream, g,
+                       swap ? b : a, swap ? a : b, split, static_cast<int32_t>(swap), io, q, static_cast<unsigned int>(chunk), rot);
 }
 
 template <bool REPLAY>

@@ -318,6 +318,10 @@ def c1_ppr(ctx, args):
     out = {"graph": "chung-lu 2^18 V, 1.5M undirected edges, gamma 2.1, dmin 2, dmax 4096, unweighted",
            "directed_edges": int(len(adj))}
     prm = T.make_params("ppr", stop_probability=0.2, rng="philox", seed=1)
+    # The queue form's chunk and blocks per CU for this graph, tuned on a sample the size of the batch
+    # (a batch of one walker per vertex: the grid's residency sets how many walkers a lane runs).
+    out["autotune"] = tune_record(dg.autotune(prm, sample_walkers=1 << 18, seed=1, with_paths=False))
+    log("C1 autotune: %s" % out["autotune"])
     for label, src in (("all_sources", T.sources_all(deg)), ("single_source", T.sources_single(deg, len(deg)))):
         b = Batch(ctx, src, 1, paths=False)
         io = b.io()
@@ -766,6 +770,9 @@ def main():
     rec = (details.get("ppr_serving") or {}).get("autotune")
     if rec:
         at[rec["move"] + "_queue_serving"] = rec
+    rec = (details.get("c1_ppr") or {}).get("autotune")
+    if rec:
+        at[rec["move"] + "_queue_c1"] = rec
     for key, name in (("autotune", "node2vec"), ("weighted_autotune", "node2vec_weighted")):
         rec = (details.get("c3_node2vec") or {}).get(key)
         if rec:

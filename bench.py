@@ -482,6 +482,28 @@ def mix_forms(ctx, dg, deg, args):
     return out
 
 
+def latency_parts(srv, ids):
+    """Where the served queries' time went: per query, arrival -> first launch (queue), first
+    launch -> its last batch seen done on the device (device), and the host-side completion after
+    it; p50 and p99 of each, and their means over the slowest 1% of queries."""
+    rows = []
+    for q in ids:
+        i = srv.info(int(q))
+        if i["latency_ms"] >= 0 and i["device_ms"] >= 0:
+            rows.append((i["latency_ms"], i["queue_ms"], i["device_ms"],
+                         i["latency_ms"] - i["queue_ms"] - i["device_ms"]))
+    if not rows:
+        return None
+    a = np.array(rows)
+    slow = a[a[:, 0] >= np.percentile(a[:, 0], 99)]
+    out = {}
+    for j, name in ((1, "queue"), (2, "device"), (3, "completion")):
+        out[name + "_p50_ms"] = float(np.percentile(a[:, j], 50))
+        out[name + "_p99_ms"] = float(np.percentile(a[:, j], 99))
+        out[name + "_slowest1pct_mean_ms"] = float(slow[:, j].mean())
+    return out
+
+
 def serving(ctx, args):
     """PPR query serving on a 2^24-vertex Chung-Lu graph (exponent 2.1, average degree 16,
     dmin 2, dmax 65536, FP64 U[1,5) weights, weighted alias tables resident).  Poisson
@@ -515,7 +537,13 @@ def serving(ctx, args):
             k = int(np.exp(rng.uniform(0.0, np.log(4096.0))))
             nw = int(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
             qs.append(srv.query(prm, rng.choice(live, k), nw, (i + 1) << 32))
-        # Warm-up at the same rate (first-touch of pinned staging, kernel loads).
+        # Warm-up: 4,000 queries at the sweep's highest rate, every slot busy with full batches.  The
+        # first time the load reaches a new level the device stalls once for 7 - 12 ms (measured with
+        # per-query device times: every batch then in flight is seen done that much late); the
+        # server's own warm-up at creation removes most of these, a burst at full load the rest.
+        _, ids, _ = srv.run_open_loop(qs[:4000], max(args.sweep + [args.qps]), seed=1)
+        for q in ids:
+            srv.release(int(q))
         srv.run_open_loop(qs[:200], args.qps, seed=1)
         rep, ids, lat = srv.run_open_loop(qs, args.qps, seed=2)
         st = srv.stats()
@@ -526,12 +554,15 @@ def serving(ctx, args):
         sweep = []
         for qps in args.sweep:
             r, ids, _ = srv.run_open_loop(qs, qps, seed=3)
+            parts = latency_parts(srv, ids)
             for q in ids:
                 srv.release(int(q))
-            sweep.append({k: r[k] for k in ("offered_qps",) + keys})
+            sweep.append(dict({k: r[k] for k in ("offered_qps",) + keys}, parts=parts))
             log("serving sweep: offered %.0f qps -> achieved %.0f qps, p50 %.3f ms, p99 %.3f ms, %.3e steps/s, "
-                "%.1f queries per batch" % (qps, r["achieved_qps"], r["p50_ms"], r["p99_ms"], r["steps_per_s"],
-                                            r["mean_batch_queries"]))
+                "%.1f queries per batch; slowest 1%%: queue %.3f, device %.3f, completion %.3f ms"
+                % (qps, r["achieved_qps"], r["p50_ms"], r["p99_ms"], r["steps_per_s"], r["mean_batch_queries"],
+                   parts["queue_slowest1pct_mean_ms"], parts["device_slowest1pct_mean_ms"],
+                   parts["completion_slowest1pct_mean_ms"]))
     dg.close()
     out.update({k: rep[k] for k in keys})
     out["mean_batch_walkers"] = st["mean_batch_walkers"]

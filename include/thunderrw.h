@@ -465,6 +465,8 @@ typedef struct {
     int64_t ppr_entries;      /* distinct end vertices (TRW_RESULT_PPR) */
     double queue_ms;          /* arrival -> first walkers launched */
     double latency_ms;        /* arrival -> result ready */
+    double device_ms;         /* first walkers launched -> the last batch holding its walkers seen done
+                                 on the device; latency - queue - device is the host-side completion */
 } trw_query_info;
 
 typedef struct {

@@ -87,7 +87,8 @@ class Query(C.Structure):
 
 class QueryInfo(C.Structure):
     _fields_ = [("state", C.c_int32), ("n_walkers", C.c_int32), ("steps", C.c_uint64), ("errors", C.c_uint64),
-                ("ppr_entries", C.c_int64), ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+                ("ppr_entries", C.c_int64), ("queue_ms", C.c_double), ("latency_ms", C.c_double),
+                ("device_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):

@@ -81,6 +81,7 @@ struct Query {
     std::vector<int32_t> ends, lengths, status, paths;  // TRW_RESULT_PATHS queries
     std::mutex merge_mu;                                // PPR runs of a query split over batches
     std::vector<Run> ppr;                               // ascending end vertices with walker counts
+    Clock::time_point device_done{};                    // latest batch of its walkers seen done (merge_mu)
     // Guarded by the server mutex.
     Clock::time_point arrival, dispatched, finished;
     bool was_dispatched = false;
@@ -155,6 +156,7 @@ struct Slot {
     int32_t pool_used = 0;
     bool ppr = false;  // the batch's queries want end-vertex counts
     SlotState state = SlotState::Free;
+    Clock::time_point done_at{};  // when the dispatcher saw the batch done on the device
 };
 
 class WorkerPool {
@@ -234,6 +236,7 @@ struct trw_server {
     void launch(Slot &sl);
     void drain(Slot *sl, hipError_t err, std::string msg);
     void finalize(const std::shared_ptr<Query> &q);
+    int warm_up();
     void free_slots();
 };
 
@@ -357,6 +360,7 @@ void trw_server::launch(Slot &sl) {
         if (msg.empty()) msg = std::string("serving batch launch: ") + hipGetErrorString(e);
         std::lock_guard<std::mutex> lk(mu);
         sl.state = SlotState::Draining;
+        sl.done_at = Clock::now();
         Slot *p = &sl;
         workers->push([this, p, e, msg] { drain(p, e, msg); });
     }
@@ -376,6 +380,7 @@ void trw_server::dispatch_loop() {
                 continue;
             }
             sl.state = SlotState::Draining;
+            sl.done_at = Clock::now();
             Slot *p = &sl;
             std::string msg = e == hipSuccess ? std::string() : std::string("serving batch: ") + hipGetErrorString(e);
             workers->push([this, p, e, msg] { drain(p, e, msg); });
@@ -456,6 +461,7 @@ void trw_server::drain(Slot *sl, hipError_t err, std::string msg) {
                 std::lock_guard<std::mutex> lk(q.merge_mu);
                 if (q.ppr.empty()) q.ppr.swap(runs);
                 else merge_runs(q.ppr, runs.data(), nr);
+                q.device_done = std::max(q.device_done, sl->done_at);
             }
             q.steps += st.steps;
             q.errors += st.errors;
@@ -476,6 +482,8 @@ void trw_server::drain(Slot *sl, hipError_t err, std::string msg) {
             }
             q.steps += steps;
             q.errors += errors;
+            std::lock_guard<std::mutex> lk(q.merge_mu);
+            q.device_done = std::max(q.device_done, sl->done_at);
         } else {
             std::lock_guard<std::mutex> lk(mu);
             if (q.error_code == TRW_OK) {
@@ -637,11 +645,72 @@ void fill_info(const Query &q, trw_query_info *info) {
     info->ppr_entries = finished ? static_cast<int64_t>(q.ppr.size()) : 0;
     info->queue_ms = q.was_dispatched || finished ? ms_between(q.arrival, q.dispatched) : -1.0;
     info->latency_ms = finished ? ms_between(q.arrival, q.finished) : -1.0;
+    // Written by the completion threads before the one that finished the query took the server
+    // mutex to mark it done; read here under that mutex.
+    info->device_ms = finished && q.was_dispatched ? ms_between(q.dispatched, q.device_done) : -1.0;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------- C ABI
+
+// The first time many full-size batches are in flight together, the runtime grows what it keeps
+// for them, once (measured on the 2^24-vertex serving graph: one stall of 7 - 12 ms inside the
+// latency of the batches then in flight, the first time an open-loop run reached 25,000 or 50,000
+// queries/s, and none after a burst of full-size batches on every slot).  Before the first query
+// every slot runs two full-size PPR batches at once (uniform moves, which need no table, from the
+// highest-degree vertex); then the statistics start from zero.
+int trw_server::warm_up() {
+    // Sources: the graph's vertices in degree order (most of them non-isolated), as many as a
+    // batch's queries bring at most, so that batches carry many segments and a full source pool.
+    const int32_t per_query = std::min<int32_t>(std::min<int32_t>(kPoolInts / std::max(max_segments, 1), 4096), g->nv);
+    std::vector<int32_t> src(static_cast<size_t>(std::max(per_query, 1)), 0);
+    if (g->order && per_query > 0 &&
+        hipMemcpy(src.data(), g->order, sizeof(int32_t) * src.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return trw::set_error(TRW_E_HIP, "reading the graph's degree order");
+    const int32_t queries_per_batch = std::max(1, std::min(max_segments, 64));
+    trw_query wq;
+    std::memset(&wq, 0, sizeof(wq));
+    wq.sources = src.data();
+    wq.n_sources = static_cast<int32_t>(src.size());
+    wq.n_walkers = std::max(1, max_walkers / queries_per_batch);
+    wq.result = TRW_RESULT_PPR;
+    wq.params.app = TRW_APP_PPR;
+    wq.params.execution = TRW_EXEC_UNIFORM;
+    wq.params.sampling = TRW_SM_UNIFORM;
+    wq.params.length = 1;
+    wq.params.stop_probability = 0.2;
+    wq.params.rng = TRW_RNG_PHILOX;
+    wq.params.form = TRW_FORM_QUEUE;
+    // Submitted as fast as admission takes them (a full pending list: the oldest is waited for).
+    const size_t total = slots.size() * 2 * static_cast<size_t>(queries_per_batch);
+    std::deque<uint64_t> open;
+    int rc = TRW_OK;
+    const auto retire = [&] {
+        trw_query_info info;
+        const int w = trw_server_wait(this, open.front(), -1.0, &info);
+        if (rc == TRW_OK && w != TRW_OK) rc = w;
+        (void)trw_server_release(this, open.front());
+        open.pop_front();
+    };
+    for (size_t i = 0; i < total && rc == TRW_OK;) {
+        uint64_t id = 0;
+        const int r = submit(&wq, Clock::now(), &id);
+        if (r == TRW_OK) {
+            open.push_back(id);
+            ++i;
+        } else if (r == TRW_E_BUSY && !open.empty()) {
+            retire();
+        } else {
+            rc = r;
+        }
+    }
+    while (!open.empty()) retire();
+    std::lock_guard<std::mutex> lk(mu);
+    stats = trw_server_stats{};
+    batch_walkers = batch_queries = 0;
+    return rc;
+}
 
 int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server **out) {
     if (!g || !out) return trw::set_error(TRW_E_INVALID, "null argument");
@@ -718,9 +787,30 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
         s->free_slots();
         return trw::set_hip_error(e, "allocating the server's slots");
     }
+    // Every slot's stream does once what a batch does -- copies both ways between its pinned staging
+    // and the device, a kernel, its event -- before the first query: the runtime sets a stream up
+    // on its first use, which otherwise lands inside the latency of the first batches on that slot.
+    for (Slot &sl : s->slots) {
+        if (e == hipSuccess) e = hipMemcpyAsync(sl.d_segs, sl.h_segs, sizeof(trw::ServeSegment), hipMemcpyHostToDevice, sl.stream);
+        if (e == hipSuccess) e = trw::launch_fill_u32(sl.d_queue_head, 0u, 1, sl.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(sl.h_ends, sl.d_ends, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream);
+        if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
+    }
+    for (Slot &sl : s->slots)
+        if (e == hipSuccess) e = hipEventSynchronize(sl.done);
+    if (e != hipSuccess) {
+        s->free_slots();
+        return trw::set_hip_error(e, "setting up the server's streams");
+    }
     s->workers = std::make_unique<WorkerPool>(c.completion_threads);
     trw_server *raw = s.get();
     s->dispatcher = std::thread([raw] { raw->dispatch_loop(); });
+    const int rc = raw->warm_up();
+    if (rc != TRW_OK) {
+        const std::string msg = trw_last_error();
+        trw_server_destroy(s.release());
+        return trw::set_error(rc, "server warm-up: %s", msg.c_str());
+    }
     *out = s.release();
     return TRW_OK;
 }

@@ -491,6 +491,10 @@ typedef struct {
     uint64_t rejected;
 } trw_open_loop_report;
 
+/* Allocates the slots, starts the dispatcher, then warms up: two rounds of full batches of
+ * internal PPR queries (uniform moves from the highest-degree vertex) on every slot, after which
+ * query ids continue and the statistics start from zero.  The first time many full batches run at
+ * once the device stalls for several ms, once; this keeps that out of the first queries' latency. */
 int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server **out);
 /* Stops the dispatcher after the queries in flight finish and frees every result.  No other
  * call on the server may be running; destroy servers before their graph.  The other

@@ -202,6 +202,59 @@ def test_metapath_label_alphabets_match_oracle(gpu_ctx, n_labels, schema):
         dg.close()
 
 
+def test_metapath_schema_records_on_a_directed_graph_match_oracle(gpu_ctx):
+    """A schema of distinct labels reads the graph's 16-byte schema records.  On a directed graph
+    whose sinks (vertices without out-edges) are reachable, a walk that moves onto one ends as a
+    dead end there, and one whose next label is missing as gather-terminated, as in the oracle; a
+    schema with a repeated label reads the general records; both equal the oracle in every form."""
+    base = graphs.chung_lu(6000, 30000, 2.0, 2, 2500, 41)
+    rng = np.random.default_rng(42)
+    sink = rng.random(base.nv) < 0.1
+    keep = ~np.repeat(sink, base.degree)
+    deg = np.where(sink, 0, base.degree).astype(np.int32)
+    g = graphs.CSR(deg, base.adj[keep], base.weight[keep], rng.integers(0, 5, int(keep.sum())))
+    dg = dgraph(gpu_ctx, g)
+    try:
+        live = np.nonzero(g.degree > 0)[0].astype(np.int32)
+        src = np.concatenate([live, np.full(64, int(np.argmax(g.degree)), np.int32)]).astype(np.int32)
+        for schema in ("0,1,2,3,4", "2,4", "1,0,1,3"):
+            case = {"app": "metapath", "execution": 2, "sampling": 1, "length": 30, "extra": {"-s": schema}}
+            K, KT = 120, 1
+            ref = oracle_walk(g, case, src, K, KT, 46, 47)
+            assert np.any(ref["status"] == T._abi.WALKER_DEAD_END), schema
+            assert np.any(ref["status"] == T._abi.WALKER_GATHER_TERMINATED), schema
+            draws, tdraws = T.replay_fill(46, len(src), K), T.replay_fill(47, len(src), KT)
+            for form in forms_for("metapath"):
+                r = T.walk(dg, case_params(case, form), src, draws, tdraws)
+                for key in ("paths", "lengths", "ends", "status", "draws_used"):
+                    assert np.array_equal(r[key], ref[key]), (schema, form, key)
+    finally:
+        dg.close()
+
+
+def test_mixed_metapath_kinds_with_two_schemas(gpu_ctx):
+    """Two MetaPath kinds of different schemas in one batch: the graph keeps one schema's records,
+    the other kind reads the general records; each half walks as it does alone."""
+    g = graphs.chung_lu(20000, 120000, 2.1, 2, 3000, 43)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        live = np.nonzero(g.degree > 0)[0]
+        n, split = 60000, 30000
+        src = np.random.default_rng(9).choice(live, n).astype(np.int32)
+        for form in ("queue", "step", "graph"):
+            a = T.make_params("metapath", length=40, schema=[0, 1, 2, 3, 4], rng="philox", seed=8, form=form)
+            b = T.make_params("metapath", length=40, schema=[3, 1, 4], rng="philox", seed=8, form=form)
+            mixed = T.walk(dg, a, src, mix=(b, split))
+            ra = T.walk(dg, a, src[:split])
+            rb = T.walk(dg, b, src[split:], walker_id_base=split)
+            for key in ("paths", "lengths", "ends", "status"):
+                assert np.array_equal(mixed[key][:split], ra[key]), (form, key)
+                assert np.array_equal(mixed[key][split:], rb[key]), (form, key)
+            assert mixed["steps"] == ra["steps"] + rb["steps"]
+    finally:
+        dg.close()
+
+
 def test_tuning_changes_no_walk(gpu_ctx):
     """ITS through guide records, and with them switched off through the search tree on every
     vertex, on hubs only, and off; the queue form with several chunk sizes; the fused form with

@@ -283,6 +283,7 @@ static void graph_release(trw_graph *g) {
     dfree(g->lbl_pos);
     dfree(g->lstart);
     dfree(g->lerec);
+    dfree(g->msrec);
     dfree(g->its_lines);
     dfree(g->its_root);
     dfree(g->grec);
@@ -966,6 +967,44 @@ static bool reads_static_table(const trw_walk_params *p) {
 
 // Maps an app + g_para to a device mode, building the static tables the mode
 // reads when they are missing.
+// A MetaPath schema whose labels are distinct and all in [0, 8): its labels as nibbles and its
+// length in bits 60-63 (never 0); 0 for any other schema.
+static uint64_t schema_key(const trw_walk_params *p) {
+    if (p->schema_length <= 0 || p->schema_length > 8) return 0;
+    uint32_t seen = 0;
+    uint64_t key = static_cast<uint64_t>(p->schema_length) << 60;
+    for (int i = 0; i < p->schema_length; ++i) {
+        const int32_t l = p->schema[i];
+        if (l < 0 || l >= 8 || (seen >> l & 1u)) return 0;
+        seen |= 1u << l;
+        key |= static_cast<uint64_t>(l) << (4 * i);
+    }
+    return key;
+}
+
+static int build_schema_records(trw_graph *g, const trw_walk_params *p, uint64_t key) {
+    uint32_t next_map = 0xffffffffu;  // nibble l: the label after l in the schema (15: l not in it)
+    for (int i = 0; i < p->schema_length; ++i) {
+        const uint32_t l = static_cast<uint32_t>(p->schema[i]);
+        const uint32_t nl = static_cast<uint32_t>(p->schema[(i + 1) % p->schema_length]);
+        next_map = (next_map & ~(15u << (4 * l))) | (nl << (4 * l));
+    }
+    if (!g->msrec) TRW_HIP(dmalloc(&g->msrec, static_cast<size_t>(g->ne) + 1), "hipMalloc MetaPath schema records");
+    g->msrec_key = 0;
+    TRW_HIP(trw::launch_schema_records(g->view(), next_map, g->msrec, g->ctx->stream), "MetaPath schema records");
+    TRW_HIP(hipStreamSynchronize(g->ctx->stream), "MetaPath schema records");
+    g->msrec_key = key;
+    return TRW_OK;
+}
+
+int trw::prepare_records(trw_graph *g, const trw_walk_params *p) {
+    if (p->app == TRW_APP_METAPATH && p->execution != TRW_EXEC_UNIFORM && g->lerec) {
+        const uint64_t key = schema_key(p);
+        if (key != 0 && g->msrec_key != key) return build_schema_records(g, p, key);
+    }
+    return TRW_OK;
+}
+
 static int resolve_move(trw_graph *g, const trw_walk_params *p, KindParams &k, bool allow_build) {
     std::memset(&k, 0, sizeof(k));
     if (p->app < TRW_APP_DEEPWALK || p->app > TRW_APP_GENERIC) return fail(TRW_E_INVALID, "unknown app %d", p->app);
@@ -1032,6 +1071,16 @@ static int resolve_move(trw_graph *g, const trw_walk_params *p, KindParams &k, b
             k.schema[i] = p->schema[i];
             const uint64_t nib = (p->schema[i] >= 0 && p->schema[i] < 8) ? static_cast<uint64_t>(p->schema[i]) : 8u;
             k.schema_nibbles |= nib << (4 * i);
+        }
+        // Distinct labels, all in [0, 8): 16-byte schema records (built once per schema and kept
+        // with the graph; another schema replaces them).
+        const uint64_t key = schema_key(p);
+        if (key != 0 && g->lerec) {
+            if (g->msrec_key != key && allow_build) {
+                int rc = build_schema_records(g, p, key);
+                if (rc != TRW_OK) return rc;
+            }
+            k.schema_rec = g->msrec_key == key;
         }
         return TRW_OK;
     }
@@ -1352,6 +1401,10 @@ int trw_walk_mix_async(trw_graph *g, const trw_walk_params *pa, const trw_walk_p
     if (rc != TRW_OK) return rc;
     rc = resolve_kind(g, pb, b);
     if (rc != TRW_OK) return rc;
+    // Two MetaPath kinds with different schemas: the graph keeps the records of the one resolved
+    // last, and the other reads the general label-ordered records.
+    if (a.schema_rec) a.schema_rec = g->msrec_key == schema_key(pa);
+    if (b.schema_rec) b.schema_rec = g->msrec_key == schema_key(pb);
     BatchIO bio;
     rc = make_io(g, pa, io, a, b, bio);
     if (rc != TRW_OK) return rc;

@@ -100,6 +100,8 @@ int tune(trw_graph *g, const trw_walk_params *params, const trw_walk_params *pb,
     if (hipSetDevice(g->ctx->device) != hipSuccess) return trw::set_error(TRW_E_HIP, "hipSetDevice");
     int rc = trw::walk_check(g, params);
     if (rc == TRW_OK && pb) rc = trw::walk_check(g, pb);
+    if (rc == TRW_OK) rc = trw::prepare_records(g, params);
+    if (rc == TRW_OK && pb) rc = trw::prepare_records(g, pb);
     if (rc != TRW_OK) return rc;
 
     // Sources: non-isolated vertices drawn by a seeded generator (splitmix64).

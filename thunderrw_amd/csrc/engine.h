@@ -64,6 +64,11 @@ struct DevGraph {
     const uint4 *lerec;        // 4*ne with lstart: 64-byte neighbour records in label order, entry j of v:
                                //       lerec[4 (off(v) + j)] = erec[off(v) + lbl_pos[off(v) + j]], then the
                                //       neighbour's 8 label starts (2 uint4), then 16 bytes of zero
+    const uint4 *msrec;        // ne with lerec, built for one MetaPath schema whose labels are distinct and in
+                               //       [0, 8): entry j of v's label order {neighbour u, u's first edge, the
+                               //       range [lo, hi) of the schema label after this edge's at u (label
+                               //       order), kNoRange twice when u has no edges}: a MetaPath step reads 16
+                               //       bytes (a table a quarter of lerec's, within the TLB's reach)
     const uint4 *its_lines;    // ITS search structures (128-byte lines, 8 uint4 each), or null
     const uint32_t *its_root;  // nv: first line of v's structure, kNoRange if v has none
     const uint4 *rrec;         // 2*ne: rejection walk records {weight lo, weight hi, adj[e], first edge of adj[e]}
@@ -141,6 +146,7 @@ struct KindParams {
     int32_t n2v_weighted;   // Node2Vec over edge weights: a candidate's weight is w_e times its factor
     int32_t schema[16];
     uint64_t schema_nibbles;  // schema[i] as nibble i when in [0, 8), else 8 (no neighbour carries it)
+    int32_t schema_rec;       // MetaPath: the graph's msrec were built for this schema
     int32_t its_tree_min;   // ITS: vertices of at least this degree search their tree (when built)
     int32_t queue_chunk;    // QUEUE: walker indices a wavefront takes per atomic (0 = from the batch size)
     int32_t its_guide;      // ITS: read the guide records when the graph has them
@@ -283,6 +289,9 @@ hipError_t launch_alias_records(const DevGraph &g, uint4 *arec, hipStream_t stre
 // Rejection walk records from the weights the maxima were built from (w, or 1.0 when unit).
 hipError_t launch_rs_records(const DevGraph &g, const double *w, int32_t unit, uint4 *rrec, hipStream_t stream);
 hipError_t launch_label_records(const DevGraph &g, uint32_t *lstart, uint4 *lerec, hipStream_t stream);
+// MetaPath schema records for the schema whose next label after label l is nibble l of next_map (15:
+// l is not in the schema).
+hipError_t launch_schema_records(const DevGraph &g, uint32_t next_map, uint4 *msrec, hipStream_t stream);
 // ITS search structures: line counts per vertex into root (then the caller turns them
 // into offsets), the structures themselves, and the roots copied into the neighbour
 // records' .w.

@@ -33,6 +33,8 @@ struct trw_graph {
     uint4 *arec = nullptr;
     uint32_t *lstart = nullptr;
     uint4 *lerec = nullptr;
+    uint4 *msrec = nullptr;        // MetaPath schema records (16 bytes per edge) for the schema msrec_key
+    uint64_t msrec_key = 0;        // the schema's labels as nibbles, its length in bits 60-63 (0: none built)
     uint4 *its_lines = nullptr;    // ITS search structures (null: not built, or sums not non-decreasing)
     uint32_t *its_root = nullptr;  // nv: first line of each vertex's structure
     uint4 *rrec = nullptr;         // rejection walk records (2 per edge), with the rejection table
@@ -79,6 +81,7 @@ struct trw_graph {
         g.arec = arec;
         g.lstart = lstart;
         g.lerec = lerec;
+        g.msrec = msrec;
         g.its_lines = its_lines;
         g.its_root = its_root;
         g.rrec = rrec;
@@ -107,5 +110,9 @@ int walk_launch(trw_graph *g, const trw_walk_params *params, const trw_walk_io *
 // Validates a walk's parameters against the graph without launching or building
 // anything (the tables it reads must already be built).
 int walk_check(trw_graph *g, const trw_walk_params *params);
+
+// Builds the per-walk records a walk would build on its first launch (MetaPath schema records),
+// so that a walk launched without building (the autotuner's) reads what the real walk reads.
+int prepare_records(trw_graph *g, const trw_walk_params *params);
 
 }  // namespace trw

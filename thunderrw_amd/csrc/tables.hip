@@ -580,6 +580,31 @@ __global__ void label_records_kernel(const int64_t *offset, int32_t nv, const in
     }
 }
 
+// MetaPath schema records: entry e of the label order (lerec's), {neighbour u, u's first edge, the
+// label range at u of the label that follows this edge's in the schema, in u's label order}.  A
+// schema with distinct labels fixes a walker's position from the label of the edge it moved along,
+// so the next step's label is known per edge.  Edges whose label is not in the schema are never
+// taken; their records are zero.  A neighbour without edges gets kNoRange twice (a dead end).
+__global__ void schema_records_kernel(const uint4 *lerec, const int32_t *lbl_sorted, const uint32_t *lstart,
+                                      int64_t ne, uint32_t next_map, uint4 *msrec) {
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < ne;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int32_t l = lbl_sorted[e];
+        const uint32_t nl = (l >= 0 && l < 8) ? (next_map >> (4 * l)) & 15u : 15u;
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        if (nl < 8u) {
+            const uint32_t u = lerec[4 * e].x;
+            const uint32_t *L = lstart + kLabelRecWords * static_cast<int64_t>(u);
+            const uint32_t first = L[8], deg = L[9];
+            r.x = u;
+            r.y = first;
+            r.z = deg == 0u ? kNoRange : L[nl];
+            r.w = deg == 0u ? kNoRange : (nl < 7u ? L[nl + 1] : deg);
+        }
+        msrec[e] = r;
+    }
+}
+
 // ITS search structures.  Sizes: one thread per vertex writes its line count.
 __global__ void its_tree_sizes_kernel(const int64_t *offset, int32_t nv, uint32_t *count, uint64_t *start) {
     const int32_t v = static_cast<int32_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -802,6 +827,13 @@ hipError_t launch_rs_records(const DevGraph &g, const double *w, int32_t unit, u
     if (g.ne <= 0) return hipSuccess;
     hipLaunchKernelGGL(rs_records_kernel, dim3(8192), dim3(256), 0, stream, g.adj, g.vrec, g.rs_max, w, unit, g.ne,
                        rrec);
+    return hipGetLastError();
+}
+
+hipError_t launch_schema_records(const DevGraph &g, uint32_t next_map, uint4 *msrec, hipStream_t stream) {
+    if (g.ne > 0)
+        hipLaunchKernelGGL(schema_records_kernel, dim3(8192), dim3(256), 0, stream, g.lerec, g.lbl_sorted, g.lstart,
+                           g.ne, next_map, msrec);
     return hipGetLastError();
 }
 

@@ -690,7 +690,16 @@ __device__ __forceinline__ bool advance_mode(const DevGraph &g, const KindParams
         uint32_t u;
         if (!draw_one<REPLAY>(io, w, s, blk, u)) return false;
         const int32_t t = static_cast<int32_t>(real2(u) * static_cast<double>(K));
-        if (g.lerec) {
+        if (P.schema_rec) {
+            // The schema's 16-byte record: the neighbour, its first edge and the next step's label
+            // range there (kNoRange: the neighbour has no edges, a dead end next step).
+            const uint4 r = g.msrec[off + a + t];
+            next = static_cast<int32_t>(r.x);
+            noff = r.y;
+            ndeg = r.z == kNoRange ? 0u : 1u;  // only its sign is read: the range comes with the record
+            nlo = r.z;
+            nhi = r.w;
+        } else if (g.lerec) {
             const uint4 *R = g.lerec + 4 * (off + a + t);
             const uint4 r = R[0];
             next = static_cast<int32_t>(r.x);

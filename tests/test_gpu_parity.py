@@ -558,6 +558,34 @@ def test_mixed_batch_rows_per_kind(gpu_ctx, pair, unpadded):
         dg.close()
 
 
+def test_mixed_batch_row_strides_are_checked(gpu_ctx):
+    """Each kind's rows must hold its walks: path_stride >= the first kind's length and
+    path_stride_b >= the second's (or, with path_stride_b 0, path_stride >= both)."""
+    g = graphs.chung_lu(2000, 8000, 2.1, 2, 300, 75)
+    dg = dgraph(gpu_ctx, g)
+    try:
+        src = np.nonzero(g.degree > 0)[0][:100].astype(np.int32)
+        n, split = len(src), 50
+        a = T.make_params("ppr", length=1, stop_probability=0.2, rng="philox", seed=1, form="queue")
+        b = T.make_params("metapath", length=20, schema=[0, 1, 2], rng="philox", seed=1, form="queue")
+        d_src = T.DeviceBuffer.from_array(gpu_ctx, src)
+        paths = T.DeviceBuffer(gpu_ctx, n * 20 * 4)
+        try:
+            for stride, stride_b in ((1, 19), (1, -1), (0, 20), (19, 0)):
+                io = T.make_io(d_src, n, stride, paths=paths, path_stride_b=stride_b)
+                with pytest.raises(T.TrwError) as e:
+                    T.walk_mix_async(dg, a, b, split, io)
+                assert e.value.code == T._abi.E_INVALID, (stride, stride_b)
+            for stride, stride_b in ((1, 20), (20, 0), (3, 25)):
+                T.walk_mix_async(dg, a, b, split, T.make_io(d_src, n, stride, paths=paths, path_stride_b=stride_b))
+            T.synchronize(gpu_ctx)
+        finally:
+            d_src.free()
+            paths.free()
+    finally:
+        dg.close()
+
+
 def test_mixed_kinds_on_one_static_table_from_different_weights_are_refused(gpu_ctx):
     """A graph holds one table per sampler: PPR's unit-weight alias table and DeepWalk's
     edge-weight one cannot serve one batch (the second build would replace the first under the

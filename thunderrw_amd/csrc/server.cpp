@@ -661,6 +661,7 @@ void fill_info(const Query &q, trw_query_info *info) {
 // every slot runs two full-size PPR batches at once (uniform moves, which need no table, from the
 // highest-degree vertex); then the statistics start from zero.
 int trw_server::warm_up() {
+    if (g->nv <= 0) return TRW_OK;  // no vertex to start from
     // Sources: the graph's vertices in degree order (most of them non-isolated), as many as a
     // batch's queries bring at most, so that batches carry many segments and a full source pool.
     const int32_t per_query = std::min<int32_t>(std::min<int32_t>(kPoolInts / std::max(max_segments, 1), 4096), g->nv);

@@ -278,7 +278,8 @@ int trw_build_membership(trw_graph *g);
 #define TRW_TUNE_QUEUE_BLOCKS_PER_CU 6 /* QUEUE: blocks of the persistent grid per CU, 1..16 (0: as many as
                                           are resident at once) */
 #define TRW_TUNE_N2V_BLOCK 7           /* FUSED Node2Vec over membership tables: threads per block, 128 or 256 */
-#define TRW_TUNE_KNOBS 8
+#define TRW_TUNE_GRAPH_BURST 8         /* GRAPH: steps a launch takes each live walker, 1..256 (0: 16) */
+#define TRW_TUNE_KNOBS 9
 /* The Moves the shape knobs are kept for (the walk kernels' Move of a walk). */
 #define TRW_MOVE_UNIFORM 0
 #define TRW_MOVE_ALIAS 1
@@ -287,11 +288,11 @@ int trw_build_membership(trw_graph *g);
 #define TRW_MOVE_NODE2VEC 4
 #define TRW_MOVE_METAPATH 5
 #define TRW_MOVES 6
-/* knobs 0-3 are per graph; a shape knob (4-7) set here applies to every Move, and reads
+/* knobs 0-3 are per graph; a shape knob (4-8) set here applies to every Move, and reads
  * back the uniform Move's value. */
 int trw_graph_set_tuning(trw_graph *g, int32_t knob, int64_t value);
 int trw_graph_get_tuning(const trw_graph *g, int32_t knob, int64_t *value);
-/* A shape knob (4-7) of one Move. */
+/* A shape knob (4-8) of one Move. */
 int trw_graph_set_move_tuning(trw_graph *g, int32_t move, int32_t knob, int64_t value);
 int trw_graph_get_move_tuning(const trw_graph *g, int32_t move, int32_t knob, int64_t *value);
 

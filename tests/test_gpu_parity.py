@@ -288,7 +288,10 @@ def test_tuning_changes_no_walk(gpu_ctx):
                  ({"app": "metapath", "execution": 2, "sampling": 1, "length": 40, "extra": {"-s": "0,1,2,3,4"}},
                   [dict(queue_chunk=0, queue_blocks_per_cu=v) for v in (1, 2)] +
                   [dict(queue_blocks_per_cu=0, fused_block=b, fused_tile=t) for b, t in ((64, 16), (256, 0))],
-                  ["queue", "fused"])]
+                  ["queue", "fused"]),
+                 # GRAPH-form steps per launch (0: the built-in 16) over MetaPath's uneven walks.
+                 ({"app": "metapath", "execution": 2, "sampling": 1, "length": 40, "extra": {"-s": "0,1,2,3,4"}},
+                  [dict(fused_block=0, fused_tile=0, graph_burst=v) for v in (1, 3, 0, 64)], ["graph"])]
         for case, knobs, forms in cases:
             ref = oracle_walk(g, case, src, K, 1, 72, 73)
             for kn in knobs:
@@ -298,7 +301,7 @@ def test_tuning_changes_no_walk(gpu_ctx):
                     for key in ("paths", "lengths", "ends", "status", "draws_used"):
                         assert np.array_equal(r[key], ref[key]), (case["app"], kn, form, key)
             dg.set_tuning(walkers_per_thread=0, its_tree_min_degree=0, queue_chunk=0, its_guide_off=0, fused_block=0,
-                          fused_tile=0, queue_blocks_per_cu=0, n2v_block=0)
+                          fused_tile=0, queue_blocks_per_cu=0, n2v_block=0, graph_burst=0)
     finally:
         dg.close()
     # Negative weights: no tree; the walk still equals the oracle's (binary search, clamped pick).

@@ -16,7 +16,7 @@ FORM_AUTO, FORM_FUSED, FORM_QUEUE, FORM_STEP, FORM_GRAPH = 0, 1, 2, 3, 4
 WEIGHTS_EDGE, WEIGHTS_UNIT, WEIGHTS_CUSTOM = 0, 1, 2
 WALK_WEIGHTS_APP, WALK_WEIGHTS_EDGE, WALK_WEIGHTS_UNIT = 0, 1, 2
 TUNE_WALKERS_PER_THREAD, TUNE_ITS_TREE_MIN_DEGREE, TUNE_QUEUE_CHUNK, TUNE_ITS_GUIDE_OFF = 0, 1, 2, 3
-TUNE_FUSED_BLOCK, TUNE_FUSED_TILE, TUNE_QUEUE_BLOCKS_PER_CU, TUNE_N2V_BLOCK = 4, 5, 6, 7
+TUNE_FUSED_BLOCK, TUNE_FUSED_TILE, TUNE_QUEUE_BLOCKS_PER_CU, TUNE_N2V_BLOCK, TUNE_GRAPH_BURST = 4, 5, 6, 7, 8
 RESULT_PPR, RESULT_PATHS = 0, 1
 PATHS_PADDED, PATHS_UNPADDED = 0, 1
 QUERY_PENDING, QUERY_RUNNING, QUERY_DONE, QUERY_FAILED = 0, 1, 2, 3
@@ -63,7 +63,7 @@ class WalkIO(C.Structure):
                 ("path_stride_b", C.c_int32)]
 
 
-TUNE_KNOBS = 8
+TUNE_KNOBS = 9
 MOVES = ("uniform", "alias", "its", "rejection", "node2vec", "metapath")  # TRW_MOVE_* order
 
 

@@ -171,6 +171,7 @@ int tune(trw_graph *g, const trw_walk_params *params, const trw_walk_params *pb,
         knobs.push_back({TRW_TUNE_QUEUE_CHUNK, {0, 64, 256, 1024, 4096}});
         knobs.push_back({TRW_TUNE_QUEUE_BLOCKS_PER_CU, {0, 1, 2, 4}});
     }
+    if (form == TRW_FORM_GRAPH) knobs.push_back({TRW_TUNE_GRAPH_BURST, {0, 4, 8, 32, 64}});
     if (form == TRW_FORM_FUSED) {
         if (node2vec) knobs.push_back({TRW_TUNE_N2V_BLOCK, {256, 128}});
         knobs.push_back({TRW_TUNE_FUSED_BLOCK, {0, 64, 128, 256}});

@@ -155,6 +155,7 @@ struct KindParams {
     int32_t fused_tile;           // FUSED: most path positions a lane stages in LDS per write-out
     int32_t queue_blocks_per_cu;  // QUEUE: resident blocks per CU of the persistent grid
     int32_t n2v_block;            // Node2Vec state machine: threads per block (128 or 256)
+    int32_t graph_burst;          // GRAPH: steps per live walker per launch (0: the built-in choice)
 };
 
 // Walker batch (device pointers).

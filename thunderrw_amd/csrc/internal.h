@@ -48,7 +48,7 @@ struct trw_graph {
     bool weights_nonneg = true;    // every edge weight is finite and >= 0
     int32_t build_hubs = 0;        // vertices of degree >= kWaveBuildMin (a prefix of order)
     int64_t tune[4] = {0, 0, 0, 0};  // TRW_TUNE_* knobs 0-3
-    int64_t shape[TRW_MOVES][TRW_TUNE_KNOBS - 4] = {};  // launch-shape knobs 4-7 per Move
+    int64_t shape[TRW_MOVES][TRW_TUNE_KNOBS - 4] = {};  // launch-shape knobs 4-8 per Move
     trw::AliasSlot *alias = nullptr;
     double *prefix = nullptr;
     double *rs_max = nullptr;

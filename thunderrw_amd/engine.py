@@ -179,7 +179,8 @@ class DeviceGraph:
               "its_guide_off": A.TUNE_ITS_GUIDE_OFF,
               "queue_chunk": A.TUNE_QUEUE_CHUNK}
     SHAPES = {"fused_block": A.TUNE_FUSED_BLOCK, "fused_tile": A.TUNE_FUSED_TILE,
-              "queue_blocks_per_cu": A.TUNE_QUEUE_BLOCKS_PER_CU, "n2v_block": A.TUNE_N2V_BLOCK}
+              "queue_blocks_per_cu": A.TUNE_QUEUE_BLOCKS_PER_CU, "n2v_block": A.TUNE_N2V_BLOCK,
+              "graph_burst": A.TUNE_GRAPH_BURST}
 
     def set_tuning(self, move=None, **knobs):
         """Launch choices that change speed, never results (trw_graph_set_tuning); the shape

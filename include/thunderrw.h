@@ -60,8 +60,9 @@ extern "C" {
 #define TRW_FORM_FUSED 1 /* thread per walker, whole walk in one launch, LDS-staged path tiles */
 #define TRW_FORM_QUEUE 2 /* persistent kernel, lanes refill from a device work queue */
 #define TRW_FORM_STEP 3  /* plain form: one launch per step over walker state in HBM */
-#define TRW_FORM_GRAPH 4 /* launches over a compacted list of live walkers, up to 4 steps each,
-                            replayed from a hipGraph of 6 launches kept with the graph */
+#define TRW_FORM_GRAPH 4 /* launches over a compacted list of live walkers, up to 16 steps each
+                            (TRW_TUNE_GRAPH_BURST), replayed from a hipGraph of 6 launches kept
+                            with the graph */
 
 /* Per-walker outcome (status[] output). */
 #define TRW_WALKER_DONE 0              /* update() ended it (length reached / PPR stop draw) */

@@ -76,6 +76,8 @@ def parse_args():
     ap.add_argument("--qps", type=float, default=10000.0, help="offered load of the PPR serving run")
     ap.add_argument("--queries", type=int, default=20000, help="queries in the PPR serving run")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-ppr-l1", dest="ppr_l1", action="store_false",
+                    help="skip the served PPR estimates' L1 check against an FP64 power iteration")
     ap.add_argument("--sweep", type=float, nargs="*", default=[10000.0, 25000.0, 50000.0, 100000.0, 200000.0, 400000.0],
                     help="offered loads (queries/s) of the serving sweep")
     ap.add_argument("--only", choices=["its", "mix", "serving", "c1", "c3", "functor"],
@@ -504,6 +506,125 @@ def latency_parts(srv, ids):
     return out
 
 
+def symmetric_sample(deg, adj, w, off, n=4096, seed=7):
+    """True when n random edges (u, v, w) each have their reverse (v, u, w): the serving graph is
+    symmetrised with mirrored weights by construction, which lets the power iteration use the
+    weighted adjacency itself for the transpose."""
+    rng = np.random.default_rng(seed)
+    e = rng.integers(0, len(adj), n)
+    u = np.searchsorted(off, e, side="right") - 1
+    for ei, ui in zip(e, u):
+        v = int(adj[ei])
+        row = adj[off[v]:off[v + 1]]
+        j = int(np.searchsorted(row, ui))
+        if j >= len(row) or row[j] != ui or w[off[v] + j] != w[ei]:
+            return False
+    return True
+
+
+def ppr_power_host(A, strength, starts, alpha, tol=1e-4):
+    """FP64 power iteration of the PPR end-vertex law for several start distributions at once (one
+    column each): sum_{k>=1} a (1-a)^(k-1) x0 P^k with P(u->v) = w_uv / sum_u w, the reference
+    moving before it tests termination (creeper.h:306, ppr.cpp:27-29); a vertex without edges keeps
+    its mass.  Iterated until the mass not yet placed is below tol.  A is the weighted adjacency of
+    a symmetric graph, so y = A (x / s) is P^T x; its row blocks are multiplied on the host's cores
+    in parallel.  Returns (pi, iterations, mass left)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import scipy.sparse as sp
+    n = A.shape[0]
+    nt = max(1, min(16, os.cpu_count() or 1))
+    cuts = np.searchsorted(A.indptr, np.linspace(0, A.nnz, nt + 1)).clip(0, n)
+    cuts[0], cuts[-1] = 0, n
+    blocks = []
+    for r0, r1 in zip(cuts[:-1], cuts[1:]):
+        o0, o1 = int(A.indptr[r0]), int(A.indptr[r1])
+        blocks.append(sp.csr_matrix((A.data[o0:o1], A.indices[o0:o1], A.indptr[r0:r1 + 1] - o0),
+                                    shape=(int(r1 - r0), n)))
+    x = np.zeros((n, len(starts)))
+    for j, s in enumerate(starts):
+        np.add.at(x[:, j], s, 1.0 / len(s))
+    dead = strength == 0
+    inv = np.where(dead, 0.0, 1.0 / np.where(dead, 1.0, strength))[:, None]
+    pi = np.zeros_like(x)
+    coef, left, iters = alpha, 1.0, 0
+    with ThreadPoolExecutor(nt) as pool:
+        while left > tol:
+            xs = x * inv
+            y = np.concatenate(list(pool.map(lambda b: b @ xs, blocks)))
+            y[dead] += x[dead]
+            pi += coef * y
+            x = y
+            coef *= 1.0 - alpha
+            left *= 1.0 - alpha
+            iters += 1
+    return pi, iters, left
+
+
+def judge_ppr(pi, checks):
+    """Each served estimate (end-vertex walker counts / N) against its law pi (one column per
+    check): the L1 distance beside what sampling alone gives (expected_l1), and Pearson's
+    chi-square over the vertices where the law expects 5 or more walks.  ok: L1 within 4 / sqrt(N)
+    (about four standard deviations of L1) of the sampling figure and the chi-square tail above
+    1e-4."""
+    out = []
+    for j, (src, nw, v, c) in enumerate(checks):
+        counts = np.zeros(pi.shape[0])
+        counts[v] = c
+        p = pi[:, j]
+        l1 = float(np.abs(counts / nw - p).sum())
+        el1 = expected_l1(p, nw)
+        stat, dof, pval = head_chi_square(counts, p, nw)
+        out.append({"sources": int(len(src)), "walkers": int(nw), "l1": round(l1, 5),
+                    "expected_l1": round(el1, 5), "chi2": round(stat, 2), "chi2_dof": dof, "chi2_p": pval,
+                    "ok": bool(l1 <= el1 + 4.0 / np.sqrt(nw) and pval > 1e-4)})
+    return out
+
+
+def ppr_l1(A, strength, checks, alpha, tol=1e-4):
+    """Served PPR queries (sources, walkers, vertices, counts) judged against the FP64 power
+    iteration from the walkers' own starts (walker i starts at sources[i % |S|])."""
+    t0 = time.time()
+    pi, iters, left = ppr_power_host(A, strength, [src[np.arange(nw) % len(src)] for src, nw, _, _ in checks],
+                                     alpha, tol)
+    out = judge_ppr(pi, checks)
+    log("served PPR vs FP64 power iteration (%d iterations, %.1fs): %s"
+        % (iters, time.time() - t0,
+           ", ".join("|S| %d N %d: L1 %.4f (sampling alone %.4f), head chi2 %.1f / %d dof, p %.3g"
+                     % (o["sources"], o["walkers"], o["l1"], o["expected_l1"], o["chi2"], o["chi2_dof"], o["chi2_p"])
+                     for o in out)))
+    return {"iterations": iters, "mass_left": left, "queries": out}
+
+
+def expected_l1(p, n):
+    """E sum_v |X_v / n - p_v| for multinomial counts X of n draws from p: per vertex the binomial's
+    mean absolute deviation, 2 (m + 1) C(n, m + 1) p^(m+1) (1 - p)^(n-m) with m = floor(n p)
+    (de Moivre), in logs.  A walk batch's counts are a sum of such draws from each source's law,
+    whose spread is at most this, so it bounds what sampling alone gives."""
+    from scipy.special import gammaln
+    p = p[(p > 0) & (p < 1)]
+    m = np.floor(n * p)
+    lg = (np.log(2.0) + np.log(m + 1) + gammaln(n + 1) - gammaln(m + 2) - gammaln(n - m)
+          + (m + 1) * np.log(p) + (n - m) * np.log1p(-p))
+    return float(np.exp(lg).sum() / n)
+
+
+def head_chi_square(counts, p, n, min_expected=5.0):
+    """Pearson's statistic over the vertices expected to end at least min_expected walks, the rest
+    lumped into one bin; its degrees of freedom and the chi-square upper tail."""
+    from scipy.stats import chi2
+    e = n * p
+    head = e >= min_expected
+    obs, exp = list(counts[head]), list(e[head])
+    if n - sum(exp) >= min_expected:
+        obs.append(n - sum(obs))
+        exp.append(n - sum(exp))
+    obs, exp = np.array(obs), np.array(exp)
+    if len(obs) < 2:
+        return 0.0, 0, 1.0
+    stat = float(((obs - exp) ** 2 / exp).sum())
+    return stat, len(obs) - 1, float(chi2.sf(stat, len(obs) - 1))
+
+
 def serving(ctx, args):
     """PPR query serving on a 2^24-vertex Chung-Lu graph (exponent 2.1, average degree 16,
     dmin 2, dmax 65536, FP64 U[1,5) weights, weighted alias tables resident).  Poisson
@@ -516,7 +637,20 @@ def serving(ctx, args):
     log("serving graph: %d directed edges (%.1fs)" % (len(adj), time.time() - t0))
     dg = T.DeviceGraph(ctx, deg, adj, w)
     live = np.nonzero(deg > 0)[0].astype(np.int32)
-    del deg, adj, w  # views into the host graph, which is freed next
+    # The weighted adjacency and vertex strengths for the PPR L1 check (copies: the arrays are views
+    # into the host graph, which is freed next).
+    adjacency = None
+    if args.ppr_l1:
+        import scipy.sparse as sp
+        off = np.zeros(len(deg) + 1, np.int64)
+        np.cumsum(deg, out=off[1:])
+        if not symmetric_sample(deg, adj, w, off):
+            raise RuntimeError("the serving graph is not symmetric with mirrored weights")
+        strength = np.zeros(len(deg))
+        nz = deg > 0
+        strength[nz] = np.add.reduceat(w, off[:-1][nz])
+        adjacency = sp.csr_matrix((w.copy(), adj.copy(), off), shape=(len(deg), len(deg)))
+    del deg, adj, w
     hg.close()
     dg.build_tables("alias", "edge")
     rng = np.random.default_rng(5)
@@ -547,6 +681,15 @@ def serving(ctx, args):
         srv.run_open_loop(qs[:200], args.qps, seed=1)
         rep, ids, lat = srv.run_open_loop(qs, args.qps, seed=2)
         st = srv.stats()
+        # Served estimates kept for the L1 check: the queries with the fewest and the most walkers
+        # and with the fewest and the most sources.
+        checks = []
+        if adjacency is not None:
+            nws = np.array([q.n_walkers for q in qs])
+            nss = np.array([q.n_sources for q in qs])
+            for i in dict.fromkeys([int(nws.argmin()), int(nws.argmax()), int(nss.argmin()), int(nss.argmax())]):
+                v, c = srv.ppr(int(ids[i]))
+                checks.append((qs[i]._src, int(qs[i].n_walkers), v, c.astype(np.float64)))
         for q in ids:
             srv.release(int(q))
         # Offered-load sweep: the same queries at rising Poisson rates; where the server saturates the
@@ -564,6 +707,9 @@ def serving(ctx, args):
                    parts["queue_slowest1pct_mean_ms"], parts["device_slowest1pct_mean_ms"],
                    parts["completion_slowest1pct_mean_ms"]))
     dg.close()
+    if checks:
+        out["ppr_l1"] = ppr_l1(adjacency, strength, checks, 0.2)
+    del adjacency
     out.update({k: rep[k] for k in keys})
     out["mean_batch_walkers"] = st["mean_batch_walkers"]
     out["sweep"] = sweep

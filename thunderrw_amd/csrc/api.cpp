@@ -1307,7 +1307,7 @@ static int run_graph_form(trw_graph *g, const KindParams &a, const KindParams &b
 }
 
 int trw::walk_launch(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, hipStream_t stream,
-                     unsigned int *queue_head, bool allow_table_build) {
+                     unsigned int *queue_head, bool allow_table_build, bool head_zeroed) {
     if (!g || !params) return fail(TRW_E_INVALID, "null argument");
     TRW_HIP(hipSetDevice(g->ctx->device), "hipSetDevice");
     KindParams k;
@@ -1326,7 +1326,8 @@ int trw::walk_launch(trw_graph *g, const trw_walk_params *params, const trw_walk
         TRW_HIP(trw::launch_walk_fused(dv, k, bio, st), "fused walk launch");
         return TRW_OK;
     case TRW_FORM_QUEUE:
-        TRW_HIP(trw::launch_walk_queue(dv, k, k, bio.n, bio, queue_head ? queue_head : g->counters, st),
+        TRW_HIP(trw::launch_walk_queue(dv, k, k, bio.n, bio, queue_head ? queue_head : g->counters, st,
+                                       head_zeroed && queue_head != nullptr),
                 "queue walk launch");
         return TRW_OK;
     case TRW_FORM_STEP:

@@ -206,8 +206,11 @@ constexpr int64_t kLabelRecWords = 16;  // DevGraph::lstart words per vertex
 
 // Kernel launchers (walk.hip / tables.hip).  All asynchronous on `stream`.
 hipError_t launch_walk_fused(const DevGraph &g, const KindParams &p, const BatchIO &io, hipStream_t stream);
+// The QUEUE form zeroes its queue head first unless the caller's own earlier work on the
+// stream has (head_zeroed).
 hipError_t launch_walk_queue(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
-                             const BatchIO &io, unsigned int *queue_counter, hipStream_t stream);
+                             const BatchIO &io, unsigned int *queue_counter, hipStream_t stream,
+                             bool head_zeroed = false);
 hipError_t launch_walk_step_init(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
                                  const BatchIO &io, WalkerState *state, hipStream_t stream);
 hipError_t launch_walk_step(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
@@ -255,7 +258,7 @@ struct PprScratch {
     int32_t *bucketed;           // [walkers]: end vertices grouped by (segment, chunk)
     int2 *pairs;                 // [walkers]: (vertex, count) runs at their bucket's positions
     uint2 *bucket;               // [segments][C]: {first position, walkers}
-    uint32_t *bucket_runs;       // [segments][C]
+    uint32_t *tickets;           // [segments]: finished tiles, zero when the batch starts
     int32_t nv;
     int32_t C;
 };

@@ -103,9 +103,10 @@ int set_error(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3
 int set_hip_error(hipError_t e, const char *what);
 
 // trw_walk_async with an explicit work-queue head (one per concurrent QUEUE-form
-// launch; null = the graph's own) and control over building missing tables.
+// launch; null = the graph's own), control over building missing tables, and whether
+// earlier work on the stream has already zeroed that head (head_zeroed).
 int walk_launch(trw_graph *g, const trw_walk_params *params, const trw_walk_io *io, hipStream_t stream,
-                unsigned int *queue_head, bool allow_table_build);
+                unsigned int *queue_head, bool allow_table_build, bool head_zeroed = false);
 
 // Validates a walk's parameters against the graph without launching or building
 // anything (the tables it reads must already be built).

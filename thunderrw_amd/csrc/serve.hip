@@ -54,15 +54,19 @@ hipError_t launch_serve_expand(const ServeSegment *segs, int32_t n_segs, const i
 // performed at the memory side, far below the HBM rate), as a partition of each
 // segment's end vertices into chunks of 2^17 vertex ids, then LDS work per chunk:
 //   hist    - per tile (2048 walkers of one segment): walkers per chunk (LDS counters),
-//             the tile's steps and errors;
-//   scan    - per segment: each (tile, chunk) pair's first position in the segment's
-//             bucket order (chunk-major, tile-minor) and each bucket's extent;
+//             the tile's steps and errors; the segment's last tile to finish (a ticket
+//             per segment, taken with acquire-release ordering) then scans the segment: each
+//             (tile, chunk) pair's first position in the segment's bucket order
+//             (chunk-major, tile-minor), each bucket's extent, the segment's totals;
 //   scatter - per tile: each walker's end vertex into its bucket (LDS cursors);
 //   chunk   - per (segment, chunk) bucket: the bucket's vertices into a 16 KiB LDS
 //             bitmap and their walker counts into LDS; reading the bitmap in order, with
 //             a popcount prefix for each thread's first rank, writes the (vertex, count)
-//             runs ascending at the bucket's first position;
-//   export  - per bucket: the runs and the bucket's extent into host memory.
+//             runs ascending at the bucket's first position, then copies them and the
+//             bucket's extent into host memory with consecutive threads on consecutive
+//             runs (whole lines over PCIe rather than one 8-byte write per run).
+// Three launches per batch: each one's fixed cost (a few microseconds) is latency a
+// lightly loaded server's queries pay in full.
 // Runs of segment s lie in its own walker positions, bucket by bucket: buckets are in
 // vertex order and runs within a bucket ascend, so the segment's runs read in bucket
 // order are ascending.
@@ -81,7 +85,6 @@ constexpr int kAux = 4096;                                 // LDS words beside t
 constexpr int kHashMax = kAux / 2;                         // buckets up to this many walkers count in a hash
 constexpr uint32_t kHashEmpty = 0xffffffffu;               // never a packed entry: counts stay <= kHashMax
 constexpr int kRankCap = kAux - kChunkWords;               // run counters per pass of the large-bucket path
-constexpr int kScanBlock = 256;
 
 __device__ inline uint64_t block_sum_u64(uint64_t x, uint64_t *lds) {
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
@@ -144,18 +147,67 @@ __device__ inline int32_t tile_range(const ServeSegment *__restrict__ segs, int3
 // A walker that ended at a vertex (DONE, GATHER_TERMINATED, DEAD_END) is counted at it.
 __device__ inline bool counted(int32_t status, int32_t v, int32_t nv) { return status < 3 && v >= 0 && v < nv; }
 
+// Segment s's scan, by one block of kTileBlock threads: tile_hist[t][c] becomes the first
+// bucket position of tile t's walkers in chunk c; bucket[s][c] = {first position, walkers};
+// the segment's totals go to host memory.
+__device__ void scan_segment(const ServeSegment &sg, int32_t s, const PprScratch &p,
+                             PprSegmentStats *__restrict__ host_stats, uint32_t *lds, uint64_t *red) {
+    const int tid = static_cast<int>(threadIdx.x);
+    const int32_t t0 = sg.first_tile, nt = (sg.count + kTileWalkers - 1) / kTileWalkers;
+    uint32_t carry = static_cast<uint32_t>(sg.batch_offset);
+    for (int32_t c0 = 0; c0 < p.C; c0 += kTileBlock) {
+        const int32_t c = c0 + tid;
+        uint32_t col = 0;
+        if (c < p.C)
+            for (int32_t t = 0; t < nt; ++t) col += p.tile_hist[static_cast<int64_t>(t0 + t) * p.C + c];
+        uint32_t total;
+        const uint32_t first = carry + block_exclusive_u32(col, lds, &total);
+        if (c < p.C) {
+            p.bucket[static_cast<int64_t>(s) * p.C + c] = make_uint2(first, col);
+            uint32_t pos = first;
+            for (int32_t t = 0; t < nt; ++t) {
+                uint32_t *h = p.tile_hist + static_cast<int64_t>(t0 + t) * p.C + c;
+                const uint32_t k = *h;
+                *h = pos;
+                pos += k;
+            }
+        }
+        carry += total;
+    }
+    uint64_t steps = 0, errors = 0;
+    for (int32_t t = tid; t < nt; t += kTileBlock) {
+        steps += p.tile_stats[t0 + t].steps;
+        errors += p.tile_stats[t0 + t].errors;
+    }
+    steps = block_sum_u64(steps, red);
+    errors = block_sum_u64(errors, red);
+    if (tid == 0) {
+        PprSegmentStats o;
+        o.steps = steps;
+        o.errors = errors;
+        host_stats[s] = o;
+    }
+}
+
+// One block per tile.  p.tickets[s] (zeroed with the batch's control block) counts the
+// segment's finished tiles; the block that takes the last ticket scans the segment.  Each
+// block's counts are released (device scope) with its ticket and acquired by the scanning
+// block with its own, so the scan sees every tile of the segment whichever XCD ran it.
 __global__ __launch_bounds__(kTileBlock) void ppr_hist_kernel(const ServeSegment *__restrict__ segs, int32_t n_segs,
                                                               const int32_t *__restrict__ ends,
                                                               const int32_t *__restrict__ lengths,
-                                                              const int32_t *__restrict__ status, PprScratch p) {
+                                                              const int32_t *__restrict__ status, PprScratch p,
+                                                              PprSegmentStats *__restrict__ host_stats) {
     extern __shared__ uint32_t hist[];  // C counters
     __shared__ uint64_t red[kTileBlock / 64];
+    __shared__ uint32_t lds[kTileBlock / 64];
     __shared__ int32_t s_lds;
+    __shared__ int32_t last;
     const int tid = static_cast<int>(threadIdx.x);
     const int32_t t = static_cast<int32_t>(blockIdx.x);
     for (int32_t c = tid; c < p.C; c += kTileBlock) hist[c] = 0u;
     int32_t i0, i1;
-    tile_range(segs, n_segs, t, &s_lds, &i0, &i1);  // its barrier also orders the zeroing
+    const int32_t s = tile_range(segs, n_segs, t, &s_lds, &i0, &i1);  // its barrier also orders the zeroing
     int32_t v[kTilePerThread], st[kTilePerThread], len[kTilePerThread];
 #pragma unroll
     for (int k = 0; k < kTilePerThread; ++k) {
@@ -179,52 +231,19 @@ __global__ __launch_bounds__(kTileBlock) void ppr_hist_kernel(const ServeSegment
         p.tile_stats[t].errors = errors;
     }
     for (int32_t c = tid; c < p.C; c += kTileBlock) p.tile_hist[static_cast<int64_t>(t) * p.C + c] = hist[c];
-}
-
-// One block per segment: tile_hist[t][c] becomes the first bucket position of tile t's
-// walkers in chunk c; bucket[s][c] = {first position, walkers}; the segment's totals go
-// to host memory.
-__global__ __launch_bounds__(kScanBlock) void ppr_scan_kernel(const ServeSegment *__restrict__ segs, PprScratch p,
-                                                              PprSegmentStats *__restrict__ host_stats) {
-    __shared__ uint32_t lds[kScanBlock / 64];
-    __shared__ uint64_t red[kScanBlock / 64];
-    const int tid = static_cast<int>(threadIdx.x);
-    const int32_t s = static_cast<int32_t>(blockIdx.x);
     const ServeSegment sg = segs[s];
-    const int32_t t0 = sg.first_tile, nt = (sg.count + kTileWalkers - 1) / kTileWalkers;
-    uint32_t carry = static_cast<uint32_t>(sg.batch_offset);
-    for (int32_t c0 = 0; c0 < p.C; c0 += kScanBlock) {
-        const int32_t c = c0 + tid;
-        uint32_t col = 0;
-        if (c < p.C)
-            for (int32_t t = 0; t < nt; ++t) col += p.tile_hist[static_cast<int64_t>(t0 + t) * p.C + c];
-        uint32_t total;
-        const uint32_t first = carry + block_exclusive_u32(col, lds, &total);
-        if (c < p.C) {
-            p.bucket[static_cast<int64_t>(s) * p.C + c] = make_uint2(first, col);
-            uint32_t pos = first;
-            for (int32_t t = 0; t < nt; ++t) {
-                uint32_t *h = p.tile_hist + static_cast<int64_t>(t0 + t) * p.C + c;
-                const uint32_t k = *h;
-                *h = pos;
-                pos += k;
-            }
-        }
-        carry += total;
-    }
-    uint64_t steps = 0, errors = 0;
-    for (int32_t t = tid; t < nt; t += kScanBlock) {
-        steps += p.tile_stats[t0 + t].steps;
-        errors += p.tile_stats[t0 + t].errors;
-    }
-    steps = block_sum_u64(steps, red);
-    errors = block_sum_u64(errors, red);
+    // Every wave's counts are written at the barrier; one thread's acquire-release ticket then
+    // releases them (this XCD's L2 written back) and, for the last block, acquires the other
+    // blocks' (stale lines invalidated).  A fence in every wave instead (each one a write-back
+    // and an invalidate of the L2 under the other slots' walks) cost 9% of the saturated rate.
+    __syncthreads();
     if (tid == 0) {
-        PprSegmentStats o;
-        o.steps = steps;
-        o.errors = errors;
-        host_stats[s] = o;
+        const uint32_t nt = static_cast<uint32_t>((sg.count + kTileWalkers - 1) / kTileWalkers);
+        last = __hip_atomic_fetch_add(&p.tickets[s], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nt - 1u;
     }
+    __syncthreads();
+    if (!last) return;
+    scan_segment(sg, s, p, host_stats, lds, red);
 }
 
 __global__ __launch_bounds__(kTileBlock) void ppr_scatter_kernel(const ServeSegment *__restrict__ segs,
@@ -256,7 +275,8 @@ __device__ inline uint32_t hash_slot(uint32_t lv) { return (lv * 2654435761u) >>
 // rank, are the bucket's runs in ascending order.  Counts: a bucket of at most kHashMax
 // walkers counts in one pass, in an LDS hash of packed (vertex in chunk, count) words; a
 // larger one ranks each walker (word prefix + popcount) and counts kRankCap ranks a pass.
-__global__ __launch_bounds__(kChunkBlock) void ppr_chunk_kernel(PprScratch p) {
+__global__ __launch_bounds__(kChunkBlock) void ppr_chunk_kernel(PprScratch p, int2 *__restrict__ host_pairs,
+                                                                uint2 *__restrict__ host_buckets) {
     __shared__ unsigned long long bm[kChunkWords];
     __shared__ uint32_t aux[kAux];
     __shared__ uint32_t lds[kChunkBlock / 64];
@@ -264,7 +284,7 @@ __global__ __launch_bounds__(kChunkBlock) void ppr_chunk_kernel(PprScratch p) {
     const int32_t c = static_cast<int32_t>(blockIdx.x), s = static_cast<int32_t>(blockIdx.y);
     const uint2 b = p.bucket[static_cast<int64_t>(s) * p.C + c];  // {first position, walkers}
     if (b.y == 0) {
-        if (tid == 0) p.bucket_runs[static_cast<int64_t>(s) * p.C + c] = 0u;
+        if (tid == 0) host_buckets[static_cast<int64_t>(s) * p.C + c] = make_uint2(b.x, 0u);
         return;
     }
     const int32_t vbase = c << kChunkShift;
@@ -353,18 +373,11 @@ __global__ __launch_bounds__(kChunkBlock) void ppr_chunk_kernel(PprScratch p) {
             __syncthreads();
         }
     }
-    if (tid == 0) p.bucket_runs[static_cast<int64_t>(s) * p.C + c] = runs;
-}
-
-// grid (C, segments), one wave per bucket: its runs and extent into host memory.
-__global__ __launch_bounds__(64) void ppr_export_kernel(PprScratch p, int2 *__restrict__ host_pairs,
-                                                        uint2 *__restrict__ host_buckets) {
-    const int32_t c = static_cast<int32_t>(blockIdx.x), s = static_cast<int32_t>(blockIdx.y);
-    const int64_t k = static_cast<int64_t>(s) * p.C + c;
-    const uint2 b = p.bucket[k];
-    const uint32_t runs = p.bucket_runs[k];
-    for (uint32_t j = threadIdx.x; j < runs; j += 64) host_pairs[b.x + j] = p.pairs[b.x + j];
-    if (threadIdx.x == 0) host_buckets[k] = make_uint2(b.x, runs);
+    // The runs into host memory, consecutive threads on consecutive runs (the block's own
+    // device writes, visible to it after the barrier).
+    __syncthreads();
+    for (uint32_t j = static_cast<uint32_t>(tid); j < runs; j += kChunkBlock) host_pairs[b.x + j] = p.pairs[b.x + j];
+    if (tid == 0) host_buckets[static_cast<int64_t>(s) * p.C + c] = make_uint2(b.x, runs);
 }
 
 }  // namespace
@@ -381,12 +394,10 @@ hipError_t launch_serve_ppr_reduce(const ServeSegment *segs, int32_t n_segs, int
     const size_t per_chunk = sizeof(uint32_t) * static_cast<size_t>(p.C);
     const dim3 buckets(static_cast<unsigned>(p.C), static_cast<unsigned>(n_segs));
     hipLaunchKernelGGL(ppr_hist_kernel, dim3(n_tiles), dim3(kTileBlock), per_chunk, stream, segs, n_segs, ends,
-                       lengths, status, p);
-    hipLaunchKernelGGL(ppr_scan_kernel, dim3(n_segs), dim3(kScanBlock), 0, stream, segs, p, host_stats);
+                       lengths, status, p, host_stats);
     hipLaunchKernelGGL(ppr_scatter_kernel, dim3(n_tiles), dim3(kTileBlock), per_chunk, stream, segs, n_segs, ends,
                        status, p);
-    hipLaunchKernelGGL(ppr_chunk_kernel, buckets, dim3(kChunkBlock), 0, stream, p);
-    hipLaunchKernelGGL(ppr_export_kernel, buckets, dim3(64), 0, stream, p, host_pairs, host_buckets);
+    hipLaunchKernelGGL(ppr_chunk_kernel, buckets, dim3(kChunkBlock), 0, stream, p, host_pairs, host_buckets);
     return hipGetLastError();
 }
 

@@ -51,8 +51,12 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 }
 
 constexpr int32_t kPoolInts = 1 << 22;  // source-set pool of one batch (16 MiB)
-// A slot's control block in u32 words: the walk's counters (64 bytes).
+// A slot's control block in u32 words, zeroed by each batch's expand launch: the walk's
+// counters (64 bytes), the QUEUE form's work-queue head (its own 64 bytes), then one
+// PPR ticket per segment.
 constexpr int32_t kCtlWords = 16;
+constexpr int32_t kCtlHead = 16;
+constexpr int32_t kCtlTickets = 32;
 // Bound on (segments of a PPR batch) x (chunks of 2^17 vertex ids): the per-bucket
 // arrays of a slot (8 MiB each at most).
 constexpr int64_t kPprBuckets = int64_t(1) << 20;
@@ -135,14 +139,13 @@ enum class SlotState { Free, InFlight, Draining };
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    trw::ServeSegment *d_segs = nullptr, *h_segs = nullptr;
-    int32_t *d_pool = nullptr, *h_pool = nullptr;
+    // A batch's segment table and then its source pool, one copy to the device.
+    unsigned char *d_stage = nullptr, *h_stage = nullptr;
     int32_t *d_sources = nullptr;
     uint64_t *d_ids = nullptr;
     int32_t *d_ends = nullptr, *d_lengths = nullptr, *d_status = nullptr, *d_paths = nullptr;
-    // Control block zeroed by the expand launch: the walk's counters.
+    // Control block zeroed by the expand launch (kCtlWords, kCtlHead, kCtlTickets).
     uint32_t *d_ctl = nullptr;
-    unsigned int *d_queue_head = nullptr;
     int32_t *h_ends = nullptr, *h_lengths = nullptr, *h_status = nullptr, *h_paths = nullptr;
     // PPR aggregation: device scratch, and the pinned memory the device writes each
     // segment's totals, each bucket's {first position, runs} and the runs into.
@@ -292,6 +295,11 @@ bool trw_server::form_batch(Slot &sl, std::unique_lock<std::mutex> &) {
 void trw_server::launch(Slot &sl) {
     // Runs without mu: the slot is InFlight and owned by the dispatcher until its
     // completion event fires.
+    const size_t seg_bytes = sizeof(trw::ServeSegment) * sl.segs.size();  // a multiple of 16
+    trw::ServeSegment *h_segs = reinterpret_cast<trw::ServeSegment *>(sl.h_stage);
+    int32_t *h_pool = reinterpret_cast<int32_t *>(sl.h_stage + seg_bytes);
+    const trw::ServeSegment *d_segs = reinterpret_cast<const trw::ServeSegment *>(sl.d_stage);
+    const int32_t *d_pool = reinterpret_cast<const int32_t *>(sl.d_stage + seg_bytes);
     int32_t pool = 0, tiles = 0;
     for (size_t i = 0; i < sl.segs.size(); ++i) {
         const Segment &s = sl.segs[i];
@@ -304,8 +312,8 @@ void trw_server::launch(Slot &sl) {
         d.first_tile = tiles;
         if (sl.ppr) tiles += trw::ppr_tiles(s.count);
         d.id_base = s.q->id_base;
-        sl.h_segs[i] = d;
-        std::memcpy(sl.h_pool + pool, s.q->sources.data(), sizeof(int32_t) * s.q->sources.size());
+        h_segs[i] = d;
+        std::memcpy(h_pool + pool, s.q->sources.data(), sizeof(int32_t) * s.q->sources.size());
         pool += d.n_sources;
     }
     const int32_t nseg = static_cast<int32_t>(sl.segs.size());
@@ -313,12 +321,10 @@ void trw_server::launch(Slot &sl) {
     const size_t N = static_cast<size_t>(sl.n);
     hipStream_t st = sl.stream;
     std::string msg;
-    hipError_t e = hipMemcpyAsync(sl.d_segs, sl.h_segs, sizeof(trw::ServeSegment) * sl.segs.size(),
+    hipError_t e = hipMemcpyAsync(sl.d_stage, sl.h_stage, seg_bytes + sizeof(int32_t) * static_cast<size_t>(pool),
                                   hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(sl.d_pool, sl.h_pool, sizeof(int32_t) * static_cast<size_t>(pool), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = trw::launch_serve_expand(sl.d_segs, nseg, sl.d_pool, sl.n, sl.d_sources, sl.d_ids, sl.d_ctl, kCtlWords,
+        e = trw::launch_serve_expand(d_segs, nseg, d_pool, sl.n, sl.d_sources, sl.d_ids, sl.d_ctl, kCtlTickets + nseg,
                                      st);
     if (e == hipSuccess) {
         trw_walk_io io;
@@ -332,7 +338,7 @@ void trw_server::launch(Slot &sl) {
         io.ends = sl.d_ends;
         io.status = sl.d_status;
         io.counters = reinterpret_cast<uint64_t *>(sl.d_ctl);
-        const int rc = trw::walk_launch(g, &params, &io, st, sl.d_queue_head, false);
+        const int rc = trw::walk_launch(g, &params, &io, st, sl.d_ctl + kCtlHead, false, true);
         if (rc != TRW_OK) {
             e = hipErrorLaunchFailure;
             msg = trw_last_error();
@@ -341,7 +347,7 @@ void trw_server::launch(Slot &sl) {
     if (sl.ppr) {
         // The device writes the totals, bucket extents and runs straight into the pinned staging.
         if (e == hipSuccess)
-            e = trw::launch_serve_ppr_reduce(sl.d_segs, nseg, tiles, sl.d_ends, sl.d_lengths, sl.d_status, sl.agg,
+            e = trw::launch_serve_ppr_reduce(d_segs, nseg, tiles, sl.d_ends, sl.d_lengths, sl.d_status, sl.agg,
                                              sl.h_stats, sl.h_buckets, sl.h_pairs, st);
     } else {
         if (e == hipSuccess) e = hipMemcpyAsync(sl.h_ends, sl.d_ends, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st);
@@ -587,8 +593,7 @@ int trw_server::submit(const trw_query *q, Clock::time_point arrival, uint64_t *
 void trw_server::free_slots() {
     for (Slot &sl : slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        (void)hipFree(sl.d_segs);
-        (void)hipFree(sl.d_pool);
+        (void)hipFree(sl.d_stage);
         (void)hipFree(sl.d_sources);
         (void)hipFree(sl.d_ids);
         (void)hipFree(sl.d_ends);
@@ -596,9 +601,7 @@ void trw_server::free_slots() {
         (void)hipFree(sl.d_status);
         (void)hipFree(sl.d_paths);
         (void)hipFree(sl.d_ctl);
-        (void)hipFree(sl.d_queue_head);
-        (void)hipHostFree(sl.h_segs);
-        (void)hipHostFree(sl.h_pool);
+        (void)hipHostFree(sl.h_stage);
         (void)hipHostFree(sl.h_ends);
         (void)hipHostFree(sl.h_lengths);
         (void)hipHostFree(sl.h_status);
@@ -608,7 +611,6 @@ void trw_server::free_slots() {
         (void)hipFree(sl.agg.bucketed);
         (void)hipFree(sl.agg.pairs);
         (void)hipFree(sl.agg.bucket);
-        (void)hipFree(sl.agg.bucket_runs);
         (void)hipHostFree(sl.h_stats);
         (void)hipHostFree(sl.h_buckets);
         (void)hipHostFree(sl.h_pairs);
@@ -745,6 +747,7 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
     s->slots.resize(static_cast<size_t>(c.streams));
     const size_t W = static_cast<size_t>(c.max_batch_walkers), P = static_cast<size_t>(s->path_ints),
                  S = static_cast<size_t>(c.max_queries_per_batch);
+    const size_t stage_bytes = sizeof(trw::ServeSegment) * S + sizeof(int32_t) * static_cast<size_t>(kPoolInts);
     // PPR batches: buckets of (segment, chunk of 2^17 vertex ids), kPprBuckets at most.
     s->ppr_chunks = trw::ppr_chunks(g->nv);
     s->ppr_segments = static_cast<int32_t>(std::max<int64_t>(
@@ -755,18 +758,15 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
         // Default stream priority: every stream of the library runs at the same one.
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-        if (e == hipSuccess) e = dev(&sl.d_segs, S);
-        if (e == hipSuccess) e = dev(&sl.d_pool, kPoolInts);
+        if (e == hipSuccess) e = dev(&sl.d_stage, stage_bytes);
         if (e == hipSuccess) e = dev(&sl.d_sources, W);
         if (e == hipSuccess) e = dev(&sl.d_ids, W);
         if (e == hipSuccess) e = dev(&sl.d_ends, W);
         if (e == hipSuccess) e = dev(&sl.d_lengths, W);
         if (e == hipSuccess) e = dev(&sl.d_status, W);
         if (e == hipSuccess) e = dev(&sl.d_paths, P);
-        if (e == hipSuccess) e = dev(&sl.d_ctl, static_cast<size_t>(kCtlWords));
-        if (e == hipSuccess) e = dev(&sl.d_queue_head, 4);
-        if (e == hipSuccess) e = pinned(&sl.h_segs, S);
-        if (e == hipSuccess) e = pinned(&sl.h_pool, kPoolInts);
+        if (e == hipSuccess) e = dev(&sl.d_ctl, static_cast<size_t>(kCtlTickets) + S);
+        if (e == hipSuccess) e = pinned(&sl.h_stage, stage_bytes);
         if (e == hipSuccess) e = pinned(&sl.h_ends, W);
         if (e == hipSuccess) e = pinned(&sl.h_lengths, W);
         if (e == hipSuccess) e = pinned(&sl.h_status, W);
@@ -779,7 +779,7 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
         if (e == hipSuccess) e = dev(&x.bucketed, W);
         if (e == hipSuccess) e = dev(&x.pairs, W);
         if (e == hipSuccess) e = dev(&x.bucket, PS * PC);
-        if (e == hipSuccess) e = dev(&x.bucket_runs, PS * PC);
+        x.tickets = sl.d_ctl ? sl.d_ctl + kCtlTickets : nullptr;
         if (e == hipSuccess) e = pinned(&sl.h_stats, PS);
         if (e == hipSuccess) e = pinned(&sl.h_buckets, PS * PC);
         if (e == hipSuccess) e = pinned(&sl.h_pairs, W);
@@ -792,8 +792,8 @@ int trw_server_create(trw_graph *g, const trw_server_config *config, trw_server 
     // and the device, a kernel, its event -- before the first query: the runtime sets a stream up
     // on its first use, which otherwise lands inside the latency of the first batches on that slot.
     for (Slot &sl : s->slots) {
-        if (e == hipSuccess) e = hipMemcpyAsync(sl.d_segs, sl.h_segs, sizeof(trw::ServeSegment), hipMemcpyHostToDevice, sl.stream);
-        if (e == hipSuccess) e = trw::launch_fill_u32(sl.d_queue_head, 0u, 1, sl.stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(sl.d_stage, sl.h_stage, sizeof(trw::ServeSegment), hipMemcpyHostToDevice, sl.stream);
+        if (e == hipSuccess) e = trw::launch_fill_u32(sl.d_ctl, 0u, kCtlTickets, sl.stream);
         if (e == hipSuccess) e = hipMemcpyAsync(sl.h_ends, sl.d_ends, sizeof(int32_t), hipMemcpyDeviceToHost, sl.stream);
         if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
     }

@@ -2116,14 +2116,15 @@ hipError_t launch_walk_fused(const DevGraph &g, const KindParams &p, const Batch
 }
 
 hipError_t launch_walk_queue(const DevGraph &g, const KindParams &a, const KindParams &b, int32_t split,
-                             const BatchIO &io, unsigned int *queue_counter, hipStream_t stream) {
+                             const BatchIO &io, unsigned int *queue_counter, hipStream_t stream,
+                             bool head_zeroed) {
     if (io.n <= 0) return hipSuccess;
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
     }
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, queue_counter, 0u, int64_t(1));
+    if (!head_zeroed) hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, stream, queue_counter, 0u, int64_t(1));
     if (io.replay) queue_dispatch<true>(g, a, b, split, io, queue_counter, stream, cus);
     else queue_dispatch<false>(g, a, b, split, io, queue_counter, stream, cus);
     return hipGetLastError();

@@ -752,6 +752,44 @@ def test_full_size_deepwalk_alias_graph(gpu_ctx):
         dg.close()
 
 
+@pytest.mark.parametrize("sm", ["alias", "its"])
+def test_full_size_replay_of_every_walker_is_bit_exact(gpu_ctx, sm):
+    """The DeepWalk configuration at full size, as the headline runs it: the 2^22-vertex Chung-Lu graph
+    (exponent 2.1, average degree 16, dmin 2, dmax 65536, FP64 U[1,5) weights), one walker per
+    non-isolated vertex (4,077,405), length 80, static alias and static ITS (guide records), replay
+    draws from the SFMT stream: every walker's path, length, end, status and draws used equal the
+    oracle's (run over walker ranges on the host's cores)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    h = T.HostGraph.chung_lu(1 << 22, 8 << 22, 2.1, 2, 65536, 2, True, 0)
+    deg, adj, w, _ = h.arrays()
+    g = graphs.CSR(deg.copy(), adj.copy(), w.copy())
+    h.close()
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight)
+    try:
+        dg.build_tables(sm, "edge")
+        og = ob.Graph(g, ob.alias_build(g, g.weight)[0]) if sm == "alias" else ob.Graph(g, prefix=ob.its_build(g, g.weight))
+        src = T.sources_all(g.degree)
+        n, K, L = len(src), 158, 80
+        case = {"app": "deepwalk", "execution": 1, "sampling": 2 if sm == "alias" else 1, "length": L, "extra": {}}
+        r = T.walk(dg, case_params(case, "fused"), src, T.replay_fill(1234, n, K))
+        print("full-size replay (%s): %d walkers, %d steps" % (sm, n, r["steps"]), flush=True)
+        assert r["steps"] == n * (L - 1)
+        d = ob.sfmt_stream(1234, n * K).reshape(n, K)
+        cuts = np.linspace(0, n, 33).astype(np.int64)
+
+        def part(i):
+            a, b = int(cuts[i]), int(cuts[i + 1])
+            return ob.walk(og, "deepwalk", 1, case["sampling"], L, src[a:b], d[a:b], K)
+
+        with ThreadPoolExecutor(16) as pool:
+            parts = list(pool.map(part, range(len(cuts) - 1)))
+        for key in ("paths", "lengths", "ends", "status", "draws_used"):
+            assert np.array_equal(r[key], np.concatenate([p[key] for p in parts])), key
+    finally:
+        dg.close()
+
+
 def test_upload_rejects_vertex_ids_out_of_range(gpu_ctx):
     g = graphs.chung_lu(3000, 20000, 2.1, 2, 300, 17)
     for bad in (-1, g.nv):

@@ -790,6 +790,132 @@ def test_full_size_replay_of_every_walker_is_bit_exact(gpu_ctx, sm):
         dg.close()
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_full_size_node2vec_replay_is_bit_exact(gpu_ctx, weighted):
+    """The Node2Vec configuration at full size: the 2^22-vertex Chung-Lu graph (exponent 2.0, average
+    degree 32, dmin 2, dmax 65536, FP64 U[1,5) weights; hubs of degree up to 57,998 take the hashed
+    membership tables), p = 2, q = 0.5, length 80, max-weight rejection as the reference's functor
+    (weights ignored) and over the edge weights; every 8th of the one-walker-per-vertex sources
+    (519,518 walkers), replay draws (1,600 per walker, 2,400 over the weights: a walker that runs out
+    ends as exhausted in both): paths, lengths, ends, statuses and draws used equal the oracle's."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    h = T.HostGraph.chung_lu(1 << 22, 16 << 22, 2.0, 2.0, 65536.0, 3, True, 0)
+    deg, adj, w, _ = h.arrays()
+    g = graphs.CSR(deg.copy(), adj.copy(), w.copy())
+    h.close()
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight)
+    try:
+        src = T.sources_all(g.degree)[::8].copy()
+        n, L, K = len(src), 80, 2400 if weighted else 1600
+        prm = T.make_params("node2vec", length=L, p=2.0, q=0.5, rng="replay", weights="edge" if weighted else "app")
+        r = T.walk(dg, prm, src, T.replay_fill(5, n, K))
+        print("full-size node2vec replay (weighted %s): %d walkers, %d steps" % (weighted, n, r["steps"]), flush=True)
+        og = ob.Graph(g)
+        d = ob.sfmt_stream(5, n * K).reshape(n, K)
+        cuts = np.linspace(0, n, 33).astype(np.int64)
+
+        def part(i):
+            a, b = int(cuts[i]), int(cuts[i + 1])
+            return ob.walk(og, "node2vec", 2, 4, L, src[a:b], d[a:b], K, p=2.0, q=0.5, weighted=weighted)
+
+        with ThreadPoolExecutor(16) as pool:
+            parts = list(pool.map(part, range(len(cuts) - 1)))
+        st = np.concatenate([p["status"] for p in parts])
+        print("oracle statuses:", np.bincount(st), flush=True)
+        assert np.mean(st == 0) > 0.9999  # the draws cover (nearly) every walk in full
+        for key in ("paths", "lengths", "ends", "status", "draws_used"):
+            assert np.array_equal(r[key], np.concatenate([p[key] for p in parts])), key
+        assert r["steps"] == sum(int(p["steps"]) for p in parts)
+    finally:
+        dg.close()
+
+
+def test_full_size_ppr_metapath_mix(gpu_ctx):
+    """The uneven mix at full size: the 2^22-vertex headline graph with edge labels U{0..4}, 2^23
+    walkers from uniform random sources, walkers [0, 2^22) PPR (termination 0.15, length 1), the rest
+    MetaPath on schema 0,1,2,3,4 (length 80), each kind's rows at its own length.
+      * Free-running: the QUEUE form (padded and unpadded rows), the hipGraph step form and the plain
+        step form give the same walks, step for step, and each kind alone gives what it gives in the mix.
+      * Replay: every 16th walker of the batch (2^19, the same split) in the QUEUE form equals the
+        oracle's PPR and MetaPath walks."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    h = T.HostGraph.chung_lu(1 << 22, 8 << 22, 2.1, 2.0, 65536.0, 2, True, 5)
+    deg, adj, w, lab = h.arrays()
+    g = graphs.CSR(deg.copy(), adj.copy(), w.copy(), lab.copy())
+    h.close()
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight, g.label)
+    schema = [0, 1, 2, 3, 4]
+    try:
+        n = 1 << 23
+        split = n // 2
+        src = T.sources_random(g.degree, n, 4)
+
+        def kinds(form, rng="philox"):
+            return (T.make_params("ppr", length=1, stop_probability=0.15, rng=rng, seed=4, form=form),
+                    T.make_params("metapath", length=80, schema=schema, rng=rng, seed=4, form=form))
+
+        res = {}
+        for form, unpadded in (("queue", False), ("queue", True), ("graph", False), ("step", False)):
+            a, b = kinds(form)
+            res[(form, unpadded)] = T.walk(dg, a, src, mix=(b, split), kind_rows=True, unpadded=unpadded)
+            print("full-size mix, %s%s: %d steps" % (form, " unpadded" if unpadded else "",
+                                                    res[(form, unpadded)]["steps"]), flush=True)
+        ref = res[("queue", False)]
+        for k, r in res.items():
+            for key in ("lengths", "ends", "status", "paths_a"):
+                assert np.array_equal(r[key], ref[key]), (k, key)
+            assert r["steps"] == ref["steps"], k
+            pb = r["paths_b"]
+            if k[1]:  # unpadded: the entries of each walk, the rest untouched
+                inside = np.arange(80)[None, :] < ref["lengths"][split:, None]
+                assert np.array_equal(np.where(inside, pb, -1), ref["paths_b"]), k
+            else:
+                assert np.array_equal(pb, ref["paths_b"]), k
+        a, b = kinds("queue")
+        ra = T.walk(dg, a, src[:split])
+        rb = T.walk(dg, b, src[split:], walker_id_base=split)
+        assert np.array_equal(ra["paths"][:, :1], ref["paths_a"]) and np.array_equal(rb["paths"], ref["paths_b"])
+        for key in ("lengths", "ends", "status"):
+            assert np.array_equal(np.concatenate([ra[key], rb[key]]), ref[key]), key
+        del res, ra, rb
+
+        sub = src[::16].copy()
+        m, ms = len(sub), len(sub) // 2
+        K = KT = 160
+        a, b = kinds("queue", "replay")
+        r = T.walk(dg, a, sub, T.replay_fill(41, m, K), T.replay_fill(42, m, KT), mix=(b, ms), kind_rows=True)
+        d = ob.sfmt_stream(41, m * K).reshape(m, K)
+        td = ob.sfmt_stream(42, m * KT).reshape(m, KT)
+        og = ob.Graph(g)
+        cuts = np.linspace(0, m, 33).astype(np.int64)
+
+        def part(i):
+            lo, hi = int(cuts[i]), int(cuts[i + 1])
+            out = []
+            if lo < ms:
+                e = min(hi, ms)
+                out.append(ob.walk(og, "ppr", 0, 0, 1, sub[lo:e], d[lo:e], K, td[lo:e], KT, stop_prob=0.15))
+            if hi > ms:
+                s0 = max(lo, ms)
+                out.append(ob.walk(og, "metapath", 2, 1, 80, sub[s0:hi], d[s0:hi], K, td[s0:hi], KT, schema=schema))
+            return out
+
+        with ThreadPoolExecutor(16) as pool:
+            parts = [x for p in pool.map(part, range(len(cuts) - 1)) for x in p]
+        oracle = {key: np.concatenate([p[key] for p in parts]) for key in ("lengths", "ends", "status", "draws_used")}
+        print("full-size mix replay: %d walkers, oracle statuses %s" % (m, np.bincount(oracle["status"])), flush=True)
+        for key in oracle:
+            assert np.array_equal(r[key], oracle[key]), key
+        pa = np.concatenate([p["paths"] for p in parts if p["paths"].shape[1] == 1])
+        pb = np.concatenate([p["paths"] for p in parts if p["paths"].shape[1] == 80])
+        assert np.array_equal(r["paths_a"], pa) and np.array_equal(r["paths_b"], pb)
+        assert r["steps"] == sum(int(p["steps"]) for p in parts)
+    finally:
+        dg.close()
+
+
 def test_upload_rejects_vertex_ids_out_of_range(gpu_ctx):
     g = graphs.chung_lu(3000, 20000, 2.1, 2, 300, 17)
     for bad in (-1, g.nv):

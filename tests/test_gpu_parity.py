@@ -831,6 +831,49 @@ def test_full_size_node2vec_replay_is_bit_exact(gpu_ctx, weighted):
         dg.close()
 
 
+@pytest.mark.parametrize("sources", ["all", "single"])
+def test_full_size_c1_ppr_replay_is_bit_exact(gpu_ctx, sources):
+    """The reference's own CPU-runnable PPR configuration at full size: the 2^18-vertex Chung-Lu graph
+    (exponent 2.1, dmin 2, dmax 4096, 1.5 M undirected edges, unweighted), uniform Moves, termination
+    0.2, one walker per non-isolated vertex (generate_all_source) or |V| walkers from the max-degree
+    vertex (the app's default single source); replay draws: every walker's end, length, status, draws
+    used and source entry equal the oracle's in the QUEUE, STEP and GRAPH forms."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    h = T.HostGraph.chung_lu(1 << 18, 1500000, 2.1, 2.0, 4096.0, 1, False, 0)
+    deg, adj, _, _ = h.arrays()
+    g = graphs.CSR(deg.copy(), adj.copy())
+    h.close()
+    dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj)
+    try:
+        src = T.sources_all(g.degree) if sources == "all" else T.sources_single(g.degree, g.nv)
+        n, K = len(src), 128
+        draws, tdraws = T.replay_fill(51, n, K), T.replay_fill(52, n, K)
+        d = ob.sfmt_stream(51, n * K).reshape(n, K)
+        td = ob.sfmt_stream(52, n * K).reshape(n, K)
+        og = ob.Graph(g)
+        cuts = np.linspace(0, n, 17).astype(np.int64)
+
+        def part(i):
+            a, b = int(cuts[i]), int(cuts[i + 1])
+            return ob.walk(og, "ppr", 0, 0, 1, src[a:b], d[a:b], K, td[a:b], K, stop_prob=0.2)
+
+        with ThreadPoolExecutor(16) as pool:
+            parts = list(pool.map(part, range(len(cuts) - 1)))
+        ref = {key: np.concatenate([p[key] for p in parts]) for key in ("paths", "lengths", "ends", "status",
+                                                                       "draws_used")}
+        assert np.all(ref["status"] == 0)
+        for form in ("queue", "step", "graph"):
+            prm = T.make_params("ppr", stop_probability=0.2, rng="replay", form=form)
+            r = T.walk(dg, prm, src, draws, tdraws)
+            for key in ref:
+                assert np.array_equal(r[key], ref[key]), (form, key)
+            assert r["steps"] == sum(int(p["steps"]) for p in parts), form
+        print("C1 replay (%s sources): %d walkers, %d steps" % (sources, n, r["steps"]), flush=True)
+    finally:
+        dg.close()
+
+
 def test_full_size_ppr_metapath_mix(gpu_ctx):
     """The uneven mix at full size: the 2^22-vertex headline graph with edge labels U{0..4}, 2^23
     walkers from uniform random sources, walkers [0, 2^22) PPR (termination 0.15, length 1), the rest

@@ -52,9 +52,8 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
 
 constexpr int32_t kPoolInts = 1 << 22;  // source-set pool of one batch (16 MiB)
 // A slot's control block in u32 words, zeroed by each batch's expand launch: the walk's
-// counters (64 bytes), the QUEUE form's work-queue head (its own 64 bytes), then one
-// PPR ticket per segment.
-constexpr int32_t kCtlWords = 16;
+// counters in words [0, 16) (64 bytes), the QUEUE form's work-queue head at kCtlHead (its own
+// 64 bytes), then one PPR ticket per segment from kCtlTickets.
 constexpr int32_t kCtlHead = 16;
 constexpr int32_t kCtlTickets = 32;
 // Bound on (segments of a PPR batch) x (chunks of 2^17 vertex ids): the per-bucket
@@ -144,7 +143,7 @@ struct Slot {
     int32_t *d_sources = nullptr;
     uint64_t *d_ids = nullptr;
     int32_t *d_ends = nullptr, *d_lengths = nullptr, *d_status = nullptr, *d_paths = nullptr;
-    // Control block zeroed by the expand launch (kCtlWords, kCtlHead, kCtlTickets).
+    // Control block zeroed by the expand launch (walk counters, kCtlHead, kCtlTickets).
     uint32_t *d_ctl = nullptr;
     int32_t *h_ends = nullptr, *h_lengths = nullptr, *h_status = nullptr, *h_paths = nullptr;
     // PPR aggregation: device scratch, and the pinned memory the device writes each

@@ -752,13 +752,18 @@ def test_full_size_deepwalk_alias_graph(gpu_ctx):
         dg.close()
 
 
-@pytest.mark.parametrize("sm", ["alias", "its"])
-def test_full_size_replay_of_every_walker_is_bit_exact(gpu_ctx, sm):
+# (Move, execution, sampling, every how many of the walkers, draws per walker)
+FULL_SIZE_MOVES = [("alias", 1, 2, 1, 158), ("its", 1, 1, 1, 158), ("uniform", 0, 0, 1, 79), ("rejection", 1, 3, 4, 400)]
+
+
+@pytest.mark.parametrize("sm,execution,sampling,every,K", FULL_SIZE_MOVES, ids=[m[0] for m in FULL_SIZE_MOVES])
+def test_full_size_replay_of_every_walker_is_bit_exact(gpu_ctx, sm, execution, sampling, every, K):
     """The DeepWalk configuration at full size, as the headline runs it: the 2^22-vertex Chung-Lu graph
     (exponent 2.1, average degree 16, dmin 2, dmax 65536, FP64 U[1,5) weights), one walker per
-    non-isolated vertex (4,077,405), length 80, static alias and static ITS (guide records), replay
-    draws from the SFMT stream: every walker's path, length, end, status and draws used equal the
-    oracle's (run over walker ranges on the host's cores)."""
+    non-isolated vertex (4,077,405), length 80, replay draws from the SFMT stream; static alias, static
+    ITS (guide records) and uniform on every walker, static rejection (a trial per two draws) on every
+    4th: every walker's path, length, end, status and draws used equal the oracle's (run over walker
+    ranges on the host's cores)."""
     from concurrent.futures import ThreadPoolExecutor
 
     h = T.HostGraph.chung_lu(1 << 22, 8 << 22, 2.1, 2, 65536, 2, True, 0)
@@ -767,11 +772,15 @@ def test_full_size_replay_of_every_walker_is_bit_exact(gpu_ctx, sm):
     h.close()
     dg = T.DeviceGraph(gpu_ctx, g.degree, g.adj, g.weight)
     try:
-        dg.build_tables(sm, "edge")
-        og = ob.Graph(g, ob.alias_build(g, g.weight)[0]) if sm == "alias" else ob.Graph(g, prefix=ob.its_build(g, g.weight))
-        src = T.sources_all(g.degree)
-        n, K, L = len(src), 158, 80
-        case = {"app": "deepwalk", "execution": 1, "sampling": 2 if sm == "alias" else 1, "length": L, "extra": {}}
+        if sm != "uniform":
+            dg.build_tables(sm, "edge")
+        og = (ob.Graph(g, ob.alias_build(g, g.weight)[0]) if sm == "alias"
+              else ob.Graph(g, prefix=ob.its_build(g, g.weight)) if sm == "its"
+              else ob.Graph(g, rs_weight=g.weight, rs_max=ob.rs_build(g, g.weight)) if sm == "rejection"
+              else ob.Graph(g))
+        src = T.sources_all(g.degree)[::every].copy()
+        n, L = len(src), 80
+        case = {"app": "deepwalk", "execution": execution, "sampling": sampling, "length": L, "extra": {}}
         r = T.walk(dg, case_params(case, "fused"), src, T.replay_fill(1234, n, K))
         print("full-size replay (%s): %d walkers, %d steps" % (sm, n, r["steps"]), flush=True)
         assert r["steps"] == n * (L - 1)
@@ -780,7 +789,7 @@ def test_full_size_replay_of_every_walker_is_bit_exact(gpu_ctx, sm):
 
         def part(i):
             a, b = int(cuts[i]), int(cuts[i + 1])
-            return ob.walk(og, "deepwalk", 1, case["sampling"], L, src[a:b], d[a:b], K)
+            return ob.walk(og, "deepwalk", execution, sampling, L, src[a:b], d[a:b], K)
 
         with ThreadPoolExecutor(16) as pool:
             parts = list(pool.map(part, range(len(cuts) - 1)))

@@ -302,8 +302,11 @@ int trw_graph_get_move_tuning(const trw_graph *g, int32_t move, int32_t knob, in
  * of every knob that applies -- walkers per lane for FUSED uniform/alias/ITS records,
  * the ITS tree degree threshold, the QUEUE chunk, and the launch shapes of the walk's
  * Move: FUSED threads per block and path tile, QUEUE blocks per CU, Node2Vec threads per
- * block -- one knob at a time, and keeps the fastest on the graph (shape knobs for the
- * walk's Move).  Walks are identical under every setting, so tuning never changes results. */
+ * block, GRAPH steps per launch -- one knob at a time (the QUEUE chunk and blocks per CU as
+ * every pair of the two), and keeps the fastest on the graph (shape knobs for the walk's
+ * Move).  Each candidate is timed at least three times, and a short sample until 20 ms have
+ * been timed (at most 63 runs); the median counts.  Walks are identical under every setting,
+ * so tuning never changes results. */
 typedef struct {
     int32_t knobs;                 /* entries used below */
     int32_t move;                  /* TRW_MOVE_* of the walk (the shape knobs were kept for it) */

@@ -53,7 +53,10 @@ int time_walk(trw_graph *g, const trw_walk_params &p, const trw_walk_params *pb,
     std::vector<double> ms;
     uint64_t steps = 0;
     int rc = TRW_OK;
-    for (int rep = 0; rep < 4 && rc == TRW_OK; ++rep) {
+    // At least three timed runs after a warm one, and more for a short walk (a 0.1 ms batch moves by
+    // a few microseconds from run to run): until 20 ms have been timed, at most 64 runs.
+    double timed = 0.0;
+    for (int rep = 0; rc == TRW_OK && (rep < 4 || (timed < 20.0 && rep < 64)); ++rep) {
         if (trw::launch_fill_u32(reinterpret_cast<unsigned int *>(s.counters), 0u, 8, st) != hipSuccess) {
             rc = trw::set_error(TRW_E_HIP, "autotune counter reset");
             break;
@@ -68,7 +71,10 @@ int time_walk(trw_graph *g, const trw_walk_params &p, const trw_walk_params *pb,
         }
         float t = 0.f;
         (void)hipEventElapsedTime(&t, a, b);
-        if (rep > 0) ms.push_back(t);
+        if (rep > 0) {
+            ms.push_back(t);
+            timed += t;
+        }
         if (hipMemcpy(&steps, s.counters, sizeof(steps), hipMemcpyDeviceToHost) != hipSuccess)
             rc = trw::set_error(TRW_E_HIP, "autotune counters");
     }
@@ -167,10 +173,9 @@ int tune(trw_graph *g, const trw_walk_params *params, const trw_walk_params *pb,
     std::vector<Knob> knobs;
     if (fused_records) knobs.push_back({TRW_TUNE_WALKERS_PER_THREAD, {1, 2, 4}});
     if (its) knobs.push_back({TRW_TUNE_ITS_TREE_MIN_DEGREE, {0, 32, 128, 512, INT32_MAX}});
-    if (form == TRW_FORM_QUEUE) {
-        knobs.push_back({TRW_TUNE_QUEUE_CHUNK, {0, 64, 256, 1024, 4096}});
-        knobs.push_back({TRW_TUNE_QUEUE_BLOCKS_PER_CU, {0, 1, 2, 4}});
-    }
+    // The QUEUE form's chunk and blocks per CU act together (a smaller grid wants other chunks), so
+    // every pair of them is timed rather than one knob after the other.
+    const std::vector<int64_t> queue_chunks = {0, 64, 256, 1024, 4096}, queue_bpcs = {0, 1, 2, 4};
     if (form == TRW_FORM_GRAPH) knobs.push_back({TRW_TUNE_GRAPH_BURST, {0, 4, 8, 32, 64}});
     if (form == TRW_FORM_FUSED) {
         if (node2vec) knobs.push_back({TRW_TUNE_N2V_BLOCK, {256, 128}});
@@ -193,6 +198,37 @@ int tune(trw_graph *g, const trw_walk_params *params, const trw_walk_params *pb,
     }
     report->move = move;
     double best = report->default_steps_per_s;
+    if (form == TRW_FORM_QUEUE) {
+        int64_t best_c = get(TRW_TUNE_QUEUE_CHUNK), best_b = get(TRW_TUNE_QUEUE_BLOCKS_PER_CU);
+        for (int64_t b : queue_bpcs) {
+            for (int64_t c : queue_chunks) {
+                get(TRW_TUNE_QUEUE_CHUNK) = c;
+                get(TRW_TUNE_QUEUE_BLOCKS_PER_CU) = b;
+                double sps = 0.0;
+                rc = time_walk(g, *params, pb, split, s, &sps);
+                if (rc != TRW_OK) {
+                    restore();
+                    return rc;
+                }
+                if (sps > best * 1.01) {  // keep the earlier (default-first) pair unless clearly faster
+                    best = sps;
+                    best_c = c;
+                    best_b = b;
+                }
+            }
+        }
+        get(TRW_TUNE_QUEUE_CHUNK) = best_c;
+        get(TRW_TUNE_QUEUE_BLOCKS_PER_CU) = best_b;
+        const int64_t chosen[2] = {best_c, best_b};
+        const int32_t ids[2] = {TRW_TUNE_QUEUE_CHUNK, TRW_TUNE_QUEUE_BLOCKS_PER_CU};
+        const int32_t counts[2] = {static_cast<int32_t>(queue_chunks.size()), static_cast<int32_t>(queue_bpcs.size())};
+        for (int k = 0; k < 2; ++k) {
+            const int i = report->knobs++;
+            report->knob[i] = ids[k];
+            report->chosen[i] = chosen[k];
+            report->candidates[i] = counts[k];
+        }
+    }
     for (const Knob &kn : knobs) {
         int64_t best_v = get(kn.id);
         for (int64_t v : kn.values) {

@@ -456,11 +456,13 @@ void trw_server::drain(Slot *sl, hipError_t err, std::string msg) {
             const uint2 *bk = sl->h_buckets + j * C;
             size_t nr = 0;
             for (size_t c = 0; c < C; ++c) nr += bk[c].y;
-            std::vector<Run> runs(nr);
-            size_t k = 0;
+            // Appended bucket by bucket into reserved storage: a sized vector would first zero
+            // every run (half a megabyte for a query of 10^5 walkers) only to overwrite it.
+            std::vector<Run> runs;
+            runs.reserve(nr);
             for (size_t c = 0; c < C; ++c) {
-                std::memcpy(runs.data() + k, sl->h_pairs + bk[c].x, sizeof(Run) * bk[c].y);
-                k += bk[c].y;
+                const Run *from = reinterpret_cast<const Run *>(sl->h_pairs + bk[c].x);
+                runs.insert(runs.end(), from, from + bk[c].y);
             }
             {
                 std::lock_guard<std::mutex> lk(q.merge_mu);

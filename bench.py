@@ -803,8 +803,8 @@ def main():
         record_traffic(args.record_traffic)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU; this build has no multi-GPU path (walkers would shard by "
-              "contiguous ranges with the graph replicated, see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU; the library has no multi-GPU path (walkers would shard by "
+              "contiguous ranges, the graph and its tables replicated on every GPU)", file=sys.stderr)
         sys.exit(2)
     if args.only in ("serving", "c1", "c3"):
         ctx = T.Context(0)

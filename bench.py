@@ -415,13 +415,19 @@ def tune_record(tune):
             "tuned_steps_per_s": round(tune["tuned_steps_per_s"], 1)}
 
 
-def its_walk(ctx, dg, batch, length, per_launch, args):
+def its_walk(ctx, dg, batch, length, per_launch, args, hg=None):
     """DeepWalk with static ITS and with static rejection sampling on the same graph and walkers
     (paths written).  ITS: prefix sums, search trees and guide records built on the device (a
     fused step is one guide-record read); rejection: maxima and rejection records (a trial is
-    one record read).  Launch knobs autotuned."""
+    one record read).  Launch knobs autotuned.  Beside each, the reference's own deepwalk.out
+    with the same sampler (-em 1 -sm 1 / 3) on the host cores, unless --no-cpu-baseline."""
     out = {}
-    for sm, key in (("its", "its"), ("rejection", "rejection")):
+    for sm, key, code in (("its", "its", "1"), ("rejection", "rejection", "3")):
+        if hg is not None and not args.no_cpu_baseline:
+            ref = reference_run(hg, "deepwalk", ["-ew", "-l", str(length), "-em", "1", "-sm", code])
+            if ref:
+                out["reference_" + key + "_steps_per_s"] = ref["value"]
+                out["reference_" + key + "_cores"] = ref["cores"]
         dg.build_tables(sm, "edge")
         out[key + "_table_build_ms"] = round(dg.last_build_ms(), 3)
         prm = T.make_params("deepwalk", execution="static", sampling=sm, length=length, rng="philox", seed=7,
@@ -431,7 +437,9 @@ def its_walk(ctx, dg, batch, length, per_launch, args):
         batch.reset()
         ms = time_launches(ctx, dg, prm, batch.io(), 1, args.steps)
         out[key + "_steps_per_s"] = per_launch / (ms / 1e3 / args.steps)
-        log("fused %s (philox, tuned %s): %.3e steps/s" % (sm, tune["chosen"], out[key + "_steps_per_s"]))
+        log("fused %s (philox, tuned %s): %.3e steps/s, reference %s" % (
+            sm, tune["chosen"], out[key + "_steps_per_s"],
+            "%.3e" % out["reference_" + key + "_steps_per_s"] if "reference_" + key + "_steps_per_s" in out else "n/a"))
     return out
 
 
@@ -839,7 +847,7 @@ def main():
     n = len(src)
     batch = Batch(ctx, src, length)
     if args.only == "its":
-        print(json.dumps(its_walk(ctx, dg, batch, length, (length - 1) * n, args)), flush=True)
+        print(json.dumps(its_walk(ctx, dg, batch, length, (length - 1) * n, args, hg)), flush=True)
         batch.free()
         dg.close()
         ctx.close()
@@ -880,7 +888,7 @@ def main():
         details["alias_replay_roofline_frac"] = v_r * B_STEP["alias_replay"] / 1e9 / HBM_PEAK_GBS
         log("fused alias (replay): %.3e steps/s" % v_r)
 
-        details.update(its_walk(ctx, dg, batch, length, per_launch, args))
+        details.update(its_walk(ctx, dg, batch, length, per_launch, args, hg))
 
         # The same walk through the host-memory entry point: sources in and 1.3 GB of paths out over
         # PCIe (pageable host buffers), device buffers allocated by the call.  Reported beside the

@@ -443,7 +443,7 @@ def its_walk(ctx, dg, batch, length, per_launch, args, hg=None):
     return out
 
 
-def mix_forms(ctx, dg, deg, args):
+def mix_forms(ctx, dg, deg, args, hg=None):
     """Uneven PPR + MetaPath mix: 2^23 walkers from uniform random non-isolated sources
     (generate_random_source), walkers [0, 2^22) PPR with termination 0.15 and length 1
     (g_para.length_ = 1, ppr.cpp:51: the path row holds the source), the rest MetaPath on
@@ -489,6 +489,18 @@ def mix_forms(ctx, dg, deg, args):
     out["batched_vs_plain"] = out["queue_steps_per_s"] / out["step_steps_per_s"]
     out["graph_vs_plain"] = out["graph_steps_per_s"] / out["step_steps_per_s"]
     batch.free()
+    # The reference's CPU path for each kind on the same graph (it has no mixed batch): its ppr.out at
+    # termination 0.15 (|V| walkers from the max-degree vertex, the app's default) and its
+    # metapath.out on schema 0,1,2,3,4 with length 80 (one walker per non-isolated vertex).
+    if hg is not None and not args.no_cpu_baseline:
+        for app, flags in (("ppr", ["-sp", "0.15"]), ("metapath", ["-el", "-s", "0,1,2,3,4", "-l", "80"])):
+            ref = reference_run(hg, app, flags)
+            if ref:
+                out["reference_%s_steps_per_s" % app] = ref["value"]
+                out["reference_%s_cores" % app] = ref["cores"]
+        log("mix kinds on the reference's CPU path: PPR %s, MetaPath %s steps/s"
+            % tuple("%.3e" % out["reference_%s_steps_per_s" % a] if "reference_%s_steps_per_s" % a in out else "n/a"
+                    for a in ("ppr", "metapath")))
     return out
 
 
@@ -837,7 +849,7 @@ def main():
 
     if args.only in ("mix", "functor"):
         if args.only == "mix":
-            print(json.dumps({"mix_ppr_metapath": mix_forms(ctx, dg, deg, args)}), flush=True)
+            print(json.dumps({"mix_ppr_metapath": mix_forms(ctx, dg, deg, args, hg)}), flush=True)
         else:
             print(json.dumps({"functor_engine": functor_engine(ctx, dg, hg, deg, args)}), flush=True)
         dg.close()
@@ -918,7 +930,7 @@ def main():
             % (details["probe_gathers_per_s_4GiB"], details["probe_32B_records_per_s_2GiB"]))
 
     if not args.quick:
-        details["mix_ppr_metapath"] = mix_forms(ctx, dg, deg, args)
+        details["mix_ppr_metapath"] = mix_forms(ctx, dg, deg, args, hg)
         details["functor_engine"] = functor_engine(ctx, dg, hg, deg, args)
     batch.free()
     dg.close()
